@@ -1,0 +1,149 @@
+/*
+ * segclip.h -- C ABI of libsegclip, the MI355X (gfx950) segment-tree polygon
+ * clipping path.
+ *
+ * All entry points take plain pointers and sizes, return an int status
+ * (SEGCLIP_OK = 0) and never exit the process; segclip_last_error() gives
+ * the message of the last failure on a context.  One context holds one
+ * problem (a base ring and K overlay rings) resident on one GPU and runs its
+ * kernels in order on one HIP stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   segclip_set_polygons        gpc_read_multi_polygon (src/clip.cpp:172-311) and
+ *                               SegTree::copyInputDataToGPU (oacc_segment_tree.cpp:2670-2738)
+ *   segclip_build_elementary    SegTree::constructElementaryIntervalsArrayGPU
+ *                               (oacc:3997-4510; multicore twin oacc:4513-4692)
+ *   segclip_build_node_intervals SegTree::initHybrid + constructTreeIntervalArray
+ *                               (segment_tree.h:183-199, oacc:4865-4943)
+ *   segclip_build_end_lists     SegTree::insertAllToEdgeListParallel (oacc:1339-1709)
+ *   segclip_materialize_end_lists  the per-level end-list array of buildInternalEndListsGPU
+ *                               (oacc:1404-1616), for parity checks and drop-in callers
+ *   segclip_build_cover_lists   SegTree::constructCoverListGPU (oacc:2958-3128)
+ *   segclip_find_candidates     countIntersectionsParallel + thrustDeviceExclusiveScan +
+ *                               polyClipArrayGPUParallel (oacc:6250-6681, clip:1032-1089)
+ *                               and radixsort/removeDuplicatesAtEdge (clip:368-493)
+ *   segclip_intersections       countEdgeIntersections (clip:522-601), saveIntersectionsParallel,
+ *                               sortIntersectionsAtEdgesParallel, copyIntersectionsParallel,
+ *                               calculateInitLabelParallel (oacc:7257-7988)
+ *   segclip_label_traverse      copyToLinkedList (clip:606-695), labelIntersections2,
+ *                               createResult2, cleanUpResult2 (foster/polyclip.h:337-787)
+ *   segclip_clip                the per-overlay body of hybrid_clipping (clip:1026-1297)
+ *   segclip_stable_sort_by_key_* / segclip_exclusive_scan_*
+ *                               thrust_func.h:4-12 (p_seg_tree/thrust_func.cu:19-42)
+ */
+#ifndef SEGCLIP_H
+#define SEGCLIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEGCLIP_OK 0
+#define SEGCLIP_EINVAL 1  /* bad argument or input polygon */
+#define SEGCLIP_EHIP 2    /* HIP runtime failure */
+#define SEGCLIP_ENOMEM 3  /* device allocation failed */
+#define SEGCLIP_ESTATE 4  /* stage called before the one it depends on */
+#define SEGCLIP_ERANGE 5  /* sizes beyond the supported range */
+#define SEGCLIP_EUNDEF 6  /* the reference's result is undefined on this input */
+#define SEGCLIP_ESPACE 7  /* output buffer too small; the needed size was stored */
+
+typedef struct segclip_ctx segclip_ctx;
+
+/* Create a context on HIP device 'device'.  'stream' is a hipStream_t or
+ * NULL (the context then creates its own default-priority stream). */
+int segclip_create(segclip_ctx **out, int device, void *stream);
+void segclip_destroy(segclip_ctx *ctx);
+const char *segclip_last_error(const segclip_ctx *ctx);
+const char *segclip_version(void);
+
+/* Input rings as x,y pairs without repeated closing vertex.  ov_xy holds the
+ * K overlays back to back with ov_sizes[k] vertices each.  Host pointers
+ * (set_polygons) or device pointers (set_polygons_device). */
+int segclip_set_polygons(segclip_ctx *ctx, const double *base_xy, int nb, const double *ov_xy,
+                         const int *ov_sizes, int K);
+int segclip_set_polygons_device(segclip_ctx *ctx, const double *base_xy, int nb, const double *ov_xy,
+                                const int *ov_sizes, int K);
+
+/* Tree stages.  segclip_build_tree runs elementary, node intervals, end lists
+ * and cover lists in that order. */
+int segclip_build_elementary(segclip_ctx *ctx);
+int segclip_build_node_intervals(segclip_ctx *ctx);
+int segclip_build_end_lists(segclip_ctx *ctx);
+int segclip_build_cover_lists(segclip_ctx *ctx);
+int segclip_materialize_end_lists(segclip_ctx *ctx);
+int segclip_build_tree(segclip_ctx *ctx);
+
+/* Candidate pairs of every overlay in one pass (stab + dedupe). */
+int segclip_find_candidates(segclip_ctx *ctx);
+/* Per-overlay bookkeeping (k is 1-based) and labelling / traversal. */
+int segclip_intersections(segclip_ctx *ctx, int k);
+int segclip_label_traverse(segclip_ctx *ctx);
+/* Whole path for overlay k: the tree and candidates are built on first use. */
+int segclip_clip(segclip_ctx *ctx, int k);
+
+typedef struct {
+    int64_t num_edges;        /* N */
+    int64_t epc;              /* elementary points (2^k + 1) */
+    int64_t tree_size;        /* T */
+    int64_t cover_entries;    /* L */
+    int64_t leaf_end_entries; /* leaf-level end-list entries */
+    int64_t end_entries;      /* E, after segclip_materialize_end_lists */
+    int64_t tests;            /* W, pair tests of the stab */
+    int64_t candidates;       /* K, emitted (with duplicates) */
+    int64_t pairs;            /* unique pairs, all overlays */
+    int32_t overlay;          /* overlay of the last segclip_intersections */
+    int32_t overlay_pairs;    /* I of that overlay */
+    int32_t nb_aug, nc_aug;   /* augmented ring sizes */
+    int32_t rings;            /* output components after clean-up */
+    int32_t undefined;        /* bit 1: an intersection got no slot; 2: ring walk; 4: traversal */
+    int64_t out_vertices;
+} segclip_counts;
+int segclip_get_counts(segclip_ctx *ctx, segclip_counts *out);
+
+/* Copy-out (device -> host).  Each returns SEGCLIP_ESPACE and stores the
+ * needed element count in *need when 'cap' is too small. */
+int segclip_copy_elementary(segclip_ctx *ctx, double *out, int64_t cap, int64_t *need);
+int segclip_copy_node_intervals(segclip_ctx *ctx, double *out, int64_t cap, int64_t *need);
+/* cover lists: off has tree_size+1 entries; edges are edge indices (base
+ * edges first, then overlays in order), unordered within a node */
+int segclip_copy_cover_lists(segclip_ctx *ctx, uint32_t *off, uint32_t *edges, int64_t cap, int64_t *need);
+/* end lists per node sorted by (id,type); needs segclip_materialize_end_lists */
+int segclip_copy_end_lists(segclip_ctx *ctx, uint32_t *off, uint32_t *edges, int64_t cap, int64_t *need);
+/* per edge "after"/"before" leaf node of its leaf end entries (-1 = none) */
+int segclip_copy_leaf_targets(segclip_ctx *ctx, int32_t *after, int32_t *before, int64_t cap, int64_t *need);
+/* unique pairs of overlay k as reference ids (2*vertex offset) and types */
+int segclip_copy_pairs(segclip_ctx *ctx, int k, int32_t *b_id, int32_t *c_id, int32_t *type, int64_t cap,
+                       int64_t *need);
+/* intersections of the current overlay, in pair order */
+int segclip_copy_intersections(segclip_ctx *ctx, double *xy, int32_t *alpha_b, int32_t *alpha_c, int64_t cap,
+                               int64_t *need);
+/* augmented rings: xy (2*n), neighbours (1-based, -100), alpha values (-100 = plain), initial labels */
+int segclip_copy_augmented(segclip_ctx *ctx, int side, double *xy, int32_t *nbr, int32_t *alpha, int32_t *label,
+                           int64_t cap, int64_t *need);
+/* output rings: ring_off has rings+1 entries, xy 2*out_vertices */
+int segclip_copy_rings(segclip_ctx *ctx, int32_t *ring_off, int64_t off_cap, double *xy, int64_t xy_cap,
+                       int64_t *need);
+
+/* thrust_func.h primitives on device pointers (stable, in place) */
+int segclip_stable_sort_by_key_f64(double *keys, int32_t *vals, int64_t n, void *stream);
+int segclip_stable_sort_by_key_i32(int32_t *keys, int32_t *vals, int64_t n, void *stream);
+int segclip_stable_sort_by_key_i64(int64_t *keys, int32_t *vals, int64_t n, void *stream);
+int segclip_exclusive_scan_i32(int32_t *data, int64_t n, void *stream);
+int segclip_exclusive_scan_i64(int64_t *data, int64_t n, void *stream);
+
+/* Seeded synthetic rings (host code; see the workload definitions) */
+int segclip_synth_star(double *xy, int n, uint64_t seed, double cx, double cy, double rmin, double rmax,
+                       double rot);
+int segclip_synth_harmonic(double *xy, int n, uint64_t seed, double cx, double cy, double scale, double rot);
+int segclip_synth_convex(double *xy, int n, uint64_t seed, double cx, double cy, double scale, double rot);
+void segclip_synth_uniform(double *out, int n, uint64_t seed);
+int segclip_count_vertical(const double *xy, int n);
+
+/* Per-stage device times in ms of the last segclip_clip with timing enabled */
+int segclip_set_timing(segclip_ctx *ctx, int on);
+int segclip_get_stage_times(segclip_ctx *ctx, float *ms8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

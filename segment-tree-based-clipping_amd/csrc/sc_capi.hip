@@ -1,0 +1,317 @@
+// sc_capi.hip -- the extern "C" boundary of libsegclip (declared in include/segclip.h).
+#include "../../include/segclip.h"
+#include "sc_ctx.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+struct segclip_ctx {
+    sc::Ctx c;
+};
+
+namespace sc {
+Ctx::~Ctx() {
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+} // namespace sc
+
+namespace {
+
+template <class F> int guarded(segclip_ctx *h, F &&f) {
+    if (!h) return SEGCLIP_EINVAL;
+    try {
+        SC_HIP(hipSetDevice(h->c.device));
+        f(h->c);
+        h->c.last_error.clear();
+        return SEGCLIP_OK;
+    } catch (const sc::Error &e) {
+        h->c.last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        h->c.last_error = e.what();
+        return SEGCLIP_EHIP;
+    }
+}
+
+int need_space(int64_t n, int64_t cap, int64_t *need) {
+    if (need) *need = n;
+    return n > cap ? SEGCLIP_ESPACE : SEGCLIP_OK;
+}
+
+template <class T> void d2h(sc::Ctx &c, T *dst, const T *src, int64_t n) {
+    if (n <= 0) return;
+    SC_HIP(hipMemcpyAsync(dst, src, sizeof(T) * size_t(n), hipMemcpyDeviceToHost, c.stream));
+}
+
+void tree_all(sc::Ctx &c) {
+    sc::build_edges(c);
+    sc::build_elementary(c);
+    sc::build_node_intervals(c);
+    sc::build_leaf_end_entries(c);
+    sc::build_cover_lists(c);
+}
+
+} // namespace
+
+extern "C" {
+
+const char *segclip_version(void) { return "segclip 0.1 (gfx950)"; }
+
+int segclip_create(segclip_ctx **out, int device, void *stream) {
+    if (!out) return SEGCLIP_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return SEGCLIP_EHIP;
+    }
+    if (device < 0 || device >= n) return SEGCLIP_EINVAL;
+    segclip_ctx *h = new (std::nothrow) segclip_ctx();
+    if (!h) return SEGCLIP_ENOMEM;
+    h->c.device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete h;
+        return SEGCLIP_EHIP;
+    }
+    if (stream) {
+        h->c.stream = static_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking) != hipSuccess) {
+            delete h;
+            return SEGCLIP_EHIP;
+        }
+        h->c.own_stream = true;
+    }
+    *out = h;
+    return SEGCLIP_OK;
+}
+
+void segclip_destroy(segclip_ctx *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->c.device);
+    (void)hipStreamSynchronize(h->c.stream);
+    delete h;
+}
+
+const char *segclip_last_error(const segclip_ctx *h) { return h ? h->c.last_error.c_str() : "null context"; }
+
+int segclip_set_polygons(segclip_ctx *h, const double *b, int nb, const double *c, const int *csz, int K) {
+    return guarded(h, [&](sc::Ctx &x) { sc::ctx_set_polygons(x, b, nb, c, csz, K, false); });
+}
+int segclip_set_polygons_device(segclip_ctx *h, const double *b, int nb, const double *c, const int *csz, int K) {
+    return guarded(h, [&](sc::Ctx &x) { sc::ctx_set_polygons(x, b, nb, c, csz, K, true); });
+}
+int segclip_build_elementary(segclip_ctx *h) {
+    return guarded(h, [&](sc::Ctx &x) {
+        sc::build_edges(x);
+        sc::build_elementary(x);
+    });
+}
+int segclip_build_node_intervals(segclip_ctx *h) {
+    return guarded(h, [&](sc::Ctx &x) { sc::build_node_intervals(x); });
+}
+int segclip_build_end_lists(segclip_ctx *h) {
+    return guarded(h, [&](sc::Ctx &x) { sc::build_leaf_end_entries(x); });
+}
+int segclip_build_cover_lists(segclip_ctx *h) {
+    return guarded(h, [&](sc::Ctx &x) { sc::build_cover_lists(x); });
+}
+int segclip_materialize_end_lists(segclip_ctx *h) {
+    return guarded(h, [&](sc::Ctx &x) { sc::materialize_end_lists(x); });
+}
+int segclip_build_tree(segclip_ctx *h) { return guarded(h, [&](sc::Ctx &x) { tree_all(x); }); }
+int segclip_find_candidates(segclip_ctx *h) {
+    return guarded(h, [&](sc::Ctx &x) { sc::find_candidates(x); });
+}
+int segclip_intersections(segclip_ctx *h, int k) {
+    int rc = guarded(h, [&](sc::Ctx &x) { sc::post_overlay(x, k); });
+    return rc;
+}
+int segclip_label_traverse(segclip_ctx *h) {
+    return guarded(h, [&](sc::Ctx &x) { sc::label_and_traverse(x); });
+}
+int segclip_clip(segclip_ctx *h, int k) {
+    return guarded(h, [&](sc::Ctx &x) { sc::clip_all(x, k); });
+}
+
+int segclip_get_counts(segclip_ctx *h, segclip_counts *o) {
+    if (!h || !o) return SEGCLIP_EINVAL;
+    const sc::Ctx &c = h->c;
+    std::memset(o, 0, sizeof(*o));
+    o->num_edges = c.ne;
+    o->epc = c.epc;
+    o->tree_size = c.T;
+    o->cover_entries = c.n_cov;
+    o->leaf_end_entries = c.n_lend;
+    o->end_entries = c.n_mend;
+    o->tests = c.n_tests;
+    o->candidates = c.n_cand;
+    o->pairs = c.n_pairs;
+    o->overlay = c.cur_k;
+    o->overlay_pairs = c.I;
+    o->nb_aug = c.nbaug;
+    o->nc_aug = c.ncaug;
+    o->rings = c.nrings;
+    o->undefined = c.undefined;
+    o->out_vertices = c.n_out;
+    return SEGCLIP_OK;
+}
+
+int segclip_copy_elementary(segclip_ctx *h, double *out, int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    int rc = need_space(h->c.epc, cap, need);
+    if (rc || !out) return rc;
+    return guarded(h, [&](sc::Ctx &c) {
+        d2h(c, out, c.ea.p, c.epc);
+        SC_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int segclip_copy_node_intervals(segclip_ctx *h, double *out, int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    int rc = need_space(2 * int64_t(h->c.T), cap, need);
+    if (rc || !out) return rc;
+    return guarded(h, [&](sc::Ctx &c) {
+        d2h(c, out, c.niv.p, 2 * int64_t(c.T));
+        SC_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int segclip_copy_cover_lists(segclip_ctx *h, uint32_t *off, uint32_t *edges, int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    int rc = need_space(h->c.n_cov, cap, need);
+    if (rc || !off || !edges) return rc;
+    return guarded(h, [&](sc::Ctx &c) {
+        d2h(c, off, c.cov_off.p, int64_t(c.T) + 1);
+        d2h(c, edges, c.cov_e.p, c.n_cov);
+        SC_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int segclip_copy_end_lists(segclip_ctx *h, uint32_t *off, uint32_t *edges, int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    int rc = need_space(h->c.n_mend, cap, need);
+    if (rc || !off || !edges) return rc;
+    return guarded(h, [&](sc::Ctx &c) {
+        d2h(c, off, c.mend_off.p, int64_t(c.T) + 1);
+        d2h(c, edges, c.mend_e.p, c.n_mend);
+        SC_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int segclip_copy_leaf_targets(segclip_ctx *h, int32_t *after, int32_t *before, int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    int rc = need_space(h->c.ne, cap, need);
+    if (rc || !after || !before) return rc;
+    return guarded(h, [&](sc::Ctx &c) {
+        d2h(c, after, c.after_leaf.p, c.ne);
+        d2h(c, before, c.before_leaf.p, c.ne);
+        SC_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int segclip_copy_pairs(segclip_ctx *h, int k, int32_t *b_id, int32_t *c_id, int32_t *type, int64_t cap,
+                       int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    sc::Ctx &c = h->c;
+    if (int(c.pair_off.size()) != c.K + 1 || k < 1 || k > c.K) return SEGCLIP_ESTATE;
+    int64_t n = c.pair_off[k] - c.pair_off[k - 1];
+    int rc = need_space(n, cap, need);
+    if (rc || !b_id || !c_id) return rc;
+    return guarded(h, [&](sc::Ctx &x) {
+        std::vector<uint64_t> keys(size_t(n > 0 ? n : 1));
+        d2h(x, keys.data(), x.pairs.p + x.pair_off[k - 1], n);
+        SC_HIP(hipStreamSynchronize(x.stream));
+        const int cStart = x.coff[k - 1];
+        std::vector<double> bxy(2 * size_t(x.nb + 1));
+        d2h(x, bxy.data(), x.bpoly.p, 2 * int64_t(x.nb + 1));
+        std::vector<double> cxy;
+        if (type) {
+            cxy.resize(2 * size_t(x.csz[k - 1] + 1));
+            d2h(x, cxy.data(), x.cpoly.p + 2 * int64_t(cStart), 2 * int64_t(x.csz[k - 1] + 1));
+        }
+        SC_HIP(hipStreamSynchronize(x.stream));
+        for (int64_t i = 0; i < n; ++i) {
+            int b = int((keys[i] >> x.bits_c) & ((1ull << x.bits_b) - 1ull));
+            int cl = int(keys[i] & ((1ull << x.bits_c) - 1ull));
+            b_id[i] = 2 * b;
+            c_id[i] = 2 * (cStart + cl);
+            if (type) type[i] = -1; // filled by segclip_intersections order; see copy_intersections
+        }
+        if (type) {
+            // type recomputed on the device during segclip_intersections; copy if it matches this overlay
+            if (x.cur_k == k && x.I == n) d2h(x, type, x.p_type.p, n);
+            SC_HIP(hipStreamSynchronize(x.stream));
+        }
+    });
+}
+
+int segclip_copy_intersections(segclip_ctx *h, double *xy, int32_t *alpha_b, int32_t *alpha_c, int64_t cap,
+                               int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    int rc = need_space(h->c.I, cap, need);
+    if (rc || !xy) return rc;
+    return guarded(h, [&](sc::Ctx &c) {
+        int n = c.I;
+        std::vector<double> x(size_t(n > 0 ? n : 1)), y(size_t(n > 0 ? n : 1));
+        d2h(c, x.data(), c.p_x.p, n);
+        d2h(c, y.data(), c.p_y.p, n);
+        if (alpha_b) d2h(c, alpha_b, c.p_qa.p, n);
+        if (alpha_c) d2h(c, alpha_c, c.p_qb.p, n);
+        SC_HIP(hipStreamSynchronize(c.stream));
+        for (int i = 0; i < n; ++i) {
+            xy[2 * i] = x[i];
+            xy[2 * i + 1] = y[i];
+        }
+    });
+}
+
+int segclip_copy_augmented(segclip_ctx *h, int side, double *xy, int32_t *nbr, int32_t *alpha, int32_t *label,
+                           int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    int64_t n = side == 0 ? h->c.nbaug : h->c.ncaug;
+    int rc = need_space(n, cap, need);
+    if (rc || !xy) return rc;
+    return guarded(h, [&](sc::Ctx &c) {
+        std::vector<double> x(size_t(n > 0 ? n : 1)), y(size_t(n > 0 ? n : 1));
+        d2h(c, x.data(), side == 0 ? c.aug_bx.p : c.aug_cx.p, n);
+        d2h(c, y.data(), side == 0 ? c.aug_by.p : c.aug_cy.p, n);
+        if (nbr) d2h(c, nbr, side == 0 ? c.nbr_b.p : c.nbr_c.p, n);
+        if (alpha) d2h(c, alpha, side == 0 ? c.av_b.p : c.av_c.p, n);
+        if (label) d2h(c, label, side == 0 ? c.lab_b.p : c.lab_c.p, n);
+        SC_HIP(hipStreamSynchronize(c.stream));
+        for (int64_t i = 0; i < n; ++i) {
+            xy[2 * i] = x[i];
+            xy[2 * i + 1] = y[i];
+        }
+    });
+}
+
+int segclip_copy_rings(segclip_ctx *h, int32_t *ring_off, int64_t off_cap, double *xy, int64_t xy_cap,
+                       int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    const sc::Ctx &c = h->c;
+    if (need) need[0] = c.nrings + 1;
+    if (need) need[1] = int64_t(c.ring_xy.size() / 2);
+    if (int64_t(c.nrings) + 1 > off_cap || int64_t(c.ring_xy.size() / 2) > xy_cap) return SEGCLIP_ESPACE;
+    if (!ring_off || !xy) return SEGCLIP_EINVAL;
+    std::copy(c.ring_off.begin(), c.ring_off.end(), ring_off);
+    std::copy(c.ring_xy.begin(), c.ring_xy.end(), xy);
+    return SEGCLIP_OK;
+}
+
+int segclip_set_timing(segclip_ctx *h, int on) {
+    if (!h) return SEGCLIP_EINVAL;
+    h->c.timing = on != 0;
+    return SEGCLIP_OK;
+}
+
+int segclip_get_stage_times(segclip_ctx *h, float *ms8) {
+    if (!h || !ms8) return SEGCLIP_EINVAL;
+    const sc::StageTimes &t = h->c.times;
+    float v[8] = {t.elementary, t.tree, t.endlist, t.cover, t.stab, t.dedupe, t.post, t.label};
+    std::memcpy(ms8, v, sizeof(v));
+    return SEGCLIP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,119 @@
+// sc_common.hpp -- shared host/device helpers of libsegclip (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sc {
+
+// Error raised by any failing HIP call or violated precondition; the C-ABI
+// layer turns it into a status code plus a message (segclip_last_error).
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+enum Status : int {
+    SC_OK = 0,
+    SC_EINVAL = 1,    // bad argument / input
+    SC_EHIP = 2,      // HIP runtime failure
+    SC_ENOMEM = 3,    // device allocation failed
+    SC_ESTATE = 4,    // call out of order
+    SC_ERANGE = 5,    // size beyond what the kernels support
+    SC_EUNDEF = 6,    // the reference's behaviour is undefined on this input
+};
+
+#define SC_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw ::sc::Error(::sc::SC_EHIP, std::string(#call) + ": " + hipGetErrorString(e_) + \
+                                                 " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+    } while (0)
+
+#define SC_REQUIRE(cond, code, msg)                   \
+    do {                                              \
+        if (!(cond)) throw ::sc::Error((code), (msg)); \
+    } while (0)
+
+// Growable device buffer; contents are not preserved across growth.
+template <class T> struct DBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    T *ensure(size_t n) {
+        if (n == 0) n = 1;
+        if (n > cap) {
+            release();
+            size_t want = n + n / 8;
+            hipError_t e = hipMalloc(&p, want * sizeof(T));
+            if (e != hipSuccess) {
+                p = nullptr;
+                (void)hipGetLastError();
+                throw Error(SC_ENOMEM, "hipMalloc of " + std::to_string(want * sizeof(T)) + " bytes failed");
+            }
+            cap = want;
+        }
+        return p;
+    }
+    operator T *() const { return p; }
+};
+
+constexpr int WAVE = 64;
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline int next_pow2_int(int n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+inline int ilog2_exact(int64_t n) {
+    int l = 0;
+    while ((int64_t(1) << l) < n) l++;
+    return l;
+}
+
+// ---------------------------------------------------------------------------
+// primitives (sc_prims.hip)
+// ---------------------------------------------------------------------------
+
+// Exclusive prefix sum of n values into out (in == out allowed); returns the
+// total through *d_total when non-null (device pointer).
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total, hipStream_t s);
+void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *d_total, hipStream_t s);
+
+// Stable LSD radix sort of (keys, vals) by key bits [bit_lo, bit_hi).
+// vals may be null (keys only).  The sorted data ends in keys/vals; the
+// alternate buffers are scratch of the same length.
+struct RadixScratch {
+    DBuf<uint32_t> tile_hist;
+    DBuf<uint32_t> global_hist;
+    DBuf<uint64_t> keys_alt;
+    DBuf<uint32_t> vals_alt;
+};
+void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs,
+                    hipStream_t s);
+
+// Order-preserving map of an IEEE double to uint64 (-0.0 and +0.0 map to the
+// same key, so a stable sort on it orders ties by original position as a
+// comparison sort with operator< does).
+__host__ __device__ inline uint64_t f64_order_key(double x) {
+    if (x == 0.0) x = 0.0;
+    uint64_t u;
+    __builtin_memcpy(&u, &x, 8);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+} // namespace sc

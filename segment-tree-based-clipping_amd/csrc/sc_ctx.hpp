@@ -1,0 +1,116 @@
+// sc_ctx.hpp -- device-resident state of one clipping problem (one base ring,
+// K overlay rings) and the pipeline stage entry points.
+#pragma once
+#include "sc_common.hpp"
+
+namespace sc {
+
+// Per-stage timings filled when Ctx::timing is on (milliseconds, HIP events).
+struct StageTimes {
+    float elementary = 0, tree = 0, endlist = 0, cover = 0, stab = 0, dedupe = 0, post = 0, label = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string last_error;
+
+    // ---- input (reference layout of clip:172-311) ----
+    int nb = 0, K = 0, nct = 0, ne = 0, maxcsz = 0;
+    std::vector<int> csz, coff;     // overlay sizes; vertex offsets with closing vertices
+    std::vector<int> eoff;          // overlay edge offsets in the edge index space (K+1), eoff[0] = nb
+    DBuf<double> bpoly, cpoly;      // (nb+1)*2, (nct+K)*2
+    DBuf<int> d_coff, d_eoff;
+    // per-edge SoA (edge index e in [0,ne): base edges, then overlay 1, 2, ...)
+    DBuf<double> ex0, ey0, ex1, ey1; // endpoints in ring order
+    DBuf<double> estart, eend;       // x-extent
+    DBuf<int> etype, eid;
+
+    // ---- A3 elementary intervals ----
+    DBuf<double> ea;
+    int epc = 0, nleaves = 0, T = 0, depth = 0;
+    // ---- A4 ----
+    DBuf<double> niv;
+    // ---- A6 leaf end entries, split by class (0 base, 1 overlay) ----
+    DBuf<int> after_leaf, before_leaf;   // node index or -1
+    DBuf<uint32_t> lend_off;             // 2*(nleaves+1): class-major leaf offsets
+    DBuf<uint32_t> lend_e;               // entries: edge index | 0x80000000 for the "after" entry,
+                                         // sorted by (class, leaf), edge order inside a leaf
+    DBuf<uint64_t> lend_key;             // (class, leaf) key of each entry
+    int64_t n_lend = 0;
+    // materialised end lists (parity / drop-in only)
+    DBuf<uint32_t> mend_off;
+    DBuf<uint32_t> mend_e;
+    int64_t n_mend = 0;
+    // ---- A5 cover lists ----
+    DBuf<uint32_t> cov_off;              // T+1
+    DBuf<uint32_t> cov_e;                // edge indices: base run then overlay run, order within a run unspecified
+    DBuf<uint32_t> cov_split;            // per node: count of base entries
+    int64_t n_cov = 0;
+    // ---- A8/A10 candidates and unique pairs ----
+    DBuf<uint64_t> cand;                 // packed (k, b, c) keys
+    int64_t n_cand = 0, cand_cap = 0;
+    int64_t n_tests = 0;                 // W
+    DBuf<uint64_t> pairs;                // unique packed keys, sorted
+    int64_t n_pairs = 0;
+    int bits_b = 0, bits_c = 0, bits_k = 0;
+    std::vector<int64_t> pair_off;       // per overlay [K+1] offsets into pairs (host)
+    DBuf<uint32_t> d_pair_off;
+
+    // ---- per-overlay post-processing (A11-A17) for the overlay last clipped ----
+    int cur_k = 0;
+    int I = 0;                           // unique pairs of this overlay
+    DBuf<int> p_b, p_c, p_type;          // bId, cId (reference ids), type
+    DBuf<double> p_x, p_y;               // intersection point
+    DBuf<int> p_qa, p_qb;                // quantised alpha (B side), beta (C side)
+    DBuf<int> ordB, ordC;                // per-edge alpha order (positions into the pair list)
+    DBuf<int> nd_b, nd_c;                // non-degenerate counts -> exclusive scans (nb+1, nc+1)
+    DBuf<int> all_b, all_c;
+    int nbaug = 0, ncaug = 0;
+    DBuf<double> aug_bx, aug_by, aug_cx, aug_cy;
+    DBuf<int> nbr_b, nbr_c, av_b, av_c, lab_b, lab_c;
+    DBuf<int> slot_b, slot_c;            // per pair: augmented slot on each ring
+    DBuf<int> flags;                     // device status words
+    // labelling / traversal
+    DBuf<int> nx_b, pv_b, nx_c, pv_c;    // ring links over augmented slots (+ appended vertices)
+    DBuf<int> enex_b, enex_c, inter_b, inter_c;
+    int nrings = 0;
+    std::vector<int> ring_off;           // host copy of the result
+    std::vector<double> ring_xy;
+    DBuf<int> d_ring_off;
+    DBuf<double> d_ring_x, d_ring_y;
+    int64_t n_out = 0;
+    int undefined = 0;                   // bitmask like the oracle's status
+
+    // scratch
+    RadixScratch rs;
+    DBuf<uint64_t> k64a, k64b;
+    DBuf<uint32_t> u32a, u32b, u32c, u32d;
+    DBuf<uint64_t> u64a, u64b;
+    DBuf<uint32_t> d_u32_total;
+    DBuf<uint64_t> d_u64_total;
+
+    bool timing = false;
+    StageTimes times;
+
+    ~Ctx();
+};
+
+// stage entry points (throw sc::Error)
+void ctx_set_polygons(Ctx &c, const double *bxy, int nb, const double *cxy, const int *csz, int K, bool device_ptrs);
+void build_edges(Ctx &c);
+void build_elementary(Ctx &c);
+void build_node_intervals(Ctx &c);
+void build_leaf_end_entries(Ctx &c);
+void materialize_end_lists(Ctx &c);
+void build_cover_lists(Ctx &c);
+void find_candidates(Ctx &c);          // all overlays at once, then unique pairs
+void post_overlay(Ctx &c, int k);      // A11-A15 for overlay k
+void label_and_traverse(Ctx &c);       // A16-A17 for the overlay of post_overlay
+void clip_all(Ctx &c, int k);          // full pipeline for one overlay
+
+uint32_t read_u32(Ctx &c, const uint32_t *d);
+uint64_t read_u64(Ctx &c, const uint64_t *d);
+
+} // namespace sc

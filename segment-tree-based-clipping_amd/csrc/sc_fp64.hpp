@@ -1,0 +1,93 @@
+// sc_fp64.hpp -- FP64 geometric predicates of the clipping path.
+//
+// Every expression keeps the operation order of the reference
+// (oacc_segment_tree.cpp:5282-5316 LSMF, 5334-5337 A(), 5520-5611
+// lineSegmentIntersection2, 7247-7252 alphaToInt, 7847-7920 oracle /
+// getInitialLabel) and the library is compiled with -ffp-contract=off, so
+// the GPU reproduces the host's IEEE double results bit for bit: no fused
+// multiply-add, correctly rounded division.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sc {
+
+constexpr double EPS = 0.000000001;
+
+__host__ __device__ inline double areaA(double px, double py, double qx, double qy, double rx, double ry) {
+    double t1 = (qx - px) * (ry - py);
+    double t2 = (qy - py) * (rx - px);
+    return t1 - t2;
+}
+
+__host__ __device__ inline bool lsmf(double l1p1x, double l1p1y, double l1p2x, double l1p2y, double l2p1x,
+                                     double l2p1y, double l2p2x, double l2p2y) {
+    double minPX = l1p1x, minPY = l1p1y, maxPX = l1p2x, maxPY = l1p2y;
+    double minQX = l2p1x, minQY = l2p1y, maxQX = l2p2x, maxQY = l2p2y;
+    if (minPX > l1p2x) { minPX = l1p2x; maxPX = l1p1x; }
+    if (minPY > l1p2y) { minPY = l1p2y; maxPY = l1p1y; }
+    if (minQX > l2p2x) { minQX = l2p2x; maxQX = l2p1x; }
+    if (minQY > l2p2y) { minQY = l2p2y; maxQY = l2p1y; }
+    if (minPX > maxQX || maxPX < minQX) return false;
+    if (minPY > maxQY || maxPY < minQY) return false;
+    return true;
+}
+
+// intersection type 0..8 of [b1,b2] x [c1,c2] and the parameters alpha (on b)
+// and beta (on c); the collinear branch keeps the reference's beta_is_0 test
+// on alpha (oacc:5595)
+__host__ __device__ inline int classify(double b1x, double b1y, double b2x, double b2y, double c1x, double c1y,
+                                        double c2x, double c2y, double &alpha, double &beta) {
+    double AP1 = areaA(b1x, b1y, c1x, c1y, c2x, c2y);
+    double AP2 = areaA(b2x, b2y, c1x, c1y, c2x, c2y);
+    alpha = 0.0;
+    beta = 0.0;
+    if (fabs(AP1 - AP2) > EPS) {
+        double AQ1 = areaA(c1x, c1y, b1x, b1y, b2x, b2y);
+        double AQ2 = areaA(c2x, c2y, b1x, b1y, b2x, b2y);
+        alpha = AP1 / (AP1 - AP2);
+        beta = AQ1 / (AQ1 - AQ2);
+        bool a0 = false, a01 = false, b0 = false, b01 = false;
+        if (alpha > EPS && alpha < 1.0 - EPS) a01 = true;
+        else if (fabs(alpha) <= EPS) a0 = true;
+        if (beta > EPS && beta < 1.0 - EPS) b01 = true;
+        else if (fabs(beta) <= EPS) b0 = true;
+        if (a01 && b01) return 1;
+        if (a0 && b01) return 2;
+        if (b0 && a01) return 3;
+        if (a0 && b0) return 4;
+    } else if (fabs(AP1) < EPS) {
+        double dPx = b2x - b1x, dPy = b2y - b1y;
+        double dQx = c2x - c1x, dQy = c2y - c1y;
+        double PQx = c1x - b1x, PQy = c1y - b1y;
+        double m1 = PQx * dPx, m2 = PQy * dPy;
+        double n1 = dPx * dPx, n2 = dPy * dPy;
+        alpha = (m1 + m2) / (n1 + n2);
+        double k1 = PQx * dQx, k2 = PQy * dQy;
+        double l1 = dQx * dQx, l2 = dQy * dQy;
+        beta = -(k1 + k2) / (l1 + l2);
+        bool a0 = false, a01 = false, an = false, b0 = false, b01 = false, bn = false;
+        if (alpha > EPS && alpha < 1.0 - EPS) a01 = true;
+        else if (fabs(alpha) <= EPS) a0 = true;
+        else an = true;
+        if (beta > EPS && beta < 1.0 - EPS) b01 = true;
+        else if (fabs(alpha) <= EPS) b0 = true;
+        else bn = true;
+        if (a01 && b01) return 5;
+        if (an && b01) return 6;
+        if (bn && a01) return 7;
+        if (a0 && b0) return 8;
+    }
+    return 0;
+}
+
+__host__ __device__ inline int alpha_to_int(double a) {
+    if (a < EPS) return 0;
+    return int(1000000000.0 * a);
+}
+
+__host__ __device__ inline bool nondeg_b(int t) { return t == 1 || t == 3 || t == 5 || t == 7; }
+__host__ __device__ inline bool nondeg_c(int t) { return t == 1 || t == 2 || t == 6; }
+__host__ __device__ inline bool degen_b_first(int t) { return t == 2 || t == 4 || t == 6 || t == 8; }
+__host__ __device__ inline bool degen_c_first(int t) { return t == 3 || t == 4 || t == 5 || t == 7 || t == 8; }
+
+} // namespace sc

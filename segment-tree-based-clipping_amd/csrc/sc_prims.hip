@@ -1,0 +1,297 @@
+// sc_prims.hip -- device-wide scan and stable LSD radix sort for gfx950.
+//
+// Scan: reduce / scan-of-block-sums / downsweep, 2048 items per 256-thread
+// block, wave64 shuffles inside a block.
+//
+// Radix sort: 8-bit digits, 4096 keys per tile.  One pass over the keys first
+// builds every digit histogram at once so that passes whose digit is the same
+// for all keys are skipped.  The scatter ranks keys stably inside a tile with
+// a 64-lane match (eight ballots per key) and per-wave digit counts in LDS.
+#include "sc_common.hpp"
+
+namespace sc {
+
+namespace {
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+template <class T> __device__ inline T wave_incl_scan(T x, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        T y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of one thread value over the block; returns the block total.
+template <class T> __device__ inline T block_excl_scan(T x, T *s_wave, T &total) {
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    T inc = wave_incl_scan(x, lane);
+    if (lane == WAVE - 1) s_wave[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        T w = lane < SCAN_BLOCK / WAVE ? s_wave[lane] : T(0);
+        T wi = wave_incl_scan(w, lane);
+        if (lane < SCAN_BLOCK / WAVE) s_wave[lane] = wi - w;
+        if (lane == SCAN_BLOCK / WAVE - 1) s_wave[SCAN_BLOCK / WAVE] = wi;
+    }
+    __syncthreads();
+    T r = s_wave[wid] + inc - x;
+    total = s_wave[SCAN_BLOCK / WAVE];
+    __syncthreads();
+    return r;
+}
+
+template <class T> __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_reduce(const T *in, int64_t n, T *sums) {
+    __shared__ T s_wave[SCAN_BLOCK / WAVE + 1];
+    int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
+    T acc = 0;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
+        if (i < n) acc += in[i];
+    }
+    T total;
+    (void)block_excl_scan(acc, s_wave, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of 'n' block sums in place, total -> *d_total
+template <class T> __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(T *sums, int64_t n, T *d_total) {
+    __shared__ T s_wave[SCAN_BLOCK / WAVE + 1];
+    T carry = 0;
+    for (int64_t base = 0; base < n; base += SCAN_BLOCK) {
+        int64_t i = base + threadIdx.x;
+        T v = i < n ? sums[i] : T(0);
+        T total;
+        T ex = block_excl_scan(v, s_wave, total);
+        if (i < n) sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = carry;
+}
+
+template <class T>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(const T *in, T *out, int64_t n, const T *offsets,
+                                                          T *d_total_single) {
+    __shared__ T s_data[SCAN_TILE];
+    __shared__ T s_wave[SCAN_BLOCK / WAVE + 1];
+    int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
+        s_data[it * SCAN_BLOCK + threadIdx.x] = i < n ? in[i] : T(0);
+    }
+    __syncthreads();
+    T loc[SCAN_ITEMS];
+    T acc = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        loc[j] = s_data[threadIdx.x * SCAN_ITEMS + j];
+        acc += loc[j];
+    }
+    T total;
+    T ex = block_excl_scan(acc, s_wave, total);
+    T off = offsets ? offsets[blockIdx.x] : T(0);
+    T run = off + ex;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        T v = loc[j];
+        s_data[threadIdx.x * SCAN_ITEMS + j] = run;
+        run += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
+        if (i < n) out[i] = s_data[it * SCAN_BLOCK + threadIdx.x];
+    }
+    if (d_total_single && threadIdx.x == 0) *d_total_single = total;
+}
+
+template <class T> struct ScanScratch {
+    DBuf<T> sums;
+};
+
+template <class T> void scan_impl(const T *in, T *out, int64_t n, T *d_total, hipStream_t s, ScanScratch<T> &ss) {
+    if (n <= 0) {
+        if (d_total) SC_HIP(hipMemsetAsync(d_total, 0, sizeof(T), s));
+        return;
+    }
+    int64_t nb = ceil_div(n, SCAN_TILE);
+    if (nb == 1) {
+        k_tile_scan<T><<<1, SCAN_BLOCK, 0, s>>>(in, out, n, nullptr, d_total);
+        SC_HIP(hipGetLastError());
+        return;
+    }
+    SC_REQUIRE(nb < (int64_t(1) << 31), SC_ERANGE, "scan too long");
+    T *sums = ss.sums.ensure(size_t(nb));
+    k_tile_reduce<T><<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, n, sums);
+    SC_HIP(hipGetLastError());
+    k_scan_sums<T><<<1, SCAN_BLOCK, 0, s>>>(sums, nb, d_total);
+    SC_HIP(hipGetLastError());
+    k_tile_scan<T><<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, out, n, sums, nullptr);
+    SC_HIP(hipGetLastError());
+}
+
+// per-thread scratch, intentionally never freed: releasing device memory from
+// thread-exit destructors can run after the HIP runtime has shut down
+template <class T> ScanScratch<T> &scan_scratch() {
+    thread_local ScanScratch<T> *p = new ScanScratch<T>();
+    return *p;
+}
+
+// ---------------------------------------------------------------------------
+// radix sort
+// ---------------------------------------------------------------------------
+constexpr int RS_BLOCK = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+constexpr int RS_WAVES = RS_BLOCK / WAVE;
+constexpr int RS_MAXPASS = 8;
+
+__global__ __launch_bounds__(RS_BLOCK) void k_global_hist(const uint64_t *keys, int64_t n, int bit_lo, int bit_hi,
+                                                          int npass, uint32_t *ghist) {
+    __shared__ uint32_t h[RS_MAXPASS][256];
+    for (int i = threadIdx.x; i < RS_MAXPASS * 256; i += RS_BLOCK) (&h[0][0])[i] = 0;
+    __syncthreads();
+    for (int64_t i = int64_t(blockIdx.x) * RS_BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * RS_BLOCK) {
+        uint64_t k = keys[i];
+        for (int p = 0; p < npass; ++p) {
+            int sh = bit_lo + 8 * p;
+            int bits = bit_hi - sh < 8 ? bit_hi - sh : 8;
+            uint32_t d = uint32_t(k >> sh) & ((1u << bits) - 1u);
+            atomicAdd(&h[p][d], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npass * 256; i += RS_BLOCK) {
+        uint32_t v = (&h[0][0])[i];
+        if (v) atomicAdd(&ghist[i], v);
+    }
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_upsweep(const uint64_t *keys, int64_t n, int shift, uint32_t mask,
+                                                      uint32_t *tile_hist, int ntiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = int64_t(blockIdx.x) * RS_TILE;
+#pragma unroll 4
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        int64_t i = base + int64_t(it) * RS_BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    tile_hist[int64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_downsweep(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                        uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                        int64_t n, int shift, uint32_t mask,
+                                                        const uint32_t *__restrict__ tile_off, int ntiles) {
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_cnt[RS_WAVES][256];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+    s_base[threadIdx.x] = tile_off[int64_t(threadIdx.x) * ntiles + blockIdx.x];
+    for (int w = 0; w < RS_WAVES; ++w) s_cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = int64_t(blockIdx.x) * RS_TILE;
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        int64_t i = base + int64_t(it) * RS_BLOCK + threadIdx.x;
+        bool valid = i < n;
+        uint64_t k = valid ? kin[i] : 0ull;
+        uint32_t v = (valid && vin) ? vin[i] : 0u;
+        uint32_t d = uint32_t(k >> shift) & mask;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        uint32_t r = __popcll(m & lt);
+        if (valid && r == 0) s_cnt[wid][d] = __popcll(m);
+        __syncthreads();
+        uint32_t pos = 0;
+        if (valid) {
+            pos = s_base[d] + r;
+            for (int w = 0; w < wid; ++w) pos += s_cnt[w][d];
+        }
+        __syncthreads();
+        {
+            uint32_t add = 0;
+#pragma unroll
+            for (int w = 0; w < RS_WAVES; ++w) {
+                add += s_cnt[w][threadIdx.x];
+                s_cnt[w][threadIdx.x] = 0;
+            }
+            s_base[threadIdx.x] += add;
+        }
+        __syncthreads();
+        if (valid) {
+            kout[pos] = k;
+            if (vout) vout[pos] = v;
+        }
+    }
+}
+
+} // namespace
+
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total, hipStream_t s) {
+    scan_impl<uint32_t>(in, out, n, d_total, s, scan_scratch<uint32_t>());
+}
+void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *d_total, hipStream_t s) {
+    scan_impl<uint64_t>(in, out, n, d_total, s, scan_scratch<uint64_t>());
+}
+
+void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs,
+                    hipStream_t s) {
+    if (n <= 1 || bit_hi <= bit_lo) return;
+    SC_REQUIRE(n < (int64_t(1) << 32), SC_ERANGE, "radix sort longer than 2^32 keys");
+    SC_REQUIRE(bit_lo >= 0 && bit_hi <= 64, SC_EINVAL, "radix bit range");
+    int npass = int(ceil_div(bit_hi - bit_lo, 8));
+    SC_REQUIRE(npass <= RS_MAXPASS, SC_EINVAL, "radix passes");
+    uint32_t *gh = rs.global_hist.ensure(RS_MAXPASS * 256);
+    SC_HIP(hipMemsetAsync(gh, 0, sizeof(uint32_t) * RS_MAXPASS * 256, s));
+    int ghb = int(ceil_div(n, RS_BLOCK * 16));
+    if (ghb > 1024) ghb = 1024;
+    k_global_hist<<<ghb, RS_BLOCK, 0, s>>>(keys, n, bit_lo, bit_hi, npass, gh);
+    SC_HIP(hipGetLastError());
+    std::vector<uint32_t> hh(size_t(npass) * 256);
+    SC_HIP(hipMemcpyAsync(hh.data(), gh, sizeof(uint32_t) * hh.size(), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipStreamSynchronize(s));
+
+    int ntiles = int(ceil_div(n, RS_TILE));
+    uint32_t *th = rs.tile_hist.ensure(size_t(ntiles) * 256);
+    uint64_t *kalt = rs.keys_alt.ensure(size_t(n));
+    uint32_t *valt = vals ? rs.vals_alt.ensure(size_t(n)) : nullptr;
+    uint64_t *kc = keys, *ko = kalt;
+    uint32_t *vc = vals, *vo = valt;
+    for (int p = 0; p < npass; ++p) {
+        bool trivial = false;
+        for (int d = 0; d < 256; ++d)
+            if (hh[size_t(p) * 256 + d] == uint32_t(n)) trivial = true;
+        if (trivial) continue;
+        int sh = bit_lo + 8 * p;
+        int bits = bit_hi - sh < 8 ? bit_hi - sh : 8;
+        uint32_t mask = (1u << bits) - 1u;
+        k_upsweep<<<ntiles, RS_BLOCK, 0, s>>>(kc, n, sh, mask, th, ntiles);
+        SC_HIP(hipGetLastError());
+        scan_exclusive_u32(th, th, int64_t(ntiles) * 256, nullptr, s);
+        k_downsweep<<<ntiles, RS_BLOCK, 0, s>>>(kc, vc, ko, vo, n, sh, mask, th, ntiles);
+        SC_HIP(hipGetLastError());
+        std::swap(kc, ko);
+        std::swap(vc, vo);
+    }
+    if (kc != keys) {
+        SC_HIP(hipMemcpyAsync(keys, kc, sizeof(uint64_t) * size_t(n), hipMemcpyDeviceToDevice, s));
+        if (vals) SC_HIP(hipMemcpyAsync(vals, vc, sizeof(uint32_t) * size_t(n), hipMemcpyDeviceToDevice, s));
+    }
+}
+
+} // namespace sc

@@ -1,0 +1,547 @@
+// sc_tree.hip -- input layout, elementary intervals, node intervals, leaf end
+// entries and cover lists (rows A1-A7 of the hot path) for gfx950.
+//
+// Reference behaviour restated here (file:line in the reference repository):
+//  - input layout and Edge records: src/clip.cpp:172-311, edge.h:190-201
+//  - elementary intervals: oacc_segment_tree.cpp:4513-4692 (stable sort of
+//    the 2N x-endpoints, keep the last of each run of equal values, the
+//    vertical-edge duplicate rule with its own indexing, the special case
+//    at position 2N-2, padding with the last kept value to 2^k+1 points)
+//  - node intervals: oacc:4948-4994 (leaf l = [ea[l], ea[l+1]], parent =
+//    [start(left), end(right)]), computed here in closed form per node
+//  - leaf end entries: oacc:1970-2113 ("after" = leaf starting at e.end,
+//    "before" = leaf ending at e.start, point-interval skip, degenerate
+//    leaves excluded)
+//  - cover lists: oacc:2496-2607.  e covers v iff Int(v) is inside
+//    [e.start,e.end] and Int(parent) is not: the canonical decomposition of
+//    the leaf range [L,R) with L = first point >= start, R = last point <= end.
+#include "sc_ctx.hpp"
+
+namespace sc {
+
+namespace {
+
+constexpr int BLK = 256;
+
+inline unsigned grid_for(int64_t n, int per = BLK) {
+    int64_t g = ceil_div(n, per);
+    if (g < 1) g = 1;
+    if (g > (int64_t(1) << 30)) g = int64_t(1) << 30;
+    return unsigned(g);
+}
+
+__device__ inline int find_overlay(const int *eoff, int K, int e) {
+    // largest j with eoff[j] <= e, j in [0,K)
+    int lo = 0, hi = K - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (eoff[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// overlay input vertex g (concatenated, no closing vertices) -> reference cPoly slot
+__global__ void k_layout_overlays(const double *__restrict__ cxy, int nct, const int *__restrict__ eoff,
+                                  const int *__restrict__ coff, int K, int nb, double *__restrict__ cpoly) {
+    int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= nct) return;
+    int j = find_overlay(eoff, K, int(g) + nb);
+    int local = int(g) + nb - eoff[j];
+    int dst = coff[j] + local;
+    double x = cxy[2 * g], y = cxy[2 * g + 1];
+    cpoly[2 * int64_t(dst)] = x;
+    cpoly[2 * int64_t(dst) + 1] = y;
+    if (local == 0) {
+        int cl = coff[j] + (eoff[j + 1] - eoff[j]);
+        cpoly[2 * int64_t(cl)] = x;
+        cpoly[2 * int64_t(cl) + 1] = y;
+    }
+}
+
+__global__ void k_close_base(double *bpoly, int nb) {
+    bpoly[2 * nb] = bpoly[0];
+    bpoly[2 * nb + 1] = bpoly[1];
+}
+
+__global__ void k_edges(const double *__restrict__ bpoly, const double *__restrict__ cpoly, int nb, int ne,
+                        const int *__restrict__ eoff, const int *__restrict__ coff, int K, double *x0, double *y0,
+                        double *x1, double *y1, double *es, double *ee, int *ety, int *eid) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const double *P;
+    int id, ty;
+    if (e < nb) {
+        P = bpoly;
+        id = 2 * e;
+        ty = 0;
+    } else {
+        int j = find_overlay(eoff, K, e);
+        P = cpoly;
+        id = 2 * (coff[j] + (e - eoff[j]));
+        ty = j + 1;
+    }
+    double a = P[id], b = P[id + 1], c = P[id + 2], d = P[id + 3];
+    x0[e] = a;
+    y0[e] = b;
+    x1[e] = c;
+    y1[e] = d;
+    if (a < c) {
+        es[e] = a;
+        ee[e] = c;
+    } else {
+        ee[e] = a;
+        es[e] = c;
+    }
+    ety[e] = ty;
+    eid[e] = id;
+}
+
+// ---- A3 ----
+__global__ void k_endpoint_keys(const double *__restrict__ es, const double *__restrict__ ee, int ne, uint64_t *keys,
+                                uint32_t *vals) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= 2 * int64_t(ne)) return;
+    double v = (i & 1) ? ee[i >> 1] : es[i >> 1];
+    keys[i] = f64_order_key(v);
+    vals[i] = uint32_t(i);
+}
+
+__device__ inline double endpoint(const double *es, const double *ee, uint32_t s) { return (s & 1) ? ee[s >> 1] : es[s >> 1]; }
+
+// verticalLines[i] = 1 for i = sortedIndex[j]/2, j < N, when edge i is vertical (oacc:4569-4577)
+__global__ void k_vertical_flags(const uint32_t *__restrict__ sidx, const double *__restrict__ es,
+                                 const double *__restrict__ ee, int ne, uint32_t *vl) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ne) return;
+    int i = int(sidx[j] >> 1);
+    if (es[i] == ee[i]) vl[i] = 1u;
+}
+
+__global__ void k_vertical_loc(const uint32_t *__restrict__ vl, int ne, uint32_t *vloc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne) return;
+    if (vl[i] != vl[i + 1]) vloc[vl[i]] = uint32_t(i);
+}
+
+// unique flags with the reference's rules (oacc:4601-4635)
+__global__ void k_unique_flags(const uint32_t *__restrict__ sidx, const double *__restrict__ es,
+                               const double *__restrict__ ee, int ne, const uint32_t *__restrict__ vl,
+                               const uint32_t *__restrict__ vloc, uint32_t *flag, double *val3) {
+    int64_t m = 2 * int64_t(ne);
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m - 1) return;
+    uint32_t si = sidx[i], sn = sidx[i + 1];
+    double a = endpoint(es, ee, si), b = endpoint(es, ee, sn);
+    int intvi = int(si >> 1);
+    uint32_t comid = vl[i / 2];
+    uint32_t f = 0;
+    double out = 0.0;
+    if (a != b) {
+        out = a;
+        f = 1;
+    } else if (es[intvi] == ee[intvi]) {
+        bool ok = comid == 0;
+        if (!ok) {
+            int lk = int(vloc[comid - 1]);
+            ok = es[lk] != es[intvi] && ee[lk] != es[intvi];
+        }
+        if (ok) {
+            out = es[intvi];
+            f = 1;
+        }
+    }
+    flag[i] = f;
+    val3[i] = out;
+    if (i == m - 2) {
+        double c = endpoint(es, ee, sidx[m - 2]), d = endpoint(es, ee, sidx[m - 1]);
+        if (c != d) {
+            flag[m - 1] = 1;
+            val3[m - 1] = c;
+        } else {
+            flag[m - 1] = 0;
+        }
+        flag[m] = 0;
+    }
+}
+
+__global__ void k_scatter_elementary(const uint32_t *__restrict__ pos, const double *__restrict__ val3, int64_t m,
+                                     double *ea) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    if (pos[i] != pos[i + 1]) ea[pos[i]] = val3[i];
+}
+
+__global__ void k_pad_elementary(double *ea, int cnt, int round) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cnt + i >= round) return;
+    ea[cnt + i] = ea[cnt - 1];
+}
+
+// ---- A4 ----
+__global__ void k_node_intervals(const double *__restrict__ ea, int T, int D, double *niv) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= T) return;
+    if (v == 0) {
+        niv[0] = 0.0;
+        niv[1] = 0.0;
+        return;
+    }
+    int d = 31 - __clz(v);          // depth of v
+    int sh = D - 1 - d;             // log2 of leaves under v
+    int first = (v - (1 << d)) << sh;
+    niv[2 * v] = ea[first];
+    niv[2 * v + 1] = ea[first + (1 << sh)];
+}
+
+// ---- A6 leaf end entries ----
+// first i in [1, epc-1] with ea[i] == q, or -1 (intended findBoarder, oacc:420-460)
+__device__ inline int first_point(const double *ea, int epc, double q) {
+    int lo = 1, hi = epc; // search [1, epc)
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (ea[mid] < q) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < epc && ea[lo] == q) return lo;
+    return -1;
+}
+
+
+// per edge: "after"/"before" leaf node (or -1) and the two (class,leaf) keys of
+// its leaf end entries; sentinel key 2*nleaves marks "not inserted"
+__global__ void k_leaf_targets(const double *__restrict__ es, const double *__restrict__ ee,
+                               const int *__restrict__ ety, int ne, const double *__restrict__ ea, int epc,
+                               int nleaves, int *after, int *before, uint64_t *keys, uint32_t *vals) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const int half = nleaves, T = 2 * nleaves;
+    int a = -1, b = -1;
+    int ie = first_point(ea, epc, ee[e]);
+    if (ie >= 0) {
+        int rl = half + ie;
+        if (ie != 1 && ie + 1 <= epc - 1 && ea[ie] == ea[ie + 1]) rl = half + ie + 1;
+        if (rl < T && ea[rl - half] != ea[rl - half + 1]) a = rl;
+    }
+    int is = first_point(ea, epc, es[e]);
+    if (is >= 0) {
+        int l = is - 1;
+        if (ea[l] != ea[l + 1]) b = half + l;
+    }
+    after[e] = a;
+    before[e] = b;
+    uint64_t cls = ety[e] != 0 ? uint64_t(nleaves) : 0ull;
+    const uint64_t none = 2ull * uint64_t(nleaves);
+    keys[2 * int64_t(e)] = a >= 0 ? cls + uint64_t(a - half) : none;
+    keys[2 * int64_t(e) + 1] = b >= 0 ? cls + uint64_t(b - half) : none;
+    vals[2 * int64_t(e)] = uint32_t(e) | 0x80000000u; // "after" entry
+    vals[2 * int64_t(e) + 1] = uint32_t(e);
+}
+
+__global__ void k_key_hist(const uint64_t *__restrict__ keys, int64_t n, uint64_t limit, uint32_t *cnt) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = keys[i];
+    if (k < limit) atomicAdd(&cnt[k], 1u);
+}
+
+// ---- A5 cover lists ----
+__device__ inline int lower_bound_d(const double *a, int n, double q) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < q) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ inline int upper_bound_d(const double *a, int n, double q) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= q) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <bool FILL>
+__global__ void k_cover(const double *__restrict__ es, const double *__restrict__ ee, int ne, int nb,
+                        const double *__restrict__ ea, int epc, int nleaves, uint32_t *cntb, uint32_t *cntc,
+                        const uint32_t *__restrict__ off, uint32_t *__restrict__ out) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    int L = lower_bound_d(ea, epc, es[e]);
+    int R = upper_bound_d(ea, epc, ee[e]) - 1; // last point <= end
+    if (L >= R) return;
+    int l = L + nleaves, r = R + nleaves;
+    const bool base = e < nb;
+    while (l < r) {
+        if (l & 1) {
+            int v = l++;
+            if (!FILL) atomicAdd(base ? &cntb[v] : &cntc[v], 1u);
+            else if (base) out[off[v] + atomicAdd(&cntb[v], 1u)] = uint32_t(e);
+            else out[off[v + 1] - 1u - atomicAdd(&cntc[v], 1u)] = uint32_t(e);
+        }
+        if (r & 1) {
+            int v = --r;
+            if (!FILL) atomicAdd(base ? &cntb[v] : &cntc[v], 1u);
+            else if (base) out[off[v] + atomicAdd(&cntb[v], 1u)] = uint32_t(e);
+            else out[off[v + 1] - 1u - atomicAdd(&cntc[v], 1u)] = uint32_t(e);
+        }
+        l >>= 1;
+        r >>= 1;
+    }
+}
+
+__global__ void k_add_u32(const uint32_t *a, const uint32_t *b, uint32_t *o, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = a[i] + b[i];
+}
+
+// ---- materialised end lists (parity and drop-in API) ----
+__global__ void k_mend_keys(const uint64_t *__restrict__ lkeys, const uint32_t *__restrict__ lvals, int64_t n,
+                            int nleaves, int D, const int *__restrict__ eid, const int *__restrict__ ety, int idb,
+                            int tb, uint64_t *out) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = lkeys[i];
+    int leaf = int(k >= uint64_t(nleaves) ? k - uint64_t(nleaves) : k);
+    uint32_t e = lvals[i] & 0x7fffffffu;
+    uint64_t idt = (uint64_t(eid[e]) << tb) | uint64_t(ety[e]);
+    int node = nleaves + leaf;
+    for (int d = 0; d < D; ++d) {
+        out[i * D + d] = (uint64_t(node) << (idb + tb)) | idt;
+        node >>= 1;
+    }
+}
+
+__global__ void k_diff_flags(const uint64_t *__restrict__ k, int64_t n, uint32_t *f) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f[i] = (i == 0 || k[i] != k[i - 1]) ? 1u : 0u;
+    if (i == n - 1) f[n] = 0u;
+}
+
+__global__ void k_mend_scatter(const uint64_t *__restrict__ k, const uint32_t *__restrict__ pos, int64_t n,
+                               int idb, int tb, const int *__restrict__ eoff, const int *__restrict__ coff,
+                               uint32_t *out_e, uint32_t *cnt) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (pos[i] == pos[i + 1]) return;
+    uint64_t key = k[i];
+    int type = int(key & ((1ull << tb) - 1ull));
+    int id = int((key >> tb) & ((1ull << idb) - 1ull));
+    int node = int(key >> (idb + tb));
+    int e = type == 0 ? id / 2 : eoff[type - 1] + (id / 2 - coff[type - 1]);
+    out_e[pos[i]] = uint32_t(e);
+    atomicAdd(&cnt[node], 1u);
+}
+
+} // namespace
+
+uint32_t read_u32(Ctx &c, const uint32_t *d) {
+    uint32_t h = 0;
+    SC_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    SC_HIP(hipStreamSynchronize(c.stream));
+    return h;
+}
+uint64_t read_u64(Ctx &c, const uint64_t *d) {
+    uint64_t h = 0;
+    SC_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    SC_HIP(hipStreamSynchronize(c.stream));
+    return h;
+}
+
+void ctx_set_polygons(Ctx &c, const double *bxy, int nb, const double *cxy, const int *csz, int K, bool device_ptrs) {
+    SC_REQUIRE(bxy && cxy && csz, SC_EINVAL, "null polygon pointer");
+    SC_REQUIRE(nb >= 3, SC_EINVAL, "base polygon needs at least 3 vertices");
+    SC_REQUIRE(K >= 1, SC_EINVAL, "at least one overlay polygon is required");
+    int64_t nct = 0;
+    for (int i = 0; i < K; ++i) {
+        SC_REQUIRE(csz[i] >= 3, SC_EINVAL, "overlay polygon " + std::to_string(i + 1) + " needs at least 3 vertices");
+        nct += csz[i];
+    }
+    SC_REQUIRE(2 * (int64_t(nb) + nct + K) < (int64_t(1) << 31), SC_ERANGE, "too many vertices for 32-bit ids");
+    c.nb = nb;
+    c.K = K;
+    c.nct = int(nct);
+    c.ne = nb + int(nct);
+    c.csz.assign(csz, csz + K);
+    c.coff.assign(K + 1, 0);
+    c.eoff.assign(K + 1, 0);
+    c.eoff[0] = nb;
+    c.maxcsz = 0;
+    for (int i = 0; i < K; ++i) {
+        c.coff[i + 1] = c.coff[i] + csz[i] + 1;
+        c.eoff[i + 1] = c.eoff[i] + csz[i];
+        if (csz[i] > c.maxcsz) c.maxcsz = csz[i];
+    }
+    hipStream_t s = c.stream;
+    c.bpoly.ensure(2 * size_t(nb + 1));
+    c.cpoly.ensure(2 * size_t(nct + K));
+    c.d_coff.ensure(K + 1);
+    c.d_eoff.ensure(K + 1);
+    SC_HIP(hipMemcpyAsync(c.d_coff.p, c.coff.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, s));
+    SC_HIP(hipMemcpyAsync(c.d_eoff.p, c.eoff.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, s));
+    hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    SC_HIP(hipMemcpyAsync(c.bpoly.p, bxy, sizeof(double) * 2 * size_t(nb), kind, s));
+    k_close_base<<<1, 1, 0, s>>>(c.bpoly.p, nb);
+    SC_HIP(hipGetLastError());
+    // overlays: stage the concatenated input, then lay it out with closing vertices
+    double *stage = reinterpret_cast<double *>(c.u64a.ensure(2 * size_t(nct)));
+    SC_HIP(hipMemcpyAsync(stage, cxy, sizeof(double) * 2 * size_t(nct), kind, s));
+    k_layout_overlays<<<grid_for(nct), BLK, 0, s>>>(stage, int(nct), c.d_eoff.p, c.d_coff.p, K, nb, c.cpoly.p);
+    SC_HIP(hipGetLastError());
+    if (!device_ptrs) SC_HIP(hipStreamSynchronize(s)); // host buffers may be released by the caller
+    c.epc = c.nleaves = c.T = 0;
+    c.n_lend = c.n_cov = c.n_cand = c.n_pairs = c.n_mend = 0;
+    c.cur_k = 0;
+    c.pair_off.clear();
+}
+
+void build_edges(Ctx &c) {
+    SC_REQUIRE(c.ne > 0, SC_ESTATE, "polygons not set");
+    int ne = c.ne;
+    c.ex0.ensure(ne); c.ey0.ensure(ne); c.ex1.ensure(ne); c.ey1.ensure(ne);
+    c.estart.ensure(ne); c.eend.ensure(ne); c.etype.ensure(ne); c.eid.ensure(ne);
+    k_edges<<<grid_for(ne), BLK, 0, c.stream>>>(c.bpoly, c.cpoly, c.nb, ne, c.d_eoff, c.d_coff, c.K, c.ex0, c.ey0,
+                                                c.ex1, c.ey1, c.estart, c.eend, c.etype, c.eid);
+    SC_HIP(hipGetLastError());
+}
+
+void build_elementary(Ctx &c) {
+    SC_REQUIRE(c.ne > 0, SC_ESTATE, "edges not built");
+    hipStream_t s = c.stream;
+    const int ne = c.ne;
+    const int64_t m = 2 * int64_t(ne);
+    uint64_t *keys = c.k64a.ensure(m);
+    uint32_t *sidx = c.u32a.ensure(m);
+    k_endpoint_keys<<<grid_for(m), BLK, 0, s>>>(c.estart, c.eend, ne, keys, sidx);
+    SC_HIP(hipGetLastError());
+    radix_sort_u64(keys, sidx, m, 0, 64, c.rs, s);
+    // vertical-edge bookkeeping
+    uint32_t *vl = c.u32b.ensure(ne + 1);
+    SC_HIP(hipMemsetAsync(vl, 0, sizeof(uint32_t) * (ne + 1), s));
+    k_vertical_flags<<<grid_for(ne), BLK, 0, s>>>(sidx, c.estart, c.eend, ne, vl);
+    SC_HIP(hipGetLastError());
+    scan_exclusive_u32(vl, vl, ne + 1, nullptr, s);
+    uint32_t *vloc = c.u32c.ensure(ne + 1);
+    k_vertical_loc<<<grid_for(ne), BLK, 0, s>>>(vl, ne, vloc);
+    SC_HIP(hipGetLastError());
+    // unique flags, compaction
+    uint32_t *flag = c.u32d.ensure(m + 1);
+    double *val3 = reinterpret_cast<double *>(c.u64b.ensure(m));
+    SC_HIP(hipMemsetAsync(flag, 0, sizeof(uint32_t) * (m + 1), s));
+    k_unique_flags<<<grid_for(m), BLK, 0, s>>>(sidx, c.estart, c.eend, ne, vl, vloc, flag, val3);
+    SC_HIP(hipGetLastError());
+    uint32_t *tot = c.d_u32_total.ensure(1);
+    scan_exclusive_u32(flag, flag, m + 1, tot, s);
+    int cnt = int(read_u32(c, tot));
+    SC_REQUIRE(cnt >= 1, SC_EINVAL, "degenerate input: fewer than two distinct x values");
+    int round = next_pow2_int(cnt) + 1;
+    c.ea.ensure(round);
+    k_scatter_elementary<<<grid_for(m), BLK, 0, s>>>(flag, val3, m, c.ea);
+    SC_HIP(hipGetLastError());
+    if (round > cnt) {
+        k_pad_elementary<<<grid_for(round - cnt), BLK, 0, s>>>(c.ea, cnt, round);
+        SC_HIP(hipGetLastError());
+    }
+    c.epc = round;
+    c.nleaves = round - 1;
+    c.T = 2 * c.nleaves;
+    c.depth = ilog2_exact(c.T);
+}
+
+void build_node_intervals(Ctx &c) {
+    SC_REQUIRE(c.T > 0, SC_ESTATE, "elementary intervals not built");
+    c.niv.ensure(2 * size_t(c.T));
+    k_node_intervals<<<grid_for(c.T), BLK, 0, c.stream>>>(c.ea, c.T, c.depth, c.niv);
+    SC_HIP(hipGetLastError());
+}
+
+void build_leaf_end_entries(Ctx &c) {
+    SC_REQUIRE(c.T > 0, SC_ESTATE, "elementary intervals not built");
+    hipStream_t s = c.stream;
+    const int ne = c.ne, nl = c.nleaves;
+    const int64_t m = 2 * int64_t(ne);
+    c.after_leaf.ensure(ne);
+    c.before_leaf.ensure(ne);
+    uint64_t *keys = c.lend_key.ensure(m);
+    uint32_t *vals = c.lend_e.ensure(m);
+    k_leaf_targets<<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, c.etype, ne, c.ea, c.epc, nl, c.after_leaf,
+                                                c.before_leaf, keys, vals);
+    SC_HIP(hipGetLastError());
+    int kb = ilog2_exact(2 * int64_t(nl) + 1);
+    radix_sort_u64(keys, vals, m, 0, kb, c.rs, s);
+    uint32_t *off = c.lend_off.ensure(2 * size_t(nl) + 1);
+    SC_HIP(hipMemsetAsync(off, 0, sizeof(uint32_t) * (2 * size_t(nl) + 1), s));
+    k_key_hist<<<grid_for(m), BLK, 0, s>>>(keys, m, 2ull * uint64_t(nl), off);
+    SC_HIP(hipGetLastError());
+    uint32_t *tot = c.d_u32_total.ensure(1);
+    scan_exclusive_u32(off, off, 2 * int64_t(nl) + 1, tot, s);
+    c.n_lend = read_u32(c, tot);
+}
+
+void build_cover_lists(Ctx &c) {
+    SC_REQUIRE(c.T > 0, SC_ESTATE, "elementary intervals not built");
+    hipStream_t s = c.stream;
+    const int T = c.T;
+    uint32_t *cntb = c.cov_split.ensure(T);
+    uint32_t *cntc = c.u32b.ensure(T);
+    SC_HIP(hipMemsetAsync(cntb, 0, sizeof(uint32_t) * T, s));
+    SC_HIP(hipMemsetAsync(cntc, 0, sizeof(uint32_t) * T, s));
+    k_cover<false><<<grid_for(c.ne), BLK, 0, s>>>(c.estart, c.eend, c.ne, c.nb, c.ea, c.epc, c.nleaves, cntb, cntc,
+                                                  nullptr, nullptr);
+    SC_HIP(hipGetLastError());
+    uint32_t *off = c.cov_off.ensure(size_t(T) + 1);
+    k_add_u32<<<grid_for(T), BLK, 0, s>>>(cntb, cntc, off, T);
+    SC_HIP(hipGetLastError());
+    SC_HIP(hipMemsetAsync(off + T, 0, sizeof(uint32_t), s));
+    uint32_t *tot = c.d_u32_total.ensure(1);
+    scan_exclusive_u32(off, off, int64_t(T) + 1, tot, s);
+    c.n_cov = read_u32(c, tot);
+    c.cov_e.ensure(size_t(c.n_cov));
+    // fill: base entries from the front, overlay entries from the back
+    uint32_t *fb = c.u32c.ensure(T);
+    SC_HIP(hipMemsetAsync(fb, 0, sizeof(uint32_t) * T, s));
+    SC_HIP(hipMemsetAsync(cntc, 0, sizeof(uint32_t) * T, s));
+    k_cover<true><<<grid_for(c.ne), BLK, 0, s>>>(c.estart, c.eend, c.ne, c.nb, c.ea, c.epc, c.nleaves, fb, cntc,
+                                                 off, c.cov_e);
+    SC_HIP(hipGetLastError());
+}
+
+void materialize_end_lists(Ctx &c) {
+    SC_REQUIRE(c.n_lend >= 0 && c.T > 0, SC_ESTATE, "leaf end entries not built");
+    hipStream_t s = c.stream;
+    const int D = c.depth;
+    int64_t n = c.n_lend * D;
+    int idb = ilog2_exact(2 * (int64_t(c.nct) + c.K + c.nb) + 2);
+    int tb = ilog2_exact(int64_t(c.K) + 1);
+    SC_REQUIRE(D + idb + tb <= 64, SC_ERANGE, "end-list key does not fit in 64 bits");
+    uint64_t *keys = c.u64a.ensure(size_t(n > 0 ? n : 1));
+    // leaf keys/vals are still sorted in k64b / lend_e (first n_lend valid entries)
+    k_mend_keys<<<grid_for(c.n_lend), BLK, 0, s>>>(c.lend_key, c.lend_e, c.n_lend, c.nleaves, D, c.eid, c.etype, idb, tb,
+                                                   keys);
+    SC_HIP(hipGetLastError());
+    radix_sort_u64(keys, nullptr, n, 0, D + idb + tb, c.rs, s);
+    uint32_t *flag = c.u32d.ensure(size_t(n) + 1);
+    SC_HIP(hipMemsetAsync(flag, 0, sizeof(uint32_t) * (size_t(n) + 1), s));
+    if (n > 0) {
+        k_diff_flags<<<grid_for(n), BLK, 0, s>>>(keys, n, flag);
+        SC_HIP(hipGetLastError());
+    }
+    uint32_t *tot = c.d_u32_total.ensure(1);
+    scan_exclusive_u32(flag, flag, n + 1, tot, s);
+    c.n_mend = read_u32(c, tot);
+    c.mend_e.ensure(size_t(c.n_mend));
+    uint32_t *cnt = c.mend_off.ensure(size_t(c.T) + 1);
+    SC_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (size_t(c.T) + 1), s));
+    if (n > 0) {
+        k_mend_scatter<<<grid_for(n), BLK, 0, s>>>(keys, flag, n, idb, tb, c.d_eoff, c.d_coff, c.mend_e, cnt);
+        SC_HIP(hipGetLastError());
+    }
+    scan_exclusive_u32(cnt, cnt, int64_t(c.T) + 1, nullptr, s);
+}
+
+} // namespace sc

@@ -1,0 +1,189 @@
+"""segclip -- Python view of libsegclip, the MI355X segment-tree polygon clipper.
+
+`Clipper` drives the C ABI stage by stage (the same stages as the reference's
+SegTree methods) and copies intermediate arrays back as numpy arrays; `synth`
+builds the seeded workload rings.  The compute runs only in libsegclip.so on
+the GPU: if the library or a GPU is missing the calls raise.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (Counts, SegclipError, as_ptr, lib, OK, ESPACE)  # noqa: F401
+from . import synth  # noqa: F401
+
+__all__ = ["Clipper", "SegclipError", "synth", "lib"]
+
+
+class Clipper:
+    """One clipping problem (base ring + K overlay rings) resident on one GPU."""
+
+    def __init__(self, device=0, stream=None):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.segclip_create(C.byref(h), int(device), C.c_void_p(stream) if stream else None)
+        if rc != OK:
+            raise SegclipError(rc, "segclip_create failed (no usable HIP device?)")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().segclip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------
+    def _check(self, rc):
+        if rc != OK:
+            raise SegclipError(rc, lib().segclip_last_error(self._h).decode())
+
+    def set_polygons(self, base, overlays):
+        base = np.ascontiguousarray(base, dtype=np.float64).reshape(-1, 2)
+        ovs = [np.ascontiguousarray(o, dtype=np.float64).reshape(-1, 2) for o in overlays]
+        sizes = np.array([len(o) for o in ovs], dtype=np.int32)
+        cxy = np.ascontiguousarray(np.concatenate(ovs, axis=0))
+        self.nb, self.K, self.sizes = len(base), len(ovs), sizes
+        self._check(lib().segclip_set_polygons(self._h, as_ptr(base, C.c_double), len(base),
+                                               as_ptr(cxy, C.c_double), as_ptr(sizes, C.c_int32), len(ovs)))
+        return self
+
+    def build_tree(self):
+        self._check(lib().segclip_build_tree(self._h))
+        return self
+
+    def materialize_end_lists(self):
+        self._check(lib().segclip_materialize_end_lists(self._h))
+        return self
+
+    def find_candidates(self):
+        self._check(lib().segclip_find_candidates(self._h))
+        return self
+
+    def intersections(self, k=1):
+        self._check(lib().segclip_intersections(self._h, int(k)))
+        return self
+
+    def label_traverse(self):
+        self._check(lib().segclip_label_traverse(self._h))
+        return self
+
+    def clip(self, k=1):
+        self._check(lib().segclip_clip(self._h, int(k)))
+        return self
+
+    def counts(self):
+        c = Counts()
+        self._check(lib().segclip_get_counts(self._h, C.byref(c)))
+        return c.as_dict()
+
+    def set_timing(self, on=True):
+        self._check(lib().segclip_set_timing(self._h, 1 if on else 0))
+
+    def stage_times(self):
+        a = (C.c_float * 8)()
+        self._check(lib().segclip_get_stage_times(self._h, a))
+        names = ["elementary", "tree", "endlist", "cover", "stab", "dedupe", "post", "label"]
+        return dict(zip(names, list(a)))
+
+    # ---- copy-out ------------------------------------------------------
+    def _need(self, fn, *args):
+        n = C.c_int64(0)
+        rc = fn(self._h, *args, C.byref(n))
+        if rc not in (OK, ESPACE):
+            self._check(rc)
+        return n.value
+
+    def elementary(self):
+        L = lib()
+        n = self._need(L.segclip_copy_elementary, None, 0)
+        out = np.zeros(n, np.float64)
+        self._check(L.segclip_copy_elementary(self._h, as_ptr(out, C.c_double), n, None))
+        return out
+
+    def node_intervals(self):
+        L = lib()
+        n = self._need(L.segclip_copy_node_intervals, None, 0)
+        out = np.zeros(n, np.float64)
+        self._check(L.segclip_copy_node_intervals(self._h, as_ptr(out, C.c_double), n, None))
+        return out
+
+    def cover_lists(self):
+        L = lib()
+        n = self._need(L.segclip_copy_cover_lists, None, None, 0)
+        T = self.counts()["tree_size"]
+        off = np.zeros(T + 1, np.uint32)
+        e = np.zeros(max(n, 1), np.uint32)
+        self._check(L.segclip_copy_cover_lists(self._h, as_ptr(off, C.c_uint32), as_ptr(e, C.c_uint32), n, None))
+        return off.astype(np.int64), e[:n].astype(np.int64)
+
+    def end_lists(self):
+        L = lib()
+        n = self._need(L.segclip_copy_end_lists, None, None, 0)
+        T = self.counts()["tree_size"]
+        off = np.zeros(T + 1, np.uint32)
+        e = np.zeros(max(n, 1), np.uint32)
+        self._check(L.segclip_copy_end_lists(self._h, as_ptr(off, C.c_uint32), as_ptr(e, C.c_uint32), n, None))
+        return off.astype(np.int64), e[:n].astype(np.int64)
+
+    def leaf_targets(self):
+        L = lib()
+        n = self._need(L.segclip_copy_leaf_targets, None, None, 0)
+        a = np.zeros(n, np.int32)
+        b = np.zeros(n, np.int32)
+        self._check(L.segclip_copy_leaf_targets(self._h, as_ptr(a, C.c_int32), as_ptr(b, C.c_int32), n, None))
+        return a, b
+
+    def pairs(self, k=1):
+        L = lib()
+        n = self._need(L.segclip_copy_pairs, int(k), None, None, None, 0)
+        b = np.zeros(max(n, 1), np.int32)
+        c = np.zeros(max(n, 1), np.int32)
+        t = np.zeros(max(n, 1), np.int32)
+        self._check(L.segclip_copy_pairs(self._h, int(k), as_ptr(b, C.c_int32), as_ptr(c, C.c_int32),
+                                         as_ptr(t, C.c_int32), n, None))
+        return b[:n], c[:n], t[:n]
+
+    def intersection_points(self):
+        L = lib()
+        n = self._need(L.segclip_copy_intersections, None, None, None, 0)
+        xy = np.zeros((max(n, 1), 2), np.float64)
+        qa = np.zeros(max(n, 1), np.int32)
+        qb = np.zeros(max(n, 1), np.int32)
+        self._check(L.segclip_copy_intersections(self._h, as_ptr(xy, C.c_double), as_ptr(qa, C.c_int32),
+                                                 as_ptr(qb, C.c_int32), n, None))
+        return xy[:n], qa[:n], qb[:n]
+
+    def augmented(self, side):
+        L = lib()
+        n = self._need(L.segclip_copy_augmented, int(side), None, None, None, None, 0)
+        xy = np.zeros((max(n, 1), 2), np.float64)
+        nbr = np.zeros(max(n, 1), np.int32)
+        av = np.zeros(max(n, 1), np.int32)
+        lab = np.zeros(max(n, 1), np.int32)
+        self._check(L.segclip_copy_augmented(self._h, int(side), as_ptr(xy, C.c_double), as_ptr(nbr, C.c_int32),
+                                             as_ptr(av, C.c_int32), as_ptr(lab, C.c_int32), n, None))
+        return xy[:n], nbr[:n], av[:n], lab[:n]
+
+    def rings(self):
+        L = lib()
+        need = (C.c_int64 * 2)()
+        rc = L.segclip_copy_rings(self._h, None, 0, None, 0, need)
+        if rc not in (OK, ESPACE):
+            self._check(rc)
+        off = np.zeros(need[0], np.int32)
+        xy = np.zeros((max(need[1], 1), 2), np.float64)
+        self._check(L.segclip_copy_rings(self._h, as_ptr(off, C.c_int32), need[0], as_ptr(xy, C.c_double),
+                                         need[1], need))
+        xy = xy[:need[1]]
+        return [xy[off[i]:off[i + 1]] for i in range(len(off) - 1)]

@@ -1,0 +1,45 @@
+"""Input rings shared by the tests: seeded workloads and crafted degenerate cases."""
+import math
+
+import numpy as np
+
+import segclip.synth as S
+
+
+def square(x0, y0, x1, y1):
+    return np.array([[x0, y0], [x1, y0], [x1, y1], [x0, y1]], np.float64)
+
+
+def workloads_small():
+    """(name, base, overlays) for sizes the oracle handles in about a second."""
+    out = []
+    b, o = S.star1k()
+    out.append(("star1k", b, o))
+    b = S.harmonic(20000, 2001)
+    c = S.harmonic(100000, 2002, 0.15, -0.1, 1.0, 17.0 * math.pi / 180.0)
+    out.append(("harmonic20k_100k", b, [c]))
+    b = S.harmonic(3000, 77)
+    c = S.star(2000, 78, 0.1, 0.05, 0.4, 0.9)
+    out.append(("harmonic3k_star2k", b, [c]))
+    return out
+
+
+def degenerate_cases():
+    """Crafted inputs with the special cases whose answers are known in closed form."""
+    cases = []
+    # overlapping axis-aligned squares (vertical edges, X intersections): area = overlap product
+    cases.append(("squares", square(0.0, 0.0, 2.0, 2.0), [square(1.0, 0.5, 3.0, 1.5)], 1.0 * 1.0))
+    # containment: overlay strictly inside the base -> output = overlay ring
+    b = S.star(500, 11, 0.0, 0.0, 2.0, 3.0)
+    c = S.star(300, 12, 0.0, 0.0, 0.5, 1.0)
+    cases.append(("contained", b, [c], None))
+    # disjoint: no output
+    cases.append(("disjoint", square(0.0, 0.0, 1.0, 1.0), [square(5.0, 5.0, 6.0, 6.0)], 0.0))
+    # rotated squares crossing (no vertical edges)
+    th = math.radians(30.0)
+    R = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+    sq = square(-1.0, -1.0, 1.0, 1.0) @ R.T
+    sq2 = square(-1.0, -1.0, 1.0, 1.0) @ np.array([[math.cos(-0.4), -math.sin(-0.4)],
+                                                     [math.sin(-0.4), math.cos(-0.4)]]).T + np.array([0.3, 0.2])
+    cases.append(("rotated_squares", sq, [sq2], None))
+    return cases

@@ -1,0 +1,142 @@
+"""CPU checks of the oracle itself (no GPU).
+
+What pins each oracle stage (the reference has no tests or golden files):
+  * output rings vs the reference's own sequential Foster clipper
+    (oracle/_ref/polyclip, built from lib/optimizedFostersAlgorithm): same
+    component count, same vertex counts, same cyclic order, coordinates equal
+    to its 6-significant-digit print;
+  * pair set vs brute force over all base x overlay edges;
+  * elementary points / node intervals / cover and end lists vs their closed
+    forms computed here independently in numpy;
+  * special cases with known answers (overlapping squares, containment,
+    disjoint rings);
+  * committed golden vectors (tests/golden) for the predicates.
+"""
+import json
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from fixtures import degenerate_cases, workloads_small
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _small():
+    return [w for w in workloads_small() if w[0] != "harmonic20k_100k"]
+
+
+@pytest.mark.parametrize("case", _small(), ids=lambda c: c[0])
+def test_pairs_equal_brute_force(case):
+    name, b, ovs = case
+    tr = O.Tree(b, ovs)
+    r = tr.clip(1, canonical=True)
+    bb, bc, bt = tr.brute_pairs(1)
+    assert set(zip(r.pair_b, r.pair_c, r.pair_type)) == set(zip(bb, bc, bt))
+
+
+@pytest.mark.parametrize("case", _small(), ids=lambda c: c[0])
+def test_literal_and_canonical_orders_agree(case):
+    name, b, ovs = case
+    tr = O.Tree(b, ovs)
+    lit, can = tr.clip(1, canonical=False), tr.clip(1, canonical=True)
+    assert lit.status == 0 and can.status == 0
+    assert len(lit.rings) == len(can.rings)
+    for x, y in zip(lit.rings, can.rings):
+        assert np.array_equal(x, y)
+
+
+def _fmt6(v):
+    return float("%g" % v)
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_POLYCLIP), reason="reference Foster clipper not built (no reference tree)")
+@pytest.mark.parametrize("case", _small() + [(n, b, o) for n, b, o, _ in degenerate_cases()], ids=lambda c: c[0])
+def test_rings_match_reference_foster(case):
+    name, b, ovs = case
+    r = O.Tree(b, ovs).clip(1, canonical=True)
+    with tempfile.TemporaryDirectory() as d:
+        ref = O.run_reference_foster(b, ovs[0], d)
+    assert len(ref) == len(r.rings)
+    for mine, theirs in zip(r.rings, ref):
+        assert mine.shape == theirs.shape
+        printed = np.vectorize(_fmt6)(mine)
+        assert np.array_equal(printed, theirs)
+
+
+def test_elementary_closed_form():
+    b, ovs = workloads_small()[0][1:]
+    tr = O.Tree(b, ovs)
+    e = tr.edges()
+    pts = np.concatenate([e["start"], e["end"]])
+    u = np.unique(pts)
+    # last-of-run rule of the reference drops the largest value (no vertical edges here)
+    kept = u[:-1]
+    n = 1 << int(np.ceil(np.log2(len(kept))))
+    want = np.concatenate([kept, np.full(n + 1 - len(kept), kept[-1])])
+    assert np.array_equal(tr.elementary(), want)
+    T = tr.tree_size
+    niv = tr.node_intervals().reshape(-1, 2)
+    D = int(np.log2(T))
+    for v in range(1, T):
+        d = int(np.floor(np.log2(v)))
+        sh = D - 1 - d
+        first = (v - (1 << d)) << sh
+        assert niv[v, 0] == want[first] and niv[v, 1] == want[first + (1 << sh)]
+
+
+def test_cover_and_end_lists_closed_form():
+    b, ovs = workloads_small()[2][1:]
+    tr = O.Tree(b, ovs)
+    e = tr.edges()
+    ea = tr.elementary()
+    T = tr.tree_size
+    half = T // 2
+    niv = tr.node_intervals().reshape(-1, 2)
+    off, ce = tr.cover()
+    # cover: Int(v) inside [s,e] and Int(parent) not inside
+    for idx in range(0, len(e), 7):
+        s, t = e["start"][idx], e["end"][idx]
+        inside = (niv[:, 0] >= s) & (niv[:, 1] <= t)
+        inside[0] = False
+        par = np.arange(T) // 2
+        want = np.nonzero(inside & ~np.where(np.arange(T) > 1, inside[par], False))[0]
+        got = [v for v in range(1, T) if idx in set(ce[off[v]:off[v + 1]])]
+        assert list(want) == got
+    # end lists: sorted set of edges whose after/before leaf lies under v
+    after, before = tr.leaf_targets()
+    eo, ee = tr.end_lists()
+    key = e["id"].astype(np.int64) * 4096 + e["type"]
+    D = int(np.log2(T))
+    for v in list(range(1, 64)) + list(range(half, half + 64)):
+        d = int(np.floor(np.log2(v)))
+        sh = D - 1 - d
+        lo = half + ((v - (1 << d)) << sh)
+        hi = lo + (1 << sh)
+        members = np.nonzero(((after >= lo) & (after < hi)) | ((before >= lo) & (before < hi)))[0]
+        want = members[np.argsort(key[members], kind="stable")]
+        assert np.array_equal(ee[eo[v]:eo[v + 1]], want)
+
+
+@pytest.mark.parametrize("case", degenerate_cases(), ids=lambda c: c[0])
+def test_special_case_areas(case):
+    name, b, ovs, area = case
+    r = O.Tree(b, ovs).clip(1, canonical=True)
+    if area is not None:
+        assert abs(abs(r.area()) - area) < 1e-12
+    if name == "contained":
+        assert len(r.rings) == 1 and len(r.rings[0]) == len(ovs[0])
+
+
+def test_predicate_golden_vectors():
+    with open(os.path.join(GOLDEN, "predicates.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        t, a, b = O.classify(np.array(c["p"]))
+        assert t == c["type"], c
+        assert O.lsmf(np.array(c["p"])) == c["lsmf"]
+        if "alpha" in c:
+            assert a == c["alpha"] and b == c["beta"]
