@@ -80,6 +80,9 @@ int segclip_intersections(segclip_ctx *ctx, int k);
 int segclip_label_traverse(segclip_ctx *ctx);
 /* Whole path for overlay k: the tree and candidates are built on first use. */
 int segclip_clip(segclip_ctx *ctx, int k);
+/* Drop everything derived from the polygons (tree, candidates, results) so
+ * that the next segclip_clip recomputes it; the resident polygons stay. */
+int segclip_reset(segclip_ctx *ctx);
 
 typedef struct {
     int64_t num_edges;        /* N */
@@ -139,7 +142,10 @@ int segclip_synth_convex(double *xy, int n, uint64_t seed, double cx, double cy,
 void segclip_synth_uniform(double *out, int n, uint64_t seed);
 int segclip_count_vertical(const double *xy, int n);
 
-/* Per-stage device times in ms of the last segclip_clip with timing enabled */
+/* Per-stage device times in ms of the last segclip_clip with timing enabled,
+ * from HIP events on the context's stream: elementary (edges + sort + unique),
+ * node intervals, leaf end lists, cover lists, candidates (stab + dedupe),
+ * the stab kernel alone, intersections (A11-A15), labelling + traversal. */
 int segclip_set_timing(segclip_ctx *ctx, int on);
 int segclip_get_stage_times(segclip_ctx *ctx, float *ms8);
 

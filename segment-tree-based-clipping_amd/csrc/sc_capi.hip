@@ -11,7 +11,19 @@ struct segclip_ctx {
 
 namespace sc {
 Ctx::~Ctx() {
+    if (ev_ready)
+        for (auto &e : ev) (void)hipEventDestroy(e);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+void Ctx::ensure_events() {
+    if (ev_ready) return;
+    for (auto &e : ev) SC_HIP(hipEventCreate(&e));
+    ev_ready = true;
+}
+float Ctx::elapsed(int a, int b) {
+    float ms = 0;
+    SC_HIP(hipEventElapsedTime(&ms, ev[a], ev[b]));
+    return ms;
 }
 } // namespace sc
 
@@ -132,6 +144,14 @@ int segclip_label_traverse(segclip_ctx *h) {
 }
 int segclip_clip(segclip_ctx *h, int k) {
     return guarded(h, [&](sc::Ctx &x) { sc::clip_all(x, k); });
+}
+int segclip_reset(segclip_ctx *h) {
+    return guarded(h, [&](sc::Ctx &x) {
+        x.epc = x.nleaves = x.T = 0;
+        x.n_lend = x.n_cov = x.n_cand = x.n_pairs = x.n_mend = 0;
+        x.cur_k = 0;
+        x.pair_off.clear();
+    });
 }
 
 int segclip_get_counts(segclip_ctx *h, segclip_counts *o) {
@@ -290,26 +310,38 @@ int segclip_copy_augmented(segclip_ctx *h, int side, double *xy, int32_t *nbr, i
 int segclip_copy_rings(segclip_ctx *h, int32_t *ring_off, int64_t off_cap, double *xy, int64_t xy_cap,
                        int64_t *need) {
     if (!h) return SEGCLIP_EINVAL;
-    const sc::Ctx &c = h->c;
-    if (need) need[0] = c.nrings + 1;
-    if (need) need[1] = int64_t(c.ring_xy.size() / 2);
-    if (int64_t(c.nrings) + 1 > off_cap || int64_t(c.ring_xy.size() / 2) > xy_cap) return SEGCLIP_ESPACE;
+    const sc::Ctx &c0 = h->c;
+    if (need) {
+        need[0] = int64_t(c0.nrings) + 1;
+        need[1] = c0.n_out;
+    }
+    if (int64_t(c0.nrings) + 1 > off_cap || c0.n_out > xy_cap) return SEGCLIP_ESPACE;
     if (!ring_off || !xy) return SEGCLIP_EINVAL;
-    std::copy(c.ring_off.begin(), c.ring_off.end(), ring_off);
-    std::copy(c.ring_xy.begin(), c.ring_xy.end(), xy);
-    return SEGCLIP_OK;
+    return guarded(h, [&](sc::Ctx &c) {
+        int64_t n = c.n_out;
+        std::vector<double> x(size_t(n > 0 ? n : 1)), y(size_t(n > 0 ? n : 1));
+        d2h(c, ring_off, c.d_ring_off.p, int64_t(c.nrings) + 1);
+        d2h(c, x.data(), c.d_ring_x.p, n);
+        d2h(c, y.data(), c.d_ring_y.p, n);
+        SC_HIP(hipStreamSynchronize(c.stream));
+        for (int64_t i = 0; i < n; ++i) {
+            xy[2 * i] = x[i];
+            xy[2 * i + 1] = y[i];
+        }
+    });
 }
 
 int segclip_set_timing(segclip_ctx *h, int on) {
-    if (!h) return SEGCLIP_EINVAL;
-    h->c.timing = on != 0;
-    return SEGCLIP_OK;
+    return guarded(h, [&](sc::Ctx &c) {
+        if (on) c.ensure_events();
+        c.timing = on != 0;
+    });
 }
 
 int segclip_get_stage_times(segclip_ctx *h, float *ms8) {
     if (!h || !ms8) return SEGCLIP_EINVAL;
     const sc::StageTimes &t = h->c.times;
-    float v[8] = {t.elementary, t.tree, t.endlist, t.cover, t.stab, t.dedupe, t.post, t.label};
+    float v[8] = {t.elementary, t.tree, t.endlist, t.cover, t.candidates, t.stab_kernel, t.post, t.label};
     std::memcpy(ms8, v, sizeof(v));
     return SEGCLIP_OK;
 }

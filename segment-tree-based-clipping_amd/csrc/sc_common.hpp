@@ -93,6 +93,8 @@ inline int ilog2_exact(int64_t n) {
 // total through *d_total when non-null (device pointer).
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total, hipStream_t s);
 void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *d_total, hipStream_t s);
+// Inclusive running maximum of n values (in == out allowed).
+void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s);
 
 // Stable LSD radix sort of (keys, vals) by key bits [bit_lo, bit_hi).
 // vals may be null (keys only).  The sorted data ends in keys/vals; the

@@ -7,7 +7,7 @@ namespace sc {
 
 // Per-stage timings filled when Ctx::timing is on (milliseconds, HIP events).
 struct StageTimes {
-    float elementary = 0, tree = 0, endlist = 0, cover = 0, stab = 0, dedupe = 0, post = 0, label = 0;
+    float elementary = 0, tree = 0, endlist = 0, cover = 0, candidates = 0, stab_kernel = 0, post = 0, label = 0;
 };
 
 struct Ctx {
@@ -93,6 +93,13 @@ struct Ctx {
 
     bool timing = false;
     StageTimes times;
+    hipEvent_t ev[10] = {};
+    bool ev_ready = false;
+    void ensure_events();
+    void mark(int i) {
+        if (timing) SC_HIP(hipEventRecord(ev[i], stream));
+    }
+    float elapsed(int a, int b);
 
     ~Ctx();
 };

@@ -240,6 +240,92 @@ __global__ __launch_bounds__(RS_BLOCK) void k_downsweep(const uint64_t *__restri
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// inclusive max-scan (u32): propagates the last head marker to the right
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t wave_incl_max(uint32_t x, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x = x > y ? x : y;
+    }
+    return x;
+}
+
+__device__ inline uint32_t block_incl_max(uint32_t x, uint32_t *s_wave) {
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    uint32_t inc = wave_incl_max(x, lane);
+    if (lane == WAVE - 1) s_wave[wid] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < wid; ++w) before = before > s_wave[w] ? before : s_wave[w];
+    __syncthreads();
+    return inc > before ? inc : before;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_max_tile(const uint32_t *in, int64_t n, uint32_t *sums) {
+    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
+    int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
+    uint32_t acc = 0;
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
+        if (i < n) acc = acc > in[i] ? acc : in[i];
+    }
+    uint32_t r = block_incl_max(acc, s_wave);
+    if (threadIdx.x == SCAN_BLOCK - 1) sums[blockIdx.x] = r;
+}
+
+// exclusive running max over the tile maxima (few entries: one thread)
+__global__ void k_max_sums(uint32_t *sums, int64_t n) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t carry = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t v = sums[i];
+        sums[i] = carry;
+        carry = carry > v ? carry : v;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_max_apply(const uint32_t *in, uint32_t *out, int64_t n,
+                                                          const uint32_t *prefix) {
+    __shared__ uint32_t s_data[SCAN_TILE];
+    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
+    int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
+        s_data[it * SCAN_BLOCK + threadIdx.x] = i < n ? in[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t loc[SCAN_ITEMS];
+    uint32_t acc = 0;
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        loc[j] = s_data[threadIdx.x * SCAN_ITEMS + j];
+        acc = acc > loc[j] ? acc : loc[j];
+    }
+    uint32_t inc = block_incl_max(acc, s_wave);
+    // exclusive within the block: max of the previous threads
+    uint32_t prevthr = __shfl_up(inc, 1, WAVE);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    uint32_t before = 0;
+    if (lane == 0) {
+        for (int w = 0; w < wid; ++w) before = before > s_wave[w] ? before : s_wave[w];
+    } else {
+        before = prevthr;
+    }
+    uint32_t run = prefix ? prefix[blockIdx.x] : 0u;
+    run = run > before ? run : before;
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        run = run > loc[j] ? run : loc[j];
+        s_data[threadIdx.x * SCAN_ITEMS + j] = run;
+    }
+    __syncthreads();
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
+        if (i < n) out[i] = s_data[it * SCAN_BLOCK + threadIdx.x];
+    }
+}
+
 } // namespace
 
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total, hipStream_t s) {
@@ -247,6 +333,22 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 }
 void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *d_total, hipStream_t s) {
     scan_impl<uint64_t>(in, out, n, d_total, s, scan_scratch<uint64_t>());
+}
+
+void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t nb = ceil_div(n, SCAN_TILE);
+    SC_REQUIRE(nb < (int64_t(1) << 31), SC_ERANGE, "scan too long");
+    uint32_t *sums = nullptr;
+    if (nb > 1) {
+        sums = scan_scratch<uint32_t>().sums.ensure(size_t(nb));
+        k_max_tile<<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, n, sums);
+        SC_HIP(hipGetLastError());
+        k_max_sums<<<1, 64, 0, s>>>(sums, nb);
+        SC_HIP(hipGetLastError());
+    }
+    k_max_apply<<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, out, n, sums);
+    SC_HIP(hipGetLastError());
 }
 
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs,

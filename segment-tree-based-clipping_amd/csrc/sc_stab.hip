@@ -260,10 +260,12 @@ void find_candidates(Ctx &c) {
         a.nl = c.nleaves; a.D = c.depth; a.T = T; a.nunits = nunits;
         a.bits_b = c.bits_b; a.bits_c = c.bits_c;
         a.out = c.cand; a.cap = uint64_t(c.cand_cap); a.counter = counter;
+        c.mark(8);
         if (nunits > 0) {
             k_stab<<<grid_for(nunits, BLK / WAVE), BLK, 0, s>>>(a);
             SC_HIP(hipGetLastError());
         }
+        c.mark(9);
         c.n_cand = int64_t(read_u64(c, c.d_u64_total.p));
         if (c.n_cand <= c.cand_cap) break;
         c.cand_cap = c.n_cand + c.n_cand / 4 + 1024;

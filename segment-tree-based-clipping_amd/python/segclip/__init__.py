@@ -82,6 +82,18 @@ class Clipper:
         self._check(lib().segclip_clip(self._h, int(k)))
         return self
 
+    def reset(self):
+        self._check(lib().segclip_reset(self._h))
+        return self
+
+    def set_polygons_device(self, base_ptr, nb, ov_ptr, sizes):
+        """Polygons already in device memory (pointers to x,y doubles)."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        self.nb, self.K, self.sizes = int(nb), len(sizes), sizes
+        self._check(lib().segclip_set_polygons_device(self._h, C.c_void_p(base_ptr), int(nb), C.c_void_p(ov_ptr),
+                                                      as_ptr(sizes, C.c_int32), len(sizes)))
+        return self
+
     def counts(self):
         c = Counts()
         self._check(lib().segclip_get_counts(self._h, C.byref(c)))
@@ -93,7 +105,7 @@ class Clipper:
     def stage_times(self):
         a = (C.c_float * 8)()
         self._check(lib().segclip_get_stage_times(self._h, a))
-        names = ["elementary", "tree", "endlist", "cover", "stab", "dedupe", "post", "label"]
+        names = ["elementary", "tree", "endlist", "cover", "candidates", "stab_kernel", "post", "label"]
         return dict(zip(names, list(a)))
 
     # ---- copy-out ------------------------------------------------------

@@ -17,7 +17,7 @@ SYMBOLS = [
     "segclip_create", "segclip_destroy", "segclip_last_error", "segclip_version", "segclip_set_polygons",
     "segclip_set_polygons_device", "segclip_build_elementary", "segclip_build_node_intervals",
     "segclip_build_end_lists", "segclip_build_cover_lists", "segclip_materialize_end_lists", "segclip_build_tree",
-    "segclip_find_candidates", "segclip_intersections", "segclip_label_traverse", "segclip_clip",
+    "segclip_find_candidates", "segclip_intersections", "segclip_label_traverse", "segclip_clip", "segclip_reset",
     "segclip_get_counts", "segclip_copy_elementary", "segclip_copy_node_intervals", "segclip_copy_cover_lists",
     "segclip_copy_end_lists", "segclip_copy_leaf_targets", "segclip_copy_pairs", "segclip_copy_intersections",
     "segclip_copy_augmented", "segclip_copy_rings", "segclip_stable_sort_by_key_f64",

@@ -42,4 +42,15 @@ def degenerate_cases():
     sq2 = square(-1.0, -1.0, 1.0, 1.0) @ np.array([[math.cos(-0.4), -math.sin(-0.4)],
                                                      [math.sin(-0.4), math.cos(-0.4)]]).T + np.array([0.3, 0.2])
     cases.append(("rotated_squares", sq, [sq2], None))
+    # identical rings: every vertex a V-overlap (ON_ON labels), output = the base ring
+    r = S.star(200, 21, 0.0, 0.0, 0.5, 1.0)
+    cases.append(("identical", r, [r.copy()], None))
+    # shared collinear edge pieces: squares sliding along x (overlap chains), area 2
+    cases.append(("shared_edges", square(0.0, 0.0, 2.0, 2.0), [square(1.0, 0.0, 3.0, 2.0)], 2.0))
+    # overlay touching the base boundary from inside at one vertex (bouncing)
+    tri = np.array([[1.0, 1.0], [3.0, 1.0], [2.0, 0.0]])
+    cases.append(("touch_inside", square(0.0, 0.0, 4.0, 4.0), [tri], 1.0))
+    # overlay vertex on a base edge, overlay crossing the base (T intersection)
+    q = np.array([[2.0, 1.0], [3.0, -1.0], [4.0, 1.0], [3.0, 3.0]])
+    cases.append(("vertex_on_edge", square(0.0, 0.0, 3.0, 2.0), [q], None))
     return cases

@@ -1,0 +1,67 @@
+"""GPU parity of the whole clip (rows A8-A17): output rings bit-exact to the oracle.
+
+Covers the parallel traversal (generic crossings) and the sequential device
+path (intersection chains, ON_ON rings, bouncing touches); the oracle itself
+is pinned to the reference's Foster clipper in test_oracle.py.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from fixtures import degenerate_cases, workloads_small
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    out = [(n, b, o) for n, b, o in workloads_small()]
+    out += [(n, b, o) for n, b, o, _ in degenerate_cases()]
+    return out
+
+
+CASES = _cases()
+
+
+def _rings_equal(g, o):
+    if len(g) != len(o):
+        return "ring count %d != %d" % (len(g), len(o))
+    for i, (a, b) in enumerate(zip(g, o)):
+        if a.shape != b.shape:
+            return "ring %d: %d vs %d vertices" % (i, len(a), len(b))
+        if not np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64)):
+            return "ring %d: coordinates differ (max %g)" % (i, np.max(np.abs(a - b)))
+    return None
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_clip_rings_bit_exact(gpu, case):
+    import segclip
+    name, b, ovs = case
+    print("[gpu-clip]", name, len(b), [len(o) for o in ovs], flush=True)
+    tr = O.Tree(b, ovs)
+    cl = segclip.Clipper()
+    cl.set_polygons(b, ovs)
+    for k in range(1, len(ovs) + 1):
+        r = tr.clip(k, canonical=True)
+        cl.clip(k)
+        cnt = cl.counts()
+        assert (cnt["undefined"] & 1) == (r.status & 1)
+        err = _rings_equal(cl.rings(), r.rings)
+        assert err is None, err
+        assert cnt["rings"] == len(r.rings)
+
+
+def test_one_to_many_matches_per_overlay_oracle(gpu):
+    import segclip
+    b = segclip.synth.harmonic(4000, 31)
+    ovs = [segclip.synth.star(n, 40 + i, cx, cy, 0.1, 0.4) for i, (n, cx, cy) in
+           enumerate([(50, 0.2, 0.1), (300, -0.5, 0.2), (1000, 0.0, 0.0), (12, 0.9, -0.3), (700, -0.2, -0.8)])]
+    tr = O.Tree(b, ovs)
+    cl = segclip.Clipper().set_polygons(b, ovs)
+    for k in range(1, len(ovs) + 1):
+        r = tr.clip(k, canonical=True)
+        cl.clip(k)
+        pb, pc, pt = cl.pairs(k)
+        assert np.array_equal(pb, r.pair_b) and np.array_equal(pc, r.pair_c) and np.array_equal(pt, r.pair_type)
+        err = _rings_equal(cl.rings(), r.rings)
+        assert err is None, "overlay %d: %s" % (k, err)
