@@ -25,6 +25,7 @@ struct Ctx {
     // per-edge SoA (edge index e in [0,ne): base edges, then overlay 1, 2, ...)
     DBuf<double> ex0, ey0, ex1, ey1; // endpoints in ring order
     DBuf<double> estart, eend;       // x-extent
+    DBuf<double> eymin, eymax;       // y-extent
     DBuf<int> etype, eid;
 
     // ---- A3 elementary intervals ----

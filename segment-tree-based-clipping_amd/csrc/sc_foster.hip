@@ -65,19 +65,26 @@ struct Ring {
 
 __device__ inline bool inter_of(const Ring &r, int i) { return r.av ? r.av[i] != -100 : r.nbr[i] != -100; }
 
-__global__ void k_classify(Ring r, int side, int n_other, uint32_t *cross, int *st) {
+__global__ __launch_bounds__(BLK) void k_classify(Ring r, int side, int n_other, uint32_t *cross, int *st) {
+    __shared__ int s_min;
+    if (threadIdx.x == 0) s_min = INT_MAX;
+    __syncthreads();
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= r.n) return;
-    bool in = inter_of(r, i);
-    int lab = r.lab[i];
-    if (in) {
-        if (lab != L_CROSSING && lab != L_BOUNCING) atomicOr(&st[ST_SLOW], SLOW_LABEL);
-        int nb = r.nbr[i];
-        if (nb < 1 || nb > n_other) atomicOr(&st[ST_SLOW], SLOW_LINK);
-    } else {
-        atomicMin(&st[side == 0 ? ST_PLAIN_P : ST_PLAIN_Q], i);
+    if (i < r.n) {
+        bool in = inter_of(r, i);
+        int lab = r.lab[i];
+        if (in) {
+            if (lab != L_CROSSING && lab != L_BOUNCING) atomicOr(&st[ST_SLOW], SLOW_LABEL);
+            int nb = r.nbr[i];
+            if (nb < 1 || nb > n_other) atomicOr(&st[ST_SLOW], SLOW_LINK);
+        } else {
+            atomicMin(&s_min, i);
+        }
+        cross[i] = (in && lab == L_CROSSING) ? 1u : 0u;
     }
-    cross[i] = (in && lab == L_CROSSING) ? 1u : 0u;
+    __syncthreads();
+    // first plain vertex: one global atomic per block
+    if (threadIdx.x == 0 && s_min != INT_MAX) atomicMin(&st[side == 0 ? ST_PLAIN_P : ST_PLAIN_Q], s_min);
 }
 
 // winding test of polygon::pointInPoly (polygon.h:435-457): the query point is
@@ -349,10 +356,11 @@ __global__ void k_cleanup_seq(int nr, const int *off, const int *rstart, const d
     }
 }
 
-__global__ void k_ring_ids(int nr, const int *off, int *ringid) {
-    int r = blockIdx.x;
+// ring heads carry their ring index; an inclusive max-scan spreads it
+__global__ void k_ring_heads(int nr, const int *off, uint32_t *head) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
-    for (int i = off[r] + threadIdx.x; i < off[r + 1]; i += blockDim.x) ringid[i] = r;
+    if (off[r + 1] > off[r]) head[off[r]] = uint32_t(r);
 }
 
 __global__ void k_ring_counts(int nr, const int *off, const uint32_t *kscan, uint32_t *cnt) {
@@ -751,7 +759,9 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n) {
     int *rstart = f.rstart.ensure(nr);
     int *ringid = f.ringid.ensure(size_t(n));
     k_root_loop<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, f.ox, f.oy, rstart);
-    k_ring_ids<<<nr, 64, 0, s>>>(nr, f.off, ringid);
+    SC_HIP(hipMemsetAsync(ringid, 0, sizeof(int) * size_t(n), s));
+    k_ring_heads<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, reinterpret_cast<uint32_t *>(ringid));
+    scan_inclusive_max_u32(reinterpret_cast<uint32_t *>(ringid), reinterpret_cast<uint32_t *>(ringid), n, s);
     uint32_t *keep = f.keep.ensure(size_t(n) + 1);
     uint32_t *mark = f.mark.ensure(size_t(n));
     k_keep0<<<grid_for(n), BLK, 0, s>>>(n, ringid, f.off, rstart, f.ox, f.oy, keep);

@@ -80,6 +80,42 @@ __host__ __device__ inline int classify(double b1x, double b1y, double b2x, doub
     return 0;
 }
 
+// classify() with division-free early rejection, for the stab's many tests:
+// in the non-parallel branch a pair can only get a non-zero type when
+// alpha = AP1/(AP1-AP2) and beta = AQ1/(AQ1-AQ2) both lie in [-eps, 1-eps).
+// A pair with alpha < -0.5 or alpha > 2 (likewise beta) is rejected before the
+// divisions; the bounds are tested exactly (x0.5 and x2 are exact, |d| > eps
+// excludes subnormals) and the division is monotone, so every rejected pair
+// also gets type 0 from classify().  Accepted pairs run classify()'s exact
+// arithmetic on the same values, hence the same type.
+__host__ __device__ inline int classify_pre(double b1x, double b1y, double b2x, double b2y, double c1x,
+                                            double c1y, double c2x, double c2y) {
+    double AP1 = areaA(b1x, b1y, c1x, c1y, c2x, c2y);
+    double AP2 = areaA(b2x, b2y, c1x, c1y, c2x, c2y);
+    double d = AP1 - AP2;
+    if (fabs(d) > EPS) {
+        if (d > 0 ? (AP1 < -0.5 * d || AP1 > 2.0 * d) : (AP1 > -0.5 * d || AP1 < 2.0 * d)) return 0;
+        double AQ1 = areaA(c1x, c1y, b1x, b1y, b2x, b2y);
+        double AQ2 = areaA(c2x, c2y, b1x, b1y, b2x, b2y);
+        double e = AQ1 - AQ2;
+        if (e > 0 ? (AQ1 < -0.5 * e || AQ1 > 2.0 * e) : (AQ1 > -0.5 * e || AQ1 < 2.0 * e)) return 0;
+        double alpha = AP1 / d;
+        double beta = AQ1 / e;
+        bool a0 = false, a01 = false, b0 = false, b01 = false;
+        if (alpha > EPS && alpha < 1.0 - EPS) a01 = true;
+        else if (fabs(alpha) <= EPS) a0 = true;
+        if (beta > EPS && beta < 1.0 - EPS) b01 = true;
+        else if (fabs(beta) <= EPS) b0 = true;
+        if (a01 && b01) return 1;
+        if (a0 && b01) return 2;
+        if (b0 && a01) return 3;
+        if (a0 && b0) return 4;
+        return 0;
+    }
+    double al, be;
+    return classify(b1x, b1y, b2x, b2y, c1x, c1y, c2x, c2y, al, be);
+}
+
 __host__ __device__ inline int alpha_to_int(double a) {
     if (a < EPS) return 0;
     return int(1000000000.0 * a);

@@ -76,30 +76,28 @@ __global__ void k_pair_decode(const uint64_t *__restrict__ pairs, int I, int bit
     if (nondeg_c(t)) atomicAdd(&nd_c[cl], 1);
 }
 
-// stable insertion sort of one edge's intersections by quantised alpha mod 1e9
-__device__ inline void sort_segment(int *ord, int n, const int *q) {
-    for (int i = 1; i < n; ++i) {
-        int x = ord[i];
-        int kx = q[x] % 1000000000;
-        int j = i;
-        while (j > 0 && (q[ord[j - 1]] % 1000000000) > kx) {
-            ord[j] = ord[j - 1];
-            --j;
-        }
-        ord[j] = x;
-    }
+// keys for the per-edge alpha order: quantised alpha modulo 1e9 (the nine
+// decimal digits the reference's LSD sort looks at) in the low 30 bits
+__global__ void k_alpha_keys(const int *__restrict__ q, const uint32_t *__restrict__ order, int I, uint64_t *keys,
+                             uint32_t *vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= I) return;
+    uint32_t p = order ? order[i] : uint32_t(i);
+    keys[i] = uint64_t(uint32_t(q[p] % 1000000000));
+    vals[i] = p;
 }
 
-// B side: pairs are already grouped by base edge; one thread per edge with intersections
-__global__ void k_order_b(const int *__restrict__ pb, int I, const int *__restrict__ all_b_scan,
-                          const int *__restrict__ qa, int *ordB) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= I) return;
-    if (p > 0 && pb[p - 1] == pb[p]) return;
-    int b = pb[p];
-    int s = all_b_scan[b], t = all_b_scan[b + 1];
-    for (int i = s; i < t; ++i) ordB[i] = i;
-    sort_segment(ordB + s, t - s, qa);
+// second key: the edge (stable on the alpha order from the first pass)
+__global__ void k_edge_keys(const int *__restrict__ edge_of, const uint32_t *__restrict__ vals, int I,
+                            uint64_t *keys) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= I) return;
+    keys[i] = uint64_t(uint32_t(edge_of[vals[i]]));
+}
+
+__global__ void k_copy_order(const uint32_t *__restrict__ vals, int I, int *ord) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < I) ord[i] = int(vals[i]);
 }
 
 __global__ void k_ckeys(const int *__restrict__ pb, const int *__restrict__ pc, int I, int bits_b, uint64_t *keys,
@@ -108,18 +106,6 @@ __global__ void k_ckeys(const int *__restrict__ pb, const int *__restrict__ pc, 
     if (p >= I) return;
     keys[p] = (uint64_t(pc[p]) << bits_b) | uint64_t(pb[p]);
     vals[p] = uint32_t(p);
-}
-
-__global__ void k_order_c(const uint32_t *__restrict__ byc, const int *__restrict__ pc, int I,
-                          const int *__restrict__ all_c_scan, const int *__restrict__ qb, int *ordC) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= I) return;
-    int p = int(byc[i]);
-    if (i > 0 && pc[byc[i - 1]] == pc[p]) return;
-    int cl = pc[p];
-    int s = all_c_scan[cl], t = all_c_scan[cl + 1];
-    for (int j = s; j < t; ++j) ordC[j] = int(byc[j]);
-    sort_segment(ordC + s, t - s, qb);
 }
 
 __global__ void k_init_aug(int n, double *ax, double *ay, int *av, int *nbr, int *lab) {
@@ -289,14 +275,29 @@ void post_overlay(Ctx &c, int k) {
     c.ncaug = cs + htot[1];
 
     if (I > 0) {
-        k_order_b<<<grid_for(I), BLK, 0, s>>>(c.p_b, I, all_b, c.p_qa, c.ordB);
+        // B side: pairs are in (base edge, overlay edge) order; stable sort by
+        // alpha, then stably by base edge -> per edge alpha order, ties in pair order
+        uint64_t *ak = c.k64a.ensure(Ia);
+        uint32_t *av = c.u32a.ensure(Ia);
+        k_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_qa, nullptr, I, ak, av);
         SC_HIP(hipGetLastError());
-        uint64_t *ck = c.k64a.ensure(Ia);
-        uint32_t *cv = c.u32a.ensure(Ia);
-        k_ckeys<<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_c, I, c.bits_b, ck, cv);
+        radix_sort_u64(ak, av, I, 0, 30, c.rs, s);
+        k_edge_keys<<<grid_for(I), BLK, 0, s>>>(c.p_b, av, I, ak);
         SC_HIP(hipGetLastError());
-        radix_sort_u64(ck, cv, I, 0, c.bits_b + c.bits_c, c.rs, s);
-        k_order_c<<<grid_for(I), BLK, 0, s>>>(cv, c.p_c, I, all_c, c.p_qb, c.ordC);
+        radix_sort_u64(ak, av, I, 0, c.bits_b, c.rs, s);
+        k_copy_order<<<grid_for(I), BLK, 0, s>>>(av, I, c.ordB);
+        // C side: order by (overlay edge, base edge), then stably by alpha, then by overlay edge
+        uint32_t *cv = c.u32b.ensure(Ia);
+        k_ckeys<<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_c, I, c.bits_b, ak, cv);
+        SC_HIP(hipGetLastError());
+        radix_sort_u64(ak, cv, I, 0, c.bits_b + c.bits_c, c.rs, s);
+        k_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_qb, cv, I, ak, av);
+        SC_HIP(hipGetLastError());
+        radix_sort_u64(ak, av, I, 0, 30, c.rs, s);
+        k_edge_keys<<<grid_for(I), BLK, 0, s>>>(c.p_c, av, I, ak);
+        SC_HIP(hipGetLastError());
+        radix_sort_u64(ak, av, I, 0, c.bits_c, c.rs, s);
+        k_copy_order<<<grid_for(I), BLK, 0, s>>>(av, I, c.ordC);
         SC_HIP(hipGetLastError());
     }
     const int nba = c.nbaug, nca = c.ncaug;

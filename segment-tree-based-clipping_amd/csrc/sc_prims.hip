@@ -276,14 +276,30 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_max_tile(const uint32_t *in, int
     if (threadIdx.x == SCAN_BLOCK - 1) sums[blockIdx.x] = r;
 }
 
-// exclusive running max over the tile maxima (few entries: one thread)
-__global__ void k_max_sums(uint32_t *sums, int64_t n) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint32_t carry = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        uint32_t v = sums[i];
-        sums[i] = carry;
-        carry = carry > v ? carry : v;
+// exclusive running max over the tile maxima, one block walking in chunks
+__global__ __launch_bounds__(SCAN_BLOCK) void k_max_sums(uint32_t *sums, int64_t n) {
+    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
+    __shared__ uint32_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < n; base += SCAN_BLOCK) {
+        int64_t i = base + threadIdx.x;
+        uint32_t v = i < n ? sums[i] : 0u;
+        uint32_t inc = block_incl_max(v, s_wave);
+        uint32_t prev = __shfl_up(inc, 1, WAVE);
+        const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+        uint32_t before = 0;
+        if (lane == 0) {
+            for (int w = 0; w < wid; ++w) before = before > s_wave[w] ? before : s_wave[w];
+        } else {
+            before = prev;
+        }
+        uint32_t carry = s_carry;
+        uint32_t ex = carry > before ? carry : before;
+        __syncthreads();
+        if (i < n) sums[i] = ex;
+        if (threadIdx.x == SCAN_BLOCK - 1) s_carry = carry > inc ? carry : inc;
+        __syncthreads();
     }
 }
 
@@ -344,7 +360,7 @@ void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStr
         sums = scan_scratch<uint32_t>().sums.ensure(size_t(nb));
         k_max_tile<<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, n, sums);
         SC_HIP(hipGetLastError());
-        k_max_sums<<<1, 64, 0, s>>>(sums, nb);
+        k_max_sums<<<1, SCAN_BLOCK, 0, s>>>(sums, nb);
         SC_HIP(hipGetLastError());
     }
     k_max_apply<<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, out, n, sums);

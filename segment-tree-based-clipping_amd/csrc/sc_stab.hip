@@ -23,7 +23,7 @@ namespace sc {
 namespace {
 
 constexpr int BLK = 256;
-constexpr int RCH = 256; // rows per work unit
+constexpr int RCH = 64; // rows per work unit (one per lane)
 
 inline unsigned grid_for(int64_t n, int per = BLK) {
     int64_t g = ceil_div(n, per);
@@ -77,7 +77,8 @@ __global__ void k_units(const uint32_t *__restrict__ cov_off, const uint32_t *__
 struct StabArgs {
     const uint32_t *cov_off, *cov_split, *cov_e, *lend_off, *lend_e, *uoff;
     const int *before, *etype, *eoff;
-    const double *x0, *y0, *x1, *y1;
+    const double *x0, *y0, *x1, *y1;          // endpoints (classification)
+    const double *bx0, *bx1, *by0, *by1;      // bounding boxes (LSMF)
     int nl, D, T;
     uint32_t nunits;
     int bits_b, bits_c;
@@ -86,11 +87,22 @@ struct StabArgs {
     unsigned long long *counter;
 };
 
+// value of lane 'src' broadcast to the wave (src is uniform)
+__device__ inline double bcast(double v, int src) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane(int(b), src);
+    int hi = __builtin_amdgcn_readlane(int(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// One wave per unit: up to 64 rows (one per lane, broadcast in turn) x 64
+// columns (one per lane).  LSMF is the bounding-box overlap test on the
+// precomputed boxes (min/max of the endpoints, exactly as LSMF forms them);
+// pairs that pass are classified with lineSegmentIntersection2.
 __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t wg = blockIdx.x * (BLK / WAVE) + threadIdx.x / WAVE;
     if (wg >= a.nunits) return;
-    // node owning unit wg: last v with uoff[v] <= wg
     int lo = 0, hi = a.T - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -136,6 +148,7 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
         ncol = min(uint32_t(WAVE), s.ebn - ci * WAVE);
         colsrc = a.lend_e;
     }
+    // this lane's column
     bool valid = uint32_t(lane) < ncol;
     uint32_t ce = 0;
     if (valid) {
@@ -144,47 +157,43 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
         if (kind != 1 && (raw & 0x80000000u)) {
             // "after" copy of an edge whose "before" leaf is under v as well
             int bl = a.before[ce];
-            int half = a.nl;
-            if (bl >= half + s.lo && bl < half + s.hi) valid = false;
+            if (bl >= a.nl + s.lo && bl < a.nl + s.hi) valid = false;
         }
     }
-    double cx0 = 0, cy0 = 0, cx1 = 0, cy1 = 0;
-    int ck = 0;
+    double cminx = 0, cmaxx = -1, cminy = 0, cmaxy = -1;
     if (valid) {
-        cx0 = a.x0[ce];
-        cy0 = a.y0[ce];
-        cx1 = a.x1[ce];
-        cy1 = a.y1[ce];
-        if (kind != 3) ck = a.etype[ce];
+        cminx = a.bx0[ce];
+        cmaxx = a.bx1[ce];
+        cminy = a.by0[ce];
+        cmaxy = a.by1[ce];
+    }
+    // this lane's row (broadcast to all lanes in the loop below)
+    uint32_t re_l = 0;
+    double rminx = 0, rmaxx = 0, rminy = 0, rmaxy = 0;
+    if (uint32_t(lane) < nrows) {
+        re_l = a.cov_e[rows_begin + lane];
+        rminx = a.bx0[re_l];
+        rmaxx = a.bx1[re_l];
+        rminy = a.by0[re_l];
+        rmaxy = a.by1[re_l];
     }
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
     const int sb = a.bits_b + a.bits_c;
     for (uint32_t r = 0; r < nrows; ++r) {
-        const uint32_t re = a.cov_e[rows_begin + r];
-        const double rx0 = a.x0[re], ry0 = a.y0[re], rx1 = a.x1[re], ry1 = a.y1[re];
+        const double qminx = bcast(rminx, int(r)), qmaxx = bcast(rmaxx, int(r));
+        const double qminy = bcast(rminy, int(r)), qmaxy = bcast(rmaxy, int(r));
+        bool box = valid && !(qminx > cmaxx || qmaxx < cminx) && !(qminy > cmaxy || qmaxy < cminy);
         bool hit = false;
         uint64_t key = 0;
-        if (valid) {
-            double bx0, by0, bx1, by1, qx0, qy0, qx1, qy1;
-            uint32_t be, qe;
-            int k;
-            if (kind == 3) {
-                bx0 = cx0; by0 = cy0; bx1 = cx1; by1 = cy1; be = ce;
-                qx0 = rx0; qy0 = ry0; qx1 = rx1; qy1 = ry1; qe = re;
-                k = a.etype[re];
-            } else {
-                bx0 = rx0; by0 = ry0; bx1 = rx1; by1 = ry1; be = re;
-                qx0 = cx0; qy0 = cy0; qx1 = cx1; qy1 = cy1; qe = ce;
-                k = ck;
-            }
-            if (lsmf(bx0, by0, bx1, by1, qx0, qy0, qx1, qy1)) {
-                double al, be_;
-                int t = classify(bx0, by0, bx1, by1, qx0, qy0, qx1, qy1, al, be_);
-                if (t != 0) {
-                    hit = true;
-                    uint32_t local = qe - uint32_t(a.eoff[k - 1]);
-                    key = (uint64_t(k) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
-                }
+        if (box) {
+            const uint32_t re = uint32_t(__builtin_amdgcn_readlane(int(re_l), int(r)));
+            uint32_t be = kind == 3 ? ce : re, qe = kind == 3 ? re : ce;
+            int t = classify_pre(a.x0[be], a.y0[be], a.x1[be], a.y1[be], a.x0[qe], a.y0[qe], a.x1[qe], a.y1[qe]);
+            if (t != 0) {
+                hit = true;
+                int k = a.etype[qe];
+                uint32_t local = qe - uint32_t(a.eoff[k - 1]);
+                key = (uint64_t(k) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
             }
         }
         uint64_t m = __ballot(hit);
@@ -208,14 +217,24 @@ __global__ void k_diff_flags64(const uint64_t *__restrict__ k, int64_t n, uint32
 }
 
 __global__ void k_compact_unique(const uint64_t *__restrict__ k, const uint32_t *__restrict__ pos, int64_t n,
-                                 uint64_t *out, int sh, uint32_t *kcount) {
+                                 uint64_t *out) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // pos is the exclusive scan of the head flags: a head is where pos[i+1] > pos[i]
-    if (pos[i + 1] != pos[i]) {
-        out[pos[i]] = k[i];
-        atomicAdd(&kcount[int(k[i] >> sh)], 1u);
+    if (pos[i + 1] != pos[i]) out[pos[i]] = k[i];
+}
+
+// first unique pair of every overlay k (pairs are sorted by overlay first)
+__global__ void k_overlay_offsets(const uint64_t *__restrict__ pairs, int64_t n, int K, int sh, uint32_t *off) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > K + 1) return;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (int64_t(pairs[mid] >> sh) < k) lo = mid + 1;
+        else hi = mid;
     }
+    off[k] = uint32_t(lo);
 }
 
 int bits_for(int64_t n) { // bits to hold values in [0, n)
@@ -257,6 +276,7 @@ void find_candidates(Ctx &c) {
         a.lend_off = c.lend_off; a.lend_e = c.lend_e; a.uoff = units;
         a.before = c.before_leaf; a.etype = c.etype; a.eoff = c.d_eoff;
         a.x0 = c.ex0; a.y0 = c.ey0; a.x1 = c.ex1; a.y1 = c.ey1;
+        a.bx0 = c.estart; a.bx1 = c.eend; a.by0 = c.eymin; a.by1 = c.eymax;
         a.nl = c.nleaves; a.D = c.depth; a.T = T; a.nunits = nunits;
         a.bits_b = c.bits_b; a.bits_c = c.bits_c;
         a.out = c.cand; a.cap = uint64_t(c.cand_cap); a.counter = counter;
@@ -285,18 +305,19 @@ void find_candidates(Ctx &c) {
     scan_exclusive_u32(flag, flag, n + 1, tot, s);
     c.n_pairs = read_u32(c, tot);
     c.pairs.ensure(size_t(c.n_pairs));
-    uint32_t *kcount = c.d_pair_off.ensure(size_t(c.K) + 2);
-    SC_HIP(hipMemsetAsync(kcount, 0, sizeof(uint32_t) * (size_t(c.K) + 2), s));
+    uint32_t *koff = c.d_pair_off.ensure(size_t(c.K) + 2);
     if (n > 0) {
-        k_compact_unique<<<grid_for(n), BLK, 0, s>>>(c.cand, flag, n, c.pairs, c.bits_b + c.bits_c, kcount);
+        k_compact_unique<<<grid_for(n), BLK, 0, s>>>(c.cand, flag, n, c.pairs);
         SC_HIP(hipGetLastError());
     }
-    // kcount[k] = pairs of overlay k (k >= 1); offsets per overlay
+    k_overlay_offsets<<<grid_for(int64_t(c.K) + 2), BLK, 0, s>>>(c.pairs, c.n_pairs, c.K, c.bits_b + c.bits_c, koff);
+    SC_HIP(hipGetLastError());
+    // koff[k] = first pair of overlay k (k = 1..K+1): pairs of overlay k are [koff[k], koff[k+1])
     std::vector<uint32_t> h(size_t(c.K) + 2);
-    SC_HIP(hipMemcpyAsync(h.data(), kcount, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipMemcpyAsync(h.data(), koff, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, s));
     SC_HIP(hipStreamSynchronize(s));
     c.pair_off.assign(size_t(c.K) + 1, 0);
-    for (int k = 1; k <= c.K; ++k) c.pair_off[k] = c.pair_off[k - 1] + h[k];
+    for (int k = 1; k <= c.K; ++k) c.pair_off[k] = h[k + 1];
 }
 
 } // namespace sc

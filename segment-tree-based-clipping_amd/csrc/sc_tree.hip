@@ -66,7 +66,8 @@ __global__ void k_close_base(double *bpoly, int nb) {
 
 __global__ void k_edges(const double *__restrict__ bpoly, const double *__restrict__ cpoly, int nb, int ne,
                         const int *__restrict__ eoff, const int *__restrict__ coff, int K, double *x0, double *y0,
-                        double *x1, double *y1, double *es, double *ee, int *ety, int *eid) {
+                        double *x1, double *y1, double *es, double *ee, double *eyl, double *eyh, int *ety,
+                        int *eid) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     const double *P;
@@ -93,6 +94,8 @@ __global__ void k_edges(const double *__restrict__ bpoly, const double *__restri
         ee[e] = a;
         es[e] = c;
     }
+    eyl[e] = b < d ? b : d;
+    eyh[e] = b < d ? d : b;
     ety[e] = ty;
     eid[e] = id;
 }
@@ -268,11 +271,19 @@ __device__ inline int upper_bound_d(const double *a, int n, double q) {
 template <bool FILL>
 __global__ void k_cover(const double *__restrict__ es, const double *__restrict__ ee, int ne, int nb,
                         const double *__restrict__ ea, int epc, int nleaves, uint32_t *cntb, uint32_t *cntc,
-                        const uint32_t *__restrict__ off, uint32_t *__restrict__ out) {
+                        const uint32_t *__restrict__ off, uint32_t *__restrict__ out, int2 *__restrict__ lr) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
-    int L = lower_bound_d(ea, epc, es[e]);
-    int R = upper_bound_d(ea, epc, ee[e]) - 1; // last point <= end
+    int L, R;
+    if (!FILL) {
+        L = lower_bound_d(ea, epc, es[e]);
+        R = upper_bound_d(ea, epc, ee[e]) - 1; // last point <= end
+        lr[e] = make_int2(L, R);
+    } else {
+        int2 v = lr[e];
+        L = v.x;
+        R = v.y;
+    }
     if (L >= R) return;
     int l = L + nleaves, r = R + nleaves;
     const bool base = e < nb;
@@ -405,8 +416,9 @@ void build_edges(Ctx &c) {
     int ne = c.ne;
     c.ex0.ensure(ne); c.ey0.ensure(ne); c.ex1.ensure(ne); c.ey1.ensure(ne);
     c.estart.ensure(ne); c.eend.ensure(ne); c.etype.ensure(ne); c.eid.ensure(ne);
+    c.eymin.ensure(ne); c.eymax.ensure(ne);
     k_edges<<<grid_for(ne), BLK, 0, c.stream>>>(c.bpoly, c.cpoly, c.nb, ne, c.d_eoff, c.d_coff, c.K, c.ex0, c.ey0,
-                                                c.ex1, c.ey1, c.estart, c.eend, c.etype, c.eid);
+                                                c.ex1, c.ey1, c.estart, c.eend, c.eymin, c.eymax, c.etype, c.eid);
     SC_HIP(hipGetLastError());
 }
 
@@ -491,8 +503,9 @@ void build_cover_lists(Ctx &c) {
     uint32_t *cntc = c.u32b.ensure(T);
     SC_HIP(hipMemsetAsync(cntb, 0, sizeof(uint32_t) * T, s));
     SC_HIP(hipMemsetAsync(cntc, 0, sizeof(uint32_t) * T, s));
+    int2 *lr = reinterpret_cast<int2 *>(c.u64b.ensure(c.ne));
     k_cover<false><<<grid_for(c.ne), BLK, 0, s>>>(c.estart, c.eend, c.ne, c.nb, c.ea, c.epc, c.nleaves, cntb, cntc,
-                                                  nullptr, nullptr);
+                                                  nullptr, nullptr, lr);
     SC_HIP(hipGetLastError());
     uint32_t *off = c.cov_off.ensure(size_t(T) + 1);
     k_add_u32<<<grid_for(T), BLK, 0, s>>>(cntb, cntc, off, T);
@@ -507,7 +520,7 @@ void build_cover_lists(Ctx &c) {
     SC_HIP(hipMemsetAsync(fb, 0, sizeof(uint32_t) * T, s));
     SC_HIP(hipMemsetAsync(cntc, 0, sizeof(uint32_t) * T, s));
     k_cover<true><<<grid_for(c.ne), BLK, 0, s>>>(c.estart, c.eend, c.ne, c.nb, c.ea, c.epc, c.nleaves, fb, cntc,
-                                                 off, c.cov_e);
+                                                 off, c.cov_e, lr);
     SC_HIP(hipGetLastError());
 }
 
