@@ -28,6 +28,8 @@
  *   segclip_label_traverse      copyToLinkedList (clip:606-695), labelIntersections2,
  *                               createResult2, cleanUpResult2 (foster/polyclip.h:337-787)
  *   segclip_clip                the per-overlay body of hybrid_clipping (clip:1026-1297)
+ *   segclip_clip_batch          the per-overlay loop of the one-to-many mode (clip:1702-2062)
+ *                               as one batch of launches
  *   segclip_stable_sort_by_key_* / segclip_exclusive_scan_*
  *                               thrust_func.h:4-12 (p_seg_tree/thrust_func.cu:19-42)
  */
@@ -63,6 +65,16 @@ int segclip_set_polygons(segclip_ctx *ctx, const double *base_xy, int nb, const 
                          const int *ov_sizes, int K);
 int segclip_set_polygons_device(segclip_ctx *ctx, const double *base_xy, int nb, const double *ov_xy,
                                 const int *ov_sizes, int K);
+/* P independent pairs: pair k clips base ring k (base_sizes[k] vertices, the
+ * rings back to back in base_xy) against overlay ring k.  One tree holds all
+ * pairs, their x keys moved apart so that no tree node mixes two pairs; every
+ * pair's results equal those of clipping it alone (bin/seg run once per pair,
+ * multicore_clipping clip:1415-2115).  Pair ids in segclip_copy_pairs are
+ * then local to the pair's two rings. */
+int segclip_set_pairs(segclip_ctx *ctx, const double *base_xy, const int *base_sizes, const double *ov_xy,
+                      const int *ov_sizes, int P);
+int segclip_set_pairs_device(segclip_ctx *ctx, const double *base_xy, const int *base_sizes,
+                             const double *ov_xy, const int *ov_sizes, int P);
 
 /* Tree stages.  segclip_build_tree runs elementary, node intervals, end lists
  * and cover lists in that order. */
@@ -80,6 +92,15 @@ int segclip_intersections(segclip_ctx *ctx, int k);
 int segclip_label_traverse(segclip_ctx *ctx);
 /* Whole path for overlay k: the tree and candidates are built on first use. */
 int segclip_clip(segclip_ctx *ctx, int k);
+/* Whole path for overlays (or pairs) k0..k1 as one batch: intersections,
+ * labelling, traversal and clean-up of all of them in the same launches.
+ * Results equal those of segclip_clip(k) for each k; the output rings of job
+ * k are rings [job_ring_off[k-k0], job_ring_off[k-k0+1]) of segclip_copy_rings. */
+int segclip_clip_batch(segclip_ctx *ctx, int k0, int k1);
+/* per job of the last batch (or the single job of segclip_clip): ring
+ * offsets (jobs+1 entries) and status bits (the 'undefined' bits per job) */
+int segclip_copy_batch_rings(segclip_ctx *ctx, int32_t *job_ring_off, int64_t cap, int64_t *need);
+int segclip_copy_batch_status(segclip_ctx *ctx, int32_t *status, int64_t cap, int64_t *need);
 /* Drop everything derived from the polygons (tree, candidates, results) so
  * that the next segclip_clip recomputes it; the resident polygons stay. */
 int segclip_reset(segclip_ctx *ctx);

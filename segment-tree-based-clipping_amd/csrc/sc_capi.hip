@@ -113,6 +113,13 @@ int segclip_set_polygons(segclip_ctx *h, const double *b, int nb, const double *
 int segclip_set_polygons_device(segclip_ctx *h, const double *b, int nb, const double *c, const int *csz, int K) {
     return guarded(h, [&](sc::Ctx &x) { sc::ctx_set_polygons(x, b, nb, c, csz, K, true); });
 }
+int segclip_set_pairs(segclip_ctx *h, const double *b, const int *bsz, const double *c, const int *csz, int P) {
+    return guarded(h, [&](sc::Ctx &x) { sc::ctx_set_pairs(x, b, bsz, c, csz, P, false); });
+}
+int segclip_set_pairs_device(segclip_ctx *h, const double *b, const int *bsz, const double *c, const int *csz,
+                             int P) {
+    return guarded(h, [&](sc::Ctx &x) { sc::ctx_set_pairs(x, b, bsz, c, csz, P, true); });
+}
 int segclip_build_elementary(segclip_ctx *h) {
     return guarded(h, [&](sc::Ctx &x) {
         sc::build_edges(x);
@@ -145,13 +152,37 @@ int segclip_label_traverse(segclip_ctx *h) {
 int segclip_clip(segclip_ctx *h, int k) {
     return guarded(h, [&](sc::Ctx &x) { sc::clip_all(x, k); });
 }
+int segclip_clip_batch(segclip_ctx *h, int k0, int k1) {
+    return guarded(h, [&](sc::Ctx &x) { sc::clip_jobs(x, k0, k1); });
+}
 int segclip_reset(segclip_ctx *h) {
     return guarded(h, [&](sc::Ctx &x) {
         x.epc = x.nleaves = x.T = 0;
         x.n_lend = x.n_cov = x.n_cand = x.n_pairs = x.n_mend = 0;
         x.cur_k = 0;
         x.pair_off.clear();
+        x.jobs.J = 0;
+        x.jobs.AB.clear();
+        x.jobs.ring_off.clear();
     });
+}
+int segclip_copy_batch_rings(segclip_ctx *h, int32_t *job_ring_off, int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    const sc::JobSet &js = h->c.jobs;
+    if (int(js.ring_off.size()) != js.J + 1 || js.J < 1) return SEGCLIP_ESTATE;
+    int rc = need_space(js.J + 1, cap, need);
+    if (rc || !job_ring_off) return rc;
+    std::memcpy(job_ring_off, js.ring_off.data(), sizeof(int32_t) * (js.J + 1));
+    return SEGCLIP_OK;
+}
+int segclip_copy_batch_status(segclip_ctx *h, int32_t *status, int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    const sc::JobSet &js = h->c.jobs;
+    if (int(js.flags.size()) != js.J || js.J < 1) return SEGCLIP_ESTATE;
+    int rc = need_space(js.J, cap, need);
+    if (rc || !status) return rc;
+    std::memcpy(status, js.flags.data(), sizeof(int32_t) * js.J);
+    return SEGCLIP_OK;
 }
 
 int segclip_get_counts(segclip_ctx *h, segclip_counts *o) {
@@ -242,17 +273,14 @@ int segclip_copy_pairs(segclip_ctx *h, int k, int32_t *b_id, int32_t *c_id, int3
         std::vector<uint64_t> keys(size_t(n > 0 ? n : 1));
         d2h(x, keys.data(), x.pairs.p + x.pair_off[k - 1], n);
         SC_HIP(hipStreamSynchronize(x.stream));
-        const int cStart = x.coff[k - 1];
-        std::vector<double> bxy(2 * size_t(x.nb + 1));
-        d2h(x, bxy.data(), x.bpoly.p, 2 * int64_t(x.nb + 1));
-        std::vector<double> cxy;
-        if (type) {
-            cxy.resize(2 * size_t(x.csz[k - 1] + 1));
-            d2h(x, cxy.data(), x.cpoly.p + 2 * int64_t(cStart), 2 * int64_t(x.csz[k - 1] + 1));
-        }
-        SC_HIP(hipStreamSynchronize(x.stream));
+        // one-to-many: the reference's ids (base ring first, overlay k at its
+        // vertex offset); paired: ids within the job's own two rings, as if
+        // the pair were clipped alone
+        const int br = sc::job_base(x, k);
+        const int cStart = x.paired ? 0 : x.coff[k - 1];
+        const int bFirst = x.beoff[br];
         for (int64_t i = 0; i < n; ++i) {
-            int b = int((keys[i] >> x.bits_c) & ((1ull << x.bits_b) - 1ull));
+            int b = int((keys[i] >> x.bits_c) & ((1ull << x.bits_b) - 1ull)) - bFirst;
             int cl = int(keys[i] & ((1ull << x.bits_c) - 1ull));
             b_id[i] = 2 * b;
             c_id[i] = 2 * (cStart + cl);

@@ -85,6 +85,18 @@ inline int ilog2_exact(int64_t n) {
     return l;
 }
 
+// Index of the segment holding i: the last j in [0,n) with off[j] <= i, for
+// non-decreasing offsets (empty segments are skipped).
+__device__ inline int seg_of(const int *off, int n, int64_t i) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------
 // primitives (sc_prims.hip)
 // ---------------------------------------------------------------------------

@@ -10,6 +10,17 @@ struct StageTimes {
     float elementary = 0, tree = 0, endlist = 0, cover = 0, candidates = 0, stab_kernel = 0, post = 0, label = 0;
 };
 
+// Jobs of one post-processing batch: job j clips base ring job_base(k0+j)
+// against overlay k0+j.  Host vectors and their device copies; per-edge count
+// arrays hold n+1 slots per job (JB/JC offsets), augmented rings are
+// concatenated job after job (AB/AC offsets, J+1 entries each).
+struct JobSet {
+    int k0 = 1, J = 0;
+    std::vector<int> p0, bvoff, beoff, nb, nc, cvoff, JB, JC, AB, AC, flags;
+    std::vector<int> ring_off; // J+1: rings of job j are [ring_off[j], ring_off[j+1]) of the output
+    DBuf<int> d_p0, d_bvoff, d_beoff, d_nb, d_nc, d_cvoff, d_JB, d_JC, d_AB, d_AC;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -17,14 +28,21 @@ struct Ctx {
     std::string last_error;
 
     // ---- input (reference layout of clip:172-311) ----
-    int nb = 0, K = 0, nct = 0, ne = 0, maxcsz = 0;
+    // Base rings: one shared base (one-to-many, the reference's mode) or one
+    // base per overlay ("paired": job k clips base k against overlay k).  nb
+    // counts the vertices (= edges) of all base rings together.
+    int nb = 0, K = 0, nct = 0, ne = 0, maxcsz = 0, maxbsz = 0, nbr = 1;
+    bool paired = false;
+    std::vector<int> bsz, bvoff, beoff; // base sizes; vertex offsets with closing vertices; edge offsets
     std::vector<int> csz, coff;     // overlay sizes; vertex offsets with closing vertices
     std::vector<int> eoff;          // overlay edge offsets in the edge index space (K+1), eoff[0] = nb
-    DBuf<double> bpoly, cpoly;      // (nb+1)*2, (nct+K)*2
-    DBuf<int> d_coff, d_eoff;
+    DBuf<double> bpoly, cpoly;      // (nb+nbr)*2, (nct+K)*2
+    DBuf<int> d_coff, d_eoff, d_bvoff, d_beoff;
+    DBuf<double> d_shift;           // per base/overlay ring x shift of the tree keys (paired mode)
     // per-edge SoA (edge index e in [0,ne): base edges, then overlay 1, 2, ...)
     DBuf<double> ex0, ey0, ex1, ey1; // endpoints in ring order
-    DBuf<double> estart, eend;       // x-extent
+    DBuf<double> estart, eend;       // x-extent (tree keys; shifted per pair in paired mode)
+    DBuf<double> uxs, uxe;           // paired mode: input x-extent for the bounding-box test
     DBuf<double> eymin, eymax;       // y-extent
     DBuf<int> etype, eid;
 
@@ -59,14 +77,15 @@ struct Ctx {
     std::vector<int64_t> pair_off;       // per overlay [K+1] offsets into pairs (host)
     DBuf<uint32_t> d_pair_off;
 
-    // ---- per-overlay post-processing (A11-A17) for the overlay last clipped ----
-    int cur_k = 0;
-    int I = 0;                           // unique pairs of this overlay
-    DBuf<int> p_b, p_c, p_type;          // bId, cId (reference ids), type
+    // ---- post-processing (A11-A17) of the last batch of jobs ----
+    JobSet jobs;
+    int cur_k = 0;                       // overlay of a one-job batch, else 0
+    int I = 0;                           // unique pairs of the batch
+    DBuf<int> p_j, p_b, p_c, p_type;     // job, base / overlay edge within its ring, type
     DBuf<double> p_x, p_y;               // intersection point
     DBuf<int> p_qa, p_qb;                // quantised alpha (B side), beta (C side)
     DBuf<int> ordB, ordC;                // per-edge alpha order (positions into the pair list)
-    DBuf<int> nd_b, nd_c;                // non-degenerate counts -> exclusive scans (nb+1, nc+1)
+    DBuf<int> nd_b, nd_c;                // non-degenerate counts -> exclusive scans (n+1 per job)
     DBuf<int> all_b, all_c;
     int nbaug = 0, ncaug = 0;
     DBuf<double> aug_bx, aug_by, aug_cx, aug_cy;
@@ -81,6 +100,7 @@ struct Ctx {
     std::vector<double> ring_xy;
     DBuf<int> d_ring_off;
     DBuf<double> d_ring_x, d_ring_y;
+    DBuf<int> d_ring_src;                // provenance of each output vertex (P slot g, Q slot ~g)
     int64_t n_out = 0;
     int undefined = 0;                   // bitmask like the oracle's status
 
@@ -107,6 +127,9 @@ struct Ctx {
 
 // stage entry points (throw sc::Error)
 void ctx_set_polygons(Ctx &c, const double *bxy, int nb, const double *cxy, const int *csz, int K, bool device_ptrs);
+void ctx_set_pairs(Ctx &c, const double *bxy, const int *bsz, const double *cxy, const int *csz, int P,
+                   bool device_ptrs);
+inline int job_base(const Ctx &c, int k) { return c.paired ? k - 1 : 0; } // base ring of overlay k (1-based)
 void build_edges(Ctx &c);
 void build_elementary(Ctx &c);
 void build_node_intervals(Ctx &c);
@@ -114,9 +137,11 @@ void build_leaf_end_entries(Ctx &c);
 void materialize_end_lists(Ctx &c);
 void build_cover_lists(Ctx &c);
 void find_candidates(Ctx &c);          // all overlays at once, then unique pairs
+void post_jobs(Ctx &c, int k0, int k1); // A11-A15 for overlays k0..k1 in one batch
 void post_overlay(Ctx &c, int k);      // A11-A15 for overlay k
-void label_and_traverse(Ctx &c);       // A16-A17 for the overlay of post_overlay
+void label_and_traverse(Ctx &c);       // A16-A17 for the jobs of the last post_jobs
 void clip_all(Ctx &c, int k);          // full pipeline for one overlay
+void clip_jobs(Ctx &c, int k0, int k1); // full pipeline for overlays k0..k1 as one batch
 
 uint32_t read_u32(Ctx &c, const uint32_t *d);
 uint64_t read_u64(Ctx &c, const uint64_t *d);

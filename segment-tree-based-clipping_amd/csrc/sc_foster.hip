@@ -1,26 +1,32 @@
 // sc_foster.hip -- Foster/Greiner-Hormann labelling, traversal and clean-up
-// (rows A16-A17) on the GPU.
+// (rows A16-A17) on the GPU, for a batch of jobs in the same launches.
 //
 // Reference: copyToLinkedList (src/clip.cpp:606-695), labelIntersections2,
 // createResult2, cleanUpResult2 (foster/polyclip.h:337-787), intersection
-// case (UNION = false).  The augmented rings built by segclip_intersections
-// are the linked lists of copyToLinkedList laid out in ring order: slot i of
-// the base ring P is followed by slot i+1 (mod n), likewise for the overlay
+// case (UNION = false).  The augmented rings built by post_jobs are the linked
+// lists of copyToLinkedList laid out in ring order: slot i of a job's base
+// ring P is followed by slot i+1 (mod its length), likewise for the overlay
 // ring Q; a slot is an intersection vertex when its alpha (P) or neighbour
-// (Q) is set.
+// (Q) is set.  Job j owns P slots [AB[j], AB[j+1]) and Q slots
+// [AC[j], AC[j+1]); neighbour links are 1-based within the job.
 //
-// Fast path (every label CROSSING or BOUNCING, a plain vertex on both rings,
-// crossings alternate, no interior bouncing pair): the entry/exit status of
-// a crossing vertex is the parity of the crossings met since the first plain
-// vertex (one prefix count); a traversal piece runs from a crossing vertex to
-// the next crossing vertex in the walking direction, then jumps to the
-// neighbour of that vertex on the other ring.  Pieces form cycles; a cycle's
-// output component is started at its base-ring vertex that comes first in
-// base-ring order (pointer jumping for the minimum), and the pieces of a
-// component are laid out by list ranking.  Everything else -- intersection
-// chains (LEFT_ON ... ON_RIGHT), split bouncing pairs, rings made only of
-// intersections, inconsistent labels -- runs the reference's sequential
-// algorithm in one device thread.
+// Every job takes one of three routes:
+//  - crossing (every label CROSSING or BOUNCING, a plain vertex on both
+//    rings, crossings alternate, no interior bouncing pair): the entry/exit
+//    status of a crossing vertex is the parity of the crossings met since the
+//    first plain vertex (one prefix count); a traversal piece runs from a
+//    crossing vertex to the next crossing vertex in the walking direction,
+//    then jumps to the neighbour of that vertex on the other ring.  Pieces
+//    form cycles; a cycle's output component is started at its base-ring
+//    vertex that comes first in base-ring order (pointer jumping for the
+//    minimum), and the pieces of a component are laid out by list ranking;
+//  - no crossing (same conditions, no crossing vertex): each ring is output
+//    when its first plain vertex lies inside the other ring;
+//  - sequential: everything else -- intersection chains (LEFT_ON ...
+//    ON_RIGHT), split bouncing pairs, rings made only of intersections,
+//    inconsistent labels -- runs the reference's sequential algorithm, one
+//    device thread per such job.
+// The outputs of all jobs are laid out job after job, then cleaned up.
 //
 // Clean-up: the reference removes, in one pass from the root, every vertex
 // nearly collinear (|A| < 1e-9) with the last kept vertex and its successor,
@@ -28,9 +34,13 @@
 // unique fixed point of keep[i] = |A(last kept before i, i, i+1)| >= eps,
 // iterated in parallel from the guess "predecessor kept"; the fixed point
 // equals the sequential result because it is determined left to right.
+//
+// Provenance: every output vertex carries the slot it came from (a P slot g
+// as g, a Q slot g as ~g, none as SRC_NONE) for the area gradient.
 #include "sc_ctx.hpp"
 #include "sc_fp64.hpp"
 
+#include <algorithm>
 #include <climits>
 
 namespace sc {
@@ -50,142 +60,224 @@ enum : int {
 };
 enum : int { EXIT = 0, ENTRY = 1, NEITHER = 2 };
 enum : int { UB_TRAVERSAL = 4 };
+constexpr int SRC_NONE = INT_MIN;
 
-// reasons for taking the sequential path
+// reasons for taking the sequential route
 enum : int { SLOW_LABEL = 1, SLOW_NOPLAIN = 2, SLOW_ODD = 4, SLOW_SPLIT = 8, SLOW_LINK = 16 };
+// routes
+enum : int { K_CROSS = 0, K_NOCROSS = 1, K_SEQ = 2 };
 
-// status words
-enum : int { ST_SLOW = 0, ST_PLAIN_P = 1, ST_PLAIN_Q = 2, ST_WIND_P = 3, ST_WIND_Q = 4, ST_CHANGED = 5, ST_N = 8 };
+constexpr uint32_t NIL = 0xffffffffu;
 
-struct Ring {
-    int n;
+// one side (all P rings or all Q rings) of a batch
+struct Side {
+    const int *off;  // J+1 slot offsets of this side's rings
+    const int *ooff; // J+1 slot offsets of the other side's rings
     const double *x, *y;
     const int *lab, *nbr, *av; // av null: intersection iff nbr set
 };
 
-__device__ inline bool inter_of(const Ring &r, int i) { return r.av ? r.av[i] != -100 : r.nbr[i] != -100; }
+__device__ inline bool inter_at(const Side &s, int64_t i) { return s.av ? s.av[i] != -100 : s.nbr[i] != -100; }
 
-__global__ __launch_bounds__(BLK) void k_classify(Ring r, int side, int n_other, uint32_t *cross, int *st) {
-    __shared__ int s_min;
-    if (threadIdx.x == 0) s_min = INT_MAX;
-    __syncthreads();
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < r.n) {
-        bool in = inter_of(r, i);
-        int lab = r.lab[i];
-        if (in) {
-            if (lab != L_CROSSING && lab != L_BOUNCING) atomicOr(&st[ST_SLOW], SLOW_LABEL);
-            int nb = r.nbr[i];
-            if (nb < 1 || nb > n_other) atomicOr(&st[ST_SLOW], SLOW_LINK);
-        } else {
-            atomicMin(&s_min, i);
-        }
-        cross[i] = (in && lab == L_CROSSING) ? 1u : 0u;
+// crossing flags; label and link checks
+__global__ void k_jclassify(Side s, int J, int64_t n, uint32_t *cross, int *slow) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int j = seg_of(s.off, J, i);
+    uint32_t c = 0;
+    if (inter_at(s, i)) {
+        int lab = s.lab[i];
+        if (lab != L_CROSSING && lab != L_BOUNCING) atomicOr(&slow[j], SLOW_LABEL);
+        int nb = s.nbr[i];
+        if (nb < 1 || nb > s.ooff[j + 1] - s.ooff[j]) atomicOr(&slow[j], SLOW_LINK);
+        c = lab == L_CROSSING ? 1u : 0u;
     }
-    __syncthreads();
-    // first plain vertex: one global atomic per block
-    if (threadIdx.x == 0 && s_min != INT_MAX) atomicMin(&st[side == 0 ? ST_PLAIN_P : ST_PLAIN_Q], s_min);
+    cross[i] = c;
 }
 
-// winding test of polygon::pointInPoly (polygon.h:435-457): the query point is
-// the vertex at index st[qslot] of ring q; the sum is accumulated in st[out]
-__global__ void k_winding(Ring r, Ring q, const int *st_idx, int *out) {
-    __shared__ int s_sum[BLK / WAVE];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int qi = *st_idx;
-    int w = 0;
-    if (i < r.n && qi >= 0 && qi < q.n) {
-        double rx = q.x[qi], ry = q.y[qi];
-        int j = i + 1 == r.n ? 0 : i + 1;
-        double p0x = r.x[i], p0y = r.y[i], p1x = r.x[j], p1y = r.y[j];
-        if ((p0y < ry) != (p1y < ry)) {
-            int s = 2 * (p1y > p0y) - 1;
-            if (p0x >= rx) {
-                if (p1x > rx) w = s;
-                else if ((areaA(p0x, p0y, p1x, p1y, rx, ry) > 0) == (p1y > p0y)) w = s;
-            } else if (p1x > rx) {
-                if ((areaA(p0x, p0y, p1x, p1y, rx, ry) > 0) == (p1y > p0y)) w = s;
+// first plain vertex of each job's ring (local index, -1 if none): one wave per job
+__global__ __launch_bounds__(BLK) void k_first_plain(Side s, int J, int *first) {
+    const int j = int((int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / WAVE);
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (j >= J) return; // whole wave
+    const int64_t a = s.off[j], b = s.off[j + 1];
+    int res = -1;
+    for (int64_t base = a; base < b; base += WAVE) {
+        int64_t i = base + lane;
+        bool plain = i < b && !inter_at(s, i);
+        uint64_t m = __ballot(plain);
+        if (m) {
+            res = int(base - a) + __ffsll((unsigned long long)m) - 1;
+            break;
+        }
+    }
+    if (lane == 0) first[j] = res;
+}
+
+// winding test of polygon::pointInPoly (polygon.h:435-457) of the other
+// ring's first plain vertex against this side's rings, summed per job
+__global__ __launch_bounds__(BLK) void k_jwinding(Side r, int J, int64_t n, const double *qx, const double *qy,
+                                                  const int *qfirst, int *wind) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int w = 0, j = -1;
+    if (i < n) {
+        j = seg_of(r.off, J, i);
+        int qf = qfirst[j];
+        if (qf >= 0) {
+            int64_t qg = r.ooff[j] + qf;
+            double rx = qx[qg], ry = qy[qg];
+            int64_t nx = i + 1 == r.off[j + 1] ? r.off[j] : i + 1;
+            double p0x = r.x[i], p0y = r.y[i], p1x = r.x[nx], p1y = r.y[nx];
+            if ((p0y < ry) != (p1y < ry)) {
+                int sg = 2 * (p1y > p0y) - 1;
+                if (p0x >= rx) {
+                    if (p1x > rx) w = sg;
+                    else if ((areaA(p0x, p0y, p1x, p1y, rx, ry) > 0) == (p1y > p0y)) w = sg;
+                } else if (p1x > rx) {
+                    if ((areaA(p0x, p0y, p1x, p1y, rx, ry) > 0) == (p1y > p0y)) w = sg;
+                }
             }
         }
     }
-    // wave then block reduction
-    for (int d = WAVE / 2; d > 0; d >>= 1) w += __shfl_down(w, d, WAVE);
-    if ((threadIdx.x & (WAVE - 1)) == 0) s_sum[threadIdx.x / WAVE] = w;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int k = 0; k < BLK / WAVE; ++k) t += s_sum[k];
-        if (t) atomicAdd(out, t);
+    // a wave inside one job adds once; a wave across jobs adds per lane
+    const int j0 = __shfl(j, 0, WAVE), j1 = __shfl(j, WAVE - 1, WAVE);
+    if (j0 == j1 && j0 >= 0) {
+        for (int d = WAVE / 2; d > 0; d >>= 1) w += __shfl_down(w, d, WAVE);
+        if ((threadIdx.x & (WAVE - 1)) == 0 && w) atomicAdd(&wind[j0], w);
+    } else if (w && j >= 0) {
+        atomicAdd(&wind[j], w);
     }
+}
+
+struct JobState {
+    int J;
+    const int *AB, *AC;
+    int *slow, *firstP, *firstQ, *windP, *windQ, *kind, *mP, *mQ, *cP0, *cQ0;
+};
+
+// route of each job; first crossing index of each job on both sides (J+1 threads)
+__global__ void k_jdecide(JobState s, const uint32_t *ccP, const uint32_t *ccQ) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > s.J) return;
+    s.cP0[j] = int(ccP[s.AB[j]]);
+    s.cQ0[j] = int(ccQ[s.AC[j]]);
+    if (j == s.J) return;
+    int mp = int(ccP[s.AB[j + 1]] - ccP[s.AB[j]]), mq = int(ccQ[s.AC[j + 1]] - ccQ[s.AC[j]]);
+    s.mP[j] = mp;
+    s.mQ[j] = mq;
+    int sl = s.slow[j];
+    if (s.firstP[j] < 0 || s.firstQ[j] < 0) sl |= SLOW_NOPLAIN;
+    if (mp != mq || (mp & 1)) sl |= SLOW_ODD;
+    s.slow[j] = sl;
+    s.kind[j] = sl ? K_SEQ : (mp == 0 ? K_NOCROSS : K_CROSS);
 }
 
 // entry/exit status of every crossing vertex from the parity of crossings
 // since the first plain vertex V; bouncing vertices met with status EXIT are
 // split candidates (labelIntersections2 step 4, polyclip.h:551-635)
-__global__ void k_enex(Ring r, const uint32_t *cross, const uint32_t *ccount, uint32_t total, int V, int st0,
-                       int *enex, uint8_t *split) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= r.n) return;
-    uint32_t c = ccount[i] - ccount[V] + (i < V ? total : 0u);
+__global__ void k_jenex(Side s, int J, int64_t n, const uint32_t *cross, const uint32_t *cc, const int *first,
+                        const int *wind, const int *kind, int *enex, uint8_t *split) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int j = seg_of(s.off, J, i);
+    if (kind[j] != K_CROSS) {
+        enex[i] = NEITHER;
+        split[i] = 0;
+        return;
+    }
+    int64_t V = s.off[j] + first[j];
+    uint32_t m = cc[s.off[j + 1]] - cc[s.off[j]];
+    uint32_t c = cc[i] - cc[V] + (i < V ? m : 0u);
+    int st0 = (wind[j] % 2) != 0 ? EXIT : ENTRY;
     int st = st0 ^ int(c & 1u);
     enex[i] = cross[i] ? st : NEITHER;
-    split[i] = (inter_of(r, i) && r.lab[i] == L_BOUNCING && st == EXIT) ? 1 : 0;
+    split[i] = (inter_at(s, i) && s.lab[i] == L_BOUNCING && st == EXIT) ? 1 : 0;
 }
 
-__global__ void k_split_pairs(int n, const uint8_t *splitP, const int *nbrP, const uint8_t *splitQ, int *st) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !splitP[i]) return;
-    int q = nbrP[i] - 1;
-    if (splitQ[q]) atomicOr(&st[ST_SLOW], SLOW_SPLIT);
+__global__ void k_jsplit_pairs(Side P, int J, int64_t nP, const uint8_t *splitP, const uint8_t *splitQ, int *slow) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nP || !splitP[i]) return;
+    int j = seg_of(P.off, J, i);
+    int64_t q = P.ooff[j] + P.nbr[i] - 1;
+    if (splitQ[q]) atomicOr(&slow[j], SLOW_SPLIT);
 }
 
-__global__ void k_compact_cross(int n, const uint32_t *cross, const uint32_t *ccount, int *cpos) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && cross[i]) cpos[ccount[i]] = i;
+__global__ void k_compact_cross(int64_t n, const uint32_t *cross, const uint32_t *ccount, int *cpos) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && cross[i]) cpos[ccount[i]] = int(i);
 }
 
 struct PieceArgs {
-    Ring P, Q;
+    int J;
+    Side P, Q;
     const int *cposP, *cposQ;
     const uint32_t *crossP, *crossQ, *ccountP, *ccountQ;
     const int *enexP, *enexQ;
-    int mP, mQ;
-    int *ppos, *plen, *pdir;
+    const int *cP0, *cQ0, *kind;
+    int mP, mQ; // crossing totals of the batch
+    int *ppos, *plen, *pdir, *pj;
     uint32_t *succ, *val;
-    int *st;
+    int *slow;
 };
 
-__global__ void k_pieces(PieceArgs a) {
+// pieces: crossing x (P crossings first, then Q crossings) to the next
+// crossing in its walking direction; ppos is local to the job's ring
+__global__ void k_jpieces(PieceArgs a) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int M = a.mP + a.mQ;
-    if (x >= M) return;
-    bool onP = x < a.mP;
-    const Ring &R = onP ? a.P : a.Q;
+    if (x >= a.mP + a.mQ) return;
+    const bool onP = x < a.mP;
+    const Side &R = onP ? a.P : a.Q;
     const int *cpos = onP ? a.cposP : a.cposQ;
+    const int *c0 = onP ? a.cP0 : a.cQ0;
     const int *enex = onP ? a.enexP : a.enexQ;
-    int m = onP ? a.mP : a.mQ;
-    int j = onP ? x : x - a.mP;
-    int pos = cpos[j];
-    int e = enex[pos];
-    int d = e == ENTRY ? 1 : -1;
-    int jw = j + d;
+    const int xl = onP ? x : x - a.mP;
+    const int j = seg_of(c0, a.J, xl);
+    a.pj[x] = j;
+    if (a.kind[j] != K_CROSS) {
+        a.ppos[x] = 0;
+        a.plen[x] = 0;
+        a.pdir[x] = 1;
+        a.succ[x] = uint32_t(x);
+        a.val[x] = NIL;
+        return;
+    }
+    const int m = c0[j + 1] - c0[j];
+    const int jl = xl - c0[j];
+    const int pos = cpos[xl];
+    const int e = enex[pos];
+    const int d = e == ENTRY ? 1 : -1;
+    int jw = jl + d;
     if (jw < 0) jw += m;
     if (jw >= m) jw -= m;
-    int posW = cpos[jw];
-    if (enex[posW] != (e ^ 1)) atomicOr(&a.st[ST_SLOW], SLOW_ODD);
+    const int posW = cpos[c0[j] + jw];
+    if (enex[posW] != (e ^ 1)) atomicOr(&a.slow[j], SLOW_ODD);
+    const int r0 = R.off[j], n = R.off[j + 1] - r0;
     int len = d > 0 ? posW - pos : pos - posW;
-    if (len <= 0) len += R.n;
-    int w = R.nbr[posW] - 1; // slot on the other ring
+    if (len <= 0) len += n;
+    const int64_t w = int64_t(R.ooff[j]) + R.nbr[posW] - 1; // slot on the other ring
     const uint32_t *ocross = onP ? a.crossQ : a.crossP;
     const uint32_t *occ = onP ? a.ccountQ : a.ccountP;
-    uint32_t s = 0;
-    if (!ocross[w]) atomicOr(&a.st[ST_SLOW], SLOW_LINK);
+    uint32_t s = uint32_t(x);
+    if (!ocross[w]) atomicOr(&a.slow[j], SLOW_LINK);
     else s = (onP ? uint32_t(a.mP) : 0u) + occ[w];
-    a.ppos[x] = pos;
+    a.ppos[x] = pos - r0;
     a.plen[x] = len;
     a.pdir[x] = d;
     a.succ[x] = s;
-    a.val[x] = onP ? min(2u * uint32_t(pos), 2u * uint32_t(posW) + 1u) : 0xffffffffu;
+    a.val[x] = onP ? min(2u * uint32_t(pos - r0), 2u * uint32_t(posW - r0) + 1u) : NIL;
+}
+
+// jobs found inconsistent while building pieces take the sequential route
+__global__ void k_jredecide(int J, const int *slow, int *kind) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < J && kind[j] == K_CROSS && slow[j]) kind[j] = K_SEQ;
+}
+
+__global__ void k_jmask(int M, const int *pj, const int *kind, uint32_t *succ, uint32_t *val) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= M || kind[pj[x]] == K_CROSS) return;
+    succ[x] = uint32_t(x);
+    val[x] = NIL;
 }
 
 __global__ void k_jump_min(int M, const uint32_t *val, const uint32_t *jmp, uint32_t *val2, uint32_t *jmp2) {
@@ -197,15 +289,14 @@ __global__ void k_jump_min(int M, const uint32_t *val, const uint32_t *jmp, uint
     jmp2[x] = jmp[j];
 }
 
-constexpr uint32_t NIL = 0xffffffffu;
-
-__global__ void k_rank_init(int M, const uint32_t *cmin, const uint32_t *succ, const uint32_t *ccountP,
-                            const int *plen, uint32_t *inode, uint32_t *nxt, uint32_t *dist) {
+// list ranking of the traversed cycles, broken at their start (a P crossing)
+__global__ void k_rank_init(int M, const uint32_t *cmin, const uint32_t *succ, const int *pj, const int *AB,
+                            const uint32_t *ccountP, const int *plen, uint32_t *inode, uint32_t *nxt, uint32_t *dist) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M) return;
     uint32_t m = cmin[x];
-    bool trav = m != 0xffffffffu && (m & 1u) == 0u;
-    uint32_t I = trav ? ccountP[m >> 1] : NIL;
+    bool trav = m != NIL && (m & 1u) == 0u;
+    uint32_t I = trav ? ccountP[int64_t(AB[pj[x]]) + (m >> 1)] : NIL;
     inode[x] = I;
     nxt[x] = trav ? (succ[x] == I ? NIL : succ[x]) : NIL;
     dist[x] = trav ? uint32_t(plen[x]) : 0u;
@@ -225,58 +316,121 @@ __global__ void k_rank_step(int M, const uint32_t *dist, const uint32_t *nxt, ui
 }
 
 // component totals at their starts (base-ring order)
-__global__ void k_start_totals(int mP, const int *cposP, const uint32_t *cmin, const uint32_t *dist,
-                               uint32_t *tot, uint32_t *isstart) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= mP) return;
-    uint32_t m = cmin[j];
-    bool s = m != 0xffffffffu && (m & 1u) == 0u && (m >> 1) == uint32_t(cposP[j]);
-    tot[j] = s ? dist[j] : 0u;
-    isstart[j] = s ? 1u : 0u;
+__global__ void k_start_totals(int mP, const int *cposP, const int *pj, const int *AB, const uint32_t *cmin,
+                               const uint32_t *dist, uint32_t *tot, uint32_t *isstart) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= mP) return;
+    uint32_t m = cmin[x];
+    bool s = m != NIL && (m & 1u) == 0u && (m >> 1) == uint32_t(cposP[x] - AB[pj[x]]);
+    tot[x] = s ? dist[x] : 0u;
+    isstart[x] = s ? 1u : 0u;
 }
 
-__global__ void k_ring_offsets(int mP, const uint32_t *isstart, const uint32_t *ringidx, const uint32_t *base,
-                               int *ring_off, uint32_t out_base) {
+struct CountArgs {
+    int J;
+    const int *AB, *AC, *kind, *windP, *windQ, *cP0, *sidx, *seq_res;
+    const uint32_t *base, *ridx;
+    uint32_t *jv, *jr;
+};
+
+// output vertices and rings of every job before clean-up
+__global__ void k_jcounts(CountArgs a) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= mP) return;
-    if (isstart[j]) ring_off[ringidx[j]] = int(out_base + base[j]);
+    if (j >= a.J) return;
+    uint32_t v = 0, r = 0;
+    const int k = a.kind[j];
+    if (k == K_CROSS) {
+        v = a.base[a.cP0[j + 1]] - a.base[a.cP0[j]];
+        r = a.ridx[a.cP0[j + 1]] - a.ridx[a.cP0[j]];
+    } else if (k == K_NOCROSS) {
+        bool ip = (a.windP[j] % 2) != 0, iq = (a.windQ[j] % 2) != 0;
+        v = (ip ? uint32_t(a.AB[j + 1] - a.AB[j]) : 0u) + (iq ? uint32_t(a.AC[j + 1] - a.AC[j]) : 0u);
+        r = uint32_t(ip) + uint32_t(iq);
+    } else {
+        int s = a.sidx[j];
+        r = uint32_t(a.seq_res[3 * s]);
+        v = uint32_t(a.seq_res[3 * s + 1]);
+    }
+    a.jv[j] = v;
+    a.jr[j] = r;
 }
 
-__global__ void k_heads(int M, const uint32_t *inode, const uint32_t *dist, const uint32_t *base, uint32_t out_base,
-                        uint32_t *head, uint32_t *piece_at, uint32_t *ostart) {
+__global__ void k_jring_offsets(int mP, const uint32_t *isstart, const uint32_t *ridx, const uint32_t *base,
+                                const int *pj, const int *cP0, const uint32_t *JV, const uint32_t *JR, int *off) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= mP || !isstart[x]) return;
+    int j = pj[x];
+    off[JR[j] + ridx[x] - ridx[cP0[j]]] = int(JV[j] + base[x] - base[cP0[j]]);
+}
+
+__global__ void k_jheads(int M, const uint32_t *inode, const uint32_t *dist, const uint32_t *base, const int *pj,
+                         const int *cP0, const uint32_t *JV, uint32_t *head, uint32_t *piece_at) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M) return;
     uint32_t I = inode[x];
     if (I == NIL) return;
-    uint32_t total = dist[I];
-    uint32_t o = out_base + base[I] + (total - dist[x]);
+    int j = pj[x];
+    uint32_t o = JV[j] + (base[I] - base[cP0[j]]) + (dist[I] - dist[x]);
     head[o] = o;
     piece_at[o] = uint32_t(x);
-    ostart[x] = o;
 }
 
-__global__ void k_emit(uint32_t out_base, uint32_t n_out, const uint32_t *hs, const uint32_t *piece_at, int mP,
-                       const int *ppos, const int *pdir, Ring P, Ring Q, double *ox, double *oy) {
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x + out_base;
-    if (i >= int64_t(out_base) + n_out) return;
-    uint32_t s = hs[i];
-    uint32_t x = piece_at[s];
+struct EmitArgs {
+    int J, mP;
+    const uint32_t *JV;
+    const int *kind, *ppos, *pdir, *pj;
+    const uint32_t *head, *piece_at;
+    Side P, Q;
+    double *ox, *oy;
+    int *osrc;
+};
+
+__global__ void k_jemit(EmitArgs a, int64_t n_out) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    const int j = seg_of(reinterpret_cast<const int *>(a.JV), a.J, i);
+    if (a.kind[j] != K_CROSS) return;
+    uint32_t s = a.head[i];
+    uint32_t x = a.piece_at[s];
     int k = int(i - s) + 1;
-    const Ring &R = int(x) < mP ? P : Q;
-    int pos = ppos[x] + pdir[x] * k;
-    pos %= R.n;
-    if (pos < 0) pos += R.n;
-    ox[i] = R.x[pos];
-    oy[i] = R.y[pos];
+    const bool onP = int(x) < a.mP;
+    const Side &R = onP ? a.P : a.Q;
+    int n = R.off[j + 1] - R.off[j];
+    int pos = (a.ppos[x] + a.pdir[x] * k) % n;
+    if (pos < 0) pos += n;
+    int g = R.off[j] + pos;
+    a.ox[i] = R.x[g];
+    a.oy[i] = R.y[g];
+    a.osrc[i] = onP ? g : ~g;
 }
 
-__global__ void k_copy_ring(Ring r, double *ox, double *oy, uint32_t at) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= r.n) return;
-    ox[at + i] = r.x[i];
-    oy[at + i] = r.y[i];
+// rings of no-crossing jobs: P's ring when its plain vertex lies inside Q, then Q's
+__global__ void k_jcopy_nocross(Side s, bool sideP, int J, int64_t n, const int *kind, const int *windP,
+                                const int *windQ, const int *AB, const uint32_t *JV, double *ox, double *oy,
+                                int *osrc) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int j = seg_of(s.off, J, i);
+    if (kind[j] != K_NOCROSS) return;
+    bool ip = (windP[j] % 2) != 0, iq = (windQ[j] % 2) != 0;
+    if (sideP ? !ip : !iq) return;
+    int64_t dst = JV[j] + (sideP || !ip ? 0 : (AB[j + 1] - AB[j])) + (i - s.off[j]);
+    ox[dst] = s.x[i];
+    oy[dst] = s.y[i];
+    osrc[dst] = sideP ? int(i) : ~int(i);
 }
 
+__global__ void k_jnocross_off(int J, const int *kind, const int *windP, const int *windQ, const int *AB,
+                               const uint32_t *JV, const uint32_t *JR, int *off) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= J || kind[j] != K_NOCROSS) return;
+    bool ip = (windP[j] % 2) != 0, iq = (windQ[j] % 2) != 0;
+    uint32_t r = JR[j];
+    if (ip) off[r++] = int(JV[j]);
+    if (iq) off[r] = int(JV[j] + (ip ? uint32_t(AB[j + 1] - AB[j]) : 0u));
+}
+
+// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // clean-up (cleanUpResult2)
 // ---------------------------------------------------------------------------
@@ -371,11 +525,12 @@ __global__ void k_ring_counts(int nr, const int *off, const uint32_t *kscan, uin
 }
 
 __global__ void k_compact_out(int64_t n, const uint32_t *keep, const uint32_t *kscan, const double *x,
-                              const double *y, double *x2, double *y2) {
+                              const double *y, const int *src, double *x2, double *y2, int *src2) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n || !keep[i]) return;
     x2[kscan[i]] = x[i];
     y2[kscan[i]] = y[i];
+    src2[kscan[i]] = src[i];
 }
 
 __global__ void k_ring_off_final(int nr, const uint32_t *cnt, const uint32_t *rscan, const uint32_t *nonempty,
@@ -397,17 +552,20 @@ struct SeqArgs {
     uint8_t *inter;
     int *cand; // scratch list, cap entries
     double *ox, *oy;
+    int *vsrc, *osrc; // provenance of each vertex / output vertex (slot reference)
+    int P0, Q0;       // first slot of the job's rings in the batch arrays
     int out_cap;
     int *ring_off;
     int ring_cap;
     int *result; // [0] rings, [1] vertices, [2] status bits
 };
 
-__device__ int seq_new(SeqArgs &a, int &n, double x, double y) {
+__device__ int seq_new(SeqArgs &a, int &n, double x, double y, int src) {
     if (n >= a.cap) return -1;
     int v = n++;
     a.vx[v] = x;
     a.vy[v] = y;
+    a.vsrc[v] = src;
     a.nx[v] = a.pv[v] = v;
     a.nb[v] = -1;
     a.lab[v] = L_NONE;
@@ -444,17 +602,16 @@ __device__ bool seq_point_in(SeqArgs &a, int root, double rx, double ry) {
 
 __device__ int seq_toggle(int s) { return s == ENTRY ? EXIT : (s == EXIT ? ENTRY : s); }
 
-__global__ void k_sequential(SeqArgs a) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__device__ void seq_run(SeqArgs &a) {
     int n = 0, status = 0;
     // rings in slot order; P = [0,nP), Q = [nP,nP+nQ)
     for (int i = 0; i < a.nP; ++i) {
-        int v = seq_new(a, n, a.px[i], a.py[i]);
+        int v = seq_new(a, n, a.px[i], a.py[i], a.P0 + i);
         a.lab[v] = a.labP[i];
         a.inter[v] = a.avP[i] != -100;
     }
     for (int i = 0; i < a.nQ; ++i) {
-        int v = seq_new(a, n, a.qx[i], a.qy[i]);
+        int v = seq_new(a, n, a.qx[i], a.qy[i], ~(a.Q0 + i));
         a.lab[v] = a.labQ[i];
         a.inter[v] = a.nbrQ[i] != -100;
     }
@@ -547,6 +704,7 @@ __global__ void k_sequential(SeqArgs a) {
                     if (outn >= a.out_cap) { status |= UB_TRAVERSAL; goto done; }
                     a.ox[outn] = a.vx[W];
                     a.oy[outn] = a.vy[W];
+                    a.osrc[outn] = a.vsrc[W];
                     outn++;
                     W = a.nx[W];
                 } while (W != A);
@@ -567,6 +725,7 @@ __global__ void k_sequential(SeqArgs a) {
                     if (outn >= a.out_cap) { status |= UB_TRAVERSAL; goto done; }
                     a.ox[outn] = a.vx[W];
                     a.oy[outn] = a.vy[W];
+                    a.osrc[outn] = a.vsrc[W];
                     outn++;
                     W = a.nx[W];
                 } while (W != rootP);
@@ -590,7 +749,7 @@ __global__ void k_sequential(SeqArgs a) {
                 do {
                     int nxv = a.nx[W], nbv = a.nb[W];
                     if (nbv >= 0 && a.nb[nxv] != a.pv[nbv] && a.nb[nxv] != a.nx[nbv]) {
-                        int T = seq_new(a, n, 0.5 * (a.vx[W] + a.vx[nxv]), 0.5 * (a.vy[W] + a.vy[nxv]));
+                        int T = seq_new(a, n, 0.5 * (a.vx[W] + a.vx[nxv]), 0.5 * (a.vy[W] + a.vy[nxv]), SRC_NONE);
                         if (T < 0) { status |= UB_TRAVERSAL; goto done; }
                         seq_insert_after(a, T, W);
                         V = T;
@@ -645,8 +804,8 @@ __global__ void k_sequential(SeqArgs a) {
             for (int j = 0; j < nsplit[1]; ++j)
                 if (splitL[1][j] == IQ) found = true;
             if (!found) continue;
-            int VP = seq_new(a, n, a.vx[IP], a.vy[IP]);
-            int VQ = seq_new(a, n, a.vx[IQ], a.vy[IQ]);
+            int VP = seq_new(a, n, a.vx[IP], a.vy[IP], a.vsrc[IP]);
+            int VQ = seq_new(a, n, a.vx[IQ], a.vy[IQ], a.vsrc[IQ]);
             if (VP < 0 || VQ < 0) { status |= UB_TRAVERSAL; goto done; }
             double sP = areaA(a.vx[a.pv[IP]], a.vy[a.pv[IP]], a.vx[IP], a.vy[IP], a.vx[a.nx[IP]], a.vy[a.nx[IP]]);
             double sQ = areaA(a.vx[a.pv[IQ]], a.vy[a.pv[IQ]], a.vx[IQ], a.vy[IQ], a.vx[a.nx[IQ]], a.vy[a.nx[IQ]]);
@@ -693,6 +852,7 @@ __global__ void k_sequential(SeqArgs a) {
                         if (outn >= a.out_cap || ++steps > bound) { status |= UB_TRAVERSAL; goto done; }
                         a.ox[outn] = a.vx[V];
                         a.oy[outn] = a.vy[V];
+                        a.osrc[outn] = a.vsrc[V];
                         outn++;
                     } while (!(a.enex[V] == st) && V != I);
                     if (V != I) {
@@ -718,21 +878,82 @@ done:
     a.ring_off[0] = 0;
 }
 
+
+// sequential route for several jobs: one block per job, its thread 0 runs seq_run
+struct SeqJob {
+    int j;              // job
+    int nP, nQ, P0, Q0; // ring sizes and first slots
+    int cap, out_cap, ring_cap;
+    int64_t vbase, obase, rbase; // scratch bases: vertices, output vertices, ring offsets
+};
+
+struct SeqBatch {
+    const SeqJob *jobs;
+    const double *px, *py, *qx, *qy;
+    const int *avP, *labP, *nbrP, *labQ, *nbrQ;
+    double *vx, *vy;
+    int *nx, *pv, *nb, *lab, *enex, *vsrc, *cand;
+    uint8_t *inter;
+    double *ox, *oy;
+    int *osrc, *ring_off, *result;
+};
+
+__global__ void k_sequential(SeqBatch b) {
+    if (threadIdx.x != 0) return;
+    const SeqJob jb = b.jobs[blockIdx.x];
+    SeqArgs a;
+    a.nP = jb.nP;
+    a.nQ = jb.nQ;
+    a.cap = jb.cap;
+    a.P0 = jb.P0;
+    a.Q0 = jb.Q0;
+    a.px = b.px + jb.P0; a.py = b.py + jb.P0; a.qx = b.qx + jb.Q0; a.qy = b.qy + jb.Q0;
+    a.avP = b.avP + jb.P0; a.labP = b.labP + jb.P0; a.nbrP = b.nbrP + jb.P0;
+    a.labQ = b.labQ + jb.Q0; a.nbrQ = b.nbrQ + jb.Q0;
+    a.vx = b.vx + jb.vbase; a.vy = b.vy + jb.vbase;
+    a.nx = b.nx + jb.vbase; a.pv = b.pv + jb.vbase; a.nb = b.nb + jb.vbase; a.lab = b.lab + jb.vbase;
+    a.enex = b.enex + jb.vbase; a.vsrc = b.vsrc + jb.vbase; a.cand = b.cand + jb.vbase;
+    a.inter = b.inter + jb.vbase;
+    a.ox = b.ox + jb.obase; a.oy = b.oy + jb.obase; a.osrc = b.osrc + jb.obase; a.out_cap = jb.out_cap;
+    a.ring_off = b.ring_off + jb.rbase; a.ring_cap = jb.ring_cap;
+    a.result = b.result + 3 * int64_t(blockIdx.x);
+    seq_run(a);
+}
+
+// rings of the sequential jobs into the batch output (one block per job)
+__global__ void k_jseq_copy(const SeqJob *jobs, const int *result, const uint32_t *JV, const uint32_t *JR,
+                            const double *sx, const double *sy, const int *ssrc, const int *sroff, double *ox,
+                            double *oy, int *osrc, int *off) {
+    const SeqJob jb = jobs[blockIdx.x];
+    const int rings = result[3 * blockIdx.x], verts = result[3 * blockIdx.x + 1];
+    const int64_t o = JV[jb.j];
+    for (int i = threadIdx.x; i < verts; i += blockDim.x) {
+        ox[o + i] = sx[jb.obase + i];
+        oy[o + i] = sy[jb.obase + i];
+        osrc[o + i] = ssrc[jb.obase + i];
+    }
+    for (int r = threadIdx.x; r < rings; r += blockDim.x) off[JR[jb.j] + r] = int(o + sroff[jb.rbase + r]);
+}
+
 // scratch owned by this stage
 struct FosterScratch {
     DBuf<uint32_t> crossP, crossQ, ccP, ccQ;
     DBuf<int> enexP, enexQ, cposP, cposQ;
     DBuf<uint8_t> splitP, splitQ;
-    DBuf<int> ppos, plen, pdir;
-    DBuf<uint32_t> succ, val, val2, jmp2, inode, nxt, nxt2, dist, dist2, tot, isstart, ringidx, head, piece_at,
-        ostart;
+    DBuf<int> ppos, plen, pdir, pj;
+    DBuf<uint32_t> succ, val, val2, jmp, jmp2, inode, dist, dist2, tot, isstart, ringidx, head, piece_at;
+    DBuf<int> jst;
+    DBuf<uint32_t> jv, jr;
     DBuf<int> st;
-    DBuf<double> ox, oy, ox2, oy2;
-    DBuf<int> off, off2, rstart, ringid;
-    DBuf<uint32_t> keep, mark, kscan, rcnt, rscan, nonempty;
-    // sequential path
-    DBuf<double> vx, vy;
-    DBuf<int> nx, pv, nb, lab, enex, cand, result;
+    DBuf<double> ox, oy;
+    DBuf<int> osrc;
+    DBuf<int> off, rstart, ringid;
+    DBuf<uint32_t> keep, mark, kscan, rcnt, nonempty;
+    // sequential route
+    DBuf<SeqJob> sjobs;
+    DBuf<int> sres, sroff, sosrc;
+    DBuf<double> vx, vy, sox, soy;
+    DBuf<int> nx, pv, nb, lab, enex, vsrc, cand;
     DBuf<uint8_t> inter;
 };
 
@@ -741,15 +962,17 @@ FosterScratch &fscratch() {
     return *p;
 }
 
-void read_ints(Ctx &c, int *h, const int *d, int n) {
-    SC_HIP(hipMemcpyAsync(h, d, sizeof(int) * size_t(n), hipMemcpyDeviceToHost, c.stream));
+template <class T> void d2h_sync(Ctx &c, T *h, const T *d, int64_t n) {
+    SC_HIP(hipMemcpyAsync(h, d, sizeof(T) * size_t(n), hipMemcpyDeviceToHost, c.stream));
     SC_HIP(hipStreamSynchronize(c.stream));
 }
 
-// clean-up of nr rings laid out at off[0..nr] in (ox, oy); result into Ctx
-void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n) {
+// clean-up of nr rings laid out at off[0..nr] in (ox, oy, osrc); result into
+// Ctx; ring_idx[r] = index of ring r among the non-empty rings (nr+1 entries)
+void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, std::vector<uint32_t> &ring_idx) {
     hipStream_t s = c.stream;
     if (nr == 0 || n == 0) {
+        ring_idx.assign(size_t(nr) + 1, 0u);
         c.nrings = 0;
         c.n_out = 0;
         c.d_ring_off.ensure(1);
@@ -766,16 +989,16 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n) {
     uint32_t *mark = f.mark.ensure(size_t(n));
     k_keep0<<<grid_for(n), BLK, 0, s>>>(n, ringid, f.off, rstart, f.ox, f.oy, keep);
     SC_HIP(hipGetLastError());
-    int *st = f.st.ensure(ST_N);
+    int *st = f.st.ensure(1);
     bool converged = false;
     for (int it = 0; it < 48; ++it) {
         k_keep_marks<<<grid_for(n), BLK, 0, s>>>(n, keep, mark);
         scan_inclusive_max_u32(mark, mark, n, s);
-        SC_HIP(hipMemsetAsync(st + ST_CHANGED, 0, sizeof(int), s));
-        k_keep_step<<<grid_for(n), BLK, 0, s>>>(n, ringid, f.off, rstart, f.ox, f.oy, mark, keep, st + ST_CHANGED);
+        SC_HIP(hipMemsetAsync(st, 0, sizeof(int), s));
+        k_keep_step<<<grid_for(n), BLK, 0, s>>>(n, ringid, f.off, rstart, f.ox, f.oy, mark, keep, st);
         SC_HIP(hipGetLastError());
         int ch = 0;
-        read_ints(c, &ch, st + ST_CHANGED, 1);
+        d2h_sync(c, &ch, st, 1);
         if (!ch) {
             converged = true;
             break;
@@ -793,25 +1016,22 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n) {
     uint32_t *rcnt = f.rcnt.ensure(nr + 1), *nonempty = f.nonempty.ensure(nr + 1);
     k_ring_counts<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, kscan, rcnt);
     SC_HIP(hipGetLastError());
-    // nonempty flags -> ring index
-    uint32_t *rflag = f.rscan.ensure(nr + 1);
     {
-        // rflag[r] = rcnt[r] > 0
         std::vector<uint32_t> h(static_cast<size_t>(nr));
-        SC_HIP(hipMemcpyAsync(h.data(), rcnt, sizeof(uint32_t) * nr, hipMemcpyDeviceToHost, s));
-        SC_HIP(hipStreamSynchronize(s));
-        std::vector<uint32_t> idx(size_t(nr) + 1, 0);
-        for (int r = 0; r < nr; ++r) idx[r + 1] = idx[r] + (h[r] ? 1u : 0u);
-        SC_HIP(hipMemcpyAsync(nonempty, idx.data(), sizeof(uint32_t) * (nr + 1), hipMemcpyHostToDevice, s));
-        c.nrings = int(idx[nr]);
-        (void)rflag;
+        d2h_sync(c, h.data(), rcnt, nr);
+        ring_idx.assign(size_t(nr) + 1, 0u);
+        for (int r = 0; r < nr; ++r) ring_idx[r + 1] = ring_idx[r] + (h[r] ? 1u : 0u);
+        SC_HIP(hipMemcpyAsync(nonempty, ring_idx.data(), sizeof(uint32_t) * (nr + 1), hipMemcpyHostToDevice, s));
+        c.nrings = int(ring_idx[nr]);
     }
     uint32_t nout = read_u32(c, tot);
     c.n_out = nout;
     c.d_ring_x.ensure(nout);
     c.d_ring_y.ensure(nout);
+    c.d_ring_src.ensure(nout);
     int *off2 = c.d_ring_off.ensure(size_t(c.nrings) + 1);
-    k_compact_out<<<grid_for(n), BLK, 0, s>>>(n, keep, kscan, f.ox, f.oy, c.d_ring_x, c.d_ring_y);
+    k_compact_out<<<grid_for(n), BLK, 0, s>>>(n, keep, kscan, f.ox, f.oy, f.osrc, c.d_ring_x, c.d_ring_y,
+                                              c.d_ring_src);
     k_ring_off_final<<<grid_for(nr), BLK, 0, s>>>(nr, rcnt, nullptr, nonempty, f.off, kscan, off2);
     SC_HIP(hipGetLastError());
     int last = int(nout);
@@ -819,47 +1039,41 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n) {
     SC_HIP(hipStreamSynchronize(s));
 }
 
-Ring ring_P(Ctx &c) { return Ring{c.nbaug, c.aug_bx, c.aug_by, c.lab_b, c.nbr_b, c.av_b}; }
-Ring ring_Q(Ctx &c) { return Ring{c.ncaug, c.aug_cx, c.aug_cy, c.lab_c, c.nbr_c, nullptr}; }
-
-void sequential_path(Ctx &c, FosterScratch &f) {
-    hipStream_t s = c.stream;
-    int nP = c.nbaug, nQ = c.ncaug;
-    int cap = 2 * (nP + nQ) + 64;
-    int out_cap = 2 * (nP + nQ) + 64;
-    int ring_cap = nP + nQ + 8;
-    SeqArgs a;
-    a.nP = nP; a.nQ = nQ; a.cap = cap;
-    a.px = c.aug_bx; a.py = c.aug_by; a.qx = c.aug_cx; a.qy = c.aug_cy;
-    a.avP = c.av_b; a.labP = c.lab_b; a.nbrP = c.nbr_b; a.labQ = c.lab_c; a.nbrQ = c.nbr_c;
-    a.vx = f.vx.ensure(cap); a.vy = f.vy.ensure(cap);
-    a.nx = f.nx.ensure(cap); a.pv = f.pv.ensure(cap); a.nb = f.nb.ensure(cap); a.lab = f.lab.ensure(cap);
-    a.enex = f.enex.ensure(cap); a.inter = f.inter.ensure(cap); a.cand = f.cand.ensure(cap);
-    a.ox = f.ox.ensure(out_cap); a.oy = f.oy.ensure(out_cap); a.out_cap = out_cap;
-    a.ring_off = f.off.ensure(ring_cap + 1); a.ring_cap = ring_cap;
-    a.result = f.result.ensure(4);
-    k_sequential<<<1, 1, 0, s>>>(a);
-    SC_HIP(hipGetLastError());
-    int res[3];
-    read_ints(c, res, a.result, 3);
-    c.undefined |= res[2];
-    cleanup_rings(c, f, res[0], res[1]);
-}
-
 } // namespace
 
 void label_and_traverse(Ctx &c) {
-    SC_REQUIRE(c.cur_k >= 1, SC_ESTATE, "segclip_intersections must run first");
+    JobSet &js = c.jobs;
+    SC_REQUIRE(js.J >= 1 && int(js.AB.size()) == js.J + 1, SC_ESTATE, "segclip_intersections must run first");
     hipStream_t s = c.stream;
     FosterScratch &f = fscratch();
-    Ring P = ring_P(c), Q = ring_Q(c);
-    const int nP = P.n, nQ = Q.n;
-    int *st = f.st.ensure(ST_N);
-    int init[ST_N] = {0, INT_MAX, INT_MAX, 0, 0, 0, 0, 0};
-    SC_HIP(hipMemcpyAsync(st, init, sizeof(init), hipMemcpyHostToDevice, s));
+    const int J = js.J;
+    const int64_t nP = js.AB[J], nQ = js.AC[J];
+    const int *AB = js.d_AB.p, *AC = js.d_AC.p;
+    const Side P{AB, AC, c.aug_bx, c.aug_by, c.lab_b, c.nbr_b, c.av_b};
+    const Side Q{AC, AB, c.aug_cx, c.aug_cy, c.lab_c, c.nbr_c, nullptr};
+    // per-job words
+    int *jw = f.jst.ensure(11 * int64_t(J) + 4);
+    SC_HIP(hipMemsetAsync(jw, 0, sizeof(int) * (11 * size_t(J) + 4), s));
+    JobState st;
+    st.J = J;
+    st.AB = AB;
+    st.AC = AC;
+    st.slow = jw;
+    st.firstP = jw + J;
+    st.firstQ = jw + 2 * J;
+    st.windP = jw + 3 * J;
+    st.windQ = jw + 4 * J;
+    st.kind = jw + 5 * J;
+    st.mP = jw + 6 * J;
+    st.mQ = jw + 7 * J;
+    st.cP0 = jw + 8 * J;          // J+1
+    st.cQ0 = jw + 9 * J + 1;      // J+1
+    int *sidx = jw + 10 * J + 2;  // J
     uint32_t *crossP = f.crossP.ensure(nP + 1), *crossQ = f.crossQ.ensure(nQ + 1);
-    k_classify<<<grid_for(nP), BLK, 0, s>>>(P, 0, nQ, crossP, st);
-    k_classify<<<grid_for(nQ), BLK, 0, s>>>(Q, 1, nP, crossQ, st);
+    k_jclassify<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, crossP, st.slow);
+    k_jclassify<<<grid_for(nQ), BLK, 0, s>>>(Q, J, nQ, crossQ, st.slow);
+    k_first_plain<<<grid_for(int64_t(J) * WAVE), BLK, 0, s>>>(P, J, st.firstP);
+    k_first_plain<<<grid_for(int64_t(J) * WAVE), BLK, 0, s>>>(Q, J, st.firstQ);
     SC_HIP(hipGetLastError());
     SC_HIP(hipMemsetAsync(crossP + nP, 0, sizeof(uint32_t), s));
     SC_HIP(hipMemsetAsync(crossQ + nQ, 0, sizeof(uint32_t), s));
@@ -868,123 +1082,225 @@ void label_and_traverse(Ctx &c) {
     scan_exclusive_u32(crossP, ccP, nP + 1, tot, s);
     scan_exclusive_u32(crossQ, ccQ, nQ + 1, tot + 1, s);
     // point-in-polygon of each ring's first plain vertex w.r.t. the other ring
-    k_winding<<<grid_for(nQ), BLK, 0, s>>>(Q, P, st + ST_PLAIN_P, st + ST_WIND_P);
-    k_winding<<<grid_for(nP), BLK, 0, s>>>(P, Q, st + ST_PLAIN_Q, st + ST_WIND_Q);
+    k_jwinding<<<grid_for(nQ), BLK, 0, s>>>(Q, J, nQ, c.aug_bx, c.aug_by, st.firstP, st.windP);
+    k_jwinding<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, c.aug_cx, c.aug_cy, st.firstQ, st.windQ);
+    k_jdecide<<<grid_for(J + 1), BLK, 0, s>>>(st, ccP, ccQ);
     SC_HIP(hipGetLastError());
-    int hst[ST_N];
     uint32_t htot[2];
-    SC_HIP(hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, s));
-    read_ints(c, hst, st, ST_N);
-    const uint32_t mP = htot[0], mQ = htot[1];
-    int slow = hst[ST_SLOW];
-    const bool plainP = hst[ST_PLAIN_P] != INT_MAX, plainQ = hst[ST_PLAIN_Q] != INT_MAX;
-    if (!plainP || !plainQ) slow |= SLOW_NOPLAIN;
-    if (mP != mQ || (mP & 1u)) slow |= SLOW_ODD;
-    if (slow) {
-        sequential_path(c, f);
-        return;
-    }
-    const bool insideP = (hst[ST_WIND_P] % 2) != 0; // P's plain vertex inside Q
-    const bool insideQ = (hst[ST_WIND_Q] % 2) != 0;
-    if (mP == 0) {
-        // no crossing vertex on either ring: each ring is output if it lies inside the other
-        int nr = 0;
-        int64_t n = 0;
-        int hoff[3] = {0, 0, 0};
-        f.ox.ensure(size_t(nP) + nQ);
-        f.oy.ensure(size_t(nP) + nQ);
-        if (insideP) {
-            k_copy_ring<<<grid_for(nP), BLK, 0, s>>>(P, f.ox, f.oy, uint32_t(n));
-            n += nP;
-            hoff[++nr] = int(n);
-        }
-        if (insideQ) {
-            k_copy_ring<<<grid_for(nQ), BLK, 0, s>>>(Q, f.ox, f.oy, uint32_t(n));
-            n += nQ;
-            hoff[++nr] = int(n);
-        }
-        SC_HIP(hipGetLastError());
-        int *off = f.off.ensure(3);
-        SC_HIP(hipMemcpyAsync(off, hoff, sizeof(int) * 3, hipMemcpyHostToDevice, s));
-        cleanup_rings(c, f, nr, n);
-        return;
-    }
-    // entry/exit status
+    d2h_sync(c, htot, tot, 2);
+    const int mP = int(htot[0]), mQ = int(htot[1]);
+    const int M = mP + mQ;
+    // entry/exit status, split pairs and pieces of the crossing jobs
     int *enexP = f.enexP.ensure(nP), *enexQ = f.enexQ.ensure(nQ);
     uint8_t *splitP = f.splitP.ensure(nP), *splitQ = f.splitQ.ensure(nQ);
-    k_enex<<<grid_for(nP), BLK, 0, s>>>(P, crossP, ccP, mP, hst[ST_PLAIN_P], insideP ? EXIT : ENTRY, enexP, splitP);
-    k_enex<<<grid_for(nQ), BLK, 0, s>>>(Q, crossQ, ccQ, mQ, hst[ST_PLAIN_Q], insideQ ? EXIT : ENTRY, enexQ, splitQ);
-    k_split_pairs<<<grid_for(nP), BLK, 0, s>>>(nP, splitP, c.nbr_b, splitQ, st);
-    // compact crossing vertices and build the pieces
-    int *cposP = f.cposP.ensure(mP), *cposQ = f.cposQ.ensure(mQ);
-    k_compact_cross<<<grid_for(nP), BLK, 0, s>>>(nP, crossP, ccP, cposP);
-    k_compact_cross<<<grid_for(nQ), BLK, 0, s>>>(nQ, crossQ, ccQ, cposQ);
-    const int M = int(mP + mQ);
     PieceArgs pa;
-    pa.P = P; pa.Q = Q; pa.cposP = cposP; pa.cposQ = cposQ; pa.crossP = crossP; pa.crossQ = crossQ;
-    pa.ccountP = ccP; pa.ccountQ = ccQ; pa.enexP = enexP; pa.enexQ = enexQ; pa.mP = int(mP); pa.mQ = int(mQ);
-    pa.ppos = f.ppos.ensure(M); pa.plen = f.plen.ensure(M); pa.pdir = f.pdir.ensure(M);
-    pa.succ = f.succ.ensure(M); pa.val = f.val.ensure(M); pa.st = st;
-    k_pieces<<<grid_for(M), BLK, 0, s>>>(pa);
-    SC_HIP(hipGetLastError());
-    read_ints(c, hst, st, 1);
-    if (hst[ST_SLOW]) {
-        sequential_path(c, f);
-        return;
+    if (M > 0) {
+        k_jenex<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, crossP, ccP, st.firstP, st.windP, st.kind, enexP, splitP);
+        k_jenex<<<grid_for(nQ), BLK, 0, s>>>(Q, J, nQ, crossQ, ccQ, st.firstQ, st.windQ, st.kind, enexQ, splitQ);
+        k_jsplit_pairs<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, splitP, splitQ, st.slow);
+        int *cposP = f.cposP.ensure(mP + 1), *cposQ = f.cposQ.ensure(mQ + 1);
+        k_compact_cross<<<grid_for(nP), BLK, 0, s>>>(nP, crossP, ccP, cposP);
+        k_compact_cross<<<grid_for(nQ), BLK, 0, s>>>(nQ, crossQ, ccQ, cposQ);
+        pa.J = J;
+        pa.P = P;
+        pa.Q = Q;
+        pa.cposP = cposP;
+        pa.cposQ = cposQ;
+        pa.crossP = crossP;
+        pa.crossQ = crossQ;
+        pa.ccountP = ccP;
+        pa.ccountQ = ccQ;
+        pa.enexP = enexP;
+        pa.enexQ = enexQ;
+        pa.cP0 = st.cP0;
+        pa.cQ0 = st.cQ0;
+        pa.kind = st.kind;
+        pa.mP = mP;
+        pa.mQ = mQ;
+        pa.ppos = f.ppos.ensure(M);
+        pa.plen = f.plen.ensure(M);
+        pa.pdir = f.pdir.ensure(M);
+        pa.pj = f.pj.ensure(M);
+        pa.succ = f.succ.ensure(M);
+        pa.val = f.val.ensure(M);
+        pa.slow = st.slow;
+        k_jpieces<<<grid_for(M), BLK, 0, s>>>(pa);
+        k_jredecide<<<grid_for(J), BLK, 0, s>>>(J, st.slow, st.kind);
+        k_jmask<<<grid_for(M), BLK, 0, s>>>(M, pa.pj, st.kind, pa.succ, pa.val);
+        SC_HIP(hipGetLastError());
     }
-    // cycle minima by pointer jumping
-    uint32_t *val = pa.val, *val2 = f.val2.ensure(M), *jmp = f.nxt.ensure(M), *jmp2 = f.jmp2.ensure(M);
-    SC_HIP(hipMemcpyAsync(jmp, pa.succ, sizeof(uint32_t) * M, hipMemcpyDeviceToDevice, s));
-    int rounds = ilog2_exact(M) + 1;
-    for (int r = 0; r < rounds; ++r) {
-        k_jump_min<<<grid_for(M), BLK, 0, s>>>(M, val, jmp, val2, jmp2);
-        std::swap(val, val2);
-        std::swap(jmp, jmp2);
+    // routes and crossing counts on the host
+    std::vector<int> hk(3 * size_t(J));
+    d2h_sync(c, hk.data(), st.kind, 3 * int64_t(J));
+    const int *hkind = hk.data(), *hmP = hk.data() + J, *hmQ = hk.data() + 2 * J;
+    int maxm = 0;
+    std::vector<int> seq_jobs;
+    bool any_nocross = false;
+    for (int j = 0; j < J; ++j) {
+        if (hkind[j] == K_CROSS) maxm = std::max(maxm, hmP[j] + hmQ[j]);
+        else if (hkind[j] == K_SEQ) seq_jobs.push_back(j);
+        else any_nocross = true;
     }
-    SC_HIP(hipGetLastError());
-    // list ranking of the traversed cycles, broken at their start vertex
-    uint32_t *inode = f.inode.ensure(M), *nxt = jmp2, *nxt2 = jmp, *dist = f.dist.ensure(M),
-             *dist2 = f.dist2.ensure(M);
-    k_rank_init<<<grid_for(M), BLK, 0, s>>>(M, val, pa.succ, ccP, pa.plen, inode, nxt, dist);
-    for (int r = 0; r < rounds; ++r) {
-        k_rank_step<<<grid_for(M), BLK, 0, s>>>(M, dist, nxt, dist2, nxt2);
-        std::swap(dist, dist2);
-        std::swap(nxt, nxt2);
+    // sequential route
+    const int nseq = int(seq_jobs.size());
+    std::vector<SeqJob> sj(static_cast<size_t>(nseq));
+    if (nseq > 0) {
+        std::vector<int> hsidx(static_cast<size_t>(J), -1);
+        int64_t vb = 0, ob = 0, rb = 0;
+        for (int q = 0; q < nseq; ++q) {
+            const int j = seq_jobs[q];
+            SeqJob &b = sj[q];
+            b.j = j;
+            b.nP = js.AB[j + 1] - js.AB[j];
+            b.nQ = js.AC[j + 1] - js.AC[j];
+            b.P0 = js.AB[j];
+            b.Q0 = js.AC[j];
+            b.cap = 2 * (b.nP + b.nQ) + 64;
+            b.out_cap = 2 * (b.nP + b.nQ) + 64;
+            b.ring_cap = b.nP + b.nQ + 8;
+            b.vbase = vb;
+            b.obase = ob;
+            b.rbase = rb;
+            vb += b.cap;
+            ob += b.out_cap;
+            rb += b.ring_cap + 1;
+            hsidx[j] = q;
+        }
+        SeqJob *djobs = f.sjobs.ensure(nseq);
+        SC_HIP(hipMemcpyAsync(djobs, sj.data(), sizeof(SeqJob) * nseq, hipMemcpyHostToDevice, s));
+        SC_HIP(hipMemcpyAsync(sidx, hsidx.data(), sizeof(int) * J, hipMemcpyHostToDevice, s));
+        SeqBatch b;
+        b.jobs = djobs;
+        b.px = c.aug_bx; b.py = c.aug_by; b.qx = c.aug_cx; b.qy = c.aug_cy;
+        b.avP = c.av_b; b.labP = c.lab_b; b.nbrP = c.nbr_b; b.labQ = c.lab_c; b.nbrQ = c.nbr_c;
+        b.vx = f.vx.ensure(vb); b.vy = f.vy.ensure(vb);
+        b.nx = f.nx.ensure(vb); b.pv = f.pv.ensure(vb); b.nb = f.nb.ensure(vb); b.lab = f.lab.ensure(vb);
+        b.enex = f.enex.ensure(vb); b.vsrc = f.vsrc.ensure(vb); b.cand = f.cand.ensure(vb);
+        b.inter = f.inter.ensure(vb);
+        b.ox = f.sox.ensure(ob); b.oy = f.soy.ensure(ob); b.osrc = f.sosrc.ensure(ob);
+        b.ring_off = f.sroff.ensure(rb);
+        b.result = f.sres.ensure(3 * int64_t(nseq));
+        k_sequential<<<nseq, 1, 0, s>>>(b);
+        SC_HIP(hipGetLastError());
     }
+    // cycles of the crossing jobs: minimum by pointer jumping, then list ranking
+    uint32_t *cmin = nullptr, *dist = nullptr, *inode = nullptr;
+    uint32_t *base = f.tot.ensure(int64_t(mP) + 1), *ridx = f.ringidx.ensure(int64_t(mP) + 1);
+    if (maxm > 0) {
+        uint32_t *val = pa.val, *val2 = f.val2.ensure(M), *jmp = f.jmp.ensure(M), *jmp2 = f.jmp2.ensure(M);
+        SC_HIP(hipMemcpyAsync(jmp, pa.succ, sizeof(uint32_t) * M, hipMemcpyDeviceToDevice, s));
+        const int rounds = ilog2_exact(maxm) + 1;
+        for (int r = 0; r < rounds; ++r) {
+            k_jump_min<<<grid_for(M), BLK, 0, s>>>(M, val, jmp, val2, jmp2);
+            std::swap(val, val2);
+            std::swap(jmp, jmp2);
+        }
+        SC_HIP(hipGetLastError());
+        cmin = val;
+        inode = f.inode.ensure(M);
+        uint32_t *nxt = jmp2, *nxt2 = jmp, *dist2 = f.dist2.ensure(M);
+        dist = f.dist.ensure(M);
+        k_rank_init<<<grid_for(M), BLK, 0, s>>>(M, cmin, pa.succ, pa.pj, AB, ccP, pa.plen, inode, nxt, dist);
+        for (int r = 0; r < rounds; ++r) {
+            k_rank_step<<<grid_for(M), BLK, 0, s>>>(M, dist, nxt, dist2, nxt2);
+            std::swap(dist, dist2);
+            std::swap(nxt, nxt2);
+        }
+        SC_HIP(hipGetLastError());
+        uint32_t *isstart = f.isstart.ensure(int64_t(mP) + 1);
+        k_start_totals<<<grid_for(mP), BLK, 0, s>>>(mP, pa.cposP, pa.pj, AB, cmin, dist, base, isstart);
+        SC_HIP(hipMemsetAsync(base + mP, 0, sizeof(uint32_t), s));
+        SC_HIP(hipMemsetAsync(isstart + mP, 0, sizeof(uint32_t), s));
+        SC_HIP(hipGetLastError());
+        scan_exclusive_u32(base, base, int64_t(mP) + 1, nullptr, s);
+        scan_exclusive_u32(isstart, ridx, int64_t(mP) + 1, nullptr, s);
+    }
+    // per-job output sizes before clean-up, and their offsets
+    uint32_t *jv = f.jv.ensure(J + 1), *jr = f.jr.ensure(J + 1);
+    CountArgs ca;
+    ca.J = J;
+    ca.AB = AB;
+    ca.AC = AC;
+    ca.kind = st.kind;
+    ca.windP = st.windP;
+    ca.windQ = st.windQ;
+    ca.cP0 = st.cP0;
+    ca.sidx = sidx;
+    ca.seq_res = nseq > 0 ? f.sres.p : nullptr;
+    ca.base = base;
+    ca.ridx = ridx;
+    ca.jv = jv;
+    ca.jr = jr;
+    k_jcounts<<<grid_for(J), BLK, 0, s>>>(ca);
     SC_HIP(hipGetLastError());
-    // components in base-ring order
-    uint32_t *ctot = f.tot.ensure(mP + 1), *isstart = f.isstart.ensure(mP + 1), *ridx = f.ringidx.ensure(mP + 1);
-    k_start_totals<<<grid_for(mP), BLK, 0, s>>>(int(mP), cposP, val, dist, ctot, isstart);
-    SC_HIP(hipMemsetAsync(ctot + mP, 0, sizeof(uint32_t), s));
-    SC_HIP(hipMemsetAsync(isstart + mP, 0, sizeof(uint32_t), s));
-    SC_HIP(hipGetLastError());
-    uint32_t *base = ctot; // in place
-    scan_exclusive_u32(ctot, base, int64_t(mP) + 1, tot, s);
-    scan_exclusive_u32(isstart, ridx, int64_t(mP) + 1, tot + 1, s);
-    SC_HIP(hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipMemsetAsync(jv + J, 0, sizeof(uint32_t), s));
+    SC_HIP(hipMemsetAsync(jr + J, 0, sizeof(uint32_t), s));
+    scan_exclusive_u32(jv, jv, int64_t(J) + 1, tot, s);
+    scan_exclusive_u32(jr, jr, int64_t(J) + 1, tot + 1, s);
+    std::vector<uint32_t> hJR(size_t(J) + 1);
+    SC_HIP(hipMemcpyAsync(hJR.data(), jr, sizeof(uint32_t) * (J + 1), hipMemcpyDeviceToHost, s));
+    d2h_sync(c, htot, tot, 2);
+    const int64_t NV = htot[0];
+    const int NR = int(htot[1]);
+    f.ox.ensure(NV + 1);
+    f.oy.ensure(NV + 1);
+    f.osrc.ensure(NV + 1);
+    int *off = f.off.ensure(int64_t(NR) + 1);
+    if (maxm > 0 && NV > 0) {
+        k_jring_offsets<<<grid_for(mP), BLK, 0, s>>>(mP, f.isstart, ridx, base, pa.pj, st.cP0, jv, jr, off);
+        uint32_t *head = f.head.ensure(NV + 1), *piece_at = f.piece_at.ensure(NV + 1);
+        SC_HIP(hipMemsetAsync(head, 0, sizeof(uint32_t) * size_t(NV + 1), s));
+        k_jheads<<<grid_for(M), BLK, 0, s>>>(M, inode, dist, base, pa.pj, st.cP0, jv, head, piece_at);
+        SC_HIP(hipGetLastError());
+        scan_inclusive_max_u32(head, head, NV, s);
+        EmitArgs ea;
+        ea.J = J;
+        ea.mP = mP;
+        ea.JV = jv;
+        ea.kind = st.kind;
+        ea.ppos = pa.ppos;
+        ea.pdir = pa.pdir;
+        ea.pj = pa.pj;
+        ea.head = head;
+        ea.piece_at = piece_at;
+        ea.P = P;
+        ea.Q = Q;
+        ea.ox = f.ox;
+        ea.oy = f.oy;
+        ea.osrc = f.osrc;
+        k_jemit<<<grid_for(NV), BLK, 0, s>>>(ea, NV);
+        SC_HIP(hipGetLastError());
+    }
+    if (any_nocross) {
+        k_jcopy_nocross<<<grid_for(nP), BLK, 0, s>>>(P, true, J, nP, st.kind, st.windP, st.windQ, AB, jv, f.ox,
+                                                     f.oy, f.osrc);
+        k_jcopy_nocross<<<grid_for(nQ), BLK, 0, s>>>(Q, false, J, nQ, st.kind, st.windP, st.windQ, AB, jv, f.ox,
+                                                     f.oy, f.osrc);
+        k_jnocross_off<<<grid_for(J), BLK, 0, s>>>(J, st.kind, st.windP, st.windQ, AB, jv, jr, off);
+        SC_HIP(hipGetLastError());
+    }
+    std::vector<int> hres(3 * size_t(nseq));
+    if (nseq > 0) {
+        k_jseq_copy<<<nseq, BLK, 0, s>>>(f.sjobs, f.sres, jv, jr, f.sox, f.soy, f.sosrc, f.sroff, f.ox, f.oy,
+                                          f.osrc, off);
+        SC_HIP(hipGetLastError());
+        SC_HIP(hipMemcpyAsync(hres.data(), f.sres.p, sizeof(int) * 3 * size_t(nseq), hipMemcpyDeviceToHost, s));
+    }
+    int last = int(NV);
+    SC_HIP(hipMemcpyAsync(off + NR, &last, sizeof(int), hipMemcpyHostToDevice, s));
     SC_HIP(hipStreamSynchronize(s));
-    const uint32_t n_out = htot[0];
-    const int nr = int(htot[1]);
-    int *off = f.off.ensure(size_t(nr) + 1);
-    k_ring_offsets<<<grid_for(mP), BLK, 0, s>>>(int(mP), isstart, ridx, base, off, 0u);
-    int last = int(n_out);
-    SC_HIP(hipMemcpyAsync(off + nr, &last, sizeof(int), hipMemcpyHostToDevice, s));
-    uint32_t *head = f.head.ensure(size_t(n_out) + 1), *piece_at = f.piece_at.ensure(size_t(n_out) + 1),
-             *ostart = f.ostart.ensure(M);
-    SC_HIP(hipMemsetAsync(head, 0, sizeof(uint32_t) * (size_t(n_out) + 1), s));
-    // base[] is indexed by base-ring compact index; inode is a node id < mP for starts
-    k_heads<<<grid_for(M), BLK, 0, s>>>(M, inode, dist, base, 0u, head, piece_at, ostart);
-    SC_HIP(hipGetLastError());
-    scan_inclusive_max_u32(head, head, n_out, s);
-    f.ox.ensure(size_t(n_out) + 1);
-    f.oy.ensure(size_t(n_out) + 1);
-    k_emit<<<grid_for(n_out), BLK, 0, s>>>(0u, n_out, head, piece_at, int(mP), pa.ppos, pa.pdir, P, Q, f.ox, f.oy);
-    SC_HIP(hipGetLastError());
-    cleanup_rings(c, f, nr, n_out);
+    for (int q = 0; q < nseq; ++q) {
+        js.flags[sj[q].j] |= hres[3 * q + 2];
+        c.undefined |= hres[3 * q + 2];
+    }
+    std::vector<uint32_t> ring_idx;
+    cleanup_rings(c, f, NR, NV, ring_idx);
+    js.ring_off.resize(size_t(J) + 1);
+    for (int j = 0; j <= J; ++j) js.ring_off[j] = int(ring_idx[hJR[j]]);
 }
 
-void clip_all(Ctx &c, int k) {
-    SC_REQUIRE(k >= 1 && k <= c.K, SC_EINVAL, "overlay index out of range");
+void clip_jobs(Ctx &c, int k0, int k1) {
+    SC_REQUIRE(k0 >= 1 && k1 <= c.K && k0 <= k1, SC_EINVAL, "overlay index out of range");
     StageTimes t;
     const bool tree = c.T == 0, cand = tree || int(c.pair_off.size()) != c.K + 1;
     c.mark(0);
@@ -1001,7 +1317,7 @@ void clip_all(Ctx &c, int k) {
     c.mark(4);
     if (cand) find_candidates(c);
     c.mark(5);
-    post_overlay(c, k);
+    post_jobs(c, k0, k1);
     c.mark(6);
     label_and_traverse(c);
     c.mark(7);
@@ -1018,5 +1334,7 @@ void clip_all(Ctx &c, int k) {
         c.times = t;
     }
 }
+
+void clip_all(Ctx &c, int k) { clip_jobs(c, k, k); }
 
 } // namespace sc

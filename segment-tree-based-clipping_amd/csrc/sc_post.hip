@@ -1,4 +1,8 @@
-// sc_post.hip -- per-overlay intersection bookkeeping (rows A11-A15).
+// sc_post.hip -- intersection bookkeeping (rows A11-A15) for a batch of jobs.
+//
+// A job clips one base ring against one overlay ring (overlay k, base
+// job_base(k)); all jobs of a batch run in the same launches, their arrays
+// laid out job after job.
 //
 // Reference (src/clip.cpp / oacc_segment_tree.cpp):
 //  - countEdgeIntersections + isNonDegenerateCase + exclusive scans
@@ -33,18 +37,41 @@ inline unsigned grid_for(int64_t n, int per = BLK) {
     return unsigned(g);
 }
 
-enum : int { UB_UNASSIGNED = 1, UB_WALK = 2, UB_TRAVERSAL = 4 };
+enum : int { UB_UNASSIGNED = 1 };
 
-__global__ void k_pair_decode(const uint64_t *__restrict__ pairs, int I, int bits_b, int bits_c, int cStart,
-                              const double *__restrict__ bpoly, const double *__restrict__ cpoly, int *pb, int *pc,
-                              int *pt, double *px, double *py, int *qa, int *qb, int *all_b, int *nd_b, int *all_c,
-                              int *nd_c) {
+struct JobArgs {
+    int J;
+    const int *p0;     // J+1 pair offsets (relative to the batch's first pair)
+    const int *bvoff;  // base ring vertex offset in bpoly
+    const int *beoff;  // base ring first global edge
+    const int *nb, *nc;
+    const int *cvoff;  // overlay vertex offset in cpoly
+    const int *JB, *JC; // J+1 offsets of the per-edge count arrays (n+1 per job)
+};
+
+__global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, int I, int bits_b, int bits_c,
+                              const double *__restrict__ bpoly, const double *__restrict__ cpoly, int *pj, int *pb,
+                              int *pc, int *pt, double *px, double *py, int *qa, int *qb, int *all_b, int *nd_b,
+                              int *all_c, int *nd_c, int *flags) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= I) return;
+    int j = seg_of(ja.p0, ja.J, p);
     uint64_t key = pairs[p];
-    int b = int((key >> bits_c) & ((1ull << bits_b) - 1ull));
+    int b = int((key >> bits_c) & ((1ull << bits_b) - 1ull)) - ja.beoff[j];
     int cl = int(key & ((1ull << bits_c) - 1ull));
-    int bId = 2 * b, cId = 2 * (cStart + cl);
+    pj[p] = j;
+    if (b < 0 || b >= ja.nb[j] || cl >= ja.nc[j]) { // a pair outside the job's rings
+        atomicOr(&flags[j], UB_UNASSIGNED);
+        pb[p] = 0;
+        pc[p] = 0;
+        pt[p] = 0;
+        px[p] = 0.0;
+        py[p] = 0.0;
+        qa[p] = 0;
+        qb[p] = 0;
+        return;
+    }
+    int64_t bId = 2 * int64_t(ja.bvoff[j] + b), cId = 2 * int64_t(ja.cvoff[j] + cl);
     double b1x = bpoly[bId], b1y = bpoly[bId + 1], b2x = bpoly[bId + 2], b2y = bpoly[bId + 3];
     double c1x = cpoly[cId], c1y = cpoly[cId + 1], c2x = cpoly[cId + 2], c2y = cpoly[cId + 3];
     double al, be;
@@ -70,10 +97,10 @@ __global__ void k_pair_decode(const uint64_t *__restrict__ pairs, int I, int bit
     py[p] = y;
     qa[p] = alpha_to_int(al);
     qb[p] = alpha_to_int(be);
-    atomicAdd(&all_b[b], 1);
-    atomicAdd(&all_c[cl], 1);
-    if (nondeg_b(t)) atomicAdd(&nd_b[b], 1);
-    if (nondeg_c(t)) atomicAdd(&nd_c[cl], 1);
+    atomicAdd(&all_b[ja.JB[j] + b], 1);
+    atomicAdd(&all_c[ja.JC[j] + cl], 1);
+    if (nondeg_b(t)) atomicAdd(&nd_b[ja.JB[j] + b], 1);
+    if (nondeg_c(t)) atomicAdd(&nd_c[ja.JC[j] + cl], 1);
 }
 
 // keys for the per-edge alpha order: quantised alpha modulo 1e9 (the nine
@@ -87,12 +114,13 @@ __global__ void k_alpha_keys(const int *__restrict__ q, const uint32_t *__restri
     vals[i] = p;
 }
 
-// second key: the edge (stable on the alpha order from the first pass)
-__global__ void k_edge_keys(const int *__restrict__ edge_of, const uint32_t *__restrict__ vals, int I,
-                            uint64_t *keys) {
+// edge key unique over the batch: offset of the edge's count slot
+__global__ void k_edge_keys(const int *__restrict__ edge_of, const int *__restrict__ pj, const int *__restrict__ off,
+                            const uint32_t *__restrict__ vals, int I, uint64_t *keys) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= I) return;
-    keys[i] = uint64_t(uint32_t(edge_of[vals[i]]));
+    uint32_t p = vals[i];
+    keys[i] = uint64_t(uint32_t(off[pj[p]] + edge_of[p]));
 }
 
 __global__ void k_copy_order(const uint32_t *__restrict__ vals, int I, int *ord) {
@@ -100,16 +128,26 @@ __global__ void k_copy_order(const uint32_t *__restrict__ vals, int I, int *ord)
     if (i < I) ord[i] = int(vals[i]);
 }
 
-__global__ void k_ckeys(const int *__restrict__ pb, const int *__restrict__ pc, int I, int bits_b, uint64_t *keys,
-                        uint32_t *vals) {
+// (overlay edge slot, base edge) keys: pairs grouped by overlay edge, base order inside
+__global__ void k_ckeys(const int *__restrict__ pj, const int *__restrict__ pb, const int *__restrict__ pc,
+                        const int *__restrict__ JC, int I, int bits_bl, uint64_t *keys, uint32_t *vals) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= I) return;
-    keys[p] = (uint64_t(pc[p]) << bits_b) | uint64_t(pb[p]);
+    keys[p] = (uint64_t(uint32_t(JC[pj[p]] + pc[p])) << bits_bl) | uint64_t(uint32_t(pb[p]));
     vals[p] = uint32_t(p);
 }
 
-__global__ void k_init_aug(int n, double *ax, double *ay, int *av, int *nbr, int *lab) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// per job: augmented sizes nb + (non-degenerate intersections on the base), likewise overlay
+__global__ void k_job_sizes(JobArgs ja, const int *__restrict__ nd_b, const int *__restrict__ nd_c, int *nba,
+                            int *nca) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ja.J) return;
+    nba[j] = ja.nb[j] + (nd_b[ja.JB[j + 1]] - nd_b[ja.JB[j]]);
+    nca[j] = ja.nc[j] + (nd_c[ja.JC[j + 1]] - nd_c[ja.JC[j]]);
+}
+
+__global__ void k_init_aug(int64_t n, double *ax, double *ay, int *av, int *nbr, int *lab) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     ax[i] = 0.0;
     ay[i] = 0.0;
@@ -118,19 +156,27 @@ __global__ void k_init_aug(int n, double *ax, double *ay, int *av, int *nbr, int
     lab[i] = 0;
 }
 
-__global__ void k_aug_sources(int n, const double *__restrict__ poly, const int *__restrict__ nd_scan, double *ax,
+// source vertices: count slot i of job j is vertex i - JX[j] (the last slot of a job is padding)
+__global__ void k_aug_sources(int J, const int *__restrict__ JX, const int *__restrict__ nverts,
+                              const int *__restrict__ voff, const int *__restrict__ AX,
+                              const double *__restrict__ poly, const int *__restrict__ nd_scan, double *ax,
                               double *ay) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    int pos = v + nd_scan[v];
-    ax[pos] = poly[2 * int64_t(v)];
-    ay[pos] = poly[2 * int64_t(v) + 1];
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= JX[J]) return;
+    int j = seg_of(JX, J, i);
+    int v = int(i - JX[j]);
+    if (v >= nverts[j]) return;
+    int64_t pos = AX[j] + v + (nd_scan[i] - nd_scan[JX[j]]);
+    int64_t g = 2 * int64_t(voff[j] + v);
+    ax[pos] = poly[g];
+    ay[pos] = poly[g + 1];
 }
 
-// one thread per edge with intersections: leading degenerate marks the source
-// vertex, then the non-degenerate ones fill the slots after it in alpha order
+// one thread per edge with intersections: a leading degenerate intersection
+// marks the source vertex, the non-degenerate ones fill the slots after it
 template <bool SIDE_B>
-__global__ void k_aug_segments(const int *__restrict__ seg_edge_of, int I, const int *__restrict__ ord,
+__global__ void k_aug_segments(const int *__restrict__ edge_of, const int *__restrict__ pj, int I,
+                               const int *__restrict__ ord, const int *__restrict__ JX, const int *__restrict__ AX,
                                const int *__restrict__ all_scan, const int *__restrict__ nd_scan,
                                const int *__restrict__ pt, const double *__restrict__ px,
                                const double *__restrict__ py, const int *__restrict__ q, double *ax, double *ay,
@@ -138,48 +184,51 @@ __global__ void k_aug_segments(const int *__restrict__ seg_edge_of, int I, const
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= I) return;
     int p0 = ord[i];
-    int e = seg_edge_of[p0];
-    int s = all_scan[e];
+    int j = pj[p0];
+    int e = edge_of[p0];
+    int gi = JX[j] + e;
+    int s = all_scan[gi];
     if (i != s) return; // not the first position of this edge's segment
-    int t = all_scan[e + 1];
-    int nd = nd_scan[e + 1] - nd_scan[e];
-    int src = e + nd_scan[e];
-    int j = 0;
+    int t = all_scan[gi + 1];
+    int nd = nd_scan[gi + 1] - nd_scan[gi];
+    int src = AX[j] + e + (nd_scan[gi] - nd_scan[JX[j]]);
+    int k0 = 0;
     int first = ord[s];
     bool lead_degen = SIDE_B ? degen_b_first(pt[first]) : degen_c_first(pt[first]);
     if (lead_degen) {
         av[src] = q[first];
         slot[first] = src;
-        j = 1;
+        k0 = 1;
     }
     for (int k = 0; k < nd; ++k) {
-        if (s + j + k >= t) {
-            atomicOr(flags, UB_UNASSIGNED);
+        if (s + k0 + k >= t) {
+            atomicOr(&flags[j], UB_UNASSIGNED);
             break;
         }
-        int p = ord[s + j + k];
+        int p = ord[s + k0 + k];
         bool ok = SIDE_B ? nondeg_b(pt[p]) : nondeg_c(pt[p]);
-        if (!ok) atomicOr(flags, UB_UNASSIGNED);
+        if (!ok) atomicOr(&flags[j], UB_UNASSIGNED);
         int sl = src + 1 + k;
         ax[sl] = px[p];
         ay[sl] = py[p];
         av[sl] = q[p];
         slot[p] = sl;
     }
-    if (j + nd != t - s) atomicOr(flags, UB_UNASSIGNED);
+    if (k0 + nd != t - s) atomicOr(&flags[j], UB_UNASSIGNED);
 }
 
-__global__ void k_link(const int *__restrict__ sb, const int *__restrict__ scn, int I, int *nbr_b, int *nbr_c,
-                       int *flags) {
+// neighbour links, 1-based and local to the job's rings (as the reference's arrays)
+__global__ void k_link(const int *__restrict__ sb, const int *__restrict__ scn, const int *__restrict__ pj, int I,
+                       const int *__restrict__ AB, const int *__restrict__ AC, int *nbr_b, int *nbr_c, int *flags) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= I) return;
-    int a = sb[p], b = scn[p];
+    int a = sb[p], b = scn[p], j = pj[p];
     if (a < 0 || b < 0) {
-        atomicOr(flags, UB_UNASSIGNED);
+        atomicOr(&flags[j], UB_UNASSIGNED);
         return;
     }
-    nbr_b[a] = b + 1;
-    nbr_c[b] = a + 1;
+    nbr_b[a] = b - AC[j] + 1;
+    nbr_c[b] = a - AB[j] + 1;
 }
 
 __device__ inline int rel_pos(int pMN, int pPN, int q, double qx, double qy, double pmx, double pmy, double cx,
@@ -193,27 +242,32 @@ __device__ inline int rel_pos(int pMN, int pPN, int q, double qx, double qy, dou
     return (s1 < 0 && s2 < 0) ? 1 : 0;
 }
 
-__global__ void k_init_labels(int nbaug, int ncaug, const double *__restrict__ bx, const double *__restrict__ by,
+__global__ void k_init_labels(int J, const int *__restrict__ AB, const int *__restrict__ AC,
+                              const double *__restrict__ bx, const double *__restrict__ by,
                               const int *__restrict__ avb, const int *__restrict__ nbrb,
                               const double *__restrict__ cx, const double *__restrict__ cy, int *labb, int *labc,
                               int *flags) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nbaug) return;
-    if (avb[i] == -100) return;
+    int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= AB[J]) return;
+    if (avb[g] == -100) return;
+    int j = seg_of(AB, J, g);
+    const int B0 = AB[j], nbaug = AB[j + 1] - B0, C0 = AC[j], ncaug = AC[j + 1] - C0;
+    const int i = int(g - B0);
     int tm = i - 1 < 0 ? nbaug - 1 : i - 1;
     int tp = i + 1 == nbaug ? 0 : i + 1;
-    int pMN = avb[tm] != -100 ? nbrb[tm] - 1 : -100;
-    int pPN = avb[tp] != -100 ? nbrb[tp] - 1 : -100;
-    int nId = nbrb[i] - 1;
-    if (nId < 0) {
-        atomicOr(flags, UB_UNASSIGNED);
+    int pMN = avb[B0 + tm] != -100 ? nbrb[B0 + tm] - 1 : -100;
+    int pPN = avb[B0 + tp] != -100 ? nbrb[B0 + tp] - 1 : -100;
+    int nId = nbrb[g] - 1;
+    if (nId < 0 || nId >= ncaug) {
+        atomicOr(&flags[j], UB_UNASSIGNED);
         return;
     }
     int qm = nId - 1 < 0 ? ncaug - 1 : nId - 1;
     int qp = nId + 1 == ncaug ? 0 : nId + 1;
-    double X = bx[i], Y = by[i];
-    int a = rel_pos(pMN, pPN, qm, cx[qm], cy[qm], bx[tm], by[tm], X, Y, bx[tp], by[tp]);
-    int b = rel_pos(pMN, pPN, qp, cx[qp], cy[qp], bx[tm], by[tm], X, Y, bx[tp], by[tp]);
+    double X = bx[g], Y = by[g];
+    double pmx = bx[B0 + tm], pmy = by[B0 + tm], ppx = bx[B0 + tp], ppy = by[B0 + tp];
+    int a = rel_pos(pMN, pPN, qm, cx[C0 + qm], cy[C0 + qm], pmx, pmy, X, Y, ppx, ppy);
+    int b = rel_pos(pMN, pPN, qp, cx[C0 + qp], cy[C0 + qp], pmx, pmy, X, Y, ppx, ppy);
     int lab;
     if ((a == 0 && b == 1) || (a == 1 && b == 0)) lab = 1;
     else if ((a == 0 && b == 0) || (a == 1 && b == 1)) lab = 2;
@@ -223,108 +277,171 @@ __global__ void k_init_labels(int nbaug, int ncaug, const double *__restrict__ b
     else if ((a == 2 && b == 1) || (b == 2 && a == 1)) lab = 6;
     else if ((a == 2 && b == 0) || (b == 2 && a == 0)) lab = 7;
     else lab = -102;
-    labb[i] = lab;
-    labc[nId] = lab;
+    labb[g] = lab;
+    labc[C0 + nId] = lab;
 }
 
-void excl_scan_int(Ctx &c, int *a, int64_t n, int *d_total) {
-    scan_exclusive_u32(reinterpret_cast<uint32_t *>(a), reinterpret_cast<uint32_t *>(a), n,
-                       reinterpret_cast<uint32_t *>(d_total), c.stream);
+int bits_for(int64_t n) {
+    int b = 0;
+    while ((int64_t(1) << b) < n) b++;
+    return b < 1 ? 1 : b;
+}
+
+template <class T> T *upload(Ctx &c, DBuf<T> &d, const std::vector<T> &h) {
+    d.ensure(h.size());
+    SC_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c.stream));
+    return d.p;
 }
 
 } // namespace
 
-void post_overlay(Ctx &c, int k) {
-    SC_REQUIRE(k >= 1 && k <= c.K, SC_EINVAL, "overlay index out of range");
+void post_jobs(Ctx &c, int k0, int k1) {
+    SC_REQUIRE(k0 >= 1 && k1 <= c.K && k0 <= k1, SC_EINVAL, "overlay index out of range");
     SC_REQUIRE(int(c.pair_off.size()) == c.K + 1, SC_ESTATE, "candidates not computed");
     hipStream_t s = c.stream;
-    const int nb = c.nb, cs = c.csz[k - 1], cStart = c.coff[k - 1];
-    const int64_t pk0 = c.pair_off[k - 1];
-    const int I = int(c.pair_off[k] - pk0);
-    c.cur_k = k;
+    const int J = k1 - k0 + 1;
+    JobSet &js = c.jobs;
+    js.k0 = k0;
+    js.J = J;
+    js.p0.assign(J + 1, 0);
+    js.bvoff.resize(J);
+    js.beoff.resize(J);
+    js.nb.resize(J);
+    js.nc.resize(J);
+    js.cvoff.resize(J);
+    js.JB.assign(J + 1, 0);
+    js.JC.assign(J + 1, 0);
+    const int64_t P0 = c.pair_off[k0 - 1];
+    for (int j = 0; j < J; ++j) {
+        int k = k0 + j, br = job_base(c, k);
+        js.p0[j + 1] = int(c.pair_off[k] - P0);
+        js.bvoff[j] = c.bvoff[br];
+        js.beoff[j] = c.beoff[br];
+        js.nb[j] = c.bsz[br];
+        js.nc[j] = c.csz[k - 1];
+        js.cvoff[j] = c.coff[k - 1];
+        js.JB[j + 1] = js.JB[j] + js.nb[j] + 1;
+        js.JC[j + 1] = js.JC[j] + js.nc[j] + 1;
+    }
+    const int I = js.p0[J];
+    c.cur_k = J == 1 ? k0 : 0;
     c.I = I;
+    JobArgs ja;
+    ja.J = J;
+    ja.p0 = upload(c, js.d_p0, js.p0);
+    ja.bvoff = upload(c, js.d_bvoff, js.bvoff);
+    ja.beoff = upload(c, js.d_beoff, js.beoff);
+    ja.nb = upload(c, js.d_nb, js.nb);
+    ja.nc = upload(c, js.d_nc, js.nc);
+    ja.cvoff = upload(c, js.d_cvoff, js.cvoff);
+    ja.JB = upload(c, js.d_JB, js.JB);
+    ja.JC = upload(c, js.d_JC, js.JC);
+    const int nJB = js.JB[J], nJC = js.JC[J];
     const int Ia = I > 0 ? I : 1;
-    c.p_b.ensure(Ia); c.p_c.ensure(Ia); c.p_type.ensure(Ia); c.p_x.ensure(Ia); c.p_y.ensure(Ia);
+    c.p_j.ensure(Ia); c.p_b.ensure(Ia); c.p_c.ensure(Ia); c.p_type.ensure(Ia); c.p_x.ensure(Ia); c.p_y.ensure(Ia);
     c.p_qa.ensure(Ia); c.p_qb.ensure(Ia); c.ordB.ensure(Ia); c.ordC.ensure(Ia);
     c.slot_b.ensure(Ia); c.slot_c.ensure(Ia);
-    int *all_b = c.all_b.ensure(nb + 1), *nd_b = c.nd_b.ensure(nb + 1);
-    int *all_c = c.all_c.ensure(cs + 1), *nd_c = c.nd_c.ensure(cs + 1);
-    int *flags = c.flags.ensure(4);
-    SC_HIP(hipMemsetAsync(flags, 0, sizeof(int) * 4, s));
-    SC_HIP(hipMemsetAsync(all_b, 0, sizeof(int) * (nb + 1), s));
-    SC_HIP(hipMemsetAsync(nd_b, 0, sizeof(int) * (nb + 1), s));
-    SC_HIP(hipMemsetAsync(all_c, 0, sizeof(int) * (cs + 1), s));
-    SC_HIP(hipMemsetAsync(nd_c, 0, sizeof(int) * (cs + 1), s));
+    int *all_b = c.all_b.ensure(nJB + 1), *nd_b = c.nd_b.ensure(nJB + 1);
+    int *all_c = c.all_c.ensure(nJC + 1), *nd_c = c.nd_c.ensure(nJC + 1);
+    int *flags = c.flags.ensure(J);
+    SC_HIP(hipMemsetAsync(flags, 0, sizeof(int) * J, s));
+    SC_HIP(hipMemsetAsync(all_b, 0, sizeof(int) * (nJB + 1), s));
+    SC_HIP(hipMemsetAsync(nd_b, 0, sizeof(int) * (nJB + 1), s));
+    SC_HIP(hipMemsetAsync(all_c, 0, sizeof(int) * (nJC + 1), s));
+    SC_HIP(hipMemsetAsync(nd_c, 0, sizeof(int) * (nJC + 1), s));
     SC_HIP(hipMemsetAsync(c.slot_b.p, 0xff, sizeof(int) * Ia, s));
     SC_HIP(hipMemsetAsync(c.slot_c.p, 0xff, sizeof(int) * Ia, s));
     if (I > 0) {
-        k_pair_decode<<<grid_for(I), BLK, 0, s>>>(c.pairs.p + pk0, I, c.bits_b, c.bits_c, cStart, c.bpoly, c.cpoly,
+        k_pair_decode<<<grid_for(I), BLK, 0, s>>>(ja, c.pairs.p + P0, I, c.bits_b, c.bits_c, c.bpoly, c.cpoly, c.p_j,
                                                   c.p_b, c.p_c, c.p_type, c.p_x, c.p_y, c.p_qa, c.p_qb, all_b, nd_b,
-                                                  all_c, nd_c);
+                                                  all_c, nd_c, flags);
         SC_HIP(hipGetLastError());
     }
-    int *tot = reinterpret_cast<int *>(c.d_u32_total.ensure(4));
-    excl_scan_int(c, all_b, nb + 1, nullptr);
-    excl_scan_int(c, nd_b, nb + 1, tot);
-    excl_scan_int(c, all_c, cs + 1, nullptr);
-    excl_scan_int(c, nd_c, cs + 1, tot + 1);
-    int htot[2];
-    SC_HIP(hipMemcpyAsync(htot, tot, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    auto scan = [&](int *a, int64_t n) {
+        scan_exclusive_u32(reinterpret_cast<uint32_t *>(a), reinterpret_cast<uint32_t *>(a), n, nullptr, s);
+    };
+    scan(all_b, nJB + 1);
+    scan(nd_b, nJB + 1);
+    scan(all_c, nJC + 1);
+    scan(nd_c, nJC + 1);
+    // augmented ring sizes and offsets per job
+    int *nba = js.d_AB.ensure(J + 1), *nca = js.d_AC.ensure(J + 1);
+    k_job_sizes<<<grid_for(J), BLK, 0, s>>>(ja, nd_b, nd_c, nba, nca);
+    SC_HIP(hipGetLastError());
+    SC_HIP(hipMemsetAsync(nba + J, 0, sizeof(int), s));
+    SC_HIP(hipMemsetAsync(nca + J, 0, sizeof(int), s));
+    uint32_t *tot = c.d_u32_total.ensure(2);
+    scan_exclusive_u32(reinterpret_cast<uint32_t *>(nba), reinterpret_cast<uint32_t *>(nba), J + 1, tot, s);
+    scan_exclusive_u32(reinterpret_cast<uint32_t *>(nca), reinterpret_cast<uint32_t *>(nca), J + 1, tot + 1, s);
+    js.AB.resize(J + 1);
+    js.AC.resize(J + 1);
+    SC_HIP(hipMemcpyAsync(js.AB.data(), nba, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipMemcpyAsync(js.AC.data(), nca, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
     SC_HIP(hipStreamSynchronize(s));
-    c.nbaug = nb + htot[0];
-    c.ncaug = cs + htot[1];
+    const int64_t nba_tot = js.AB[J], nca_tot = js.AC[J];
+    c.nbaug = int(nba_tot);
+    c.ncaug = int(nca_tot);
 
     if (I > 0) {
-        // B side: pairs are in (base edge, overlay edge) order; stable sort by
-        // alpha, then stably by base edge -> per edge alpha order, ties in pair order
+        // base side: pairs are in (job, base edge, overlay edge) order; stable
+        // sort by alpha, then stably by edge -> per edge alpha order
         uint64_t *ak = c.k64a.ensure(Ia);
         uint32_t *av = c.u32a.ensure(Ia);
+        const int bits_eb = bits_for(int64_t(nJB) + 1), bits_ec = bits_for(int64_t(nJC) + 1);
         k_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_qa, nullptr, I, ak, av);
         SC_HIP(hipGetLastError());
         radix_sort_u64(ak, av, I, 0, 30, c.rs, s);
-        k_edge_keys<<<grid_for(I), BLK, 0, s>>>(c.p_b, av, I, ak);
+        k_edge_keys<<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, ja.JB, av, I, ak);
         SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, av, I, 0, c.bits_b, c.rs, s);
+        radix_sort_u64(ak, av, I, 0, bits_eb, c.rs, s);
         k_copy_order<<<grid_for(I), BLK, 0, s>>>(av, I, c.ordB);
-        // C side: order by (overlay edge, base edge), then stably by alpha, then by overlay edge
+        // overlay side: (overlay edge, base edge) order, then stably by alpha, then by edge
         uint32_t *cv = c.u32b.ensure(Ia);
-        k_ckeys<<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_c, I, c.bits_b, ak, cv);
+        const int bits_bl = bits_for(c.maxbsz);
+        SC_REQUIRE(bits_ec + bits_bl <= 64, SC_ERANGE, "overlay order key too wide");
+        k_ckeys<<<grid_for(I), BLK, 0, s>>>(c.p_j, c.p_b, c.p_c, ja.JC, I, bits_bl, ak, cv);
         SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, cv, I, 0, c.bits_b + c.bits_c, c.rs, s);
+        radix_sort_u64(ak, cv, I, 0, bits_ec + bits_bl, c.rs, s);
         k_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_qb, cv, I, ak, av);
         SC_HIP(hipGetLastError());
         radix_sort_u64(ak, av, I, 0, 30, c.rs, s);
-        k_edge_keys<<<grid_for(I), BLK, 0, s>>>(c.p_c, av, I, ak);
+        k_edge_keys<<<grid_for(I), BLK, 0, s>>>(c.p_c, c.p_j, ja.JC, av, I, ak);
         SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, av, I, 0, c.bits_c, c.rs, s);
+        radix_sort_u64(ak, av, I, 0, bits_ec, c.rs, s);
         k_copy_order<<<grid_for(I), BLK, 0, s>>>(av, I, c.ordC);
         SC_HIP(hipGetLastError());
     }
-    const int nba = c.nbaug, nca = c.ncaug;
-    c.aug_bx.ensure(nba); c.aug_by.ensure(nba); c.av_b.ensure(nba); c.nbr_b.ensure(nba); c.lab_b.ensure(nba);
-    c.aug_cx.ensure(nca); c.aug_cy.ensure(nca); c.av_c.ensure(nca); c.nbr_c.ensure(nca); c.lab_c.ensure(nca);
-    k_init_aug<<<grid_for(nba), BLK, 0, s>>>(nba, c.aug_bx, c.aug_by, c.av_b, c.nbr_b, c.lab_b);
-    k_init_aug<<<grid_for(nca), BLK, 0, s>>>(nca, c.aug_cx, c.aug_cy, c.av_c, c.nbr_c, c.lab_c);
+    const int64_t nbaA = nba_tot > 0 ? nba_tot : 1, ncaA = nca_tot > 0 ? nca_tot : 1;
+    c.aug_bx.ensure(nbaA); c.aug_by.ensure(nbaA); c.av_b.ensure(nbaA); c.nbr_b.ensure(nbaA); c.lab_b.ensure(nbaA);
+    c.aug_cx.ensure(ncaA); c.aug_cy.ensure(ncaA); c.av_c.ensure(ncaA); c.nbr_c.ensure(ncaA); c.lab_c.ensure(ncaA);
+    k_init_aug<<<grid_for(nba_tot), BLK, 0, s>>>(nba_tot, c.aug_bx, c.aug_by, c.av_b, c.nbr_b, c.lab_b);
+    k_init_aug<<<grid_for(nca_tot), BLK, 0, s>>>(nca_tot, c.aug_cx, c.aug_cy, c.av_c, c.nbr_c, c.lab_c);
     SC_HIP(hipGetLastError());
-    k_aug_sources<<<grid_for(nb), BLK, 0, s>>>(nb, c.bpoly, nd_b, c.aug_bx, c.aug_by);
-    k_aug_sources<<<grid_for(cs), BLK, 0, s>>>(cs, c.cpoly.p + 2 * int64_t(cStart), nd_c, c.aug_cx, c.aug_cy);
+    k_aug_sources<<<grid_for(nJB), BLK, 0, s>>>(J, ja.JB, ja.nb, ja.bvoff, nba, c.bpoly, nd_b, c.aug_bx, c.aug_by);
+    k_aug_sources<<<grid_for(nJC), BLK, 0, s>>>(J, ja.JC, ja.nc, ja.cvoff, nca, c.cpoly, nd_c, c.aug_cx, c.aug_cy);
     SC_HIP(hipGetLastError());
     if (I > 0) {
-        k_aug_segments<true><<<grid_for(I), BLK, 0, s>>>(c.p_b, I, c.ordB, all_b, nd_b, c.p_type, c.p_x, c.p_y,
-                                                         c.p_qa, c.aug_bx, c.aug_by, c.av_b, c.slot_b, flags);
-        k_aug_segments<false><<<grid_for(I), BLK, 0, s>>>(c.p_c, I, c.ordC, all_c, nd_c, c.p_type, c.p_x, c.p_y,
-                                                          c.p_qb, c.aug_cx, c.aug_cy, c.av_c, c.slot_c, flags);
+        k_aug_segments<true><<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, I, c.ordB, ja.JB, nba, all_b, nd_b, c.p_type,
+                                                         c.p_x, c.p_y, c.p_qa, c.aug_bx, c.aug_by, c.av_b, c.slot_b,
+                                                         flags);
+        k_aug_segments<false><<<grid_for(I), BLK, 0, s>>>(c.p_c, c.p_j, I, c.ordC, ja.JC, nca, all_c, nd_c,
+                                                          c.p_type, c.p_x, c.p_y, c.p_qb, c.aug_cx, c.aug_cy, c.av_c,
+                                                          c.slot_c, flags);
         SC_HIP(hipGetLastError());
-        k_link<<<grid_for(I), BLK, 0, s>>>(c.slot_b, c.slot_c, I, c.nbr_b, c.nbr_c, flags);
+        k_link<<<grid_for(I), BLK, 0, s>>>(c.slot_b, c.slot_c, c.p_j, I, nba, nca, c.nbr_b, c.nbr_c, flags);
         SC_HIP(hipGetLastError());
-        k_init_labels<<<grid_for(nba), BLK, 0, s>>>(nba, nca, c.aug_bx, c.aug_by, c.av_b, c.nbr_b, c.aug_cx,
-                                                    c.aug_cy, c.lab_b, c.lab_c, flags);
+        k_init_labels<<<grid_for(nba_tot), BLK, 0, s>>>(J, nba, nca, c.aug_bx, c.aug_by, c.av_b, c.nbr_b, c.aug_cx,
+                                                        c.aug_cy, c.lab_b, c.lab_c, flags);
         SC_HIP(hipGetLastError());
     }
-    int hflags = 0;
-    SC_HIP(hipMemcpyAsync(&hflags, flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    js.flags.resize(J);
+    SC_HIP(hipMemcpyAsync(js.flags.data(), flags, sizeof(int) * J, hipMemcpyDeviceToHost, s));
     SC_HIP(hipStreamSynchronize(s));
-    c.undefined = hflags;
+    int any = 0;
+    for (int f : js.flags) any |= f;
+    c.undefined = any;
 }
+
+void post_overlay(Ctx &c, int k) { post_jobs(c, k, k); }
 
 } // namespace sc

@@ -276,7 +276,7 @@ void find_candidates(Ctx &c) {
         a.lend_off = c.lend_off; a.lend_e = c.lend_e; a.uoff = units;
         a.before = c.before_leaf; a.etype = c.etype; a.eoff = c.d_eoff;
         a.x0 = c.ex0; a.y0 = c.ey0; a.x1 = c.ex1; a.y1 = c.ey1;
-        a.bx0 = c.estart; a.bx1 = c.eend; a.by0 = c.eymin; a.by1 = c.eymax;
+        a.bx0 = c.paired ? c.uxs.p : c.estart.p; a.bx1 = c.paired ? c.uxe.p : c.eend.p; a.by0 = c.eymin; a.by1 = c.eymax;
         a.nl = c.nleaves; a.D = c.depth; a.T = T; a.nunits = nunits;
         a.bits_b = c.bits_b; a.bits_c = c.bits_c;
         a.out = c.cand; a.cap = uint64_t(c.cand_cap); a.counter = counter;

@@ -17,6 +17,8 @@
 //    the leaf range [L,R) with L = first point >= start, R = last point <= end.
 #include "sc_ctx.hpp"
 
+#include <algorithm>
+
 namespace sc {
 
 namespace {
@@ -59,40 +61,51 @@ __global__ void k_layout_overlays(const double *__restrict__ cxy, int nct, const
     }
 }
 
-__global__ void k_close_base(double *bpoly, int nb) {
-    bpoly[2 * nb] = bpoly[0];
-    bpoly[2 * nb + 1] = bpoly[1];
-}
 
+// Edge records.  In paired mode the x-extents used as tree keys are moved by a
+// per-pair shift so that different pairs occupy disjoint key ranges (no tree
+// node is then covered by edges of two pairs); the endpoints used by every
+// geometric predicate stay the input coordinates.
 __global__ void k_edges(const double *__restrict__ bpoly, const double *__restrict__ cpoly, int nb, int ne,
-                        const int *__restrict__ eoff, const int *__restrict__ coff, int K, double *x0, double *y0,
-                        double *x1, double *y1, double *es, double *ee, double *eyl, double *eyh, int *ety,
-                        int *eid) {
+                        const int *__restrict__ eoff, const int *__restrict__ coff, int K,
+                        const int *__restrict__ beoff, const int *__restrict__ bvoff, int nbr,
+                        const double *__restrict__ shift, double *x0, double *y0, double *x1, double *y1,
+                        double *es, double *ee, double *eyl, double *eyh, int *ety, int *eid, double *uxs,
+                        double *uxe) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     const double *P;
     int id, ty;
+    double sh = 0.0;
     if (e < nb) {
+        int j = find_overlay(beoff, nbr, e);
         P = bpoly;
-        id = 2 * e;
+        id = 2 * (bvoff[j] + (e - beoff[j]));
         ty = 0;
+        if (shift) sh = shift[j];
     } else {
         int j = find_overlay(eoff, K, e);
         P = cpoly;
         id = 2 * (coff[j] + (e - eoff[j]));
         ty = j + 1;
+        if (shift) sh = shift[j];
     }
     double a = P[id], b = P[id + 1], c = P[id + 2], d = P[id + 3];
     x0[e] = a;
     y0[e] = b;
     x1[e] = c;
     y1[e] = d;
-    if (a < c) {
-        es[e] = a;
-        ee[e] = c;
+    double ka = a + sh, kc = c + sh;
+    if (ka < kc) {
+        es[e] = ka;
+        ee[e] = kc;
     } else {
-        ee[e] = a;
-        es[e] = c;
+        ee[e] = ka;
+        es[e] = kc;
+    }
+    if (uxs) { // input x-extent for the bounding-box test
+        uxs[e] = a < c ? a : c;
+        uxe[e] = a < c ? c : a;
     }
     eyl[e] = b < d ? b : d;
     eyh[e] = b < d ? d : b;
@@ -364,51 +377,111 @@ uint64_t read_u64(Ctx &c, const uint64_t *d) {
     return h;
 }
 
-void ctx_set_polygons(Ctx &c, const double *bxy, int nb, const double *cxy, const int *csz, int K, bool device_ptrs) {
-    SC_REQUIRE(bxy && cxy && csz, SC_EINVAL, "null polygon pointer");
-    SC_REQUIRE(nb >= 3, SC_EINVAL, "base polygon needs at least 3 vertices");
-    SC_REQUIRE(K >= 1, SC_EINVAL, "at least one overlay polygon is required");
-    int64_t nct = 0;
-    for (int i = 0; i < K; ++i) {
-        SC_REQUIRE(csz[i] >= 3, SC_EINVAL, "overlay polygon " + std::to_string(i + 1) + " needs at least 3 vertices");
-        nct += csz[i];
-    }
-    SC_REQUIRE(2 * (int64_t(nb) + nct + K) < (int64_t(1) << 31), SC_ERANGE, "too many vertices for 32-bit ids");
-    c.nb = nb;
-    c.K = K;
-    c.nct = int(nct);
-    c.ne = nb + int(nct);
-    c.csz.assign(csz, csz + K);
-    c.coff.assign(K + 1, 0);
-    c.eoff.assign(K + 1, 0);
-    c.eoff[0] = nb;
-    c.maxcsz = 0;
-    for (int i = 0; i < K; ++i) {
-        c.coff[i + 1] = c.coff[i] + csz[i] + 1;
-        c.eoff[i + 1] = c.eoff[i] + csz[i];
-        if (csz[i] > c.maxcsz) c.maxcsz = csz[i];
-    }
+namespace {
+// lay out a set of rings (concatenated, no closing vertices) with closing vertices
+void layout_rings(Ctx &c, const double *src, int64_t nverts, const std::vector<int> &sizes,
+                  const std::vector<int> &voff, DBuf<int> &d_voff, std::vector<int> &edge_off, int edge_base,
+                  DBuf<int> &d_eoff, DBuf<double> &dst, bool device_ptrs) {
     hipStream_t s = c.stream;
-    c.bpoly.ensure(2 * size_t(nb + 1));
-    c.cpoly.ensure(2 * size_t(nct + K));
-    c.d_coff.ensure(K + 1);
-    c.d_eoff.ensure(K + 1);
-    SC_HIP(hipMemcpyAsync(c.d_coff.p, c.coff.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, s));
-    SC_HIP(hipMemcpyAsync(c.d_eoff.p, c.eoff.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, s));
-    hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    SC_HIP(hipMemcpyAsync(c.bpoly.p, bxy, sizeof(double) * 2 * size_t(nb), kind, s));
-    k_close_base<<<1, 1, 0, s>>>(c.bpoly.p, nb);
+    const int R = int(sizes.size());
+    d_voff.ensure(R + 1);
+    d_eoff.ensure(R + 1);
+    SC_HIP(hipMemcpyAsync(d_voff.p, voff.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice, s));
+    SC_HIP(hipMemcpyAsync(d_eoff.p, edge_off.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice, s));
+    dst.ensure(2 * size_t(nverts + R));
+    double *stage = reinterpret_cast<double *>(c.u64a.ensure(2 * size_t(nverts)));
+    SC_HIP(hipMemcpyAsync(stage, src, sizeof(double) * 2 * size_t(nverts),
+                          device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    k_layout_overlays<<<grid_for(nverts), BLK, 0, s>>>(stage, int(nverts), d_eoff.p, d_voff.p, R, edge_base, dst.p);
     SC_HIP(hipGetLastError());
-    // overlays: stage the concatenated input, then lay it out with closing vertices
-    double *stage = reinterpret_cast<double *>(c.u64a.ensure(2 * size_t(nct)));
-    SC_HIP(hipMemcpyAsync(stage, cxy, sizeof(double) * 2 * size_t(nct), kind, s));
-    k_layout_overlays<<<grid_for(nct), BLK, 0, s>>>(stage, int(nct), c.d_eoff.p, c.d_coff.p, K, nb, c.cpoly.p);
-    SC_HIP(hipGetLastError());
-    if (!device_ptrs) SC_HIP(hipStreamSynchronize(s)); // host buffers may be released by the caller
+    SC_HIP(hipStreamSynchronize(s)); // staging buffer is reused by the next call
+}
+
+void reset_derived(Ctx &c) {
     c.epc = c.nleaves = c.T = 0;
     c.n_lend = c.n_cov = c.n_cand = c.n_pairs = c.n_mend = 0;
     c.cur_k = 0;
     c.pair_off.clear();
+}
+
+void load(Ctx &c, const double *bxy, const int *bsz, int nbr, const double *cxy, const int *csz, int K,
+          bool paired, bool device_ptrs) {
+    SC_REQUIRE(bxy && cxy && csz && bsz, SC_EINVAL, "null polygon pointer");
+    SC_REQUIRE(K >= 1 && nbr >= 1, SC_EINVAL, "at least one base and one overlay polygon are required");
+    int64_t nb = 0, nct = 0;
+    for (int i = 0; i < nbr; ++i) {
+        SC_REQUIRE(bsz[i] >= 3, SC_EINVAL, "base polygon " + std::to_string(i) + " needs at least 3 vertices");
+        nb += bsz[i];
+    }
+    for (int i = 0; i < K; ++i) {
+        SC_REQUIRE(csz[i] >= 3, SC_EINVAL, "overlay polygon " + std::to_string(i + 1) + " needs at least 3 vertices");
+        nct += csz[i];
+    }
+    SC_REQUIRE(2 * (nb + nct + K + nbr) < (int64_t(1) << 31), SC_ERANGE, "too many vertices for 32-bit ids");
+    c.nb = int(nb);
+    c.nbr = nbr;
+    c.K = K;
+    c.paired = paired;
+    c.nct = int(nct);
+    c.ne = int(nb + nct);
+    c.bsz.assign(bsz, bsz + nbr);
+    c.bvoff.assign(nbr + 1, 0);
+    c.beoff.assign(nbr + 1, 0);
+    c.maxbsz = 0;
+    for (int i = 0; i < nbr; ++i) {
+        c.bvoff[i + 1] = c.bvoff[i] + bsz[i] + 1;
+        c.beoff[i + 1] = c.beoff[i] + bsz[i];
+        c.maxbsz = std::max(c.maxbsz, bsz[i]);
+    }
+    c.csz.assign(csz, csz + K);
+    c.coff.assign(K + 1, 0);
+    c.eoff.assign(K + 1, 0);
+    c.eoff[0] = c.nb;
+    c.maxcsz = 0;
+    for (int i = 0; i < K; ++i) {
+        c.coff[i + 1] = c.coff[i] + csz[i] + 1;
+        c.eoff[i + 1] = c.eoff[i] + csz[i];
+        c.maxcsz = std::max(c.maxcsz, csz[i]);
+    }
+    layout_rings(c, bxy, nb, c.bsz, c.bvoff, c.d_bvoff, c.beoff, 0, c.d_beoff, c.bpoly, device_ptrs);
+    layout_rings(c, cxy, nct, c.csz, c.coff, c.d_coff, c.eoff, c.nb, c.d_eoff, c.cpoly, device_ptrs);
+    if (paired) {
+        // per-pair key shift: pairs are laid side by side along x, each in a
+        // window wider than the span of all rings
+        double lo = 0, hi = 0;
+        {
+            std::vector<double> h(2 * size_t(nb + nct));
+            SC_HIP(hipMemcpyAsync(h.data(), bxy, sizeof(double) * 2 * size_t(nb),
+                                  device_ptrs ? hipMemcpyDeviceToHost : hipMemcpyHostToHost, c.stream));
+            SC_HIP(hipMemcpyAsync(h.data() + 2 * nb, cxy, sizeof(double) * 2 * size_t(nct),
+                                  device_ptrs ? hipMemcpyDeviceToHost : hipMemcpyHostToHost, c.stream));
+            SC_HIP(hipStreamSynchronize(c.stream));
+            lo = hi = h[0];
+            for (size_t i = 0; i < h.size(); i += 2) {
+                lo = std::min(lo, h[i]);
+                hi = std::max(hi, h[i]);
+            }
+        }
+        double span = (hi - lo) + 1.0;
+        std::vector<double> sh(static_cast<size_t>(K));
+        for (int k = 0; k < K; ++k) sh[k] = double(k) * span - lo;
+        c.d_shift.ensure(K);
+        SC_HIP(hipMemcpyAsync(c.d_shift.p, sh.data(), sizeof(double) * K, hipMemcpyHostToDevice, c.stream));
+        SC_HIP(hipStreamSynchronize(c.stream));
+    }
+    reset_derived(c);
+}
+} // namespace
+
+void ctx_set_polygons(Ctx &c, const double *bxy, int nb, const double *cxy, const int *csz, int K, bool device_ptrs) {
+    SC_REQUIRE(bxy && cxy && csz, SC_EINVAL, "null polygon pointer");
+    SC_REQUIRE(nb >= 3, SC_EINVAL, "base polygon needs at least 3 vertices");
+    load(c, bxy, &nb, 1, cxy, csz, K, false, device_ptrs);
+}
+
+void ctx_set_pairs(Ctx &c, const double *bxy, const int *bsz, const double *cxy, const int *csz, int P,
+                   bool device_ptrs) {
+    load(c, bxy, bsz, P, cxy, csz, P, true, device_ptrs);
 }
 
 void build_edges(Ctx &c) {
@@ -417,8 +490,14 @@ void build_edges(Ctx &c) {
     c.ex0.ensure(ne); c.ey0.ensure(ne); c.ex1.ensure(ne); c.ey1.ensure(ne);
     c.estart.ensure(ne); c.eend.ensure(ne); c.etype.ensure(ne); c.eid.ensure(ne);
     c.eymin.ensure(ne); c.eymax.ensure(ne);
-    k_edges<<<grid_for(ne), BLK, 0, c.stream>>>(c.bpoly, c.cpoly, c.nb, ne, c.d_eoff, c.d_coff, c.K, c.ex0, c.ey0,
-                                                c.ex1, c.ey1, c.estart, c.eend, c.eymin, c.eymax, c.etype, c.eid);
+    if (c.paired) {
+        c.uxs.ensure(ne);
+        c.uxe.ensure(ne);
+    }
+    k_edges<<<grid_for(ne), BLK, 0, c.stream>>>(c.bpoly, c.cpoly, c.nb, ne, c.d_eoff, c.d_coff, c.K, c.d_beoff,
+                                                c.d_bvoff, c.nbr, c.paired ? c.d_shift.p : nullptr, c.ex0, c.ey0,
+                                                c.ex1, c.ey1, c.estart, c.eend, c.eymin, c.eymax, c.etype, c.eid,
+                                                c.paired ? c.uxs.p : nullptr, c.paired ? c.uxe.p : nullptr);
     SC_HIP(hipGetLastError());
 }
 

@@ -58,6 +58,21 @@ class Clipper:
                                                as_ptr(cxy, C.c_double), as_ptr(sizes, C.c_int32), len(ovs)))
         return self
 
+    def set_pairs(self, bases, overlays):
+        """P independent pairs: pair k clips bases[k] against overlays[k]."""
+        bs = [np.ascontiguousarray(b, dtype=np.float64).reshape(-1, 2) for b in bases]
+        ovs = [np.ascontiguousarray(o, dtype=np.float64).reshape(-1, 2) for o in overlays]
+        if len(bs) != len(ovs):
+            raise ValueError("set_pairs: %d bases for %d overlays" % (len(bs), len(ovs)))
+        bsz = np.array([len(b) for b in bs], dtype=np.int32)
+        csz = np.array([len(o) for o in ovs], dtype=np.int32)
+        bxy = np.ascontiguousarray(np.concatenate(bs, axis=0))
+        cxy = np.ascontiguousarray(np.concatenate(ovs, axis=0))
+        self.nb, self.K, self.sizes = int(bsz.sum()), len(ovs), csz
+        self._check(lib().segclip_set_pairs(self._h, as_ptr(bxy, C.c_double), as_ptr(bsz, C.c_int32),
+                                            as_ptr(cxy, C.c_double), as_ptr(csz, C.c_int32), len(ovs)))
+        return self
+
     def build_tree(self):
         self._check(lib().segclip_build_tree(self._h))
         return self
@@ -80,6 +95,12 @@ class Clipper:
 
     def clip(self, k=1):
         self._check(lib().segclip_clip(self._h, int(k)))
+        return self
+
+    def clip_batch(self, k0=1, k1=None):
+        """Overlays (or pairs) k0..k1 as one batch of launches."""
+        k1 = self.K if k1 is None else k1
+        self._check(lib().segclip_clip_batch(self._h, int(k0), int(k1)))
         return self
 
     def reset(self):
@@ -199,3 +220,24 @@ class Clipper:
                                          need[1], need))
         xy = xy[:need[1]]
         return [xy[off[i]:off[i + 1]] for i in range(len(off) - 1)]
+
+    def batch_ring_offsets(self):
+        """Per job of the last batch: ring offsets (jobs+1) into rings()."""
+        L = lib()
+        n = self._need(L.segclip_copy_batch_rings, None, 0)
+        off = np.zeros(n, np.int32)
+        self._check(L.segclip_copy_batch_rings(self._h, as_ptr(off, C.c_int32), n, None))
+        return off
+
+    def batch_status(self):
+        L = lib()
+        n = self._need(L.segclip_copy_batch_status, None, 0)
+        st = np.zeros(n, np.int32)
+        self._check(L.segclip_copy_batch_status(self._h, as_ptr(st, C.c_int32), n, None))
+        return st
+
+    def batch_rings(self):
+        """Output rings of every job of the last batch: a list (per job) of lists of (n,2) arrays."""
+        r = self.rings()
+        off = self.batch_ring_offsets()
+        return [r[off[j]:off[j + 1]] for j in range(len(off) - 1)]

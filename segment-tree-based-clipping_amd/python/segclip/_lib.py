@@ -24,6 +24,8 @@ SYMBOLS = [
     "segclip_stable_sort_by_key_i32", "segclip_stable_sort_by_key_i64", "segclip_exclusive_scan_i32",
     "segclip_exclusive_scan_i64", "segclip_synth_star", "segclip_synth_harmonic", "segclip_synth_convex",
     "segclip_synth_uniform", "segclip_count_vertical", "segclip_set_timing", "segclip_get_stage_times",
+    "segclip_set_pairs", "segclip_set_pairs_device", "segclip_clip_batch", "segclip_copy_batch_rings",
+    "segclip_copy_batch_status",
 ]
 
 
@@ -67,6 +69,11 @@ def lib():
         "segclip_version": (C.c_char_p, []),
         "segclip_set_polygons": (C.c_int, [_vp, _dp, C.c_int, _dp, _ip, C.c_int]),
         "segclip_set_polygons_device": (C.c_int, [_vp, _vp, C.c_int, _vp, _ip, C.c_int]),
+        "segclip_set_pairs": (C.c_int, [_vp, _dp, _ip, _dp, _ip, C.c_int]),
+        "segclip_set_pairs_device": (C.c_int, [_vp, _vp, _ip, _vp, _ip, C.c_int]),
+        "segclip_clip_batch": (C.c_int, [_vp, C.c_int, C.c_int]),
+        "segclip_copy_batch_rings": (C.c_int, [_vp, _ip, C.c_int64, _i64p]),
+        "segclip_copy_batch_status": (C.c_int, [_vp, _ip, C.c_int64, _i64p]),
         "segclip_intersections": (C.c_int, [_vp, C.c_int]),
         "segclip_clip": (C.c_int, [_vp, C.c_int]),
         "segclip_get_counts": (C.c_int, [_vp, C.POINTER(Counts)]),

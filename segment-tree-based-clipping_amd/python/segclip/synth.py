@@ -78,3 +78,48 @@ def one_to_many(nb=100000, count=4096, seed=3000):
         cy = lo[1] + u[6 * i + 3] * (hi[1] - lo[1])
         ovs.append(star(max(n, 3), seed + 1 + i, cx, cy, 0.5 * R, R, 0.0))
     return base, ovs
+
+
+def _normal_pairs(u):
+    """Box-Muller on pairs of uniforms in (0,1]: returns len(u)//2 standard normals."""
+    u1 = np.maximum(u[0::2], 1e-300)
+    u2 = u[1::2]
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+def small_pairs(count=16384, seed=4000):
+    """Pairs with 4..64 vertices per ring: even pairs convex (sorted angles,
+    radius 1), odd pairs star (radii U[0.4,1.0]); the overlay is rotated by
+    U[0,2pi) and displaced by N(0, 0.3^2) per coordinate."""
+    u = uniform(8 * count, seed)
+    bases, ovs = [], []
+    for i in range(count):
+        nb = 4 + min(60, int(u[8 * i] * 61))
+        nc = 4 + min(60, int(u[8 * i + 1] * 61))
+        rot = 2.0 * math.pi * u[8 * i + 2]
+        d = _normal_pairs(u[8 * i + 3:8 * i + 7])
+        dx, dy = 0.3 * d[0], 0.3 * d[1]
+        sb, sc = seed + 1 + 2 * i, seed + 2 + 2 * i
+        if i % 2 == 0:
+            bases.append(convex(nb, sb, 0.0, 0.0, 1.0, 0.0))
+            ovs.append(convex(nc, sc, dx, dy, 1.0, rot))
+        else:
+            bases.append(star(nb, sb, 0.0, 0.0, 0.4, 1.0, 0.0))
+            ovs.append(star(nc, sc, dx, dy, 0.4, 1.0, rot))
+    return bases, ovs
+
+
+def large_pairs(count=256, seed=5000):
+    """Pairs of harmonic stars with log-uniform 10k..200k vertices; the overlay
+    is offset by U[-0.3,0.3]^2 and rotated by U[0,2pi)."""
+    u = uniform(5 * count, seed)
+    bases, ovs = [], []
+    lo, hi = math.log(10000), math.log(200000)
+    for i in range(count):
+        nb = int(round(math.exp(lo + u[5 * i] * (hi - lo))))
+        nc = int(round(math.exp(lo + u[5 * i + 1] * (hi - lo))))
+        dx, dy = -0.3 + 0.6 * u[5 * i + 2], -0.3 + 0.6 * u[5 * i + 3]
+        rot = 2.0 * math.pi * u[5 * i + 4]
+        bases.append(harmonic(nb, seed + 1 + 2 * i, 0.0, 0.0, 1.0, 0.0))
+        ovs.append(harmonic(nc, seed + 2 + 2 * i, dx, dy, 1.0, rot))
+    return bases, ovs

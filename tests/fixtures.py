@@ -54,3 +54,27 @@ def degenerate_cases():
     q = np.array([[2.0, 1.0], [3.0, -1.0], [4.0, 1.0], [3.0, 3.0]])
     cases.append(("vertex_on_edge", square(0.0, 0.0, 3.0, 2.0), [q], None))
     return cases
+
+
+def rings_equal(g, o):
+    """None when the two ring lists are bit-identical, else what differs."""
+    if len(g) != len(o):
+        return "ring count %d != %d" % (len(g), len(o))
+    for i, (a, b) in enumerate(zip(g, o)):
+        if a.shape != b.shape:
+            return "ring %d: %d vs %d vertices" % (i, len(a), len(b))
+        if not np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64)):
+            return "ring %d: coordinates differ (max %g)" % (i, np.max(np.abs(a - b)))
+    return None
+
+
+def mixed_overlays(seed=40):
+    """Overlays of a unit-scale base: crossing, inside, outside, enclosing, tiny."""
+    out = []
+    for i, (n, cx, cy, r0, r1) in enumerate([(50, 0.2, 0.1, 0.1, 0.4), (300, -0.5, 0.2, 0.1, 0.4),
+                                               (1000, 0.0, 0.0, 0.6, 1.4), (12, 0.9, -0.3, 0.1, 0.4),
+                                               (700, -0.2, -0.8, 0.1, 0.4), (40, 0.0, 0.0, 0.05, 0.1),
+                                               (25, 6.0, 6.0, 0.5, 1.0), (200, 0.0, 0.0, 3.0, 4.0),
+                                               (8, 0.3, 0.3, 0.005, 0.01), (5000, 0.1, -0.1, 0.3, 1.2)]):
+        out.append(S.star(n, seed + i, cx, cy, r0, r1))
+    return out

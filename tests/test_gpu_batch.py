@@ -1,0 +1,118 @@
+"""GPU parity of batched clipping (rows A16, A18): a batch of jobs run in the
+same launches gives, job by job, exactly the rings of clipping each job alone,
+and those equal the oracle's.
+
+One-to-many batches share the base ring (the reference's one-to-many mode);
+paired batches give every pair its own base ring.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import segclip.synth as S
+from fixtures import degenerate_cases, mixed_overlays, rings_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def test_one_to_many_batch_equals_per_overlay(gpu):
+    import segclip
+    b = S.harmonic(4000, 31)
+    ovs = mixed_overlays()
+    print("[gpu-batch] one-to-many", len(b), [len(o) for o in ovs], flush=True)
+    tr = O.Tree(b, ovs)
+    cl = segclip.Clipper().set_polygons(b, ovs)
+    single, status = [], []
+    for k in range(1, len(ovs) + 1):
+        cl.clip(k)
+        single.append(cl.rings())
+        status.append(cl.counts()["undefined"])
+        r = tr.clip(k, canonical=True)
+        err = rings_equal(single[-1], r.rings)
+        assert err is None, "overlay %d vs oracle: %s" % (k, err)
+    cl.clip_batch(1, len(ovs))
+    br = cl.batch_rings()
+    st = cl.batch_status()
+    assert len(br) == len(ovs)
+    for k in range(len(ovs)):
+        err = rings_equal(br[k], single[k])
+        assert err is None, "overlay %d batch vs single: %s" % (k + 1, err)
+        assert (st[k] & 1) == (status[k] & 1)
+    # a sub-range
+    cl.clip_batch(3, 7)
+    br = cl.batch_rings()
+    for k in range(3, 8):
+        err = rings_equal(br[k - 3], single[k - 1])
+        assert err is None, "overlay %d in batch 3..7: %s" % (k, err)
+
+
+def _pairs():
+    bases, ovs, names = [], [], []
+    for name, b, o, _ in degenerate_cases():
+        bases.append(b)
+        ovs.append(o[0])
+        names.append(name)
+    sb, so = S.small_pairs(24, 4100)
+    bases += sb
+    ovs += so
+    names += ["small%d" % i for i in range(len(sb))]
+    bases.append(S.harmonic(3000, 77))
+    ovs.append(S.star(2000, 78, 0.1, 0.05, 0.4, 0.9))
+    names.append("harmonic3k_star2k")
+    return bases, ovs, names
+
+
+def test_pairs_batch_equals_each_pair_alone(gpu):
+    import segclip
+    bases, ovs, names = _pairs()
+    print("[gpu-batch] pairs", len(bases), flush=True)
+    cl = segclip.Clipper().set_pairs(bases, ovs)
+    cl.clip_batch()
+    br = cl.batch_rings()
+    st = cl.batch_status()
+    assert len(br) == len(bases)
+    for i, (b, o, name) in enumerate(zip(bases, ovs, names)):
+        one = segclip.Clipper().set_polygons(b, [o]).clip(1)
+        err = rings_equal(br[i], one.rings())
+        assert err is None, "%s: batch vs alone: %s" % (name, err)
+        assert (st[i] & 1) == (one.counts()["undefined"] & 1)
+        pb, pc, pt = cl.pairs(i + 1)
+        qb, qc, qt = one.pairs(1)
+        assert np.array_equal(pb, qb) and np.array_equal(pc, qc), name
+        r = O.Tree(b, [o]).clip(1, canonical=True)
+        err = rings_equal(br[i], r.rings)
+        assert err is None, "%s: batch vs oracle: %s" % (name, err)
+
+
+def test_small_pairs_batch_vs_oracle(gpu):
+    import segclip
+    bases, ovs = S.small_pairs(512, 4000)
+    print("[gpu-batch] small pairs", len(bases), flush=True)
+    cl = segclip.Clipper().set_pairs(bases, ovs)
+    cl.clip_batch()
+    br = cl.batch_rings()
+    for i, (b, o) in enumerate(zip(bases, ovs)):
+        r = O.Tree(b, [o]).clip(1, canonical=True)
+        err = rings_equal(br[i], r.rings)
+        assert err is None, "pair %d: %s" % (i, err)
+
+
+def test_one_to_many_scaled_batch(gpu):
+    import segclip
+    b, ovs = S.one_to_many(20000, 128, 3000)
+    print("[gpu-batch] one-to-many scaled", len(b), len(ovs), flush=True)
+    cl = segclip.Clipper().set_polygons(b, ovs)
+    single = []
+    for k in range(1, len(ovs) + 1):
+        cl.clip(k)
+        single.append(cl.rings())
+    cl.clip_batch()
+    br = cl.batch_rings()
+    for k in range(len(ovs)):
+        err = rings_equal(br[k], single[k])
+        assert err is None, "overlay %d: %s" % (k + 1, err)
+    tr = O.Tree(b, ovs)
+    for k in (1, 2, 3, 64, 128):
+        r = tr.clip(k, canonical=True)
+        err = rings_equal(br[k - 1], r.rings)
+        assert err is None, "overlay %d vs oracle: %s" % (k, err)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
-from fixtures import degenerate_cases, workloads_small
+from fixtures import degenerate_cases, rings_equal as _rings_equal, workloads_small
 
 pytestmark = pytest.mark.gpu
 
@@ -20,17 +20,6 @@ def _cases():
 
 
 CASES = _cases()
-
-
-def _rings_equal(g, o):
-    if len(g) != len(o):
-        return "ring count %d != %d" % (len(g), len(o))
-    for i, (a, b) in enumerate(zip(g, o)):
-        if a.shape != b.shape:
-            return "ring %d: %d vs %d vertices" % (i, len(a), len(b))
-        if not np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64)):
-            return "ring %d: coordinates differ (max %g)" % (i, np.max(np.abs(a - b)))
-    return None
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
