@@ -97,6 +97,26 @@ int segclip_clip(segclip_ctx *ctx, int k);
  * Results equal those of segclip_clip(k) for each k; the output rings of job
  * k are rings [job_ring_off[k-k0], job_ring_off[k-k0+1]) of segclip_copy_rings. */
 int segclip_clip_batch(segclip_ctx *ctx, int k0, int k1);
+/* Differentiable area / IoU (no counterpart in the reference, which only
+ * writes the rings).  With provenance on, the intersection stage records
+ * which input vertex or intersecting pair every augmented slot came from.
+ * segclip_batch_area gives, per job of the last batch, 4*jobs doubles:
+ * intersection areas, base ring areas, overlay ring areas, IoU (shoelace,
+ * FP64).  The backward fills grad_b (2*nb doubles, the layout of the base
+ * input) and grad_c (2*sum(ov_sizes), the layout of the overlay input) with
+ * d/d(vertex) of sum_j g_area[j]*area[j] + g_iou[j]*iou[j], the output
+ * combinatorics held fixed; g_area or g_iou may be null (zero).  Device
+ * pointers, on the context's device; the call returns after its work is done. */
+int segclip_set_provenance(segclip_ctx *ctx, int on);
+int segclip_batch_area(segclip_ctx *ctx, double *out4, int64_t cap, int64_t *need);
+int segclip_batch_area_device(segclip_ctx *ctx, double *out4);
+int segclip_batch_area_backward_device(segclip_ctx *ctx, const double *g_area, const double *g_iou, double *grad_b,
+                                       double *grad_c);
+/* Free every device buffer but the resident input rings (the next call
+ * rebuilds what it needs).  segclip_memory reports the device bytes held by
+ * the library (all contexts) and their peak since the last reset. */
+int segclip_release(segclip_ctx *ctx);
+int segclip_memory(int64_t *current, int64_t *peak, int reset_peak);
 /* per job of the last batch (or the single job of segclip_clip): ring
  * offsets (jobs+1 entries) and status bits (the 'undefined' bits per job) */
 int segclip_copy_batch_rings(segclip_ctx *ctx, int32_t *job_ring_off, int64_t cap, int64_t *need);

@@ -25,6 +25,32 @@ float Ctx::elapsed(int a, int b) {
     SC_HIP(hipEventElapsedTime(&ms, ev[a], ev[b]));
     return ms;
 }
+
+void release_derived(Ctx &c) {
+    SC_HIP(hipStreamSynchronize(c.stream));
+    for (DBuf<double> *b : {&c.ex0, &c.ey0, &c.ex1, &c.ey1, &c.estart, &c.eend, &c.uxs, &c.uxe, &c.eymin, &c.eymax,
+                            &c.ea, &c.niv, &c.p_x, &c.p_y, &c.aug_bx, &c.aug_by, &c.aug_cx, &c.aug_cy})
+        b->release();
+    for (DBuf<int> *b : {&c.etype, &c.eid, &c.after_leaf, &c.before_leaf, &c.p_j, &c.p_b, &c.p_c, &c.p_type,
+                         &c.p_qa, &c.p_qb, &c.ordB, &c.ordC, &c.nd_b, &c.nd_c, &c.all_b, &c.all_c, &c.nbr_b,
+                         &c.nbr_c, &c.av_b, &c.av_c, &c.lab_b, &c.lab_c, &c.slot_b, &c.slot_c, &c.sv_b, &c.sp_b,
+                         &c.sv_c, &c.sp_c})
+        b->release();
+    for (DBuf<uint32_t> *b : {&c.lend_off, &c.lend_e, &c.mend_off, &c.mend_e, &c.cov_off, &c.cov_e, &c.cov_split,
+                              &c.d_pair_off, &c.u32a, &c.u32b, &c.u32c, &c.u32d})
+        b->release();
+    for (DBuf<uint64_t> *b : {&c.lend_key, &c.cand, &c.pairs, &c.k64a, &c.k64b, &c.u64a, &c.u64b}) b->release();
+    c.rs.tile_hist.release();
+    c.rs.global_hist.release();
+    c.rs.keys_alt.release();
+    c.rs.vals_alt.release();
+    release_foster_scratch();
+    c.epc = c.nleaves = c.T = 0;
+    c.n_lend = c.n_cov = c.n_cand = c.n_pairs = c.n_mend = 0;
+    c.cur_k = 0;
+    c.I = 0;
+    c.pair_off.clear();
+}
 } // namespace sc
 
 namespace {
@@ -151,6 +177,44 @@ int segclip_label_traverse(segclip_ctx *h) {
 }
 int segclip_clip(segclip_ctx *h, int k) {
     return guarded(h, [&](sc::Ctx &x) { sc::clip_all(x, k); });
+}
+int segclip_set_provenance(segclip_ctx *h, int on) {
+    return guarded(h, [&](sc::Ctx &x) { x.prov = on != 0; });
+}
+int segclip_release(segclip_ctx *h) {
+    return guarded(h, [&](sc::Ctx &x) { sc::release_derived(x); });
+}
+int segclip_memory(int64_t *current, int64_t *peak, int reset_peak) {
+    if (current) *current = sc::dev_bytes().load();
+    if (peak) *peak = sc::dev_peak().load();
+    if (reset_peak) sc::dev_peak().store(sc::dev_bytes().load());
+    return SEGCLIP_OK;
+}
+int segclip_batch_area(segclip_ctx *h, double *out4, int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    const int J = h->c.jobs.J;
+    if (J < 1) return SEGCLIP_ESTATE;
+    int rc = need_space(4 * int64_t(J), cap, need);
+    if (rc || !out4) return rc;
+    return guarded(h, [&](sc::Ctx &x) {
+        sc::DBuf<double> d;
+        d.ensure(4 * size_t(J));
+        sc::batch_area(x, d.p);
+        d2h(x, out4, d.p, 4 * int64_t(J));
+        SC_HIP(hipStreamSynchronize(x.stream));
+    });
+}
+int segclip_batch_area_device(segclip_ctx *h, double *out4) {
+    if (!h || !out4) return SEGCLIP_EINVAL;
+    return guarded(h, [&](sc::Ctx &x) {
+        sc::batch_area(x, out4);
+        SC_HIP(hipStreamSynchronize(x.stream));
+    });
+}
+int segclip_batch_area_backward_device(segclip_ctx *h, const double *g_area, const double *g_iou, double *grad_b,
+                                       double *grad_c) {
+    if (!h || !grad_b || !grad_c) return SEGCLIP_EINVAL;
+    return guarded(h, [&](sc::Ctx &x) { sc::batch_area_backward(x, g_area, g_iou, grad_b, grad_c); });
 }
 int segclip_clip_batch(segclip_ctx *h, int k0, int k1) {
     return guarded(h, [&](sc::Ctx &x) { sc::clip_jobs(x, k0, k1); });

@@ -1,6 +1,7 @@
 // sc_common.hpp -- shared host/device helpers of libsegclip (gfx950, wave64).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -39,6 +40,22 @@ enum Status : int {
         if (!(cond)) throw ::sc::Error((code), (msg)); \
     } while (0)
 
+// Device bytes held by the library's buffers (all contexts) and their peak.
+inline std::atomic<int64_t> &dev_bytes() {
+    static std::atomic<int64_t> v{0};
+    return v;
+}
+inline std::atomic<int64_t> &dev_peak() {
+    static std::atomic<int64_t> v{0};
+    return v;
+}
+inline void dev_account(int64_t delta) {
+    int64_t now = dev_bytes().fetch_add(delta) + delta;
+    int64_t pk = dev_peak().load();
+    while (now > pk && !dev_peak().compare_exchange_weak(pk, now)) {
+    }
+}
+
 // Growable device buffer; contents are not preserved across growth.
 template <class T> struct DBuf {
     T *p = nullptr;
@@ -48,7 +65,10 @@ template <class T> struct DBuf {
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            dev_account(-int64_t(cap * sizeof(T)));
+        }
         p = nullptr;
         cap = 0;
     }
@@ -64,6 +84,7 @@ template <class T> struct DBuf {
                 throw Error(SC_ENOMEM, "hipMalloc of " + std::to_string(want * sizeof(T)) + " bytes failed");
             }
             cap = want;
+            dev_account(int64_t(want * sizeof(T)));
         }
         return p;
     }

@@ -18,7 +18,8 @@ struct JobSet {
     int k0 = 1, J = 0;
     std::vector<int> p0, bvoff, beoff, nb, nc, cvoff, JB, JC, AB, AC, flags;
     std::vector<int> ring_off; // J+1: rings of job j are [ring_off[j], ring_off[j+1]) of the output
-    DBuf<int> d_p0, d_bvoff, d_beoff, d_nb, d_nc, d_cvoff, d_JB, d_JC, d_AB, d_AC;
+    std::vector<int> cin;      // first input vertex (no closing vertices) of each job's overlay
+    DBuf<int> d_p0, d_bvoff, d_beoff, d_nb, d_nc, d_cvoff, d_JB, d_JC, d_AB, d_AC, d_cin, d_ring;
 };
 
 struct Ctx {
@@ -91,6 +92,8 @@ struct Ctx {
     DBuf<double> aug_bx, aug_by, aug_cx, aug_cy;
     DBuf<int> nbr_b, nbr_c, av_b, av_c, lab_b, lab_c;
     DBuf<int> slot_b, slot_c;            // per pair: augmented slot on each ring
+    bool prov = false;                   // record slot provenance (for the area gradient)
+    DBuf<int> sv_b, sp_b, sv_c, sp_c;    // per slot: source vertex (local) or pair index, else -1
     DBuf<int> flags;                     // device status words
     // labelling / traversal
     DBuf<int> nx_b, pv_b, nx_c, pv_c;    // ring links over augmented slots (+ appended vertices)
@@ -142,6 +145,13 @@ void post_overlay(Ctx &c, int k);      // A11-A15 for overlay k
 void label_and_traverse(Ctx &c);       // A16-A17 for the jobs of the last post_jobs
 void clip_all(Ctx &c, int k);          // full pipeline for one overlay
 void clip_jobs(Ctx &c, int k0, int k1); // full pipeline for overlays k0..k1 as one batch
+// A19: per job of the last batch, areas (intersection, base ring, overlay ring)
+// and IoU into out4 (device, 4*J doubles); gradient of sum(g_area*area +
+// g_iou*iou) w.r.t. every input vertex into grad_b (2*nb) and grad_c (2*nct)
+void batch_area(Ctx &c, double *out4);
+void batch_area_backward(Ctx &c, const double *g_area, const double *g_iou, double *grad_b, double *grad_c);
+void release_derived(Ctx &c);          // free everything but the resident input rings
+void release_foster_scratch();
 
 uint32_t read_u32(Ctx &c, const uint32_t *d);
 uint64_t read_u64(Ctx &c, const uint64_t *d);

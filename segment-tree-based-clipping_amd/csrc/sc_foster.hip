@@ -1041,6 +1041,23 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, std::vector<uint
 
 } // namespace
 
+void release_foster_scratch() {
+    FosterScratch &f = fscratch();
+    for (DBuf<uint32_t> *b : {&f.crossP, &f.crossQ, &f.ccP, &f.ccQ, &f.succ, &f.val, &f.val2, &f.jmp, &f.jmp2,
+                              &f.inode, &f.dist, &f.dist2, &f.tot, &f.isstart, &f.ringidx, &f.head, &f.piece_at,
+                              &f.jv, &f.jr, &f.keep, &f.mark, &f.kscan, &f.rcnt, &f.nonempty})
+        b->release();
+    for (DBuf<int> *b : {&f.enexP, &f.enexQ, &f.cposP, &f.cposQ, &f.ppos, &f.plen, &f.pdir, &f.pj, &f.jst, &f.st,
+                         &f.osrc, &f.off, &f.rstart, &f.ringid, &f.sres, &f.sroff, &f.sosrc, &f.nx, &f.pv, &f.nb,
+                         &f.lab, &f.enex, &f.vsrc, &f.cand})
+        b->release();
+    for (DBuf<double> *b : {&f.ox, &f.oy, &f.vx, &f.vy, &f.sox, &f.soy}) b->release();
+    f.splitP.release();
+    f.splitQ.release();
+    f.inter.release();
+    f.sjobs.release();
+}
+
 void label_and_traverse(Ctx &c) {
     JobSet &js = c.jobs;
     SC_REQUIRE(js.J >= 1 && int(js.AB.size()) == js.J + 1, SC_ESTATE, "segclip_intersections must run first");

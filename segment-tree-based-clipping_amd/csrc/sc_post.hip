@@ -146,7 +146,7 @@ __global__ void k_job_sizes(JobArgs ja, const int *__restrict__ nd_b, const int 
     nca[j] = ja.nc[j] + (nd_c[ja.JC[j + 1]] - nd_c[ja.JC[j]]);
 }
 
-__global__ void k_init_aug(int64_t n, double *ax, double *ay, int *av, int *nbr, int *lab) {
+__global__ void k_init_aug(int64_t n, double *ax, double *ay, int *av, int *nbr, int *lab, int *sv, int *sp) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     ax[i] = 0.0;
@@ -154,13 +154,17 @@ __global__ void k_init_aug(int64_t n, double *ax, double *ay, int *av, int *nbr,
     av[i] = -100;
     nbr[i] = -100;
     lab[i] = 0;
+    if (sv) {
+        sv[i] = -1;
+        sp[i] = -1;
+    }
 }
 
 // source vertices: count slot i of job j is vertex i - JX[j] (the last slot of a job is padding)
 __global__ void k_aug_sources(int J, const int *__restrict__ JX, const int *__restrict__ nverts,
                               const int *__restrict__ voff, const int *__restrict__ AX,
                               const double *__restrict__ poly, const int *__restrict__ nd_scan, double *ax,
-                              double *ay) {
+                              double *ay, int *sv) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= JX[J]) return;
     int j = seg_of(JX, J, i);
@@ -170,6 +174,7 @@ __global__ void k_aug_sources(int J, const int *__restrict__ JX, const int *__re
     int64_t g = 2 * int64_t(voff[j] + v);
     ax[pos] = poly[g];
     ay[pos] = poly[g + 1];
+    if (sv) sv[pos] = v;
 }
 
 // one thread per edge with intersections: a leading degenerate intersection
@@ -180,7 +185,7 @@ __global__ void k_aug_segments(const int *__restrict__ edge_of, const int *__res
                                const int *__restrict__ all_scan, const int *__restrict__ nd_scan,
                                const int *__restrict__ pt, const double *__restrict__ px,
                                const double *__restrict__ py, const int *__restrict__ q, double *ax, double *ay,
-                               int *av, int *slot, int *flags) {
+                               int *av, int *slot, int *flags, int *sp) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= I) return;
     int p0 = ord[i];
@@ -213,6 +218,7 @@ __global__ void k_aug_segments(const int *__restrict__ edge_of, const int *__res
         ay[sl] = py[p];
         av[sl] = q[p];
         slot[p] = sl;
+        if (sp) sp[sl] = p;
     }
     if (k0 + nd != t - s) atomicOr(&flags[j], UB_UNASSIGNED);
 }
@@ -311,6 +317,7 @@ void post_jobs(Ctx &c, int k0, int k1) {
     js.cvoff.resize(J);
     js.JB.assign(J + 1, 0);
     js.JC.assign(J + 1, 0);
+    js.cin.resize(J + 1);
     const int64_t P0 = c.pair_off[k0 - 1];
     for (int j = 0; j < J; ++j) {
         int k = k0 + j, br = job_base(c, k);
@@ -320,6 +327,7 @@ void post_jobs(Ctx &c, int k0, int k1) {
         js.nb[j] = c.bsz[br];
         js.nc[j] = c.csz[k - 1];
         js.cvoff[j] = c.coff[k - 1];
+        js.cin[j] = c.eoff[k - 1] - c.nb;
         js.JB[j + 1] = js.JB[j] + js.nb[j] + 1;
         js.JC[j + 1] = js.JC[j] + js.nc[j] + 1;
     }
@@ -336,6 +344,8 @@ void post_jobs(Ctx &c, int k0, int k1) {
     ja.cvoff = upload(c, js.d_cvoff, js.cvoff);
     ja.JB = upload(c, js.d_JB, js.JB);
     ja.JC = upload(c, js.d_JC, js.JC);
+    js.cin[J] = c.eoff[k1] - c.nb;
+    upload(c, js.d_cin, js.cin);
     const int nJB = js.JB[J], nJC = js.JC[J];
     const int Ia = I > 0 ? I : 1;
     c.p_j.ensure(Ia); c.p_b.ensure(Ia); c.p_c.ensure(Ia); c.p_type.ensure(Ia); c.p_x.ensure(Ia); c.p_y.ensure(Ia);
@@ -414,19 +424,28 @@ void post_jobs(Ctx &c, int k0, int k1) {
     const int64_t nbaA = nba_tot > 0 ? nba_tot : 1, ncaA = nca_tot > 0 ? nca_tot : 1;
     c.aug_bx.ensure(nbaA); c.aug_by.ensure(nbaA); c.av_b.ensure(nbaA); c.nbr_b.ensure(nbaA); c.lab_b.ensure(nbaA);
     c.aug_cx.ensure(ncaA); c.aug_cy.ensure(ncaA); c.av_c.ensure(ncaA); c.nbr_c.ensure(ncaA); c.lab_c.ensure(ncaA);
-    k_init_aug<<<grid_for(nba_tot), BLK, 0, s>>>(nba_tot, c.aug_bx, c.aug_by, c.av_b, c.nbr_b, c.lab_b);
-    k_init_aug<<<grid_for(nca_tot), BLK, 0, s>>>(nca_tot, c.aug_cx, c.aug_cy, c.av_c, c.nbr_c, c.lab_c);
+    int *svb = nullptr, *spb = nullptr, *svc = nullptr, *spc = nullptr;
+    if (c.prov) {
+        svb = c.sv_b.ensure(nbaA);
+        spb = c.sp_b.ensure(nbaA);
+        svc = c.sv_c.ensure(ncaA);
+        spc = c.sp_c.ensure(ncaA);
+    }
+    k_init_aug<<<grid_for(nba_tot), BLK, 0, s>>>(nba_tot, c.aug_bx, c.aug_by, c.av_b, c.nbr_b, c.lab_b, svb, spb);
+    k_init_aug<<<grid_for(nca_tot), BLK, 0, s>>>(nca_tot, c.aug_cx, c.aug_cy, c.av_c, c.nbr_c, c.lab_c, svc, spc);
     SC_HIP(hipGetLastError());
-    k_aug_sources<<<grid_for(nJB), BLK, 0, s>>>(J, ja.JB, ja.nb, ja.bvoff, nba, c.bpoly, nd_b, c.aug_bx, c.aug_by);
-    k_aug_sources<<<grid_for(nJC), BLK, 0, s>>>(J, ja.JC, ja.nc, ja.cvoff, nca, c.cpoly, nd_c, c.aug_cx, c.aug_cy);
+    k_aug_sources<<<grid_for(nJB), BLK, 0, s>>>(J, ja.JB, ja.nb, ja.bvoff, nba, c.bpoly, nd_b, c.aug_bx, c.aug_by,
+                                                svb);
+    k_aug_sources<<<grid_for(nJC), BLK, 0, s>>>(J, ja.JC, ja.nc, ja.cvoff, nca, c.cpoly, nd_c, c.aug_cx, c.aug_cy,
+                                                svc);
     SC_HIP(hipGetLastError());
     if (I > 0) {
         k_aug_segments<true><<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, I, c.ordB, ja.JB, nba, all_b, nd_b, c.p_type,
                                                          c.p_x, c.p_y, c.p_qa, c.aug_bx, c.aug_by, c.av_b, c.slot_b,
-                                                         flags);
+                                                         flags, spb);
         k_aug_segments<false><<<grid_for(I), BLK, 0, s>>>(c.p_c, c.p_j, I, c.ordC, ja.JC, nca, all_c, nd_c,
                                                           c.p_type, c.p_x, c.p_y, c.p_qb, c.aug_cx, c.aug_cy, c.av_c,
-                                                          c.slot_c, flags);
+                                                          c.slot_c, flags, spc);
         SC_HIP(hipGetLastError());
         k_link<<<grid_for(I), BLK, 0, s>>>(c.slot_b, c.slot_c, c.p_j, I, nba, nca, c.nbr_b, c.nbr_c, flags);
         SC_HIP(hipGetLastError());

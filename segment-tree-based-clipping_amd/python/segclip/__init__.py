@@ -12,7 +12,14 @@ import numpy as np
 from ._lib import (Counts, SegclipError, as_ptr, lib, OK, ESPACE)  # noqa: F401
 from . import synth  # noqa: F401
 
-__all__ = ["Clipper", "SegclipError", "synth", "lib"]
+__all__ = ["Clipper", "SegclipError", "synth", "lib", "memory"]
+
+
+def memory(reset_peak=False):
+    """(current, peak) device bytes held by libsegclip buffers, all contexts."""
+    cur, pk = C.c_int64(0), C.c_int64(0)
+    lib().segclip_memory(C.byref(cur), C.byref(pk), 1 if reset_peak else 0)
+    return cur.value, pk.value
 
 
 class Clipper:
@@ -72,6 +79,40 @@ class Clipper:
         self._check(lib().segclip_set_pairs(self._h, as_ptr(bxy, C.c_double), as_ptr(bsz, C.c_int32),
                                             as_ptr(cxy, C.c_double), as_ptr(csz, C.c_int32), len(ovs)))
         return self
+
+    def set_pairs_device(self, base_ptr, base_sizes, ov_ptr, ov_sizes):
+        """Pairs already in device memory (pointers to x,y doubles)."""
+        bsz = np.ascontiguousarray(base_sizes, dtype=np.int32)
+        csz = np.ascontiguousarray(ov_sizes, dtype=np.int32)
+        self.nb, self.K, self.sizes = int(bsz.sum()), len(csz), csz
+        self._bsz = bsz
+        self._check(lib().segclip_set_pairs_device(self._h, C.c_void_p(base_ptr), as_ptr(bsz, C.c_int32),
+                                                   C.c_void_p(ov_ptr), as_ptr(csz, C.c_int32), len(csz)))
+        return self
+
+    def set_provenance(self, on=True):
+        self._check(lib().segclip_set_provenance(self._h, 1 if on else 0))
+        return self
+
+    def release(self):
+        self._check(lib().segclip_release(self._h))
+        return self
+
+    def batch_area(self):
+        """Per job of the last batch: (intersection area, base area, overlay area, IoU) arrays."""
+        L = lib()
+        n = self._need(L.segclip_batch_area, None, 0)
+        out = np.zeros(n, np.float64)
+        self._check(L.segclip_batch_area(self._h, as_ptr(out, C.c_double), n, None))
+        return out.reshape(4, -1)
+
+    def batch_area_device(self, out_ptr):
+        self._check(lib().segclip_batch_area_device(self._h, C.c_void_p(out_ptr)))
+
+    def batch_area_backward_device(self, g_area_ptr, g_iou_ptr, grad_b_ptr, grad_c_ptr):
+        self._check(lib().segclip_batch_area_backward_device(
+            self._h, C.c_void_p(g_area_ptr) if g_area_ptr else None, C.c_void_p(g_iou_ptr) if g_iou_ptr else None,
+            C.c_void_p(grad_b_ptr), C.c_void_p(grad_c_ptr)))
 
     def build_tree(self):
         self._check(lib().segclip_build_tree(self._h))

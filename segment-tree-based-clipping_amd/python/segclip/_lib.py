@@ -25,7 +25,8 @@ SYMBOLS = [
     "segclip_exclusive_scan_i64", "segclip_synth_star", "segclip_synth_harmonic", "segclip_synth_convex",
     "segclip_synth_uniform", "segclip_count_vertical", "segclip_set_timing", "segclip_get_stage_times",
     "segclip_set_pairs", "segclip_set_pairs_device", "segclip_clip_batch", "segclip_copy_batch_rings",
-    "segclip_copy_batch_status",
+    "segclip_copy_batch_status", "segclip_set_provenance", "segclip_batch_area", "segclip_batch_area_device",
+    "segclip_batch_area_backward_device", "segclip_release", "segclip_memory",
 ]
 
 
@@ -74,6 +75,11 @@ def lib():
         "segclip_clip_batch": (C.c_int, [_vp, C.c_int, C.c_int]),
         "segclip_copy_batch_rings": (C.c_int, [_vp, _ip, C.c_int64, _i64p]),
         "segclip_copy_batch_status": (C.c_int, [_vp, _ip, C.c_int64, _i64p]),
+        "segclip_set_provenance": (C.c_int, [_vp, C.c_int]),
+        "segclip_batch_area": (C.c_int, [_vp, _dp, C.c_int64, _i64p]),
+        "segclip_batch_area_device": (C.c_int, [_vp, _vp]),
+        "segclip_batch_area_backward_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "segclip_memory": (C.c_int, [_i64p, _i64p, C.c_int]),
         "segclip_intersections": (C.c_int, [_vp, C.c_int]),
         "segclip_clip": (C.c_int, [_vp, C.c_int]),
         "segclip_get_counts": (C.c_int, [_vp, C.POINTER(Counts)]),
