@@ -1,14 +1,24 @@
 #!/usr/bin/env python3
 """Benchmark of the segment-tree clipping path on one MI355X.
 
-Workload (default): plain pairwise clip of two harmonic star rings, base
-200,000 vertices (seed 2001), overlay 1,000,000 vertices (seed 2002, centre
-(0.15,-0.1), rotated 17 degrees), FP64.  One step = the whole hot path on the
-device with both rings already resident in HBM: edge records, elementary
-intervals, node intervals, end lists, cover lists, stab + candidate dedupe,
-intersections / augmented rings / initial labels, labelling, traversal and
-clean-up of the output rings.  Reported value: input edges clipped per second
-(N_b + N_c per step).
+Main workload (the value of the JSON line): plain pairwise clip of two
+harmonic star rings, base 200,000 vertices (seed 2001), overlay 1,000,000
+vertices (seed 2002, centre (0.15,-0.1), rotated 17 degrees), FP64.  One step
+= the whole hot path on the device with both rings already resident in HBM:
+edge records, elementary intervals, node intervals, end lists, cover lists,
+stab + candidate dedupe, intersections / augmented rings / initial labels,
+labelling, traversal and clean-up of the output rings.  Value: input edges
+clipped per second (N_b + N_c per step).
+
+Batched workloads (reported under "batched", each next to the plain per-pair
+GPU loop on a sample of the same pairs):
+  one_to_many   base 100,000-vertex harmonic star against 4,096 star overlays
+                (log-uniform 8..20,000 vertices): one tree, every overlay
+                clipped in one batch
+  small_pairs   16,384 pairs of 4..64-vertex rings: clip + area/IoU forward,
+                and forward + backward of the area / IoU
+  large_pairs   256 pairs of 10k..200k-vertex harmonic stars: forward +
+                backward with the clip recomputed in backward, peak HBM
 
 roofline: the stab kernel (the dominant kernel), FP64 work = W pair tests x 8
 FP64 comparisons (the LSMF bounding-box test every pair runs) per launch,
@@ -55,6 +65,125 @@ def cpu_baseline(nb, nc):
                       "%d rings" % (nb, nc, dt, len(r.rings))}
 
 
+def timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def plain_loop_rate(bases, ovs, sample):
+    """Pairs/s of the plain per-pair GPU path: every pair its own context, tree and clip."""
+    import segclip
+    idx = list(range(0, len(ovs), max(1, len(ovs) // sample)))[:sample]
+    cl = segclip.Clipper()
+    one = bases if isinstance(bases, list) else None
+    cl.set_polygons(one[idx[0]] if one else bases, [ovs[idx[0]]]).clip(1)  # warm-up
+    t0 = time.perf_counter()
+    for i in idx:
+        cl.set_polygons(one[i] if one else bases, [ovs[i]])
+        cl.clip(1)
+    dt = time.perf_counter() - t0
+    return len(idx) / dt, len(idx)
+
+
+def bench_one_to_many(steps):
+    import segclip
+    base, ovs = segclip.synth.one_to_many(100000, 4096, 3000)
+    cl = segclip.Clipper().set_polygons(base, ovs)
+
+    def step():
+        cl.reset()
+        cl.clip_batch(1, len(ovs))
+
+    dt = timed(step, steps, 1)
+    cnt = cl.counts()
+    plain, n = plain_loop_rate(base, ovs, 48)
+    edges = len(base) * len(ovs) + sum(len(o) for o in ovs)
+    return {"workload": "one-to-many: harmonic base 100,000 vertices x 4,096 star overlays (8..20,000 vertices)",
+            "pairs": len(ovs), "ms_per_batch": 1e3 * dt, "pairs_per_s": len(ovs) / dt,
+            "edges_per_s": edges / dt, "plain_pairs_per_s": plain, "plain_sample": n,
+            "speedup_vs_plain": (len(ovs) / dt) / plain, "rings": cnt["rings"], "out_vertices": cnt["out_vertices"]}
+
+
+def bench_small_pairs(steps):
+    import torch
+    import segclip
+    from segclip.autograd import clip_area_iou
+    bases, ovs = segclip.synth.small_pairs(16384, 4000)
+    cl = segclip.Clipper().set_pairs(bases, ovs)
+    J = len(bases)
+    out = torch.empty((4, J), dtype=torch.float64, device="cuda")
+
+    def fwd():
+        cl.reset()
+        cl.clip_batch()
+        cl.batch_area_device(out.data_ptr())
+
+    dt = timed(fwd, steps, 1)
+    import numpy as np
+    bx = torch.tensor(np.concatenate(bases), dtype=torch.float64, device="cuda", requires_grad=True)
+    ox = torch.tensor(np.concatenate(ovs), dtype=torch.float64, device="cuda", requires_grad=True)
+    bsz, csz = [len(b) for b in bases], [len(o) for o in ovs]
+
+    def fwd_bwd():
+        bx.grad = None
+        ox.grad = None
+        a, iou = clip_area_iou(bx, bsz, ox, csz)
+        (a.sum() + iou.sum()).backward()
+
+    dtb = timed(fwd_bwd, steps, 1)
+    plain, n = plain_loop_rate(bases, ovs, 256)
+    return {"workload": "16,384 pairs, 4..64 vertices (convex / star)", "pairs": J,
+            "ms_per_batch": 1e3 * dt, "pairs_per_s": J / dt, "fwd_bwd_ms": 1e3 * dtb, "fwd_bwd_pairs_per_s": J / dtb,
+            "plain_pairs_per_s": plain, "plain_sample": n, "speedup_vs_plain": (J / dt) / plain}
+
+
+def bench_large_pairs(steps):
+    import numpy as np
+    import torch
+    import segclip
+    from segclip.autograd import clip_area_iou
+    t0 = time.perf_counter()
+    bases, ovs = segclip.synth.large_pairs(256, 5000)
+    log("[bench] large pairs generated in %.1f s" % (time.perf_counter() - t0))
+    J = len(bases)
+    bx = torch.tensor(np.concatenate(bases), dtype=torch.float64, device="cuda", requires_grad=True)
+    ox = torch.tensor(np.concatenate(ovs), dtype=torch.float64, device="cuda", requires_grad=True)
+    bsz, csz = [len(b) for b in bases], [len(o) for o in ovs]
+
+    def fwd_bwd():
+        bx.grad = None
+        ox.grad = None
+        a, iou = clip_area_iou(bx, bsz, ox, csz, recompute=True)
+        (a.sum() + iou.sum()).backward()
+
+    fwd_bwd()
+    segclip.memory(reset_peak=True)
+    dt = timed(fwd_bwd, steps, 0)
+    cur, peak = segclip.memory()
+    # forward only, batched, for the same pairs
+    cl = segclip.Clipper().set_pairs(bases, ovs)
+
+    def fwd():
+        cl.reset()
+        cl.clip_batch()
+
+    dtf = timed(fwd, steps, 1)
+    cl.release()
+    plain, n = plain_loop_rate(bases, ovs, 8)
+    edges = sum(bsz) + sum(csz)
+    return {"workload": "256 pairs of harmonic stars, 10k..200k vertices", "pairs": J,
+            "fwd_bwd_ms": 1e3 * dt, "fwd_bwd_pairs_per_s": J / dt, "fwd_bwd_edges_per_s": edges / dt,
+            "peak_hbm_bytes_fwd_bwd": peak, "fwd_ms": 1e3 * dtf, "fwd_pairs_per_s": J / dtf,
+            "plain_fwd_pairs_per_s": plain, "plain_sample": n, "speedup_fwd_vs_plain": (J / dtf) / plain}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +192,8 @@ def main():
     ap.add_argument("--nb", type=int, default=200000)
     ap.add_argument("--nc", type=int, default=1000000)
     ap.add_argument("--cpu-sample", default="100000x500000", help="NBxNC of the oracle sample, or 'none'")
+    ap.add_argument("--batched", default="one_to_many,small_pairs,large_pairs",
+                    help="comma list of batched workloads to add, or 'none'")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the box has one); --gpus %d is not supported"
@@ -98,6 +229,8 @@ def main():
     log("[bench] %d steps, %.3f ms/step, counts %s" % (args.steps, ms, json.dumps(cnt)))
     avg = {k: sum(s[k] for s in stages) / len(stages) for k in stages[0]}
     log("[bench] stage ms: %s" % json.dumps({k: round(v, 4) for k, v in avg.items()}))
+    cl.release()
+    del cl
 
     stab = sum(stab_ms) / len(stab_ms)
     flops = FLOPS_PER_TEST * cnt["tests"]
@@ -105,7 +238,21 @@ def main():
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
             "kernel": "k_stab", "kernel_ms": round(stab, 4),
-            "work": "W=%d pair tests x %d FP64 ops (LSMF box test), FP64 VALU" % (cnt["tests"], FLOPS_PER_TEST)}
+            "work": "W=%d pair tests x %d FP64 ops (LSMF box test), FP64 VALU (no MFMA); the peak is the "
+                    "FP64 vector rate" % (cnt["tests"], FLOPS_PER_TEST)}
+
+    batched = {}
+    if args.batched != "none":
+        for name in args.batched.split(","):
+            fn = {"one_to_many": bench_one_to_many, "small_pairs": bench_small_pairs,
+                  "large_pairs": bench_large_pairs}[name]
+            log("[bench] batched workload %s" % name)
+            try:
+                batched[name] = fn(3)
+                log("[bench] %s: %s" % (name, json.dumps(batched[name])))
+            except Exception as e:  # reported, never hides the main line
+                batched[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+                log("[bench] %s failed: %s" % (name, e))
 
     cpu = None
     if args.cpu_sample != "none":
@@ -122,7 +269,7 @@ def main():
                                "inputs resident in HBM" % (nb, nc),
                    "pairs_per_s": 1000.0 / ms, "intersections": cnt["overlay_pairs"], "rings": cnt["rings"],
                    "out_vertices": cnt["out_vertices"], "stage_ms": {k: round(v, 4) for k, v in avg.items()}},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

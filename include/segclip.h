@@ -97,6 +97,34 @@ int segclip_clip(segclip_ctx *ctx, int k);
  * Results equal those of segclip_clip(k) for each k; the output rings of job
  * k are rings [job_ring_off[k-k0], job_ring_off[k-k0+1]) of segclip_copy_rings. */
 int segclip_clip_batch(segclip_ctx *ctx, int k0, int k1);
+/* The reference's per-overlay array stages on the caller's host arrays, with
+ * the reference's argument meaning (pair ids = 2 * vertex offset into bPoly /
+ * cPoly with closing vertices; the context's resident rings are used).  They
+ * back the SegTree facade (segtree.h) for drivers that keep their own arrays.
+ *   segclip_save_intersections_ids       SegTree::saveIntersectionsParallel (oacc:7257-7341)
+ *   segclip_sort_intersections_at_edges  SegTree::sortIntersectionsAtEdgesParallel (oacc:7436-7473);
+ *                                        ncounts = entries of all_counts / nd_counts
+ *   segclip_copy_intersections_arrays    SegTree::copyIntersectionsParallel (oacc:7508-7664);
+ *                                        all overlay source vertices are copied
+ *   segclip_init_labels_arrays           SegTree::calculateInitLabelParallel (oacc:7925-7988) */
+int segclip_save_intersections_ids(segclip_ctx *ctx, const int32_t *b_id, const int32_t *c_id, int n, double *xy,
+                                   int32_t *type, int32_t *alpha_b, int32_t *alpha_c);
+int segclip_sort_intersections_at_edges(segclip_ctx *ctx, const int32_t *alpha, const int32_t *sorted_index,
+                                        int c_start, const int32_t *all_counts, const int32_t *nd_counts,
+                                        int ncounts, const int32_t *unique_ids, int unique_count, int n,
+                                        int32_t *sorted_alpha_index);
+int segclip_copy_intersections_arrays(segclip_ctx *ctx, int b_size, int c_size, int c_start, const int32_t *b_ids,
+                                      const int32_t *c_ids, int n, const int32_t *b_sorted_index,
+                                      const int32_t *c_sorted_index, const int32_t *sorted_b,
+                                      const int32_t *sorted_c, const double *intersections, const int32_t *types,
+                                      const int32_t *b_nd_ps, const int32_t *c_nd_ps, const int32_t *alpha_b,
+                                      const int32_t *alpha_c, double *intersected_b, double *intersected_c,
+                                      int32_t *neighbors_b, int32_t *neighbors_c, int32_t *alpha_values_b,
+                                      int32_t *alpha_values_c);
+int segclip_init_labels_arrays(segclip_ctx *ctx, int nb_aug, int nc_aug, const double *intersected_b,
+                               const double *intersected_c, const int32_t *alpha_values_b,
+                               const int32_t *neighbors_b, int32_t *labels_b, int32_t *labels_c);
+
 /* Differentiable area / IoU (no counterpart in the reference, which only
  * writes the rings).  With provenance on, the intersection stage records
  * which input vertex or intersecting pair every augmented slot came from.

@@ -216,6 +216,42 @@ int segclip_batch_area_backward_device(segclip_ctx *h, const double *g_area, con
     if (!h || !grad_b || !grad_c) return SEGCLIP_EINVAL;
     return guarded(h, [&](sc::Ctx &x) { sc::batch_area_backward(x, g_area, g_iou, grad_b, grad_c); });
 }
+int segclip_save_intersections_ids(segclip_ctx *h, const int32_t *b_id, const int32_t *c_id, int n, double *xy,
+                                   int32_t *type, int32_t *alpha_b, int32_t *alpha_c) {
+    return guarded(h, [&](sc::Ctx &x) { sc::arr_save_intersections(x, b_id, c_id, n, xy, type, alpha_b, alpha_c); });
+}
+int segclip_sort_intersections_at_edges(segclip_ctx *h, const int32_t *alpha, const int32_t *sorted_index,
+                                        int c_start, const int32_t *all_counts, const int32_t *nd_counts,
+                                        int ncounts, const int32_t *unique_ids, int unique_count, int n,
+                                        int32_t *sorted_alpha_index) {
+    return guarded(h, [&](sc::Ctx &x) {
+        sc::arr_sort_at_edges(x, alpha, sorted_index, c_start, all_counts, nd_counts, ncounts, unique_ids,
+                              unique_count, n, sorted_alpha_index);
+    });
+}
+int segclip_copy_intersections_arrays(segclip_ctx *h, int b_size, int c_size, int c_start, const int32_t *b_ids,
+                                      const int32_t *c_ids, int n, const int32_t *b_sorted_index,
+                                      const int32_t *c_sorted_index, const int32_t *sorted_b,
+                                      const int32_t *sorted_c, const double *intersections, const int32_t *types,
+                                      const int32_t *b_nd_ps, const int32_t *c_nd_ps, const int32_t *alpha_b,
+                                      const int32_t *alpha_c, double *intersected_b, double *intersected_c,
+                                      int32_t *neighbors_b, int32_t *neighbors_c, int32_t *alpha_values_b,
+                                      int32_t *alpha_values_c) {
+    return guarded(h, [&](sc::Ctx &x) {
+        sc::arr_copy_intersections(x, b_size, c_size, c_start, b_ids, c_ids, n, b_sorted_index, c_sorted_index,
+                                   sorted_b, sorted_c, intersections, types, b_nd_ps, c_nd_ps, alpha_b, alpha_c,
+                                   intersected_b, intersected_c, neighbors_b, neighbors_c, alpha_values_b,
+                                   alpha_values_c);
+    });
+}
+int segclip_init_labels_arrays(segclip_ctx *h, int nb_aug, int nc_aug, const double *intersected_b,
+                               const double *intersected_c, const int32_t *alpha_values_b,
+                               const int32_t *neighbors_b, int32_t *labels_b, int32_t *labels_c) {
+    return guarded(h, [&](sc::Ctx &x) {
+        sc::arr_init_labels(x, nb_aug, nc_aug, intersected_b, intersected_c, alpha_values_b, neighbors_b, labels_b,
+                            labels_c);
+    });
+}
 int segclip_clip_batch(segclip_ctx *h, int k0, int k1) {
     return guarded(h, [&](sc::Ctx &x) { sc::clip_jobs(x, k0, k1); });
 }

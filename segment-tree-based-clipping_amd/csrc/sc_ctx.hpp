@@ -151,6 +151,20 @@ void clip_jobs(Ctx &c, int k0, int k1); // full pipeline for overlays k0..k1 as 
 void batch_area(Ctx &c, double *out4);
 void batch_area_backward(Ctx &c, const double *g_area, const double *g_iou, double *grad_b, double *grad_c);
 void release_derived(Ctx &c);          // free everything but the resident input rings
+// the reference's per-overlay array stages on caller (host) arrays
+void arr_save_intersections(Ctx &c, const int *bid, const int *cid, int n, double *xy, int *type, int *qa, int *qb);
+void arr_sort_at_edges(Ctx &c, const int *alpha, const int *sorted_index, int c_start, const int *all_counts,
+                       const int *nd_counts, int ncounts, const int *unique_ids, int unique_count, int n, int *out);
+void arr_copy_intersections(Ctx &c, int b_size, int c_size, int c_start, const int *b_ids, const int *c_ids, int n,
+                            const int *b_sorted_index, const int *c_sorted_index, const int *sorted_b,
+                            const int *sorted_c, const double *inters, const int *types, const int *b_nd_ps,
+                            const int *c_nd_ps, const int *qa, const int *qb, double *inter_b, double *inter_c,
+                            int *nbr_b, int *nbr_c, int *alpha_b, int *alpha_c);
+void arr_init_labels(Ctx &c, int nba, int nca, const double *inter_b, const double *inter_c, const int *alpha_b,
+                     const int *nbr_b, int *lab_b, int *lab_c);
+// calculateInitLabelParallel on one pair of augmented rings given as device arrays
+void init_labels_one(Ctx &c, int nba, int nca, const double *bx, const double *by, const int *avb, const int *nbrb,
+                     const double *cx, const double *cy, int *labb, int *labc, int *flags);
 void release_foster_scratch();
 
 uint32_t read_u32(Ctx &c, const uint32_t *d);

@@ -116,6 +116,27 @@ __host__ __device__ inline int classify_pre(double b1x, double b1y, double b2x, 
     return classify(b1x, b1y, b2x, b2y, c1x, c1y, c2x, c2y, al, be);
 }
 
+// point of an intersection of type t (saveIntersectionsParallel, oacc:7257-7341):
+// type 1 at (1-alpha) b1 + alpha b2, types 3,4,5,7,8 at c1, types 2,6 at b1
+__host__ __device__ inline void intersection_point(int t, double al, double b1x, double b1y, double b2x, double b2y,
+                                                  double c1x, double c1y, double &x, double &y) {
+    x = 0.0;
+    y = 0.0;
+    if (t == 1) {
+        double w = 1.0 - al;
+        double ax = w * b1x, ay = w * b1y;
+        double bx = al * b2x, by = al * b2y;
+        x = ax + bx;
+        y = ay + by;
+    } else if (t == 3 || t == 4 || t == 5 || t == 7 || t == 8) {
+        x = c1x;
+        y = c1y;
+    } else if (t == 2 || t == 6) {
+        x = b1x;
+        y = b1y;
+    }
+}
+
 __host__ __device__ inline int alpha_to_int(double a) {
     if (a < EPS) return 0;
     return int(1000000000.0 * a);

@@ -76,20 +76,8 @@ __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, in
     double c1x = cpoly[cId], c1y = cpoly[cId + 1], c2x = cpoly[cId + 2], c2y = cpoly[cId + 3];
     double al, be;
     int t = classify(b1x, b1y, b2x, b2y, c1x, c1y, c2x, c2y, al, be);
-    double x = 0, y = 0;
-    if (t == 1) {
-        double w = 1.0 - al;
-        double ax = w * b1x, ay = w * b1y;
-        double bx = al * b2x, by = al * b2y;
-        x = ax + bx;
-        y = ay + by;
-    } else if (t == 3 || t == 4 || t == 5 || t == 7 || t == 8) {
-        x = c1x;
-        y = c1y;
-    } else if (t == 2 || t == 6) {
-        x = b1x;
-        y = b1y;
-    }
+    double x, y;
+    intersection_point(t, al, b1x, b1y, b2x, b2y, c1x, c1y, x, y);
     pb[p] = b;
     pc[p] = cl;
     pt[p] = t;
@@ -462,5 +450,19 @@ void post_jobs(Ctx &c, int k0, int k1) {
 }
 
 void post_overlay(Ctx &c, int k) { post_jobs(c, k, k); }
+
+void init_labels_one(Ctx &c, int nba, int nca, const double *bx, const double *by, const int *avb, const int *nbrb,
+                     const double *cx, const double *cy, int *labb, int *labc, int *flags) {
+    hipStream_t s = c.stream;
+    std::vector<int> off = {0, nba, 0, nca};
+    thread_local DBuf<int> *dd = new DBuf<int>(); // scratch, kept for the thread's lifetime
+    int *d = dd->ensure(4);
+    SC_HIP(hipMemcpyAsync(d, off.data(), sizeof(int) * 4, hipMemcpyHostToDevice, s));
+    SC_HIP(hipMemsetAsync(labb, 0, sizeof(int) * size_t(nba), s));
+    SC_HIP(hipMemsetAsync(labc, 0, sizeof(int) * size_t(nca), s));
+    k_init_labels<<<grid_for(nba), BLK, 0, s>>>(1, d, d + 2, bx, by, avb, nbrb, cx, cy, labb, labc, flags);
+    SC_HIP(hipGetLastError());
+    SC_HIP(hipStreamSynchronize(s)); // off is a stack vector
+}
 
 } // namespace sc

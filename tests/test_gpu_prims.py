@@ -1,0 +1,73 @@
+"""GPU checks of the thrust_func.h drop-ins (stable sort by key, exclusive
+scan) against numpy's stable sort and cumulative sum: ties keep their input
+order, -0.0 equals 0.0, negative keys, empty and one-element inputs, sizes
+spanning many radix tiles."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _sort(name, keys):
+    import torch
+    from segclip import lib
+    n = len(keys)
+    vals = np.arange(n, dtype=np.int32)
+    dk, dv = _dev(keys), _dev(vals)
+    torch.cuda.synchronize()
+    rc = getattr(lib(), name)(C.c_void_p(dk.data_ptr()), C.c_void_p(dv.data_ptr()), n, None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    return dk.cpu().numpy(), dv.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 300001])
+def test_stable_sort_f64(gpu, n):
+    rng = np.random.default_rng(n)
+    keys = np.round(rng.standard_normal(n) * 50) / 4.0  # many ties, negatives
+    if n > 4:
+        keys[:4] = [0.0, -0.0, 0.0, -0.0]
+    k, v = _sort("segclip_stable_sort_by_key_f64", keys)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(v, order.astype(np.int32))
+    assert np.array_equal(k, keys[order])
+
+
+@pytest.mark.parametrize("n", [0, 1, 5000, 200003])
+def test_stable_sort_i32_i64(gpu, n):
+    rng = np.random.default_rng(100 + n)
+    k32 = rng.integers(-1000, 1000, n).astype(np.int32)
+    k, v = _sort("segclip_stable_sort_by_key_i32", k32)
+    assert np.array_equal(v, np.argsort(k32, kind="stable").astype(np.int32))
+    k64 = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    k64[::7] = 12345
+    k, v = _sort("segclip_stable_sort_by_key_i64", k64)
+    assert np.array_equal(v, np.argsort(k64, kind="stable").astype(np.int32))
+
+
+@pytest.mark.parametrize("n", [0, 1, 777, 1 << 20])
+def test_exclusive_scan(gpu, n):
+    import torch
+    from segclip import lib
+    rng = np.random.default_rng(200 + n)
+    a32 = rng.integers(-50, 100, n).astype(np.int32)
+    d = _dev(a32)
+    torch.cuda.synchronize()
+    assert lib().segclip_exclusive_scan_i32(C.c_void_p(d.data_ptr()), n, None) == 0
+    torch.cuda.synchronize()
+    want = np.concatenate([[0], np.cumsum(a32, dtype=np.int64)[:-1]]).astype(np.int32) if n else a32
+    assert np.array_equal(d.cpu().numpy(), want)
+    a64 = rng.integers(-(1 << 33), 1 << 33, n).astype(np.int64)
+    d = _dev(a64)
+    torch.cuda.synchronize()
+    assert lib().segclip_exclusive_scan_i64(C.c_void_p(d.data_ptr()), n, None) == 0
+    torch.cuda.synchronize()
+    want = np.concatenate([[0], np.cumsum(a64)[:-1]]).astype(np.int64) if n else a64
+    assert np.array_equal(d.cpu().numpy(), want)

@@ -29,7 +29,7 @@ def _small():
     return [w for w in workloads_small() if w[0] != "harmonic20k_100k"]
 
 
-@pytest.mark.parametrize("case", _small(), ids=lambda c: c[0])
+@pytest.mark.parametrize("case", _small() + [(n, b, o) for n, b, o, _ in degenerate_cases()], ids=lambda c: c[0])
 def test_pairs_equal_brute_force(case):
     name, b, ovs = case
     tr = O.Tree(b, ovs)
