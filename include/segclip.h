@@ -169,6 +169,7 @@ typedef struct {
     int32_t rings;            /* output components after clean-up */
     int32_t undefined;        /* bit 1: an intersection got no slot; 2: ring walk; 4: traversal */
     int64_t out_vertices;
+    int64_t stab_units;       /* work units (waves) of the stab */
 } segclip_counts;
 int segclip_get_counts(segclip_ctx *ctx, segclip_counts *out);
 

@@ -305,6 +305,7 @@ int segclip_get_counts(segclip_ctx *h, segclip_counts *o) {
     o->rings = c.nrings;
     o->undefined = c.undefined;
     o->out_vertices = c.n_out;
+    o->stab_units = c.n_units;
     return SEGCLIP_OK;
 }
 

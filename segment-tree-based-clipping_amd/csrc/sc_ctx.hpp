@@ -72,6 +72,7 @@ struct Ctx {
     DBuf<uint64_t> cand;                 // packed (k, b, c) keys
     int64_t n_cand = 0, cand_cap = 0;
     int64_t n_tests = 0;                 // W
+    int64_t n_units = 0;                 // stab work units (waves)
     DBuf<uint64_t> pairs;                // unique packed keys, sorted
     int64_t n_pairs = 0;
     int bits_b = 0, bits_c = 0, bits_k = 0;

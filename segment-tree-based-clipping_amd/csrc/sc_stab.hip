@@ -10,11 +10,12 @@
 // Here the end list of v is not materialised: it is the union of the leaf end
 // entries under v, a contiguous range of the (class, leaf)-sorted leaf entry
 // array, with the copy of an edge whose "before" leaf is also under v skipped.
-// Work is cut into units of (node, rectangle, <=RCH rows, 64 columns); one
-// wave runs a unit with the 64 column edges in lanes and the rows broadcast,
-// so the base and overlay classes are never tested against themselves.
-// Hits are compacted with one ballot and one atomic per wave and row.  All
-// overlays are handled in the same pass: a pair key carries the overlay.
+// Work is cut into units of (node, rectangle, <=RCH rows, one or more
+// 64-column tiles); a unit is found from a unit->node map.  One wave runs a
+// unit with the column edges in lanes and the rows broadcast, so the base and
+// overlay classes are never tested against themselves.  Hits are staged per
+// wave in LDS and written with one atomic per flush.  All overlays are
+// handled in the same pass: a pair key carries the overlay.
 #include "sc_ctx.hpp"
 #include "sc_fp64.hpp"
 
@@ -55,7 +56,36 @@ __device__ inline NodeShape node_shape(int v, const uint32_t *cov_off, const uin
     return s;
 }
 
-__device__ inline uint32_t cdiv32(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline uint32_t cdiv32(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// Work units of one rectangle (rows x cols): row blocks of RCH rows; a unit
+// of a full row block covers one 64-column tile, a unit of the last (partial)
+// row block covers enough tiles that it runs about RCH row steps.
+__host__ __device__ inline uint32_t tiles_per_unit(uint32_t rows_last) { return cdiv32(RCH, rows_last); }
+__host__ __device__ inline uint32_t rect_units(uint32_t rows, uint32_t cols) {
+    if (!rows || !cols) return 0;
+    const uint32_t rb = cdiv32(rows, RCH), ct = cdiv32(cols, WAVE);
+    const uint32_t last = rows - (rb - 1) * RCH;
+    return (rb - 1) * ct + cdiv32(ct, tiles_per_unit(last));
+}
+// unit u of a rectangle -> first row, row count, tile range [t0,t1)
+__host__ __device__ inline void rect_unit(uint32_t rows, uint32_t cols, uint32_t u, uint32_t &r0, uint32_t &nr,
+                                          uint32_t &t0, uint32_t &t1) {
+    const uint32_t rb = cdiv32(rows, RCH), ct = cdiv32(cols, WAVE);
+    const uint32_t full = (rb - 1) * ct;
+    if (u < full) {
+        r0 = (u / ct) * RCH;
+        nr = RCH;
+        t0 = u % ct;
+        t1 = t0 + 1;
+    } else {
+        const uint32_t last = rows - (rb - 1) * RCH, g = tiles_per_unit(last);
+        r0 = (rb - 1) * RCH;
+        nr = last;
+        t0 = (u - full) * g;
+        t1 = t0 + g < ct ? t0 + g : ct;
+    }
+}
 
 __global__ void k_units(const uint32_t *__restrict__ cov_off, const uint32_t *__restrict__ cov_split,
                         const uint32_t *__restrict__ lend_off, int nl, int D, int T, uint32_t *units,
@@ -66,19 +96,26 @@ __global__ void k_units(const uint32_t *__restrict__ cov_off, const uint32_t *__
     uint64_t w = 0;
     if (v >= 1 && cov_off[v + 1] > cov_off[v]) {
         NodeShape s = node_shape(v, cov_off, cov_split, lend_off, nl, D);
-        if (s.nbase) u += cdiv32(s.nbase, RCH) * (cdiv32(s.nc, WAVE) + cdiv32(s.ecn, WAVE));
-        if (s.nc) u += cdiv32(s.nc, RCH) * cdiv32(s.ebn, WAVE);
+        u = rect_units(s.nbase, s.nc) + rect_units(s.nbase, s.ecn) + rect_units(s.nc, s.ebn);
         w = uint64_t(s.nbase) * s.nc + uint64_t(s.nbase) * s.ecn + uint64_t(s.nc) * s.ebn;
     }
     units[v] = u;
     tests[v] = w;
 }
 
+// unit -> node: each node with units marks its first unit; a max-scan spreads it
+__global__ void k_unit_heads(int T, const uint32_t *__restrict__ uoff, uint32_t *head) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= T) return;
+    if (uoff[v + 1] > uoff[v]) head[uoff[v]] = uint32_t(v);
+}
+
+constexpr int STG = 256; // hits staged per wave in LDS before a flush
+
 struct StabArgs {
-    const uint32_t *cov_off, *cov_split, *cov_e, *lend_off, *lend_e, *uoff;
+    const uint32_t *cov_off, *cov_split, *cov_e, *lend_off, *lend_e, *uoff, *unode;
     const int *before, *etype, *eoff;
-    const double *x0, *y0, *x1, *y1;          // endpoints (classification)
-    const double *bx0, *bx1, *by0, *by1;      // bounding boxes (LSMF)
+    const double *x0, *y0, *x1, *y1;          // endpoints (input coordinates)
     int nl, D, T;
     uint32_t nunits;
     int bits_b, bits_c;
@@ -95,119 +132,164 @@ __device__ inline double bcast(double v, int src) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// One wave per unit: up to 64 rows (one per lane, broadcast in turn) x 64
-// columns (one per lane).  LSMF is the bounding-box overlap test on the
-// precomputed boxes (min/max of the endpoints, exactly as LSMF forms them);
-// pairs that pass are classified with lineSegmentIntersection2.
-__global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint32_t wg = blockIdx.x * (BLK / WAVE) + threadIdx.x / WAVE;
-    if (wg >= a.nunits) return;
-    int lo = 0, hi = a.T - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (a.uoff[mid] <= wg) lo = mid;
-        else hi = mid - 1;
+// LSMF's box of one segment: min/max of its endpoints formed as LSMF forms them
+struct Box {
+    double minx, maxx, miny, maxy;
+};
+__device__ inline Box box_of(double ax, double ay, double bx, double by) {
+    Box r;
+    r.minx = ax;
+    r.maxx = bx;
+    if (r.minx > bx) {
+        r.minx = bx;
+        r.maxx = ax;
     }
-    const int v = lo;
+    r.miny = ay;
+    r.maxy = by;
+    if (r.miny > by) {
+        r.miny = by;
+        r.maxy = ay;
+    }
+    return r;
+}
+
+__device__ inline void flush(const StabArgs &a, const uint64_t *stage, uint32_t cnt, int lane) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)cnt);
+    base = __shfl(base, 0, WAVE);
+    for (uint32_t i = lane; i < cnt; i += WAVE)
+        if (base + i < a.cap) a.out[base + i] = stage[i];
+}
+
+// One wave per unit: up to 64 rows (one per lane, broadcast in turn) against
+// one or more 64-column tiles (one column per lane).  LSMF is the bounding-box
+// overlap test; pairs that pass are classified with lineSegmentIntersection2
+// (exact division-free prefilter first).  Hits are staged in LDS and written
+// with one atomic per flush.
+__global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
+    __shared__ uint64_t stage_all[BLK / WAVE][STG];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = threadIdx.x / WAVE;
+    const uint32_t wg = blockIdx.x * (BLK / WAVE) + wv;
+    if (wg >= a.nunits) return; // whole wave; no block barrier below
+    uint64_t *stage = stage_all[wv];
+    const int v = int(a.unode[wg]);
     NodeShape s = node_shape(v, a.cov_off, a.cov_split, a.lend_off, a.nl, a.D);
     uint32_t u = wg - a.uoff[v];
-    const uint32_t rb = s.nbase ? cdiv32(s.nbase, RCH) : 0u;
-    const uint32_t r1 = s.nbase ? rb * cdiv32(s.nc, WAVE) : 0u;
-    const uint32_t r2 = s.nbase ? rb * cdiv32(s.ecn, WAVE) : 0u;
+    const uint32_t n1 = rect_units(s.nbase, s.nc), n2 = rect_units(s.nbase, s.ecn);
     int kind;
-    uint32_t ncol, rows_begin, nrows, col_begin;
+    uint32_t rows, cols, row_begin, col_begin;
     const uint32_t *colsrc;
-    if (u < r1) {
-        kind = 1;
-        uint32_t cc = cdiv32(s.nc, WAVE);
-        uint32_t rc = u / cc, ci = u % cc;
-        rows_begin = s.off + rc * RCH;
-        nrows = min(uint32_t(RCH), s.nbase - rc * RCH);
-        col_begin = s.off + s.nbase + ci * WAVE;
-        ncol = min(uint32_t(WAVE), s.nc - ci * WAVE);
+    if (u < n1) {
+        kind = 1; // base cover x overlay cover
+        rows = s.nbase;
+        cols = s.nc;
+        row_begin = s.off;
+        col_begin = s.off + s.nbase;
         colsrc = a.cov_e;
-    } else if (u < r1 + r2) {
-        kind = 2;
-        u -= r1;
-        uint32_t cc = cdiv32(s.ecn, WAVE);
-        uint32_t rc = u / cc, ci = u % cc;
-        rows_begin = s.off + rc * RCH;
-        nrows = min(uint32_t(RCH), s.nbase - rc * RCH);
-        col_begin = s.ec0 + ci * WAVE;
-        ncol = min(uint32_t(WAVE), s.ecn - ci * WAVE);
+    } else if (u < n1 + n2) {
+        kind = 2; // base cover x overlay end entries
+        u -= n1;
+        rows = s.nbase;
+        cols = s.ecn;
+        row_begin = s.off;
+        col_begin = s.ec0;
         colsrc = a.lend_e;
     } else {
-        kind = 3;
-        u -= r1 + r2;
-        uint32_t cc = cdiv32(s.ebn, WAVE);
-        uint32_t rc = u / cc, ci = u % cc;
-        rows_begin = s.off + s.nbase + rc * RCH;
-        nrows = min(uint32_t(RCH), s.nc - rc * RCH);
-        col_begin = s.eb0 + ci * WAVE;
-        ncol = min(uint32_t(WAVE), s.ebn - ci * WAVE);
+        kind = 3; // overlay cover x base end entries
+        u -= n1 + n2;
+        rows = s.nc;
+        cols = s.ebn;
+        row_begin = s.off + s.nbase;
+        col_begin = s.eb0;
         colsrc = a.lend_e;
     }
-    // this lane's column
-    bool valid = uint32_t(lane) < ncol;
-    uint32_t ce = 0;
-    if (valid) {
-        uint32_t raw = colsrc[col_begin + lane];
-        ce = raw & 0x7fffffffu;
-        if (kind != 1 && (raw & 0x80000000u)) {
-            // "after" copy of an edge whose "before" leaf is under v as well
-            int bl = a.before[ce];
-            if (bl >= a.nl + s.lo && bl < a.nl + s.hi) valid = false;
-        }
-    }
-    double cminx = 0, cmaxx = -1, cminy = 0, cmaxy = -1;
-    if (valid) {
-        cminx = a.bx0[ce];
-        cmaxx = a.bx1[ce];
-        cminy = a.by0[ce];
-        cmaxy = a.by1[ce];
-    }
-    // this lane's row (broadcast to all lanes in the loop below)
+    uint32_t r0, nrows, t0, t1;
+    rect_unit(rows, cols, u, r0, nrows, t0, t1);
+    // this lane's row edge
     uint32_t re_l = 0;
-    double rminx = 0, rmaxx = 0, rminy = 0, rmaxy = 0;
+    int rk = 0;
+    double rx0 = 0, ry0 = 0, rx1 = 0, ry1 = 0;
     if (uint32_t(lane) < nrows) {
-        re_l = a.cov_e[rows_begin + lane];
-        rminx = a.bx0[re_l];
-        rmaxx = a.bx1[re_l];
-        rminy = a.by0[re_l];
-        rmaxy = a.by1[re_l];
+        re_l = a.cov_e[row_begin + r0 + lane];
+        rx0 = a.x0[re_l];
+        ry0 = a.y0[re_l];
+        rx1 = a.x1[re_l];
+        ry1 = a.y1[re_l];
+        rk = a.etype[re_l];
     }
+    const Box rb = box_of(rx0, ry0, rx1, ry1);
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
     const int sb = a.bits_b + a.bits_c;
-    for (uint32_t r = 0; r < nrows; ++r) {
-        const double qminx = bcast(rminx, int(r)), qmaxx = bcast(rmaxx, int(r));
-        const double qminy = bcast(rminy, int(r)), qmaxy = bcast(rmaxy, int(r));
-        bool box = valid && !(qminx > cmaxx || qmaxx < cminx) && !(qminy > cmaxy || qmaxy < cminy);
-        bool hit = false;
-        uint64_t key = 0;
-        if (box) {
-            const uint32_t re = uint32_t(__builtin_amdgcn_readlane(int(re_l), int(r)));
-            uint32_t be = kind == 3 ? ce : re, qe = kind == 3 ? re : ce;
-            int t = classify_pre(a.x0[be], a.y0[be], a.x1[be], a.y1[be], a.x0[qe], a.y0[qe], a.x1[qe], a.y1[qe]);
-            if (t != 0) {
-                hit = true;
-                int k = a.etype[qe];
-                uint32_t local = qe - uint32_t(a.eoff[k - 1]);
-                key = (uint64_t(k) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
+    uint32_t cnt = 0;
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t cb = t * WAVE;
+        const uint32_t ncol = min(uint32_t(WAVE), cols - cb);
+        bool valid = uint32_t(lane) < ncol;
+        uint32_t ce = 0;
+        if (valid) {
+            uint32_t raw = colsrc[col_begin + cb + lane];
+            ce = raw & 0x7fffffffu;
+            if (kind != 1 && (raw & 0x80000000u)) {
+                // "after" copy of an edge whose "before" leaf is under v as well
+                int bl = a.before[ce];
+                if (bl >= a.nl + s.lo && bl < a.nl + s.hi) valid = false;
             }
         }
-        uint64_t m = __ballot(hit);
-        if (m) {
-            unsigned long long base = 0;
-            int leader = __ffsll((long long)m) - 1;
-            if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)__popcll(m));
-            base = __shfl(base, leader, WAVE);
-            if (hit) {
-                uint64_t pos = base + __popcll(m & lt);
-                if (pos < a.cap) a.out[pos] = key;
+        double cx0 = 0, cy0 = 0, cx1 = 0, cy1 = 0;
+        int ck = 0;
+        if (valid) {
+            cx0 = a.x0[ce];
+            cy0 = a.y0[ce];
+            cx1 = a.x1[ce];
+            cy1 = a.y1[ce];
+            ck = a.etype[ce];
+        }
+        const Box cbx = box_of(cx0, cy0, cx1, cy1);
+        for (uint32_t r = 0; r < nrows; ++r) {
+            const double qminx = bcast(rb.minx, int(r)), qmaxx = bcast(rb.maxx, int(r));
+            const double qminy = bcast(rb.miny, int(r)), qmaxy = bcast(rb.maxy, int(r));
+            const bool box = valid && !(qminx > cbx.maxx || qmaxx < cbx.minx) &&
+                             !(qminy > cbx.maxy || qmaxy < cbx.miny);
+            bool hit = false;
+            uint64_t key = 0;
+            if (box) {
+                const double px0 = bcast(rx0, int(r)), py0 = bcast(ry0, int(r));
+                const double px1 = bcast(rx1, int(r)), py1 = bcast(ry1, int(r));
+                const uint32_t re = uint32_t(__builtin_amdgcn_readlane(int(re_l), int(r)));
+                int tp;
+                uint32_t be, qe;
+                int k;
+                if (kind == 3) { // row = overlay edge, column = base edge
+                    tp = classify_pre(cx0, cy0, cx1, cy1, px0, py0, px1, py1);
+                    be = ce;
+                    qe = re;
+                    k = __builtin_amdgcn_readlane(rk, int(r));
+                } else {
+                    tp = classify_pre(px0, py0, px1, py1, cx0, cy0, cx1, cy1);
+                    be = re;
+                    qe = ce;
+                    k = ck;
+                }
+                if (tp != 0) {
+                    hit = true;
+                    uint32_t local = qe - uint32_t(a.eoff[k - 1]);
+                    key = (uint64_t(k) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
+                }
+            }
+            const uint64_t m = __ballot(hit);
+            if (m) {
+                const uint32_t nh = uint32_t(__popcll(m));
+                if (cnt + nh > uint32_t(STG)) {
+                    flush(a, stage, cnt, lane);
+                    cnt = 0;
+                }
+                if (hit) stage[cnt + uint32_t(__popcll(m & lt))] = key;
+                cnt += nh;
             }
         }
     }
+    if (cnt) flush(a, stage, cnt, lane);
 }
 
 __global__ void k_diff_flags64(const uint64_t *__restrict__ k, int64_t n, uint32_t *f) {
@@ -265,6 +347,14 @@ void find_candidates(Ctx &c) {
     scan_exclusive_u64(tests, tests, int64_t(T) + 1, tot64, s);
     uint32_t nunits = read_u32(c, tot);
     c.n_tests = int64_t(read_u64(c, tot64));
+    c.n_units = nunits;
+    uint32_t *unode = c.u32c.ensure(size_t(nunits) + 1);
+    if (nunits > 0) {
+        SC_HIP(hipMemsetAsync(unode, 0, sizeof(uint32_t) * nunits, s));
+        k_unit_heads<<<grid_for(T), BLK, 0, s>>>(T, units, unode);
+        SC_HIP(hipGetLastError());
+        scan_inclusive_max_u32(unode, unode, nunits, s);
+    }
 
     if (c.cand_cap == 0) c.cand_cap = std::max<int64_t>(1 << 20, 4 * int64_t(c.ne));
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(c.d_u64_total.p);
@@ -273,10 +363,9 @@ void find_candidates(Ctx &c) {
         SC_HIP(hipMemsetAsync(counter, 0, sizeof(uint64_t), s));
         StabArgs a;
         a.cov_off = c.cov_off; a.cov_split = c.cov_split; a.cov_e = c.cov_e;
-        a.lend_off = c.lend_off; a.lend_e = c.lend_e; a.uoff = units;
+        a.lend_off = c.lend_off; a.lend_e = c.lend_e; a.uoff = units; a.unode = unode;
         a.before = c.before_leaf; a.etype = c.etype; a.eoff = c.d_eoff;
         a.x0 = c.ex0; a.y0 = c.ey0; a.x1 = c.ex1; a.y1 = c.ey1;
-        a.bx0 = c.paired ? c.uxs.p : c.estart.p; a.bx1 = c.paired ? c.uxe.p : c.eend.p; a.by0 = c.eymin; a.by1 = c.eymax;
         a.nl = c.nleaves; a.D = c.depth; a.T = T; a.nunits = nunits;
         a.bits_b = c.bits_b; a.bits_c = c.bits_c;
         a.out = c.cand; a.cap = uint64_t(c.cand_cap); a.counter = counter;
