@@ -194,7 +194,19 @@ def main():
     ap.add_argument("--cpu-sample", default="100000x500000", help="NBxNC of the oracle sample, or 'none'")
     ap.add_argument("--batched", default="one_to_many,small_pairs,large_pairs",
                     help="comma list of batched workloads to add, or 'none'")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 launches), to scale HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.arange(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 8 * n, "bytes_written": 8 * n, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the box has one); --gpus %d is not supported"
               % args.gpus, file=sys.stderr)

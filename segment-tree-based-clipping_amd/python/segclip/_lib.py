@@ -63,6 +63,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError("libsegclip.so not built at %s (run __graft_entry__.build())" % LIB_PATH)
+    # A process can hold one HIP/HSA runtime only.  PyTorch bundles its own
+    # copy with the same soname as ROCm's (libamdhip64.so.7): loading torch
+    # first makes the dynamic loader reuse that copy for libsegclip, so torch
+    # tensors and this library share one runtime (loading libsegclip first
+    # would leave torch without a usable GPU).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     sig = {
         "segclip_create": (C.c_int, [C.POINTER(_vp), C.c_int, _vp]),
