@@ -115,3 +115,19 @@ def min_vertex_edge_distance(b, o):
             best = min(best, float(np.min(np.hypot(*(p - q).T))))
         return best
     return min(one(b, o), one(o, b))
+
+
+def min_consecutive_area(ring):
+    """Smallest |A(v[i-1], v[i], v[i+1])| of a ring: the clean-up removes a vertex
+    below 1e-9, so the area jumps where a triple crosses that tolerance."""
+    p, q, n = np.roll(ring, 1, 0), ring, np.roll(ring, -1, 0)
+    a = (q[:, 0] - p[:, 0]) * (n[:, 1] - p[:, 1]) - (q[:, 1] - p[:, 1]) * (n[:, 0] - p[:, 0])
+    return float(np.min(np.abs(a)))
+
+
+def fd_safe(b, o):
+    """Pairs where central differences with h = 1e-7 see a smooth area: every
+    vertex farther than 1e-4 from the other ring's edges and no consecutive
+    triple near the clean-up tolerance."""
+    return (min_vertex_edge_distance(b, o) > 1e-4 and min_consecutive_area(b) > 1e-6
+            and min_consecutive_area(o) > 1e-6)

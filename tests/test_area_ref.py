@@ -30,7 +30,7 @@ def _generic_pairs(count, seed):
     bases, ovs = S.small_pairs(count, seed)
     out = []
     for b, o in zip(bases, ovs):
-        if R.min_vertex_edge_distance(b, o) > 1e-4 and R.area(b, o) > 1e-3:
+        if R.fd_safe(b, o) and R.area(b, o) > 1e-3:
             out.append((b, o))
     return out
 

@@ -4,8 +4,9 @@ Forward areas against the CPU shoelace over the oracle's rings; gradients
 against the CPU reference gradient (area_ref.grad, itself pinned to finite
 differences of the oracle in test_area_ref.py) and against central finite
 differences of the GPU's own areas (h = 1e-7) on pairs whose minimum
-vertex-to-edge distance exceeds 1e-4; the torch.autograd binding with and
-without recompute in backward.
+vertex-to-edge distance exceeds 1e-4 and whose consecutive vertex triples
+stay clear of the clean-up tolerance (area_ref.fd_safe); the torch.autograd
+binding with and without recompute in backward.
 """
 import numpy as np
 import pytest
@@ -24,7 +25,7 @@ def _generic(count, seed, limit):
     bases, ovs = S.small_pairs(count, seed)
     out = []
     for b, o in zip(bases, ovs):
-        if R.min_vertex_edge_distance(b, o) > 1e-4:
+        if R.fd_safe(b, o):
             out.append((b, o))
         if len(out) == limit:
             break
