@@ -281,14 +281,20 @@ __device__ inline int upper_bound_d(const double *a, int n, double q) {
     return lo;
 }
 
-template <bool FILL>
+// Cover list of an edge: the canonical decomposition of its leaf range
+// [L,R) (L = first elementary point >= start, R = last point <= end), i.e. the
+// nodes v with Int(v) inside [start,end] whose parent's interval is not
+// (constructCoverListGPU, oacc:2958-3128).  EMIT = false counts them (and
+// keeps L,R), EMIT = true writes (node, class) keys edge by edge.
+template <bool EMIT>
 __global__ void k_cover(const double *__restrict__ es, const double *__restrict__ ee, int ne, int nb,
-                        const double *__restrict__ ea, int epc, int nleaves, uint32_t *cntb, uint32_t *cntc,
-                        const uint32_t *__restrict__ off, uint32_t *__restrict__ out, int2 *__restrict__ lr) {
+                        const double *__restrict__ ea, int epc, int nleaves, uint32_t *__restrict__ cnt,
+                        const uint32_t *__restrict__ off, uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                        int2 *__restrict__ lr) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     int L, R;
-    if (!FILL) {
+    if (!EMIT) {
         L = lower_bound_d(ea, epc, es[e]);
         R = upper_bound_d(ea, epc, ee[e]) - 1; // last point <= end
         lr[e] = make_int2(L, R);
@@ -297,31 +303,52 @@ __global__ void k_cover(const double *__restrict__ es, const double *__restrict_
         L = v.x;
         R = v.y;
     }
-    if (L >= R) return;
-    int l = L + nleaves, r = R + nleaves;
-    const bool base = e < nb;
-    while (l < r) {
-        if (l & 1) {
-            int v = l++;
-            if (!FILL) atomicAdd(base ? &cntb[v] : &cntc[v], 1u);
-            else if (base) out[off[v] + atomicAdd(&cntb[v], 1u)] = uint32_t(e);
-            else out[off[v + 1] - 1u - atomicAdd(&cntc[v], 1u)] = uint32_t(e);
+    uint32_t n = 0, at = EMIT ? off[e] : 0u;
+    const uint64_t cls = e < nb ? 0ull : 1ull;
+    if (L < R) {
+        int l = L + nleaves, r = R + nleaves;
+        while (l < r) {
+            if (l & 1) {
+                if (EMIT) {
+                    keys[at + n] = (uint64_t(uint32_t(l)) << 1) | cls;
+                    vals[at + n] = uint32_t(e);
+                }
+                ++n;
+                ++l;
+            }
+            if (r & 1) {
+                --r;
+                if (EMIT) {
+                    keys[at + n] = (uint64_t(uint32_t(r)) << 1) | cls;
+                    vals[at + n] = uint32_t(e);
+                }
+                ++n;
+            }
+            l >>= 1;
+            r >>= 1;
         }
-        if (r & 1) {
-            int v = --r;
-            if (!FILL) atomicAdd(base ? &cntb[v] : &cntc[v], 1u);
-            else if (base) out[off[v] + atomicAdd(&cntb[v], 1u)] = uint32_t(e);
-            else out[off[v + 1] - 1u - atomicAdd(&cntc[v], 1u)] = uint32_t(e);
-        }
-        l >>= 1;
-        r >>= 1;
     }
+    if (!EMIT) cnt[e] = n;
 }
 
-__global__ void k_add_u32(const uint32_t *a, const uint32_t *b, uint32_t *o, int64_t n) {
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) o[i] = a[i] + b[i];
+// per node: first entry and count of base entries in the (node, class)-sorted keys
+__global__ void k_cover_bounds(const uint64_t *__restrict__ keys, int64_t n, int T, uint32_t *off, uint32_t *split) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > T) return;
+    auto lb = [&](uint64_t k) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < k) lo = mid + 1;
+            else hi = mid;
+        }
+        return uint32_t(lo);
+    };
+    const uint32_t a = lb(uint64_t(uint32_t(v)) << 1);
+    off[v] = a;
+    if (v < T) split[v] = lb((uint64_t(uint32_t(v)) << 1) | 1ull) - a;
 }
+
 
 // ---- materialised end lists (parity and drop-in API) ----
 __global__ void k_mend_keys(const uint64_t *__restrict__ lkeys, const uint32_t *__restrict__ lvals, int64_t n,
@@ -578,28 +605,31 @@ void build_cover_lists(Ctx &c) {
     SC_REQUIRE(c.T > 0, SC_ESTATE, "elementary intervals not built");
     hipStream_t s = c.stream;
     const int T = c.T;
-    uint32_t *cntb = c.cov_split.ensure(T);
-    uint32_t *cntc = c.u32b.ensure(T);
-    SC_HIP(hipMemsetAsync(cntb, 0, sizeof(uint32_t) * T, s));
-    SC_HIP(hipMemsetAsync(cntc, 0, sizeof(uint32_t) * T, s));
-    int2 *lr = reinterpret_cast<int2 *>(c.u64b.ensure(c.ne));
-    k_cover<false><<<grid_for(c.ne), BLK, 0, s>>>(c.estart, c.eend, c.ne, c.nb, c.ea, c.epc, c.nleaves, cntb, cntc,
-                                                  nullptr, nullptr, lr);
+    const int ne = c.ne;
+    // per-edge entry counts -> edge-major offsets
+    uint32_t *cnt = c.u32b.ensure(size_t(ne) + 1);
+    int2 *lr = reinterpret_cast<int2 *>(c.u64b.ensure(ne));
+    k_cover<false><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, cnt, nullptr,
+                                                nullptr, nullptr, lr);
     SC_HIP(hipGetLastError());
-    uint32_t *off = c.cov_off.ensure(size_t(T) + 1);
-    k_add_u32<<<grid_for(T), BLK, 0, s>>>(cntb, cntc, off, T);
-    SC_HIP(hipGetLastError());
-    SC_HIP(hipMemsetAsync(off + T, 0, sizeof(uint32_t), s));
+    SC_HIP(hipMemsetAsync(cnt + ne, 0, sizeof(uint32_t), s));
     uint32_t *tot = c.d_u32_total.ensure(1);
-    scan_exclusive_u32(off, off, int64_t(T) + 1, tot, s);
+    scan_exclusive_u32(cnt, cnt, int64_t(ne) + 1, tot, s);
     c.n_cov = read_u32(c, tot);
-    c.cov_e.ensure(size_t(c.n_cov));
-    // fill: base entries from the front, overlay entries from the back
-    uint32_t *fb = c.u32c.ensure(T);
-    SC_HIP(hipMemsetAsync(fb, 0, sizeof(uint32_t) * T, s));
-    SC_HIP(hipMemsetAsync(cntc, 0, sizeof(uint32_t) * T, s));
-    k_cover<true><<<grid_for(c.ne), BLK, 0, s>>>(c.estart, c.eend, c.ne, c.nb, c.ea, c.epc, c.nleaves, fb, cntc,
-                                                 off, c.cov_e, lr);
+    const int64_t L = c.n_cov;
+    // (node, class) keys with the edge as value, edge-major, then a stable
+    // radix sort: within a node base entries come first, each run in edge order
+    uint64_t *keys = c.k64a.ensure(size_t(L > 0 ? L : 1));
+    uint32_t *vals = c.cov_e.ensure(size_t(L > 0 ? L : 1));
+    if (L > 0) {
+        k_cover<true><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, nullptr, cnt,
+                                                   keys, vals, lr);
+        SC_HIP(hipGetLastError());
+        radix_sort_u64(keys, vals, L, 0, ilog2_exact(T) + 1, c.rs, s);
+    }
+    uint32_t *off = c.cov_off.ensure(size_t(T) + 1);
+    uint32_t *split = c.cov_split.ensure(T);
+    k_cover_bounds<<<grid_for(int64_t(T) + 1), BLK, 0, s>>>(keys, L, T, off, split);
     SC_HIP(hipGetLastError());
 }
 
