@@ -31,6 +31,10 @@ void release_derived(Ctx &c) {
     for (DBuf<double> *b : {&c.ex0, &c.ey0, &c.ex1, &c.ey1, &c.estart, &c.eend, &c.uxs, &c.uxe, &c.eymin, &c.eymax,
                             &c.ea, &c.niv, &c.p_x, &c.p_y, &c.aug_bx, &c.aug_by, &c.aug_cx, &c.aug_cy})
         b->release();
+    for (DBuf<int> *b : {&c.sp_slot, &c.sp_ndx, &c.sp_edge, &c.sp_av, &c.sp_nbr, &c.sp_lab, &c.sp_pos, &c.sp_ndf})
+        b->release();
+    c.sp_x.release();
+    c.sp_y.release();
     for (DBuf<int> *b : {&c.etype, &c.eid, &c.after_leaf, &c.before_leaf, &c.p_j, &c.p_b, &c.p_c, &c.p_type,
                          &c.p_qa, &c.p_qb, &c.ordB, &c.ordC, &c.nd_b, &c.nd_c, &c.all_b, &c.all_c, &c.nbr_b,
                          &c.nbr_c, &c.av_b, &c.av_c, &c.lab_b, &c.lab_c, &c.slot_b, &c.slot_c, &c.sv_b, &c.sp_b,
@@ -418,6 +422,10 @@ int segclip_copy_intersections(segclip_ctx *h, double *xy, int32_t *alpha_b, int
 int segclip_copy_augmented(segclip_ctx *h, int side, double *xy, int32_t *nbr, int32_t *alpha, int32_t *label,
                            int64_t cap, int64_t *need) {
     if (!h) return SEGCLIP_EINVAL;
+    if (side == 0 && h->c.sparseP) {
+        h->c.last_error = "the base rings of a one-to-many batch are kept sparse; copy them per overlay";
+        return SEGCLIP_ESTATE;
+    }
     int64_t n = side == 0 ? h->c.nbaug : h->c.ncaug;
     int rc = need_space(n, cap, need);
     if (rc || !xy) return rc;

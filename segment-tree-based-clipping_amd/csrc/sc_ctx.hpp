@@ -2,6 +2,7 @@
 // K overlay rings) and the pipeline stage entry points.
 #pragma once
 #include "sc_common.hpp"
+#include "sc_sparse.hpp"
 
 namespace sc {
 
@@ -94,6 +95,11 @@ struct Ctx {
     DBuf<int> nbr_b, nbr_c, av_b, av_c, lab_b, lab_c;
     DBuf<int> slot_b, slot_c;            // per pair: augmented slot on each ring
     bool prov = false;                   // record slot provenance (for the area gradient)
+    // sparse base ring (one-to-many batches without provenance): per position
+    // (pair in (job, base edge, alpha) order) its virtual slot and slot data
+    bool sparseP = false;
+    DBuf<int> sp_slot, sp_ndx, sp_edge, sp_av, sp_nbr, sp_lab, sp_pos, sp_ndf;
+    DBuf<double> sp_x, sp_y;
     DBuf<int> sv_b, sp_b, sv_c, sp_c;    // per slot: source vertex (local) or pair index, else -1
     DBuf<int> flags;                     // device status words
     // labelling / traversal
@@ -142,6 +148,7 @@ void materialize_end_lists(Ctx &c);
 void build_cover_lists(Ctx &c);
 void find_candidates(Ctx &c);          // all overlays at once, then unique pairs
 void post_jobs(Ctx &c, int k0, int k1); // A11-A15 for overlays k0..k1 in one batch
+SparseP sparse_view(Ctx &c);           // the sparse base ring of the last batch
 void post_overlay(Ctx &c, int k);      // A11-A15 for overlay k
 void label_and_traverse(Ctx &c);       // A16-A17 for the jobs of the last post_jobs
 void clip_all(Ctx &c, int k);          // full pipeline for one overlay

@@ -114,29 +114,19 @@ __global__ __launch_bounds__(BLK) void k_first_plain(Side s, int J, int *first) 
     if (lane == 0) first[j] = res;
 }
 
-// winding test of polygon::pointInPoly (polygon.h:435-457) of the other
-// ring's first plain vertex against this side's rings, summed per job
+// winding test of polygon::pointInPoly (polygon.h:435-457) of each job's
+// query point (the other ring's first plain vertex) against this side's
+// rings, summed per job
 __global__ __launch_bounds__(BLK) void k_jwinding(Side r, int J, int64_t n, const double *qx, const double *qy,
-                                                  const int *qfirst, int *wind) {
+                                                  const int *qok, int *wind) {
+    // qx[2j], qy[2j]: the query point of job j (interleaved x,y)
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     int w = 0, j = -1;
     if (i < n) {
         j = seg_of(r.off, J, i);
-        int qf = qfirst[j];
-        if (qf >= 0) {
-            int64_t qg = r.ooff[j] + qf;
-            double rx = qx[qg], ry = qy[qg];
+        if (qok[j]) {
             int64_t nx = i + 1 == r.off[j + 1] ? r.off[j] : i + 1;
-            double p0x = r.x[i], p0y = r.y[i], p1x = r.x[nx], p1y = r.y[nx];
-            if ((p0y < ry) != (p1y < ry)) {
-                int sg = 2 * (p1y > p0y) - 1;
-                if (p0x >= rx) {
-                    if (p1x > rx) w = sg;
-                    else if ((areaA(p0x, p0y, p1x, p1y, rx, ry) > 0) == (p1y > p0y)) w = sg;
-                } else if (p1x > rx) {
-                    if ((areaA(p0x, p0y, p1x, p1y, rx, ry) > 0) == (p1y > p0y)) w = sg;
-                }
-            }
+            w = wind_edge(r.x[i], r.y[i], r.x[nx], r.y[nx], qx[2 * j], qy[2 * j]);
         }
     }
     // a wave inside one job adds once; a wave across jobs adds per lane
@@ -147,6 +137,137 @@ __global__ __launch_bounds__(BLK) void k_jwinding(Side r, int J, int64_t n, cons
     } else if (w && j >= 0) {
         atomicAdd(&wind[j], w);
     }
+}
+
+// query points: each ring's first plain vertex (dense rings, or the sparse base ring)
+__global__ void k_query_pts(int J, Side P, Side Q, int sparse, SparseP sp, const int *firstP, const int *firstQ,
+                            double *pq, int *pok, double *qq, int *qok) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= J) return;
+    const int fp = firstP[j], fq = firstQ[j];
+    pok[j] = fp >= 0;
+    qok[j] = fq >= 0;
+    double x = 0, y = 0;
+    if (fp >= 0) {
+        if (sparse) {
+            SlotInfo si = sp_slot(sp, j, fp);
+            x = si.x;
+            y = si.y;
+        } else {
+            x = P.x[P.off[j] + fp];
+            y = P.y[P.off[j] + fp];
+        }
+    }
+    pq[2 * j] = x;
+    pq[2 * j + 1] = y;
+    x = y = 0;
+    if (fq >= 0) {
+        x = Q.x[Q.off[j] + fq];
+        y = Q.y[Q.off[j] + fq];
+    }
+    qq[2 * j] = x;
+    qq[2 * j + 1] = y;
+}
+
+// ---- sparse base ring (one-to-many batches) ----
+
+// first plain slot of each job's virtual base ring: the first slot not held by
+// an intersection; one wave per job over its ascending slot list
+__global__ __launch_bounds__(BLK) void k_sp_first_plain(SparseP sp, int J, int *first) {
+    const int j = int((int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / WAVE);
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (j >= J) return; // whole wave
+    const int a = sp.P0[j], b = sp.P0[j + 1];
+    const int nslots = sp.AB[j + 1] - sp.AB[j];
+    int res = b - a < nslots ? b - a : -1;
+    for (int base = a; base < b; base += WAVE) {
+        const int i = base + lane;
+        const bool gap = i < b && sp.slot[i] != i - a;
+        const uint64_t m = __ballot(gap);
+        if (m) {
+            res = base - a + __ffsll((unsigned long long)m) - 1;
+            break;
+        }
+    }
+    if (lane == 0) first[j] = res < nslots ? res : -1;
+}
+
+// winding of each job's query point against every base edge (all jobs share
+// the base ring); corrected below on the edges that hold intersections
+constexpr int WCH = 4096; // base edges per block
+__global__ __launch_bounds__(BLK) void k_sp_wind_base(int J, int nb, const double *__restrict__ bpoly,
+                                                      const double *__restrict__ qq, const int *__restrict__ qok,
+                                                      int *wind) {
+    __shared__ int sh[BLK / WAVE];
+    const int chunks = (nb + WCH - 1) / WCH;
+    const int j = blockIdx.x / chunks, ch = blockIdx.x % chunks;
+    if (j >= J || !qok[j]) return; // whole block
+    const double rx = qq[2 * j], ry = qq[2 * j + 1];
+    const int e1 = min(nb, (ch + 1) * WCH);
+    int w = 0;
+    for (int e = ch * WCH + threadIdx.x; e < e1; e += BLK) {
+        const double *P = bpoly + 2 * int64_t(e);
+        w += wind_edge(P[0], P[1], P[2], P[3], rx, ry);
+    }
+    for (int d = WAVE / 2; d > 0; d >>= 1) w += __shfl_down(w, d, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0) sh[threadIdx.x / WAVE] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < BLK / WAVE; ++k) t += sh[k];
+        if (t) atomicAdd(&wind[j], t);
+    }
+}
+
+// an edge split by non-degenerate intersections: replace its full-edge term
+// by the terms of its pieces (source vertex -> X1 -> ... -> next vertex)
+__global__ void k_sp_wind_fix(int I, int J, SparseP sp, const double *__restrict__ qq, const int *__restrict__ qok,
+                              int *wind) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= I) return;
+    const int j = seg_of(sp.P0, J, i);
+    if (!qok[j]) return;
+    const int e = sp.edge[i];
+    auto is_nd = [&](int k) { return sp.slot[k] >= 0 && sp.slot[k] == sp.edge[k] + 1 + sp.ndx[k]; };
+    if (!is_nd(i)) return;
+    if (i > sp.P0[j] && sp.edge[i - 1] == e && is_nd(i - 1)) return; // not the first piece point
+    const double rx = qq[2 * j], ry = qq[2 * j + 1];
+    const int nb = sp.nbv[j];
+    const double *V = sp.bpoly + 2 * int64_t(sp.bvoff[j]);
+    const double ax = V[2 * e], ay = V[2 * e + 1], bx = V[2 * e + 2], by = V[2 * e + 3]; // closing vertex repeated
+    (void)nb;
+    int w = -wind_edge(ax, ay, bx, by, rx, ry);
+    double px = ax, py = ay;
+    int k = i;
+    while (k < sp.P0[j + 1] && sp.edge[k] == e && is_nd(k)) {
+        w += wind_edge(px, py, sp.x[k], sp.y[k], rx, ry);
+        px = sp.x[k];
+        py = sp.y[k];
+        ++k;
+    }
+    w += wind_edge(px, py, bx, by, rx, ry);
+    if (w) atomicAdd(&wind[j], w);
+}
+
+// entry/exit status on the sparse base ring (positions = its intersections)
+__global__ void k_sp_enex(int I, int J, SparseP sp, const int *lab, const uint32_t *cross, const uint32_t *cc,
+                          const int *first, const int *wind, const int *kind, int *enex, uint8_t *split) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= I) return;
+    const int j = seg_of(sp.P0, J, i);
+    if (kind[j] != K_CROSS) {
+        enex[i] = NEITHER;
+        split[i] = 0;
+        return;
+    }
+    const int V = first[j];
+    const int kV = sp_lower(sp, j, V);
+    const uint32_t m = cc[sp.P0[j + 1]] - cc[sp.P0[j]];
+    const uint32_t c = cc[i] - cc[kV] + (sp.slot[i] < V ? m : 0u);
+    const int st0 = (wind[j] % 2) != 0 ? EXIT : ENTRY;
+    const int st = st0 ^ int(c & 1u);
+    enex[i] = cross[i] ? st : NEITHER;
+    split[i] = (lab[i] == L_BOUNCING && st == EXIT) ? 1 : 0;
 }
 
 struct JobState {
@@ -210,6 +331,9 @@ __global__ void k_compact_cross(int64_t n, const uint32_t *cross, const uint32_t
 struct PieceArgs {
     int J;
     Side P, Q;
+    const int *PAB;  // J+1 slot offsets of the base rings (virtual in the sparse layout)
+    int sparse;      // base side given as positions of a sparse ring
+    SparseP sp;
     const int *cposP, *cposQ;
     const uint32_t *crossP, *crossQ, *ccountP, *ccountQ;
     const int *enexP, *enexQ;
@@ -251,20 +375,44 @@ __global__ void k_jpieces(PieceArgs a) {
     if (jw >= m) jw -= m;
     const int posW = cpos[c0[j] + jw];
     if (enex[posW] != (e ^ 1)) atomicOr(&a.slow[j], SLOW_ODD);
-    const int r0 = R.off[j], n = R.off[j + 1] - r0;
-    int len = d > 0 ? posW - pos : pos - posW;
+    // local slots of the two crossings and the ring length
+    int sl, slW, n;
+    if (onP) {
+        n = a.PAB[j + 1] - a.PAB[j];
+        sl = a.sparse ? a.sp.slot[pos] : pos - R.off[j];
+        slW = a.sparse ? a.sp.slot[posW] : posW - R.off[j];
+    } else {
+        n = R.off[j + 1] - R.off[j];
+        sl = pos - R.off[j];
+        slW = posW - R.off[j];
+    }
+    int len = d > 0 ? slW - sl : sl - slW;
     if (len <= 0) len += n;
-    const int64_t w = int64_t(R.ooff[j]) + R.nbr[posW] - 1; // slot on the other ring
-    const uint32_t *ocross = onP ? a.crossQ : a.crossP;
-    const uint32_t *occ = onP ? a.ccountQ : a.ccountP;
+    // the neighbour of the far crossing on the other ring, as a crossing index
+    const int wl = R.nbr[posW] - 1; // local slot on the other ring
     uint32_t s = uint32_t(x);
-    if (!ocross[w]) atomicOr(&a.slow[j], SLOW_LINK);
-    else s = (onP ? uint32_t(a.mP) : 0u) + occ[w];
-    a.ppos[x] = pos - r0;
+    bool ok;
+    uint32_t occw = 0;
+    if (onP) {
+        const int64_t w = int64_t(R.ooff[j]) + wl;
+        ok = a.crossQ[w] != 0;
+        occw = a.ccountQ[w];
+    } else if (a.sparse) {
+        const int k = sp_find(a.sp, j, wl);
+        ok = k >= 0 && a.crossP[k] != 0;
+        occw = k >= 0 ? a.ccountP[k] : 0u;
+    } else {
+        const int64_t w = int64_t(R.ooff[j]) + wl;
+        ok = a.crossP[w] != 0;
+        occw = a.ccountP[w];
+    }
+    if (!ok) atomicOr(&a.slow[j], SLOW_LINK);
+    else s = (onP ? uint32_t(a.mP) : 0u) + occw;
+    a.ppos[x] = sl;
     a.plen[x] = len;
     a.pdir[x] = d;
     a.succ[x] = s;
-    a.val[x] = onP ? min(2u * uint32_t(pos - r0), 2u * uint32_t(posW - r0) + 1u) : NIL;
+    a.val[x] = onP ? min(2u * uint32_t(sl), 2u * uint32_t(slW) + 1u) : NIL;
 }
 
 // jobs found inconsistent while building pieces take the sequential route
@@ -291,12 +439,18 @@ __global__ void k_jump_min(int M, const uint32_t *val, const uint32_t *jmp, uint
 
 // list ranking of the traversed cycles, broken at their start (a P crossing)
 __global__ void k_rank_init(int M, const uint32_t *cmin, const uint32_t *succ, const int *pj, const int *AB,
-                            const uint32_t *ccountP, const int *plen, uint32_t *inode, uint32_t *nxt, uint32_t *dist) {
+                            int sparse, SparseP sp, const uint32_t *ccountP, const int *plen, uint32_t *inode,
+                            uint32_t *nxt, uint32_t *dist) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M) return;
     uint32_t m = cmin[x];
     bool trav = m != NIL && (m & 1u) == 0u;
-    uint32_t I = trav ? ccountP[int64_t(AB[pj[x]]) + (m >> 1)] : NIL;
+    uint32_t I = NIL;
+    if (trav) {
+        int k = sparse ? sp_find(sp, pj[x], int(m >> 1)) : int(AB[pj[x]] + int(m >> 1));
+        I = k >= 0 ? ccountP[k] : NIL;
+        trav = k >= 0;
+    }
     inode[x] = I;
     nxt[x] = trav ? (succ[x] == I ? NIL : succ[x]) : NIL;
     dist[x] = trav ? uint32_t(plen[x]) : 0u;
@@ -316,12 +470,13 @@ __global__ void k_rank_step(int M, const uint32_t *dist, const uint32_t *nxt, ui
 }
 
 // component totals at their starts (base-ring order)
-__global__ void k_start_totals(int mP, const int *cposP, const int *pj, const int *AB, const uint32_t *cmin,
-                               const uint32_t *dist, uint32_t *tot, uint32_t *isstart) {
+__global__ void k_start_totals(int mP, const int *cposP, const int *pj, const int *AB, const int *pslot,
+                               const uint32_t *cmin, const uint32_t *dist, uint32_t *tot, uint32_t *isstart) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= mP) return;
     uint32_t m = cmin[x];
-    bool s = m != NIL && (m & 1u) == 0u && (m >> 1) == uint32_t(cposP[x] - AB[pj[x]]);
+    const int sl = pslot ? pslot[cposP[x]] : cposP[x] - AB[pj[x]];
+    bool s = m != NIL && (m & 1u) == 0u && (m >> 1) == uint32_t(sl);
     tot[x] = s ? dist[x] : 0u;
     isstart[x] = s ? 1u : 0u;
 }
@@ -377,6 +532,9 @@ __global__ void k_jheads(int M, const uint32_t *inode, const uint32_t *dist, con
 
 struct EmitArgs {
     int J, mP;
+    const int *PAB; // base ring slot offsets (virtual in the sparse layout)
+    int sparse;
+    SparseP sp;
     const uint32_t *JV;
     const int *kind, *ppos, *pdir, *pj;
     const uint32_t *head, *piece_at;
@@ -394,14 +552,64 @@ __global__ void k_jemit(EmitArgs a, int64_t n_out) {
     uint32_t x = a.piece_at[s];
     int k = int(i - s) + 1;
     const bool onP = int(x) < a.mP;
-    const Side &R = onP ? a.P : a.Q;
-    int n = R.off[j + 1] - R.off[j];
-    int pos = (a.ppos[x] + a.pdir[x] * k) % n;
-    if (pos < 0) pos += n;
-    int g = R.off[j] + pos;
-    a.ox[i] = R.x[g];
-    a.oy[i] = R.y[g];
-    a.osrc[i] = onP ? g : ~g;
+    if (onP) {
+        const int n = a.PAB[j + 1] - a.PAB[j];
+        int pos = (a.ppos[x] + a.pdir[x] * k) % n;
+        if (pos < 0) pos += n;
+        if (a.sparse) {
+            const SlotInfo si = sp_slot(a.sp, j, pos);
+            a.ox[i] = si.x;
+            a.oy[i] = si.y;
+            a.osrc[i] = SRC_NONE;
+        } else {
+            const int g = a.P.off[j] + pos;
+            a.ox[i] = a.P.x[g];
+            a.oy[i] = a.P.y[g];
+            a.osrc[i] = g;
+        }
+    } else {
+        const int n = a.Q.off[j + 1] - a.Q.off[j];
+        int pos = (a.ppos[x] + a.pdir[x] * k) % n;
+        if (pos < 0) pos += n;
+        const int g = a.Q.off[j] + pos;
+        a.ox[i] = a.Q.x[g];
+        a.oy[i] = a.Q.y[g];
+        a.osrc[i] = ~g;
+    }
+}
+
+// the base rings of no-crossing jobs in the sparse layout, by output position
+__global__ void k_sp_nocross_p(int64_t n_out, int J, const uint32_t *JV, const int *kind, const int *windP,
+                               SparseP sp, double *ox, double *oy, int *osrc) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    const int j = seg_of(reinterpret_cast<const int *>(JV), J, i);
+    if (kind[j] != K_NOCROSS || (windP[j] % 2) == 0) return;
+    const int q = int(i - JV[j]);
+    if (q >= sp.AB[j + 1] - sp.AB[j]) return; // the overlay part
+    const SlotInfo si = sp_slot(sp, j, q);
+    ox[i] = si.x;
+    oy[i] = si.y;
+    osrc[i] = SRC_NONE;
+}
+
+// dense copies of the sparse base rings of the sequential jobs
+__global__ void k_sp_materialize(int q0, int nseq, const int *jobs_of, const int64_t *moff, SparseP sp,
+                                 const int *av, const int *nbr, const int *lab, double *mx, double *my, int *mav,
+                                 int *mnbr, int *mlab) {
+    const int q = q0 + int(blockIdx.y);
+    if (q >= nseq) return;
+    const int j = jobs_of[q];
+    const int n = sp.AB[j + 1] - sp.AB[j];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const SlotInfo si = sp_slot(sp, j, t);
+        const int64_t o = moff[q] + t;
+        mx[o] = si.x;
+        my[o] = si.y;
+        mav[o] = si.pos >= 0 ? av[si.pos] : -100;
+        mnbr[o] = si.pos >= 0 ? nbr[si.pos] : -100;
+        mlab[o] = si.pos >= 0 ? lab[si.pos] : 0;
+    }
 }
 
 // rings of no-crossing jobs: P's ring when its plain vertex lies inside Q, then Q's
@@ -945,8 +1153,9 @@ struct FosterScratch {
     DBuf<int> jst;
     DBuf<uint32_t> jv, jr;
     DBuf<int> st;
-    DBuf<double> ox, oy;
-    DBuf<int> osrc;
+    DBuf<double> ox, oy, qpts, mx, my;
+    DBuf<int> osrc, qoks, mav, mnbr, mlab, mjob;
+    DBuf<int64_t> moff;
     DBuf<int> off, rstart, ringid;
     DBuf<uint32_t> keep, mark, kscan, rcnt, nonempty;
     // sequential route
@@ -1051,7 +1260,9 @@ void release_foster_scratch() {
                          &f.osrc, &f.off, &f.rstart, &f.ringid, &f.sres, &f.sroff, &f.sosrc, &f.nx, &f.pv, &f.nb,
                          &f.lab, &f.enex, &f.vsrc, &f.cand})
         b->release();
-    for (DBuf<double> *b : {&f.ox, &f.oy, &f.vx, &f.vy, &f.sox, &f.soy}) b->release();
+    for (DBuf<double> *b : {&f.ox, &f.oy, &f.vx, &f.vy, &f.sox, &f.soy, &f.qpts, &f.mx, &f.my}) b->release();
+    for (DBuf<int> *b : {&f.qoks, &f.mav, &f.mnbr, &f.mlab, &f.mjob}) b->release();
+    f.moff.release();
     f.splitP.release();
     f.splitQ.release();
     f.inter.release();
@@ -1064,16 +1275,22 @@ void label_and_traverse(Ctx &c) {
     hipStream_t s = c.stream;
     FosterScratch &f = fscratch();
     const int J = js.J;
-    const int64_t nP = js.AB[J], nQ = js.AC[J];
+    const bool sparse = c.sparseP;
     const int *AB = js.d_AB.p, *AC = js.d_AC.p;
-    const Side P{AB, AC, c.aug_bx, c.aug_by, c.lab_b, c.nbr_b, c.av_b};
+    // the base side is either its dense rings (slots) or, in the sparse layout,
+    // its intersections (positions); the overlay side is always dense
+    const SparseP sp = sparse ? sparse_view(c) : SparseP{};
+    const int *PO = sparse ? js.d_p0.p : AB; // offsets of the base-side arrays
+    const int64_t nP = sparse ? js.p0[J] : js.AB[J], nQ = js.AC[J];
+    const Side P = sparse ? Side{PO, AC, c.sp_x, c.sp_y, c.sp_lab, c.sp_nbr, c.sp_av}
+                          : Side{AB, AC, c.aug_bx, c.aug_by, c.lab_b, c.nbr_b, c.av_b};
     const Side Q{AC, AB, c.aug_cx, c.aug_cy, c.lab_c, c.nbr_c, nullptr};
     // per-job words
     int *jw = f.jst.ensure(11 * int64_t(J) + 4);
     SC_HIP(hipMemsetAsync(jw, 0, sizeof(int) * (11 * size_t(J) + 4), s));
     JobState st;
     st.J = J;
-    st.AB = AB;
+    st.AB = PO;
     st.AC = AC;
     st.slow = jw;
     st.firstP = jw + J;
@@ -1087,9 +1304,10 @@ void label_and_traverse(Ctx &c) {
     st.cQ0 = jw + 9 * J + 1;      // J+1
     int *sidx = jw + 10 * J + 2;  // J
     uint32_t *crossP = f.crossP.ensure(nP + 1), *crossQ = f.crossQ.ensure(nQ + 1);
-    k_jclassify<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, crossP, st.slow);
+    if (nP > 0) k_jclassify<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, crossP, st.slow);
     k_jclassify<<<grid_for(nQ), BLK, 0, s>>>(Q, J, nQ, crossQ, st.slow);
-    k_first_plain<<<grid_for(int64_t(J) * WAVE), BLK, 0, s>>>(P, J, st.firstP);
+    if (sparse) k_sp_first_plain<<<grid_for(int64_t(J) * WAVE), BLK, 0, s>>>(sp, J, st.firstP);
+    else k_first_plain<<<grid_for(int64_t(J) * WAVE), BLK, 0, s>>>(P, J, st.firstP);
     k_first_plain<<<grid_for(int64_t(J) * WAVE), BLK, 0, s>>>(Q, J, st.firstQ);
     SC_HIP(hipGetLastError());
     SC_HIP(hipMemsetAsync(crossP + nP, 0, sizeof(uint32_t), s));
@@ -1099,8 +1317,20 @@ void label_and_traverse(Ctx &c) {
     scan_exclusive_u32(crossP, ccP, nP + 1, tot, s);
     scan_exclusive_u32(crossQ, ccQ, nQ + 1, tot + 1, s);
     // point-in-polygon of each ring's first plain vertex w.r.t. the other ring
-    k_jwinding<<<grid_for(nQ), BLK, 0, s>>>(Q, J, nQ, c.aug_bx, c.aug_by, st.firstP, st.windP);
-    k_jwinding<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, c.aug_cx, c.aug_cy, st.firstQ, st.windQ);
+    double *qpts = f.qpts.ensure(4 * size_t(J));
+    int *qoks = f.qoks.ensure(2 * size_t(J));
+    double *pq = qpts, *qq = qpts + 2 * size_t(J);
+    int *pok = qoks, *qok = qoks + J;
+    k_query_pts<<<grid_for(J), BLK, 0, s>>>(J, P, Q, sparse ? 1 : 0, sp, st.firstP, st.firstQ, pq, pok, qq, qok);
+    SC_HIP(hipGetLastError());
+    k_jwinding<<<grid_for(nQ), BLK, 0, s>>>(Q, J, nQ, pq, pq + 1, pok, st.windP);
+    if (sparse) {
+        const int chunks = (c.nb + WCH - 1) / WCH;
+        k_sp_wind_base<<<unsigned(int64_t(J) * chunks), BLK, 0, s>>>(J, c.nb, c.bpoly, qq, qok, st.windQ);
+        if (nP > 0) k_sp_wind_fix<<<grid_for(nP), BLK, 0, s>>>(int(nP), J, sp, qq, qok, st.windQ);
+    } else {
+        k_jwinding<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, qq, qq + 1, qok, st.windQ);
+    }
     k_jdecide<<<grid_for(J + 1), BLK, 0, s>>>(st, ccP, ccQ);
     SC_HIP(hipGetLastError());
     uint32_t htot[2];
@@ -1112,7 +1342,12 @@ void label_and_traverse(Ctx &c) {
     uint8_t *splitP = f.splitP.ensure(nP), *splitQ = f.splitQ.ensure(nQ);
     PieceArgs pa;
     if (M > 0) {
-        k_jenex<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, crossP, ccP, st.firstP, st.windP, st.kind, enexP, splitP);
+        if (sparse)
+            k_sp_enex<<<grid_for(nP), BLK, 0, s>>>(int(nP), J, sp, c.sp_lab, crossP, ccP, st.firstP, st.windP,
+                                                   st.kind, enexP, splitP);
+        else
+            k_jenex<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, crossP, ccP, st.firstP, st.windP, st.kind, enexP,
+                                                 splitP);
         k_jenex<<<grid_for(nQ), BLK, 0, s>>>(Q, J, nQ, crossQ, ccQ, st.firstQ, st.windQ, st.kind, enexQ, splitQ);
         k_jsplit_pairs<<<grid_for(nP), BLK, 0, s>>>(P, J, nP, splitP, splitQ, st.slow);
         int *cposP = f.cposP.ensure(mP + 1), *cposQ = f.cposQ.ensure(mQ + 1);
@@ -1121,6 +1356,9 @@ void label_and_traverse(Ctx &c) {
         pa.J = J;
         pa.P = P;
         pa.Q = Q;
+        pa.PAB = AB;
+        pa.sparse = sparse ? 1 : 0;
+        pa.sp = sp;
         pa.cposP = cposP;
         pa.cposQ = cposQ;
         pa.crossP = crossP;
@@ -1163,14 +1401,18 @@ void label_and_traverse(Ctx &c) {
     std::vector<SeqJob> sj(static_cast<size_t>(nseq));
     if (nseq > 0) {
         std::vector<int> hsidx(static_cast<size_t>(J), -1);
-        int64_t vb = 0, ob = 0, rb = 0;
+        int64_t vb = 0, ob = 0, rb = 0, mb = 0;
+        std::vector<int64_t> moff(static_cast<size_t>(nseq));
         for (int q = 0; q < nseq; ++q) {
             const int j = seq_jobs[q];
             SeqJob &b = sj[q];
             b.j = j;
             b.nP = js.AB[j + 1] - js.AB[j];
             b.nQ = js.AC[j + 1] - js.AC[j];
-            b.P0 = js.AB[j];
+            // sparse layout: the base ring is materialised densely for this job
+            moff[q] = mb;
+            b.P0 = sparse ? int(mb) : js.AB[j];
+            mb += b.nP;
             b.Q0 = js.AC[j];
             b.cap = 2 * (b.nP + b.nQ) + 64;
             b.out_cap = 2 * (b.nP + b.nQ) + 64;
@@ -1190,6 +1432,21 @@ void label_and_traverse(Ctx &c) {
         b.jobs = djobs;
         b.px = c.aug_bx; b.py = c.aug_by; b.qx = c.aug_cx; b.qy = c.aug_cy;
         b.avP = c.av_b; b.labP = c.lab_b; b.nbrP = c.nbr_b; b.labQ = c.lab_c; b.nbrQ = c.nbr_c;
+        if (sparse) {
+            double *mx = f.mx.ensure(mb), *my = f.my.ensure(mb);
+            int *mav = f.mav.ensure(mb), *mnbr = f.mnbr.ensure(mb), *mlab = f.mlab.ensure(mb);
+            int *djob = f.mjob.ensure(nseq);
+            int64_t *dmoff = f.moff.ensure(nseq);
+            SC_HIP(hipMemcpyAsync(djob, seq_jobs.data(), sizeof(int) * nseq, hipMemcpyHostToDevice, s));
+            SC_HIP(hipMemcpyAsync(dmoff, moff.data(), sizeof(int64_t) * nseq, hipMemcpyHostToDevice, s));
+            for (int q0 = 0; q0 < nseq; q0 += 65535) {
+                dim3 g(64, unsigned(std::min(65535, nseq - q0)));
+                k_sp_materialize<<<g, BLK, 0, s>>>(q0, nseq, djob, dmoff, sp, c.sp_av, c.sp_nbr, c.sp_lab, mx, my,
+                                                   mav, mnbr, mlab);
+                SC_HIP(hipGetLastError());
+            }
+            b.px = mx; b.py = my; b.avP = mav; b.labP = mlab; b.nbrP = mnbr;
+        }
         b.vx = f.vx.ensure(vb); b.vy = f.vy.ensure(vb);
         b.nx = f.nx.ensure(vb); b.pv = f.pv.ensure(vb); b.nb = f.nb.ensure(vb); b.lab = f.lab.ensure(vb);
         b.enex = f.enex.ensure(vb); b.vsrc = f.vsrc.ensure(vb); b.cand = f.cand.ensure(vb);
@@ -1217,7 +1474,8 @@ void label_and_traverse(Ctx &c) {
         inode = f.inode.ensure(M);
         uint32_t *nxt = jmp2, *nxt2 = jmp, *dist2 = f.dist2.ensure(M);
         dist = f.dist.ensure(M);
-        k_rank_init<<<grid_for(M), BLK, 0, s>>>(M, cmin, pa.succ, pa.pj, AB, ccP, pa.plen, inode, nxt, dist);
+        k_rank_init<<<grid_for(M), BLK, 0, s>>>(M, cmin, pa.succ, pa.pj, AB, sparse ? 1 : 0, sp, ccP, pa.plen, inode,
+                                                nxt, dist);
         for (int r = 0; r < rounds; ++r) {
             k_rank_step<<<grid_for(M), BLK, 0, s>>>(M, dist, nxt, dist2, nxt2);
             std::swap(dist, dist2);
@@ -1225,7 +1483,8 @@ void label_and_traverse(Ctx &c) {
         }
         SC_HIP(hipGetLastError());
         uint32_t *isstart = f.isstart.ensure(int64_t(mP) + 1);
-        k_start_totals<<<grid_for(mP), BLK, 0, s>>>(mP, pa.cposP, pa.pj, AB, cmin, dist, base, isstart);
+        k_start_totals<<<grid_for(mP), BLK, 0, s>>>(mP, pa.cposP, pa.pj, AB, sparse ? c.sp_slot.p : nullptr, cmin,
+                                                    dist, base, isstart);
         SC_HIP(hipMemsetAsync(base + mP, 0, sizeof(uint32_t), s));
         SC_HIP(hipMemsetAsync(isstart + mP, 0, sizeof(uint32_t), s));
         SC_HIP(hipGetLastError());
@@ -1273,6 +1532,9 @@ void label_and_traverse(Ctx &c) {
         EmitArgs ea;
         ea.J = J;
         ea.mP = mP;
+        ea.PAB = AB;
+        ea.sparse = sparse ? 1 : 0;
+        ea.sp = sp;
         ea.JV = jv;
         ea.kind = st.kind;
         ea.ppos = pa.ppos;
@@ -1289,8 +1551,11 @@ void label_and_traverse(Ctx &c) {
         SC_HIP(hipGetLastError());
     }
     if (any_nocross) {
-        k_jcopy_nocross<<<grid_for(nP), BLK, 0, s>>>(P, true, J, nP, st.kind, st.windP, st.windQ, AB, jv, f.ox,
-                                                     f.oy, f.osrc);
+        if (sparse)
+            k_sp_nocross_p<<<grid_for(NV), BLK, 0, s>>>(NV, J, jv, st.kind, st.windP, sp, f.ox, f.oy, f.osrc);
+        else
+            k_jcopy_nocross<<<grid_for(nP), BLK, 0, s>>>(P, true, J, nP, st.kind, st.windP, st.windQ, AB, jv, f.ox,
+                                                         f.oy, f.osrc);
         k_jcopy_nocross<<<grid_for(nQ), BLK, 0, s>>>(Q, false, J, nQ, st.kind, st.windP, st.windQ, AB, jv, f.ox,
                                                      f.oy, f.osrc);
         k_jnocross_off<<<grid_for(J), BLK, 0, s>>>(J, st.kind, st.windP, st.windQ, AB, jv, jr, off);

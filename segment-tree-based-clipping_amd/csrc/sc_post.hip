@@ -85,9 +85,11 @@ __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, in
     py[p] = y;
     qa[p] = alpha_to_int(al);
     qb[p] = alpha_to_int(be);
-    atomicAdd(&all_b[ja.JB[j] + b], 1);
+    if (all_b) {
+        atomicAdd(&all_b[ja.JB[j] + b], 1);
+        if (nondeg_b(t)) atomicAdd(&nd_b[ja.JB[j] + b], 1);
+    }
     atomicAdd(&all_c[ja.JC[j] + cl], 1);
-    if (nondeg_b(t)) atomicAdd(&nd_b[ja.JB[j] + b], 1);
     if (nondeg_c(t)) atomicAdd(&nd_c[ja.JC[j] + cl], 1);
 }
 
@@ -130,7 +132,7 @@ __global__ void k_job_sizes(JobArgs ja, const int *__restrict__ nd_b, const int 
                             int *nca) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ja.J) return;
-    nba[j] = ja.nb[j] + (nd_b[ja.JB[j + 1]] - nd_b[ja.JB[j]]);
+    if (nd_b) nba[j] = ja.nb[j] + (nd_b[ja.JB[j + 1]] - nd_b[ja.JB[j]]);
     nca[j] = ja.nc[j] + (nd_c[ja.JC[j + 1]] - nd_c[ja.JC[j]]);
 }
 
@@ -236,6 +238,20 @@ __device__ inline int rel_pos(int pMN, int pPN, int q, double qx, double qy, dou
     return (s1 < 0 && s2 < 0) ? 1 : 0;
 }
 
+__device__ inline int label_of(int pMN, int pPN, int qm, int qp, double qmx, double qmy, double qpx, double qpy,
+                               double pmx, double pmy, double X, double Y, double ppx, double ppy) {
+    int a = rel_pos(pMN, pPN, qm, qmx, qmy, pmx, pmy, X, Y, ppx, ppy);
+    int b = rel_pos(pMN, pPN, qp, qpx, qpy, pmx, pmy, X, Y, ppx, ppy);
+    if ((a == 0 && b == 1) || (a == 1 && b == 0)) return 1;
+    if ((a == 0 && b == 0) || (a == 1 && b == 1)) return 2;
+    if ((b == 3 && a == 1) || (a == 3 && b == 1)) return 3;
+    if ((b == 3 && a == 0) || (a == 3 && b == 0)) return 4;
+    if ((b == 3 && a == 2) || (a == 3 && b == 2)) return 5;
+    if ((a == 2 && b == 1) || (b == 2 && a == 1)) return 6;
+    if ((a == 2 && b == 0) || (b == 2 && a == 0)) return 7;
+    return -102;
+}
+
 __global__ void k_init_labels(int J, const int *__restrict__ AB, const int *__restrict__ AC,
                               const double *__restrict__ bx, const double *__restrict__ by,
                               const int *__restrict__ avb, const int *__restrict__ nbrb,
@@ -258,21 +274,102 @@ __global__ void k_init_labels(int J, const int *__restrict__ AB, const int *__re
     }
     int qm = nId - 1 < 0 ? ncaug - 1 : nId - 1;
     int qp = nId + 1 == ncaug ? 0 : nId + 1;
-    double X = bx[g], Y = by[g];
-    double pmx = bx[B0 + tm], pmy = by[B0 + tm], ppx = bx[B0 + tp], ppy = by[B0 + tp];
-    int a = rel_pos(pMN, pPN, qm, cx[C0 + qm], cy[C0 + qm], pmx, pmy, X, Y, ppx, ppy);
-    int b = rel_pos(pMN, pPN, qp, cx[C0 + qp], cy[C0 + qp], pmx, pmy, X, Y, ppx, ppy);
-    int lab;
-    if ((a == 0 && b == 1) || (a == 1 && b == 0)) lab = 1;
-    else if ((a == 0 && b == 0) || (a == 1 && b == 1)) lab = 2;
-    else if ((b == 3 && a == 1) || (a == 3 && b == 1)) lab = 3;
-    else if ((b == 3 && a == 0) || (a == 3 && b == 0)) lab = 4;
-    else if ((b == 3 && a == 2) || (a == 3 && b == 2)) lab = 5;
-    else if ((a == 2 && b == 1) || (b == 2 && a == 1)) lab = 6;
-    else if ((a == 2 && b == 0) || (b == 2 && a == 0)) lab = 7;
-    else lab = -102;
+    int lab = label_of(pMN, pPN, qm, qp, cx[C0 + qm], cy[C0 + qm], cx[C0 + qp], cy[C0 + qp], bx[B0 + tm],
+                       by[B0 + tm], bx[g], by[g], bx[B0 + tp], by[B0 + tp]);
     labb[g] = lab;
     labc[C0 + nId] = lab;
+}
+
+// ---- sparse base ring ----
+
+// per position: leading-degenerate / non-degenerate class (B side)
+__global__ void k_sp_mark(int I, const int *__restrict__ ord, const int *__restrict__ pj, const int *__restrict__ pb,
+                          const int *__restrict__ pt, const int *__restrict__ p0, int *kind, uint32_t *ndf,
+                          int *flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= I) return;
+    int p = ord[i], j = pj[p], e = pb[p], t = pt[p];
+    bool start = i == p0[j] || pb[ord[i - 1]] != e;
+    bool nd = nondeg_b(t), lead = start && degen_b_first(t);
+    if (!nd && !lead) atomicOr(&flags[j], UB_UNASSIGNED);
+    kind[i] = lead ? 1 : (nd ? 2 : 0);
+    ndf[i] = nd ? 1u : 0u;
+}
+
+__global__ void k_sp_fill(int I, const int *__restrict__ ord, const int *__restrict__ pj, const int *__restrict__ pb,
+                          const int *__restrict__ p0, const uint32_t *__restrict__ ndg, const int *__restrict__ bvoff,
+                          const double *__restrict__ bpoly, const double *__restrict__ px,
+                          const double *__restrict__ py, const int *__restrict__ qa, int *slot, int *ndx, int *edge,
+                          int *av, int *nbr, int *lab, double *x, double *y, int *pos, const int *kind) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= I) return;
+    int p = ord[i], j = pj[p], e = pb[p];
+    int rel = int(ndg[i] - ndg[p0[j]]);
+    int k = kind[i];
+    slot[i] = k == 1 ? e + rel : (k == 2 ? e + 1 + rel : -1);
+    ndx[i] = rel;
+    edge[i] = e;
+    av[i] = qa[p];
+    nbr[i] = -100;
+    lab[i] = 0;
+    if (k == 2) {
+        x[i] = px[p];
+        y[i] = py[p];
+    } else {
+        const double *V = bpoly + 2 * (int64_t(bvoff[j]) + e);
+        x[i] = V[0];
+        y[i] = V[1];
+    }
+    pos[p] = i;
+}
+
+__global__ void k_sp_sizes(int J, const int *__restrict__ p0, const uint32_t *__restrict__ ndg,
+                           const int *__restrict__ nb, int *nba) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= J) return;
+    nba[j] = nb[j] + int(ndg[p0[j + 1]] - ndg[p0[j]]);
+}
+
+__global__ void k_sp_link(int I, const int *__restrict__ pos, const int *__restrict__ pj,
+                          const int *__restrict__ slot, const int *__restrict__ scn, const int *__restrict__ AC,
+                          int nca, int *nbr_p, int *nbr_c, int *flags) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= I) return;
+    int i = pos[p], j = pj[p];
+    int a = slot[i], b = scn[p];
+    if (a < 0 || b < 0 || b >= nca) {
+        atomicOr(&flags[j], UB_UNASSIGNED);
+        return;
+    }
+    nbr_p[i] = b - AC[j] + 1;
+    nbr_c[b] = a + 1;
+}
+
+__global__ void k_sp_init_labels(int I, int J, SparseP sp, const int *__restrict__ av, const int *__restrict__ nbr,
+                                 const int *__restrict__ AC, const double *__restrict__ cx,
+                                 const double *__restrict__ cy, int *lab, int *labc, int *flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= I) return;
+    const int q = sp.slot[i];
+    if (q < 0) return;
+    const int j = seg_of(sp.P0, J, i);
+    const int nbaug = sp.AB[j + 1] - sp.AB[j], C0 = AC[j], ncaug = AC[j + 1] - C0;
+    const int tm = q - 1 < 0 ? nbaug - 1 : q - 1;
+    const int tp = q + 1 == nbaug ? 0 : q + 1;
+    const SlotInfo pm = sp_slot(sp, j, tm), pp = sp_slot(sp, j, tp);
+    const int pMN = pm.pos >= 0 ? nbr[pm.pos] - 1 : -100;
+    const int pPN = pp.pos >= 0 ? nbr[pp.pos] - 1 : -100;
+    const int nId = nbr[i] - 1;
+    if (nId < 0 || nId >= ncaug) {
+        atomicOr(&flags[j], UB_UNASSIGNED);
+        return;
+    }
+    const int qm = nId - 1 < 0 ? ncaug - 1 : nId - 1;
+    const int qp = nId + 1 == ncaug ? 0 : nId + 1;
+    const int l = label_of(pMN, pPN, qm, qp, cx[C0 + qm], cy[C0 + qm], cx[C0 + qp], cy[C0 + qp], pm.x, pm.y,
+                           sp.x[i], sp.y[i], pp.x, pp.y);
+    lab[i] = l;
+    labc[C0 + nId] = l;
 }
 
 int bits_for(int64_t n) {
@@ -336,15 +433,22 @@ void post_jobs(Ctx &c, int k0, int k1) {
     upload(c, js.d_cin, js.cin);
     const int nJB = js.JB[J], nJC = js.JC[J];
     const int Ia = I > 0 ? I : 1;
+    // one-to-many batches keep the shared base ring sparse (no dense copy per job)
+    const bool sparse = !c.paired && !c.prov && J >= 2;
+    c.sparseP = sparse;
     c.p_j.ensure(Ia); c.p_b.ensure(Ia); c.p_c.ensure(Ia); c.p_type.ensure(Ia); c.p_x.ensure(Ia); c.p_y.ensure(Ia);
     c.p_qa.ensure(Ia); c.p_qb.ensure(Ia); c.ordB.ensure(Ia); c.ordC.ensure(Ia);
     c.slot_b.ensure(Ia); c.slot_c.ensure(Ia);
-    int *all_b = c.all_b.ensure(nJB + 1), *nd_b = c.nd_b.ensure(nJB + 1);
+    int *all_b = nullptr, *nd_b = nullptr;
+    if (!sparse) {
+        all_b = c.all_b.ensure(nJB + 1);
+        nd_b = c.nd_b.ensure(nJB + 1);
+        SC_HIP(hipMemsetAsync(all_b, 0, sizeof(int) * (nJB + 1), s));
+        SC_HIP(hipMemsetAsync(nd_b, 0, sizeof(int) * (nJB + 1), s));
+    }
     int *all_c = c.all_c.ensure(nJC + 1), *nd_c = c.nd_c.ensure(nJC + 1);
     int *flags = c.flags.ensure(J);
     SC_HIP(hipMemsetAsync(flags, 0, sizeof(int) * J, s));
-    SC_HIP(hipMemsetAsync(all_b, 0, sizeof(int) * (nJB + 1), s));
-    SC_HIP(hipMemsetAsync(nd_b, 0, sizeof(int) * (nJB + 1), s));
     SC_HIP(hipMemsetAsync(all_c, 0, sizeof(int) * (nJC + 1), s));
     SC_HIP(hipMemsetAsync(nd_c, 0, sizeof(int) * (nJC + 1), s));
     SC_HIP(hipMemsetAsync(c.slot_b.p, 0xff, sizeof(int) * Ia, s));
@@ -358,27 +462,12 @@ void post_jobs(Ctx &c, int k0, int k1) {
     auto scan = [&](int *a, int64_t n) {
         scan_exclusive_u32(reinterpret_cast<uint32_t *>(a), reinterpret_cast<uint32_t *>(a), n, nullptr, s);
     };
-    scan(all_b, nJB + 1);
-    scan(nd_b, nJB + 1);
+    if (!sparse) {
+        scan(all_b, nJB + 1);
+        scan(nd_b, nJB + 1);
+    }
     scan(all_c, nJC + 1);
     scan(nd_c, nJC + 1);
-    // augmented ring sizes and offsets per job
-    int *nba = js.d_AB.ensure(J + 1), *nca = js.d_AC.ensure(J + 1);
-    k_job_sizes<<<grid_for(J), BLK, 0, s>>>(ja, nd_b, nd_c, nba, nca);
-    SC_HIP(hipGetLastError());
-    SC_HIP(hipMemsetAsync(nba + J, 0, sizeof(int), s));
-    SC_HIP(hipMemsetAsync(nca + J, 0, sizeof(int), s));
-    uint32_t *tot = c.d_u32_total.ensure(2);
-    scan_exclusive_u32(reinterpret_cast<uint32_t *>(nba), reinterpret_cast<uint32_t *>(nba), J + 1, tot, s);
-    scan_exclusive_u32(reinterpret_cast<uint32_t *>(nca), reinterpret_cast<uint32_t *>(nca), J + 1, tot + 1, s);
-    js.AB.resize(J + 1);
-    js.AC.resize(J + 1);
-    SC_HIP(hipMemcpyAsync(js.AB.data(), nba, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipMemcpyAsync(js.AC.data(), nca, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipStreamSynchronize(s));
-    const int64_t nba_tot = js.AB[J], nca_tot = js.AC[J];
-    c.nbaug = int(nba_tot);
-    c.ncaug = int(nca_tot);
 
     if (I > 0) {
         // base side: pairs are in (job, base edge, overlay edge) order; stable
@@ -409,8 +498,48 @@ void post_jobs(Ctx &c, int k0, int k1) {
         k_copy_order<<<grid_for(I), BLK, 0, s>>>(av, I, c.ordC);
         SC_HIP(hipGetLastError());
     }
+    // augmented ring sizes and offsets per job
+    int *nba = js.d_AB.ensure(J + 1), *nca = js.d_AC.ensure(J + 1);
+    if (sparse) {
+        // virtual slot of every base-side intersection
+        int *kind = reinterpret_cast<int *>(c.u32c.ensure(Ia));
+        uint32_t *ndf = reinterpret_cast<uint32_t *>(c.sp_ndf.ensure(size_t(I) + 1));
+        SC_HIP(hipMemsetAsync(ndf + I, 0, sizeof(uint32_t), s));
+        if (I > 0) {
+            k_sp_mark<<<grid_for(I), BLK, 0, s>>>(I, c.ordB, c.p_j, c.p_b, c.p_type, ja.p0, kind, ndf, flags);
+            SC_HIP(hipGetLastError());
+        }
+        scan_exclusive_u32(ndf, ndf, int64_t(I) + 1, nullptr, s);
+        int *slot = c.sp_slot.ensure(Ia), *ndx = c.sp_ndx.ensure(Ia), *edge = c.sp_edge.ensure(Ia);
+        int *sav = c.sp_av.ensure(Ia), *snbr = c.sp_nbr.ensure(Ia), *pos = c.sp_pos.ensure(Ia);
+        double *sx = c.sp_x.ensure(Ia), *sy = c.sp_y.ensure(Ia);
+        c.sp_lab.ensure(Ia);
+        if (I > 0) {
+            k_sp_fill<<<grid_for(I), BLK, 0, s>>>(I, c.ordB, c.p_j, c.p_b, ja.p0, ndf, ja.bvoff, c.bpoly, c.p_x,
+                                                  c.p_y, c.p_qa, slot, ndx, edge, sav, snbr, c.sp_lab, sx, sy, pos,
+                                                  kind);
+            SC_HIP(hipGetLastError());
+        }
+        k_sp_sizes<<<grid_for(J), BLK, 0, s>>>(J, ja.p0, ndf, ja.nb, nba);
+        SC_HIP(hipGetLastError());
+    }
+    k_job_sizes<<<grid_for(J), BLK, 0, s>>>(ja, nd_b, nd_c, nba, nca);
+    SC_HIP(hipGetLastError());
+    SC_HIP(hipMemsetAsync(nba + J, 0, sizeof(int), s));
+    SC_HIP(hipMemsetAsync(nca + J, 0, sizeof(int), s));
+    uint32_t *tot = c.d_u32_total.ensure(2);
+    scan_exclusive_u32(reinterpret_cast<uint32_t *>(nba), reinterpret_cast<uint32_t *>(nba), J + 1, tot, s);
+    scan_exclusive_u32(reinterpret_cast<uint32_t *>(nca), reinterpret_cast<uint32_t *>(nca), J + 1, tot + 1, s);
+    js.AB.resize(J + 1);
+    js.AC.resize(J + 1);
+    SC_HIP(hipMemcpyAsync(js.AB.data(), nba, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipMemcpyAsync(js.AC.data(), nca, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipStreamSynchronize(s));
+    const int64_t nba_tot = js.AB[J], nca_tot = js.AC[J];
+    c.nbaug = int(nba_tot);
+    c.ncaug = int(nca_tot);
+
     const int64_t nbaA = nba_tot > 0 ? nba_tot : 1, ncaA = nca_tot > 0 ? nca_tot : 1;
-    c.aug_bx.ensure(nbaA); c.aug_by.ensure(nbaA); c.av_b.ensure(nbaA); c.nbr_b.ensure(nbaA); c.lab_b.ensure(nbaA);
     c.aug_cx.ensure(ncaA); c.aug_cy.ensure(ncaA); c.av_c.ensure(ncaA); c.nbr_c.ensure(ncaA); c.lab_c.ensure(ncaA);
     int *svb = nullptr, *spb = nullptr, *svc = nullptr, *spc = nullptr;
     if (c.prov) {
@@ -419,27 +548,39 @@ void post_jobs(Ctx &c, int k0, int k1) {
         svc = c.sv_c.ensure(ncaA);
         spc = c.sp_c.ensure(ncaA);
     }
-    k_init_aug<<<grid_for(nba_tot), BLK, 0, s>>>(nba_tot, c.aug_bx, c.aug_by, c.av_b, c.nbr_b, c.lab_b, svb, spb);
+    if (!sparse) {
+        c.aug_bx.ensure(nbaA); c.aug_by.ensure(nbaA); c.av_b.ensure(nbaA); c.nbr_b.ensure(nbaA); c.lab_b.ensure(nbaA);
+        k_init_aug<<<grid_for(nba_tot), BLK, 0, s>>>(nba_tot, c.aug_bx, c.aug_by, c.av_b, c.nbr_b, c.lab_b, svb, spb);
+        k_aug_sources<<<grid_for(nJB), BLK, 0, s>>>(J, ja.JB, ja.nb, ja.bvoff, nba, c.bpoly, nd_b, c.aug_bx,
+                                                    c.aug_by, svb);
+    }
     k_init_aug<<<grid_for(nca_tot), BLK, 0, s>>>(nca_tot, c.aug_cx, c.aug_cy, c.av_c, c.nbr_c, c.lab_c, svc, spc);
-    SC_HIP(hipGetLastError());
-    k_aug_sources<<<grid_for(nJB), BLK, 0, s>>>(J, ja.JB, ja.nb, ja.bvoff, nba, c.bpoly, nd_b, c.aug_bx, c.aug_by,
-                                                svb);
     k_aug_sources<<<grid_for(nJC), BLK, 0, s>>>(J, ja.JC, ja.nc, ja.cvoff, nca, c.cpoly, nd_c, c.aug_cx, c.aug_cy,
                                                 svc);
     SC_HIP(hipGetLastError());
     if (I > 0) {
-        k_aug_segments<true><<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, I, c.ordB, ja.JB, nba, all_b, nd_b, c.p_type,
-                                                         c.p_x, c.p_y, c.p_qa, c.aug_bx, c.aug_by, c.av_b, c.slot_b,
-                                                         flags, spb);
+        if (!sparse)
+            k_aug_segments<true><<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, I, c.ordB, ja.JB, nba, all_b, nd_b,
+                                                             c.p_type, c.p_x, c.p_y, c.p_qa, c.aug_bx, c.aug_by,
+                                                             c.av_b, c.slot_b, flags, spb);
         k_aug_segments<false><<<grid_for(I), BLK, 0, s>>>(c.p_c, c.p_j, I, c.ordC, ja.JC, nca, all_c, nd_c,
                                                           c.p_type, c.p_x, c.p_y, c.p_qb, c.aug_cx, c.aug_cy, c.av_c,
                                                           c.slot_c, flags, spc);
         SC_HIP(hipGetLastError());
-        k_link<<<grid_for(I), BLK, 0, s>>>(c.slot_b, c.slot_c, c.p_j, I, nba, nca, c.nbr_b, c.nbr_c, flags);
-        SC_HIP(hipGetLastError());
-        k_init_labels<<<grid_for(nba_tot), BLK, 0, s>>>(J, nba, nca, c.aug_bx, c.aug_by, c.av_b, c.nbr_b, c.aug_cx,
-                                                        c.aug_cy, c.lab_b, c.lab_c, flags);
-        SC_HIP(hipGetLastError());
+        if (sparse) {
+            k_sp_link<<<grid_for(I), BLK, 0, s>>>(I, c.sp_pos, c.p_j, c.sp_slot, c.slot_c, nca, int(nca_tot),
+                                                  c.sp_nbr, c.nbr_c, flags);
+            SC_HIP(hipGetLastError());
+            k_sp_init_labels<<<grid_for(I), BLK, 0, s>>>(I, J, sparse_view(c), c.sp_av, c.sp_nbr, nca, c.aug_cx,
+                                                         c.aug_cy, c.sp_lab, c.lab_c, flags);
+            SC_HIP(hipGetLastError());
+        } else {
+            k_link<<<grid_for(I), BLK, 0, s>>>(c.slot_b, c.slot_c, c.p_j, I, nba, nca, c.nbr_b, c.nbr_c, flags);
+            SC_HIP(hipGetLastError());
+            k_init_labels<<<grid_for(nba_tot), BLK, 0, s>>>(J, nba, nca, c.aug_bx, c.aug_by, c.av_b, c.nbr_b,
+                                                            c.aug_cx, c.aug_cy, c.lab_b, c.lab_c, flags);
+            SC_HIP(hipGetLastError());
+        }
     }
     js.flags.resize(J);
     SC_HIP(hipMemcpyAsync(js.flags.data(), flags, sizeof(int) * J, hipMemcpyDeviceToHost, s));
@@ -450,6 +591,21 @@ void post_jobs(Ctx &c, int k0, int k1) {
 }
 
 void post_overlay(Ctx &c, int k) { post_jobs(c, k, k); }
+
+SparseP sparse_view(Ctx &c) {
+    SparseP v;
+    v.P0 = c.jobs.d_p0;
+    v.AB = c.jobs.d_AB;
+    v.nbv = c.jobs.d_nb;
+    v.bvoff = c.jobs.d_bvoff;
+    v.slot = c.sp_slot;
+    v.ndx = c.sp_ndx;
+    v.edge = c.sp_edge;
+    v.x = c.sp_x;
+    v.y = c.sp_y;
+    v.bpoly = c.bpoly;
+    return v;
+}
 
 void init_labels_one(Ctx &c, int nba, int nca, const double *bx, const double *by, const int *avb, const int *nbrb,
                      const double *cx, const double *cy, int *labb, int *labc, int *flags) {

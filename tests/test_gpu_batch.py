@@ -116,3 +116,33 @@ def test_one_to_many_scaled_batch(gpu):
         r = tr.clip(k, canonical=True)
         err = rings_equal(br[k - 1], r.rings)
         assert err is None, "overlay %d vs oracle: %s" % (k, err)
+
+
+def test_one_to_many_degenerate_batch(gpu):
+    """Degenerate overlays of one base in one batch (sparse base ring): shared
+    edges and identical rings take the sequential route, a containing overlay
+    the no-crossing route; each equals the overlay clipped alone and the oracle."""
+    import numpy as np
+    import segclip
+    from fixtures import square
+    b = square(0.0, 0.0, 2.0, 2.0)
+    tri = np.array([[0.5, 0.5], [1.5, 0.5], [1.0, 0.0]])
+    ovs = [square(1.0, 0.5, 3.0, 1.5), square(1.0, 0.0, 3.0, 2.0), square(0.0, 0.0, 2.0, 2.0), tri,
+           square(5.0, 5.0, 6.0, 6.0), square(-1.0, -1.0, 3.0, 3.0), square(0.5, 0.5, 1.5, 1.5),
+           np.array([[2.0, 1.0], [3.0, -1.0], [4.0, 1.0], [3.0, 3.0]])]
+    print("[gpu-batch] one-to-many degenerate", len(ovs), flush=True)
+    tr = O.Tree(b, ovs)
+    cl = segclip.Clipper().set_polygons(b, ovs)
+    single, status = [], []
+    for k in range(1, len(ovs) + 1):
+        cl.clip(k)
+        single.append(cl.rings())
+        status.append(cl.counts()["undefined"])
+        r = tr.clip(k, canonical=True)
+        assert rings_equal(single[-1], r.rings) is None, k
+    cl.clip_batch(1, len(ovs))
+    br, st = cl.batch_rings(), cl.batch_status()
+    for k in range(len(ovs)):
+        err = rings_equal(br[k], single[k])
+        assert err is None, "overlay %d: %s" % (k + 1, err)
+        assert st[k] == status[k], (k + 1, st[k], status[k])
