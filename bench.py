@@ -96,6 +96,7 @@ def bench_one_to_many(steps):
     import segclip
     base, ovs = segclip.synth.one_to_many(100000, 4096, 3000)
     cl = segclip.Clipper().set_polygons(base, ovs)
+    cl.set_timing(True)
 
     def step():
         cl.reset()
@@ -103,6 +104,7 @@ def bench_one_to_many(steps):
 
     dt = timed(step, steps, 1)
     cnt = cl.counts()
+    log("[bench] one_to_many stage ms %s counts %s" % (json.dumps(cl.stage_times()), json.dumps(cnt)))
     plain, n = plain_loop_rate(base, ovs, 48)
     edges = len(base) * len(ovs) + sum(len(o) for o in ovs)
     return {"workload": "one-to-many: harmonic base 100,000 vertices x 4,096 star overlays (8..20,000 vertices)",
@@ -194,6 +196,8 @@ def main():
     ap.add_argument("--cpu-sample", default="100000x500000", help="NBxNC of the oracle sample, or 'none'")
     ap.add_argument("--batched", default="one_to_many,small_pairs,large_pairs",
                     help="comma list of batched workloads to add, or 'none'")
+    ap.add_argument("--only", default=None, metavar="WORKLOAD",
+                    help="run only one batched workload (one_to_many, small_pairs, large_pairs) and print its figures")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 launches), to scale HBM counters")
     args = ap.parse_args()
@@ -206,6 +210,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": 8 * n, "bytes_written": 8 * n, "launches": 3}))
+        return
+    if args.only:
+        fn = {"one_to_many": bench_one_to_many, "small_pairs": bench_small_pairs,
+              "large_pairs": bench_large_pairs}[args.only]
+        print(json.dumps({args.only: fn(args.steps)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the box has one); --gpus %d is not supported"

@@ -149,6 +149,12 @@ int segclip_memory(int64_t *current, int64_t *peak, int reset_peak);
  * offsets (jobs+1 entries) and status bits (the 'undefined' bits per job) */
 int segclip_copy_batch_rings(segclip_ctx *ctx, int32_t *job_ring_off, int64_t cap, int64_t *need);
 int segclip_copy_batch_status(segclip_ctx *ctx, int32_t *status, int64_t cap, int64_t *need);
+/* per job of the last batch: labelling route (0 parallel crossing traversal,
+ * 1 no crossing vertex, 2 the reference's sequential algorithm) and, for
+ * route 2, why (bits: 1 a label other than CROSSING/BOUNCING, 2 a ring without
+ * a plain vertex, 4 crossings not alternating, 8 a bouncing pair to split,
+ * 16 a neighbour link that is not a crossing) */
+int segclip_copy_batch_routes(segclip_ctx *ctx, int32_t *route, int32_t *reason, int64_t cap, int64_t *need);
 /* Drop everything derived from the polygons (tree, candidates, results) so
  * that the next segclip_clip recomputes it; the resident polygons stay. */
 int segclip_reset(segclip_ctx *ctx);

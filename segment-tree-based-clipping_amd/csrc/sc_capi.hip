@@ -279,6 +279,16 @@ int segclip_copy_batch_rings(segclip_ctx *h, int32_t *job_ring_off, int64_t cap,
     std::memcpy(job_ring_off, js.ring_off.data(), sizeof(int32_t) * (js.J + 1));
     return SEGCLIP_OK;
 }
+int segclip_copy_batch_routes(segclip_ctx *h, int32_t *route, int32_t *reason, int64_t cap, int64_t *need) {
+    if (!h) return SEGCLIP_EINVAL;
+    const sc::JobSet &js = h->c.jobs;
+    if (int(js.route.size()) != js.J || js.J < 1) return SEGCLIP_ESTATE;
+    int rc = need_space(js.J, cap, need);
+    if (rc || !route || !reason) return rc;
+    std::memcpy(route, js.route.data(), sizeof(int32_t) * js.J);
+    std::memcpy(reason, js.reason.data(), sizeof(int32_t) * js.J);
+    return SEGCLIP_OK;
+}
 int segclip_copy_batch_status(segclip_ctx *h, int32_t *status, int64_t cap, int64_t *need) {
     if (!h) return SEGCLIP_EINVAL;
     const sc::JobSet &js = h->c.jobs;

@@ -752,15 +752,18 @@ __global__ void k_ring_off_final(int nr, const uint32_t *cnt, const uint32_t *rs
 // sequential path: the reference's algorithm on index-linked vertices, one thread
 // ---------------------------------------------------------------------------
 struct SeqArgs {
-    int nP, nQ, cap;
-    const double *px, *py, *qx, *qy;
-    const int *avP, *labP, *nbrP, *labQ, *nbrQ;
-    double *vx, *vy;
-    int *nx, *pv, *nb, *lab, *enex;
+    int nP, nQ, cap; // ring nodes of P and Q (an intersection vertex or a run of plain vertices)
+    // node records: P nodes [0,nP), Q nodes [nP,nP+nQ)
+    const double *nx0, *ny0, *nx1, *ny1; // first / last vertex of the node
+    const int *nlen, *nslot, *ninter, *nlab, *nnb;
+    const double *px, *py, *qx, *qy; // ring vertices by slot (run expansion)
+    int windP, windQ;                // winding of each ring's first plain vertex w.r.t. the other ring
+    double *vx, *vy, *vx1, *vy1;
+    int *vlen, *vslot, *nx, *pv, *nb, *lab, *enex;
     uint8_t *inter;
     int *cand; // scratch list, cap entries
     double *ox, *oy;
-    int *vsrc, *osrc; // provenance of each vertex / output vertex (slot reference)
+    int *vsrc, *osrc; // provenance of inserted vertices / output vertices (slot reference)
     int P0, Q0;       // first slot of the job's rings in the batch arrays
     int out_cap;
     int *ring_off;
@@ -773,6 +776,10 @@ __device__ int seq_new(SeqArgs &a, int &n, double x, double y, int src) {
     int v = n++;
     a.vx[v] = x;
     a.vy[v] = y;
+    a.vx1[v] = x;
+    a.vy1[v] = y;
+    a.vlen[v] = 1;
+    a.vslot[v] = -1;
     a.vsrc[v] = src;
     a.nx[v] = a.pv[v] = v;
     a.nb[v] = -1;
@@ -790,50 +797,103 @@ __device__ void seq_insert_after(SeqArgs &a, int V, int curr) {
     a.pv[c] = V;
 }
 
+// vertex t of node V in ring order
+__device__ inline void seq_vertex(const SeqArgs &a, int V, int t, double &x, double &y) {
+    if (a.vslot[V] >= 0 && a.vlen[V] > 1) {
+        const bool onP = V < a.nP;
+        const int sl = a.vslot[V] + t;
+        x = onP ? a.px[sl] : a.qx[sl];
+        y = onP ? a.py[sl] : a.qy[sl];
+    } else {
+        x = a.vx[V];
+        y = a.vy[V];
+    }
+}
+
+// point-in-polygon over the ring from root, every vertex of every node
 __device__ bool seq_point_in(SeqArgs &a, int root, double rx, double ry) {
     int w = 0, one = root;
+    bool have = false;
+    double fx = 0, fy = 0, px = 0, py = 0;
     do {
-        int two = a.nx[one];
-        double p0x = a.vx[one], p0y = a.vy[one], p1x = a.vx[two], p1y = a.vy[two];
-        if ((p0y < ry) != (p1y < ry)) {
-            if (p0x >= rx) {
-                if (p1x > rx) w = w + 2 * (p1y > p0y) - 1;
-                else if ((areaA(p0x, p0y, p1x, p1y, rx, ry) > 0) == (p1y > p0y)) w = w + 2 * (p1y > p0y) - 1;
-            } else if (p1x > rx) {
-                if ((areaA(p0x, p0y, p1x, p1y, rx, ry) > 0) == (p1y > p0y)) w = w + 2 * (p1y > p0y) - 1;
+        for (int t = 0; t < a.vlen[one]; ++t) {
+            double cx, cy;
+            seq_vertex(a, one, t, cx, cy);
+            if (!have) {
+                fx = cx;
+                fy = cy;
+                have = true;
+            } else {
+                w += wind_edge(px, py, cx, cy, rx, ry);
             }
+            px = cx;
+            py = cy;
         }
-        one = two;
+        one = a.nx[one];
     } while (one != root);
+    w += wind_edge(px, py, fx, fy, rx, ry);
     return (w % 2) != 0;
+}
+
+// provenance of a single-vertex node
+__device__ inline int seq_src(const SeqArgs &a, int V) {
+    if (a.vslot[V] < 0) return a.vsrc[V];
+    return V < a.nP ? a.P0 + a.vslot[V] : ~(a.Q0 + a.vslot[V]);
+}
+
+// output the vertices of node V, walking forward (next) or backward (prev)
+__device__ bool seq_emit(SeqArgs &a, int &outn, int V, bool forward) {
+    const int len = a.vlen[V];
+    if (outn + len > a.out_cap) return false;
+    for (int t = 0; t < len; ++t) {
+        const int u = forward ? t : len - 1 - t;
+        double x, y;
+        seq_vertex(a, V, u, x, y);
+        a.ox[outn] = x;
+        a.oy[outn] = y;
+        if (a.vslot[V] >= 0) {
+            const bool onP = V < a.nP;
+            const int sl = a.vslot[V] + u;
+            a.osrc[outn] = onP ? a.P0 + sl : ~(a.Q0 + sl);
+        } else {
+            a.osrc[outn] = a.vsrc[V];
+        }
+        ++outn;
+    }
+    return true;
 }
 
 __device__ int seq_toggle(int s) { return s == ENTRY ? EXIT : (s == EXIT ? ENTRY : s); }
 
 __device__ void seq_run(SeqArgs &a) {
     int n = 0, status = 0;
-    // rings in slot order; P = [0,nP), Q = [nP,nP+nQ)
-    for (int i = 0; i < a.nP; ++i) {
-        int v = seq_new(a, n, a.px[i], a.py[i], a.P0 + i);
-        a.lab[v] = a.labP[i];
-        a.inter[v] = a.avP[i] != -100;
+    // ring nodes in ring order; P = [0,nP), Q = [nP,nP+nQ)
+    const int nn = a.nP + a.nQ;
+    if (nn > a.cap) {
+        a.result[0] = 0;
+        a.result[1] = 0;
+        a.result[2] = UB_TRAVERSAL;
+        a.ring_off[0] = 0;
+        return;
     }
-    for (int i = 0; i < a.nQ; ++i) {
-        int v = seq_new(a, n, a.qx[i], a.qy[i], ~(a.Q0 + i));
-        a.lab[v] = a.labQ[i];
-        a.inter[v] = a.nbrQ[i] != -100;
+    for (int v = 0; v < nn; ++v) {
+        a.vx[v] = a.nx0[v];
+        a.vy[v] = a.ny0[v];
+        a.vx1[v] = a.nx1[v];
+        a.vy1[v] = a.ny1[v];
+        a.vlen[v] = a.nlen[v];
+        a.vslot[v] = a.nslot[v];
+        a.vsrc[v] = SRC_NONE;
+        a.lab[v] = a.nlab[v];
+        a.inter[v] = a.ninter[v] ? 1 : 0;
+        a.enex[v] = NEITHER;
+        a.nb[v] = a.ninter[v] ? a.nnb[v] : -1;
+        const bool onP = v < a.nP;
+        const int r0 = onP ? 0 : a.nP, m = onP ? a.nP : a.nQ, i = v - r0;
+        a.nx[v] = r0 + (i + 1 == m ? 0 : i + 1);
+        a.pv[v] = r0 + (i == 0 ? m - 1 : i - 1);
     }
-    for (int i = 0; i < a.nP; ++i) {
-        a.nx[i] = i + 1 == a.nP ? 0 : i + 1;
-        a.pv[i] = i == 0 ? a.nP - 1 : i - 1;
-        if (a.inter[i]) a.nb[i] = a.nP + a.nbrP[i] - 1;
-    }
-    for (int i = 0; i < a.nQ; ++i) {
-        int v = a.nP + i;
-        a.nx[v] = a.nP + (i + 1 == a.nQ ? 0 : i + 1);
-        a.pv[v] = a.nP + (i == 0 ? a.nQ - 1 : i - 1);
-        if (a.inter[v]) a.nb[v] = a.nbrQ[i] - 1;
-    }
+    n = nn;
     const int rootP = 0, rootQ = a.nP;
     // 2) intersection chains
     {
@@ -879,13 +939,13 @@ __device__ void seq_run(SeqArgs &a) {
                 continue;
             }
             double qx = 0, qy = 0;
-            bool found = false;
+            bool found = false, plain = false;
             V = A;
             do {
                 if (!a.inter[V]) {
                     qx = a.vx[V];
                     qy = a.vy[V];
-                    found = true;
+                    found = plain = true;
                     break;
                 }
                 V = a.nx[V];
@@ -904,16 +964,14 @@ __device__ void seq_run(SeqArgs &a) {
                 } while (V != A);
             }
             if (!found) { status |= UB_TRAVERSAL; goto done; }
-            if (seq_point_in(a, B, qx, qy)) {
+            // the ring's first plain vertex: its winding against the other ring is known
+            const bool in = plain ? ((s == 0 ? a.windP : a.windQ) % 2) != 0 : seq_point_in(a, B, qx, qy);
+            if (in) {
                 if (rings >= a.ring_cap) { status |= UB_TRAVERSAL; goto done; }
                 a.ring_off[rings++] = outn;
                 int W = A;
                 do {
-                    if (outn >= a.out_cap) { status |= UB_TRAVERSAL; goto done; }
-                    a.ox[outn] = a.vx[W];
-                    a.oy[outn] = a.vy[W];
-                    a.osrc[outn] = a.vsrc[W];
-                    outn++;
+                    if (!seq_emit(a, outn, W, true)) { status |= UB_TRAVERSAL; goto done; }
                     W = a.nx[W];
                 } while (W != A);
             }
@@ -930,17 +988,14 @@ __device__ void seq_run(SeqArgs &a) {
                 a.ring_off[rings++] = outn;
                 int W = rootP;
                 do {
-                    if (outn >= a.out_cap) { status |= UB_TRAVERSAL; goto done; }
-                    a.ox[outn] = a.vx[W];
-                    a.oy[outn] = a.vy[W];
-                    a.osrc[outn] = a.vsrc[W];
-                    outn++;
+                    if (!seq_emit(a, outn, W, true)) { status |= UB_TRAVERSAL; goto done; }
                     W = a.nx[W];
                 } while (W != rootP);
             }
         }
         // 4) entry/exit flags; split and crossing candidates in scratch
         int nsplit[2] = {0, 0}, ncross[2] = {0, 0};
+        bool inserted[2] = {false, false};
         int half = a.cap / 2; // cand[0..half) split lists, [half..cap) crossing lists
         int *splitL[2] = {a.cand, a.cand + half / 2};
         int *crossL[2] = {a.cand + half, a.cand + half + half / 2};
@@ -960,6 +1015,7 @@ __device__ void seq_run(SeqArgs &a) {
                         int T = seq_new(a, n, 0.5 * (a.vx[W] + a.vx[nxv]), 0.5 * (a.vy[W] + a.vy[nxv]), SRC_NONE);
                         if (T < 0) { status |= UB_TRAVERSAL; goto done; }
                         seq_insert_after(a, T, W);
+                        inserted[s] = true;
                         V = T;
                         break;
                     }
@@ -968,7 +1024,11 @@ __device__ void seq_run(SeqArgs &a) {
             }
             if (V < 0) { status |= UB_TRAVERSAL; goto done; }
             int st = ENTRY;
-            if (seq_point_in(a, B, a.vx[V], a.vy[V])) st = seq_toggle(st);
+            // V is the ring's first plain vertex unless a midpoint had to be inserted;
+            // its winding against ring B is known while B has no inserted vertex
+            const bool known = a.vslot[V] >= 0 && !inserted[1 - s];
+            const bool in = known ? ((s == 0 ? a.windP : a.windQ) % 2) != 0 : seq_point_in(a, B, a.vx[V], a.vy[V]);
+            if (in) st = seq_toggle(st);
             bool first = true;
             int I = a.nx[V];
             while (I != V) {
@@ -1012,11 +1072,12 @@ __device__ void seq_run(SeqArgs &a) {
             for (int j = 0; j < nsplit[1]; ++j)
                 if (splitL[1][j] == IQ) found = true;
             if (!found) continue;
-            int VP = seq_new(a, n, a.vx[IP], a.vy[IP], a.vsrc[IP]);
-            int VQ = seq_new(a, n, a.vx[IQ], a.vy[IQ], a.vsrc[IQ]);
+            int VP = seq_new(a, n, a.vx[IP], a.vy[IP], seq_src(a, IP));
+            int VQ = seq_new(a, n, a.vx[IQ], a.vy[IQ], seq_src(a, IQ));
             if (VP < 0 || VQ < 0) { status |= UB_TRAVERSAL; goto done; }
-            double sP = areaA(a.vx[a.pv[IP]], a.vy[a.pv[IP]], a.vx[IP], a.vy[IP], a.vx[a.nx[IP]], a.vy[a.nx[IP]]);
-            double sQ = areaA(a.vx[a.pv[IQ]], a.vy[a.pv[IQ]], a.vx[IQ], a.vy[IQ], a.vx[a.nx[IQ]], a.vy[a.nx[IQ]]);
+            // the vertex before IP is the last vertex of its previous node, the one after the first of the next
+            double sP = areaA(a.vx1[a.pv[IP]], a.vy1[a.pv[IP]], a.vx[IP], a.vy[IP], a.vx[a.nx[IP]], a.vy[a.nx[IP]]);
+            double sQ = areaA(a.vx1[a.pv[IQ]], a.vy1[a.pv[IQ]], a.vx[IQ], a.vy[IQ], a.vx[a.nx[IQ]], a.vy[a.nx[IQ]]);
             if (sP * sQ > 0) {
                 a.nb[IP] = VQ; a.nb[VQ] = IP; a.inter[IP] = a.inter[VQ] = 1;
                 a.nb[IQ] = VP; a.nb[VP] = IQ; a.inter[IQ] = a.inter[VP] = 1;
@@ -1057,11 +1118,7 @@ __device__ void seq_run(SeqArgs &a) {
                     do {
                         V = st == EXIT ? a.nx[V] : a.pv[V];
                         a.inter[V] = 0;
-                        if (outn >= a.out_cap || ++steps > bound) { status |= UB_TRAVERSAL; goto done; }
-                        a.ox[outn] = a.vx[V];
-                        a.oy[outn] = a.vy[V];
-                        a.osrc[outn] = a.vsrc[V];
-                        outn++;
+                        if (++steps > bound || !seq_emit(a, outn, V, st == EXIT)) { status |= UB_TRAVERSAL; goto done; }
                     } while (!(a.enex[V] == st) && V != I);
                     if (V != I) {
                         V = a.nb[V];
@@ -1089,8 +1146,9 @@ done:
 
 // sequential route for several jobs: one block per job, its thread 0 runs seq_run
 struct SeqJob {
-    int j;              // job
-    int nP, nQ, P0, Q0; // ring sizes and first slots
+    int j;                     // job
+    int nP, nQ, P0, Q0;        // ring sizes (slots) and first slots in the dense ring arrays
+    int node0, nPn, nQn;       // first node, node counts of P and Q
     int cap, out_cap, ring_cap;
     int64_t vbase, obase, rbase; // scratch bases: vertices, output vertices, ring offsets
 };
@@ -1098,9 +1156,11 @@ struct SeqJob {
 struct SeqBatch {
     const SeqJob *jobs;
     const double *px, *py, *qx, *qy;
-    const int *avP, *labP, *nbrP, *labQ, *nbrQ;
-    double *vx, *vy;
-    int *nx, *pv, *nb, *lab, *enex, *vsrc, *cand;
+    const double *nx0, *ny0, *nx1, *ny1;
+    const int *nlen, *nslot, *ninter, *nlab, *nnb;
+    const int *windP, *windQ;
+    double *vx, *vy, *vx1, *vy1;
+    int *vlen, *vslot, *nx, *pv, *nb, *lab, *enex, *vsrc, *cand;
     uint8_t *inter;
     double *ox, *oy;
     int *osrc, *ring_off, *result;
@@ -1110,15 +1170,19 @@ __global__ void k_sequential(SeqBatch b) {
     if (threadIdx.x != 0) return;
     const SeqJob jb = b.jobs[blockIdx.x];
     SeqArgs a;
-    a.nP = jb.nP;
-    a.nQ = jb.nQ;
+    a.nP = jb.nPn;
+    a.nQ = jb.nQn;
     a.cap = jb.cap;
     a.P0 = jb.P0;
     a.Q0 = jb.Q0;
+    a.nx0 = b.nx0 + jb.node0; a.ny0 = b.ny0 + jb.node0; a.nx1 = b.nx1 + jb.node0; a.ny1 = b.ny1 + jb.node0;
+    a.nlen = b.nlen + jb.node0; a.nslot = b.nslot + jb.node0; a.ninter = b.ninter + jb.node0;
+    a.nlab = b.nlab + jb.node0; a.nnb = b.nnb + jb.node0;
     a.px = b.px + jb.P0; a.py = b.py + jb.P0; a.qx = b.qx + jb.Q0; a.qy = b.qy + jb.Q0;
-    a.avP = b.avP + jb.P0; a.labP = b.labP + jb.P0; a.nbrP = b.nbrP + jb.P0;
-    a.labQ = b.labQ + jb.Q0; a.nbrQ = b.nbrQ + jb.Q0;
-    a.vx = b.vx + jb.vbase; a.vy = b.vy + jb.vbase;
+    a.windP = b.windP[jb.j];
+    a.windQ = b.windQ[jb.j];
+    a.vx = b.vx + jb.vbase; a.vy = b.vy + jb.vbase; a.vx1 = b.vx1 + jb.vbase; a.vy1 = b.vy1 + jb.vbase;
+    a.vlen = b.vlen + jb.vbase; a.vslot = b.vslot + jb.vbase;
     a.nx = b.nx + jb.vbase; a.pv = b.pv + jb.vbase; a.nb = b.nb + jb.vbase; a.lab = b.lab + jb.vbase;
     a.enex = b.enex + jb.vbase; a.vsrc = b.vsrc + jb.vbase; a.cand = b.cand + jb.vbase;
     a.inter = b.inter + jb.vbase;
@@ -1126,6 +1190,95 @@ __global__ void k_sequential(SeqBatch b) {
     a.ring_off = b.ring_off + jb.rbase; a.ring_cap = jb.ring_cap;
     a.result = b.result + 3 * int64_t(blockIdx.x);
     seq_run(a);
+}
+
+// Ring nodes of the sequential jobs: every intersection slot is a node, every
+// maximal run of plain slots is one node.  Segment g = 2q (ring P of sequential
+// job q) or 2q+1 (its ring Q) covers flat slots [segoff[g], segoff[g+1]).
+struct NodeArgs {
+    int nseg;
+    const int64_t *segoff;
+    const SeqJob *jobs;
+    const double *px, *py, *qx, *qy;
+    const int *avP, *labP, *nbrP, *labQ, *nbrQ;
+};
+
+__device__ inline bool node_inter(const NodeArgs &a, int g, int s) {
+    const SeqJob &jb = a.jobs[g >> 1];
+    return (g & 1) ? a.nbrQ[jb.Q0 + s] != -100 : a.avP[jb.P0 + s] != -100;
+}
+
+__global__ void k_node_flags(NodeArgs a, int64_t S, uint32_t *flag) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= S) return;
+    int lo = 0, hi = a.nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a.segoff[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const int g = lo, s = int(t - a.segoff[g]);
+    flag[t] = (s == 0 || node_inter(a, g, s) || node_inter(a, g, s - 1)) ? 1u : 0u;
+}
+
+// node records (start slot, interior flag, first vertex, label, neighbour slot)
+__global__ void k_node_fill(NodeArgs a, int64_t S, const uint32_t *flag, const uint32_t *nidx, int *nslot,
+                            int *ninter, double *nx0, double *ny0, int *nlab, int *nnb) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= S || !flag[t]) return;
+    int lo = 0, hi = a.nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a.segoff[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const int g = lo, s = int(t - a.segoff[g]);
+    const SeqJob &jb = a.jobs[g >> 1];
+    const uint32_t k = nidx[t];
+    const bool onQ = g & 1;
+    const bool in = node_inter(a, g, s);
+    nslot[k] = s;
+    ninter[k] = in ? 1 : 0;
+    nx0[k] = onQ ? a.qx[jb.Q0 + s] : a.px[jb.P0 + s];
+    ny0[k] = onQ ? a.qy[jb.Q0 + s] : a.py[jb.P0 + s];
+    nlab[k] = onQ ? a.labQ[jb.Q0 + s] : a.labP[jb.P0 + s];
+    nnb[k] = in ? (onQ ? a.nbrQ[jb.Q0 + s] : a.nbrP[jb.P0 + s]) - 1 : -1;
+}
+
+// run lengths, last vertices and neighbour nodes (node indices local to the job)
+__global__ void k_node_finish(NodeArgs a, int64_t N, const int64_t *nodeoff, const uint32_t *flag,
+                              const uint32_t *nidx, const int *nslot, const int *ninter, int *nlen, double *nx1,
+                              double *ny1, int *nnb) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    int lo = 0, hi = a.nseg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (nodeoff[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    const int g = lo;
+    const SeqJob &jb = a.jobs[g >> 1];
+    const bool onQ = g & 1;
+    const int nsl = int(a.segoff[g + 1] - a.segoff[g]);
+    const int end = k + 1 < nodeoff[g + 1] ? nslot[k + 1] : nsl;
+    const int len = end - nslot[k];
+    nlen[k] = len;
+    const int sl = nslot[k] + len - 1;
+    nx1[k] = onQ ? a.qx[jb.Q0 + sl] : a.px[jb.P0 + sl];
+    ny1[k] = onQ ? a.qy[jb.Q0 + sl] : a.py[jb.P0 + sl];
+    if (ninter[k]) {
+        const int og = g ^ 1; // the other ring of the same job
+        const int w = nnb[k];
+        const int on = int(a.segoff[og + 1] - a.segoff[og]);
+        if (w < 0 || w >= on) {
+            nnb[k] = -1;
+        } else {
+            const int64_t t = a.segoff[og] + w;
+            const int64_t node = int64_t(nidx[t]) + int64_t(flag[t]) - 1; // node holding that slot
+            nnb[k] = int(node - nodeoff[g & ~1]);
+        }
+    }
 }
 
 // rings of the sequential jobs into the batch output (one block per job)
@@ -1153,9 +1306,10 @@ struct FosterScratch {
     DBuf<int> jst;
     DBuf<uint32_t> jv, jr;
     DBuf<int> st;
-    DBuf<double> ox, oy, qpts, mx, my;
-    DBuf<int> osrc, qoks, mav, mnbr, mlab, mjob;
-    DBuf<int64_t> moff;
+    DBuf<double> ox, oy, qpts, mx, my, nx0, ny0, nx1, ny1, vx1, vy1;
+    DBuf<int> osrc, qoks, mav, mnbr, mlab, mjob, nslot, ninter, nlab, nnb, nlen, vlen, vslot;
+    DBuf<int64_t> moff, segoff, nodeoff;
+    DBuf<uint32_t> nflag, nidx;
     DBuf<int> off, rstart, ringid;
     DBuf<uint32_t> keep, mark, kscan, rcnt, nonempty;
     // sequential route
@@ -1260,9 +1414,17 @@ void release_foster_scratch() {
                          &f.osrc, &f.off, &f.rstart, &f.ringid, &f.sres, &f.sroff, &f.sosrc, &f.nx, &f.pv, &f.nb,
                          &f.lab, &f.enex, &f.vsrc, &f.cand})
         b->release();
-    for (DBuf<double> *b : {&f.ox, &f.oy, &f.vx, &f.vy, &f.sox, &f.soy, &f.qpts, &f.mx, &f.my}) b->release();
-    for (DBuf<int> *b : {&f.qoks, &f.mav, &f.mnbr, &f.mlab, &f.mjob}) b->release();
+    for (DBuf<double> *b : {&f.ox, &f.oy, &f.vx, &f.vy, &f.sox, &f.soy, &f.qpts, &f.mx, &f.my, &f.nx0, &f.ny0,
+                             &f.nx1, &f.ny1, &f.vx1, &f.vy1})
+        b->release();
+    for (DBuf<int> *b : {&f.qoks, &f.mav, &f.mnbr, &f.mlab, &f.mjob, &f.nslot, &f.ninter, &f.nlab, &f.nnb, &f.nlen,
+                         &f.vlen, &f.vslot})
+        b->release();
     f.moff.release();
+    f.segoff.release();
+    f.nodeoff.release();
+    f.nflag.release();
+    f.nidx.release();
     f.splitP.release();
     f.splitQ.release();
     f.inter.release();
@@ -1388,6 +1550,9 @@ void label_and_traverse(Ctx &c) {
     std::vector<int> hk(3 * size_t(J));
     d2h_sync(c, hk.data(), st.kind, 3 * int64_t(J));
     const int *hkind = hk.data(), *hmP = hk.data() + J, *hmQ = hk.data() + 2 * J;
+    js.route.assign(hkind, hkind + J);
+    js.reason.resize(size_t(J));
+    SC_HIP(hipMemcpyAsync(js.reason.data(), st.slow, sizeof(int) * size_t(J), hipMemcpyDeviceToHost, s));
     int maxm = 0;
     std::vector<int> seq_jobs;
     bool any_nocross = false;
@@ -1401,8 +1566,8 @@ void label_and_traverse(Ctx &c) {
     std::vector<SeqJob> sj(static_cast<size_t>(nseq));
     if (nseq > 0) {
         std::vector<int> hsidx(static_cast<size_t>(J), -1);
-        int64_t vb = 0, ob = 0, rb = 0, mb = 0;
-        std::vector<int64_t> moff(static_cast<size_t>(nseq));
+        int64_t mb = 0;
+        std::vector<int64_t> moff(static_cast<size_t>(nseq)), segoff(2 * size_t(nseq) + 1, 0);
         for (int q = 0; q < nseq; ++q) {
             const int j = seq_jobs[q];
             SeqJob &b = sj[q];
@@ -1414,15 +1579,8 @@ void label_and_traverse(Ctx &c) {
             b.P0 = sparse ? int(mb) : js.AB[j];
             mb += b.nP;
             b.Q0 = js.AC[j];
-            b.cap = 2 * (b.nP + b.nQ) + 64;
-            b.out_cap = 2 * (b.nP + b.nQ) + 64;
-            b.ring_cap = b.nP + b.nQ + 8;
-            b.vbase = vb;
-            b.obase = ob;
-            b.rbase = rb;
-            vb += b.cap;
-            ob += b.out_cap;
-            rb += b.ring_cap + 1;
+            segoff[2 * q + 1] = segoff[2 * q] + b.nP;
+            segoff[2 * q + 2] = segoff[2 * q + 1] + b.nQ;
             hsidx[j] = q;
         }
         SeqJob *djobs = f.sjobs.ensure(nseq);
@@ -1431,7 +1589,7 @@ void label_and_traverse(Ctx &c) {
         SeqBatch b;
         b.jobs = djobs;
         b.px = c.aug_bx; b.py = c.aug_by; b.qx = c.aug_cx; b.qy = c.aug_cy;
-        b.avP = c.av_b; b.labP = c.lab_b; b.nbrP = c.nbr_b; b.labQ = c.lab_c; b.nbrQ = c.nbr_c;
+        const int *avP = c.av_b, *labP = c.lab_b, *nbrP = c.nbr_b;
         if (sparse) {
             double *mx = f.mx.ensure(mb), *my = f.my.ensure(mb);
             int *mav = f.mav.ensure(mb), *mnbr = f.mnbr.ensure(mb), *mlab = f.mlab.ensure(mb);
@@ -1445,9 +1603,67 @@ void label_and_traverse(Ctx &c) {
                                                    mav, mnbr, mlab);
                 SC_HIP(hipGetLastError());
             }
-            b.px = mx; b.py = my; b.avP = mav; b.labP = mlab; b.nbrP = mnbr;
+            b.px = mx; b.py = my;
+            avP = mav; labP = mlab; nbrP = mnbr;
         }
-        b.vx = f.vx.ensure(vb); b.vy = f.vy.ensure(vb);
+        // ring nodes
+        const int nseg = 2 * nseq;
+        const int64_t S = segoff[nseg];
+        int64_t *dseg = f.segoff.ensure(nseg + 1);
+        SC_HIP(hipMemcpyAsync(dseg, segoff.data(), sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, s));
+        NodeArgs na;
+        na.nseg = nseg;
+        na.segoff = dseg;
+        na.jobs = djobs;
+        na.px = b.px; na.py = b.py; na.qx = b.qx; na.qy = b.qy;
+        na.avP = avP; na.labP = labP; na.nbrP = nbrP; na.labQ = c.lab_c; na.nbrQ = c.nbr_c;
+        uint32_t *flag = f.nflag.ensure(S + 1), *nidx = f.nidx.ensure(S + 1);
+        k_node_flags<<<grid_for(S), BLK, 0, s>>>(na, S, flag);
+        SC_HIP(hipGetLastError());
+        SC_HIP(hipMemsetAsync(flag + S, 0, sizeof(uint32_t), s));
+        scan_exclusive_u32(flag, nidx, S + 1, nullptr, s);
+        // node offsets of the segments
+        std::vector<int64_t> nodeoff(size_t(nseg) + 1);
+        {
+            std::vector<uint32_t> h(size_t(nseg) + 1);
+            for (int g = 0; g <= nseg; ++g)
+                SC_HIP(hipMemcpyAsync(&h[g], nidx + segoff[g], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            SC_HIP(hipStreamSynchronize(s));
+            for (int g = 0; g <= nseg; ++g) nodeoff[g] = h[g];
+        }
+        const int64_t N = nodeoff[nseg];
+        int64_t *dnodeoff = f.nodeoff.ensure(nseg + 1);
+        SC_HIP(hipMemcpyAsync(dnodeoff, nodeoff.data(), sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, s));
+        int *nslot = f.nslot.ensure(N), *ninter = f.ninter.ensure(N), *nlab = f.nlab.ensure(N);
+        int *nnb = f.nnb.ensure(N), *nlen = f.nlen.ensure(N);
+        double *nx0 = f.nx0.ensure(N), *ny0 = f.ny0.ensure(N), *nx1 = f.nx1.ensure(N), *ny1 = f.ny1.ensure(N);
+        k_node_fill<<<grid_for(S), BLK, 0, s>>>(na, S, flag, nidx, nslot, ninter, nx0, ny0, nlab, nnb);
+        k_node_finish<<<grid_for(N), BLK, 0, s>>>(na, N, dnodeoff, flag, nidx, nslot, ninter, nlen, nx1, ny1, nnb);
+        SC_HIP(hipGetLastError());
+        // scratch per job, sized by its nodes
+        int64_t vb = 0, ob = 0, rb = 0;
+        for (int q = 0; q < nseq; ++q) {
+            SeqJob &j = sj[q];
+            j.node0 = int(nodeoff[2 * q]);
+            j.nPn = int(nodeoff[2 * q + 1] - nodeoff[2 * q]);
+            j.nQn = int(nodeoff[2 * q + 2] - nodeoff[2 * q + 1]);
+            j.cap = 2 * (j.nPn + j.nQn) + 64;
+            j.out_cap = 2 * (j.nP + j.nQ) + 64;
+            j.ring_cap = j.nP + j.nQ + 8;
+            j.vbase = vb;
+            j.obase = ob;
+            j.rbase = rb;
+            vb += j.cap;
+            ob += j.out_cap;
+            rb += j.ring_cap + 1;
+        }
+        SC_HIP(hipMemcpyAsync(djobs, sj.data(), sizeof(SeqJob) * nseq, hipMemcpyHostToDevice, s));
+        b.nx0 = nx0; b.ny0 = ny0; b.nx1 = nx1; b.ny1 = ny1;
+        b.nlen = nlen; b.nslot = nslot; b.ninter = ninter; b.nlab = nlab; b.nnb = nnb;
+        b.windP = st.windP;
+        b.windQ = st.windQ;
+        b.vx = f.vx.ensure(vb); b.vy = f.vy.ensure(vb); b.vx1 = f.vx1.ensure(vb); b.vy1 = f.vy1.ensure(vb);
+        b.vlen = f.vlen.ensure(vb); b.vslot = f.vslot.ensure(vb);
         b.nx = f.nx.ensure(vb); b.pv = f.pv.ensure(vb); b.nb = f.nb.ensure(vb); b.lab = f.lab.ensure(vb);
         b.enex = f.enex.ensure(vb); b.vsrc = f.vsrc.ensure(vb); b.cand = f.cand.ensure(vb);
         b.inter = f.inter.ensure(vb);

@@ -277,6 +277,15 @@ class Clipper:
         self._check(L.segclip_copy_batch_status(self._h, as_ptr(st, C.c_int32), n, None))
         return st
 
+    def batch_routes(self):
+        """Per job of the last batch: (route, reason bits) of the labelling stage."""
+        L = lib()
+        n = self._need(L.segclip_copy_batch_routes, None, None, 0)
+        r = np.zeros(n, np.int32)
+        w = np.zeros(n, np.int32)
+        self._check(L.segclip_copy_batch_routes(self._h, as_ptr(r, C.c_int32), as_ptr(w, C.c_int32), n, None))
+        return r, w
+
     def batch_rings(self):
         """Output rings of every job of the last batch: a list (per job) of lists of (n,2) arrays."""
         r = self.rings()

@@ -28,6 +28,7 @@ SYMBOLS = [
     "segclip_copy_batch_status", "segclip_set_provenance", "segclip_batch_area", "segclip_batch_area_device",
     "segclip_batch_area_backward_device", "segclip_release", "segclip_memory", "segclip_save_intersections_ids",
     "segclip_sort_intersections_at_edges", "segclip_copy_intersections_arrays", "segclip_init_labels_arrays",
+    "segclip_copy_batch_routes",
 ]
 
 
@@ -85,6 +86,7 @@ def lib():
         "segclip_clip_batch": (C.c_int, [_vp, C.c_int, C.c_int]),
         "segclip_copy_batch_rings": (C.c_int, [_vp, _ip, C.c_int64, _i64p]),
         "segclip_copy_batch_status": (C.c_int, [_vp, _ip, C.c_int64, _i64p]),
+        "segclip_copy_batch_routes": (C.c_int, [_vp, _ip, _ip, C.c_int64, _i64p]),
         "segclip_set_provenance": (C.c_int, [_vp, C.c_int]),
         "segclip_batch_area": (C.c_int, [_vp, _dp, C.c_int64, _i64p]),
         "segclip_batch_area_device": (C.c_int, [_vp, _vp]),
