@@ -13,7 +13,16 @@ namespace sc {
 Ctx::~Ctx() {
     if (ev_ready)
         for (auto &e : ev) (void)hipEventDestroy(e);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+void Ctx::ensure_side_stream() {
+    if (stream2) return;
+    SC_HIP(hipStreamCreate(&stream2));
+    SC_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    SC_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
 }
 void Ctx::ensure_events() {
     if (ev_ready) return;

@@ -123,6 +123,11 @@ struct Ctx {
     DBuf<uint32_t> d_u32_total;
     DBuf<uint64_t> d_u64_total;
 
+    // second stream (default priority) for work that runs beside the main one
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    void ensure_side_stream();
+
     bool timing = false;
     StageTimes times;
     hipEvent_t ev[10] = {};

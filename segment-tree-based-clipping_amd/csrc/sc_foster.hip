@@ -762,6 +762,7 @@ struct SeqArgs {
     int *vlen, *vslot, *nx, *pv, *nb, *lab, *enex;
     uint8_t *inter;
     int *cand; // scratch list, cap entries
+    int *seen; // 2 per vertex: the traversal component that last walked it in each direction
     double *ox, *oy;
     int *vsrc, *osrc; // provenance of inserted vertices / output vertices (slot reference)
     int P0, Q0;       // first slot of the job's rings in the batch arrays
@@ -865,18 +866,11 @@ __device__ bool seq_emit(SeqArgs &a, int &outn, int V, bool forward) {
 
 __device__ int seq_toggle(int s) { return s == ENTRY ? EXIT : (s == EXIT ? ENTRY : s); }
 
-__device__ void seq_run(SeqArgs &a) {
-    int n = 0, status = 0;
-    // ring nodes in ring order; P = [0,nP), Q = [nP,nP+nQ)
+// node records -> the job's linked rings (all threads of the block)
+__device__ void seq_init(SeqArgs &a) {
     const int nn = a.nP + a.nQ;
-    if (nn > a.cap) {
-        a.result[0] = 0;
-        a.result[1] = 0;
-        a.result[2] = UB_TRAVERSAL;
-        a.ring_off[0] = 0;
-        return;
-    }
-    for (int v = 0; v < nn; ++v) {
+    for (int v = threadIdx.x; v < 2 * a.cap; v += blockDim.x) a.seen[v] = 0;
+    for (int v = threadIdx.x; v < nn; v += blockDim.x) {
         a.vx[v] = a.nx0[v];
         a.vy[v] = a.ny0[v];
         a.vx1[v] = a.nx1[v];
@@ -893,7 +887,13 @@ __device__ void seq_run(SeqArgs &a) {
         a.nx[v] = r0 + (i + 1 == m ? 0 : i + 1);
         a.pv[v] = r0 + (i == 0 ? m - 1 : i - 1);
     }
-    n = nn;
+}
+
+// the reference's sequential algorithm on the job's rings (one thread)
+__device__ void seq_run(SeqArgs &a) {
+    int status = 0;
+    // ring nodes already laid out by seq_init: P = [0,nP), Q = [nP,nP+nQ)
+    int n = a.nP + a.nQ;
     const int rootP = 0, rootQ = a.nP;
     // 2) intersection chains
     {
@@ -1107,17 +1107,24 @@ __device__ void seq_run(SeqArgs &a) {
                 have = I != root;
             }
             long bound = 4L * n + 16;
+            int comp = 0;
             while (have) {
                 if (rings >= a.ring_cap) { status |= UB_TRAVERSAL; goto done; }
                 a.ring_off[rings++] = outn;
                 int V = I;
                 a.inter[V] = 0;
                 long steps = 0;
+                ++comp;
                 do {
                     int st = seq_toggle(a.enex[V]);
                     do {
                         V = st == EXIT ? a.nx[V] : a.pv[V];
                         a.inter[V] = 0;
+                        // within one component the walk is determined by (vertex, direction):
+                        // a repeated state can never reach I again (the reference loops forever)
+                        int &seen = a.seen[2 * V + (st == EXIT ? 1 : 0)];
+                        if (seen == comp) { status |= UB_TRAVERSAL; goto done; }
+                        seen = comp;
                         if (++steps > bound || !seq_emit(a, outn, V, st == EXIT)) { status |= UB_TRAVERSAL; goto done; }
                     } while (!(a.enex[V] == st) && V != I);
                     if (V != I) {
@@ -1160,14 +1167,15 @@ struct SeqBatch {
     const int *nlen, *nslot, *ninter, *nlab, *nnb;
     const int *windP, *windQ;
     double *vx, *vy, *vx1, *vy1;
-    int *vlen, *vslot, *nx, *pv, *nb, *lab, *enex, *vsrc, *cand;
+    int *vlen, *vslot, *nx, *pv, *nb, *lab, *enex, *vsrc, *cand, *seen;
     uint8_t *inter;
     double *ox, *oy;
     int *osrc, *ring_off, *result;
 };
 
-__global__ void k_sequential(SeqBatch b) {
-    if (threadIdx.x != 0) return;
+// one block per sequential job: its threads lay out the rings, thread 0 runs
+// the algorithm (no barrier after that point)
+__global__ __launch_bounds__(BLK) void k_sequential(SeqBatch b) {
     const SeqJob jb = b.jobs[blockIdx.x];
     SeqArgs a;
     a.nP = jb.nPn;
@@ -1185,10 +1193,23 @@ __global__ void k_sequential(SeqBatch b) {
     a.vlen = b.vlen + jb.vbase; a.vslot = b.vslot + jb.vbase;
     a.nx = b.nx + jb.vbase; a.pv = b.pv + jb.vbase; a.nb = b.nb + jb.vbase; a.lab = b.lab + jb.vbase;
     a.enex = b.enex + jb.vbase; a.vsrc = b.vsrc + jb.vbase; a.cand = b.cand + jb.vbase;
+    a.seen = b.seen + 2 * jb.vbase;
     a.inter = b.inter + jb.vbase;
     a.ox = b.ox + jb.obase; a.oy = b.oy + jb.obase; a.osrc = b.osrc + jb.obase; a.out_cap = jb.out_cap;
     a.ring_off = b.ring_off + jb.rbase; a.ring_cap = jb.ring_cap;
     a.result = b.result + 3 * int64_t(blockIdx.x);
+    if (jb.nPn + jb.nQn > jb.cap) { // cannot happen: cap is sized from the node counts
+        if (threadIdx.x == 0) {
+            a.result[0] = 0;
+            a.result[1] = 0;
+            a.result[2] = UB_TRAVERSAL;
+            a.ring_off[0] = 0;
+        }
+        return;
+    }
+    seq_init(a);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     seq_run(a);
 }
 
@@ -1307,7 +1328,7 @@ struct FosterScratch {
     DBuf<uint32_t> jv, jr;
     DBuf<int> st;
     DBuf<double> ox, oy, qpts, mx, my, nx0, ny0, nx1, ny1, vx1, vy1;
-    DBuf<int> osrc, qoks, mav, mnbr, mlab, mjob, nslot, ninter, nlab, nnb, nlen, vlen, vslot;
+    DBuf<int> osrc, qoks, mav, mnbr, mlab, mjob, nslot, ninter, nlab, nnb, nlen, vlen, vslot, seen;
     DBuf<int64_t> moff, segoff, nodeoff;
     DBuf<uint32_t> nflag, nidx;
     DBuf<int> off, rstart, ringid;
@@ -1418,7 +1439,7 @@ void release_foster_scratch() {
                              &f.nx1, &f.ny1, &f.vx1, &f.vy1})
         b->release();
     for (DBuf<int> *b : {&f.qoks, &f.mav, &f.mnbr, &f.mlab, &f.mjob, &f.nslot, &f.ninter, &f.nlab, &f.nnb, &f.nlen,
-                         &f.vlen, &f.vslot})
+                         &f.vlen, &f.vslot, &f.seen})
         b->release();
     f.moff.release();
     f.segoff.release();
@@ -1666,12 +1687,18 @@ void label_and_traverse(Ctx &c) {
         b.vlen = f.vlen.ensure(vb); b.vslot = f.vslot.ensure(vb);
         b.nx = f.nx.ensure(vb); b.pv = f.pv.ensure(vb); b.nb = f.nb.ensure(vb); b.lab = f.lab.ensure(vb);
         b.enex = f.enex.ensure(vb); b.vsrc = f.vsrc.ensure(vb); b.cand = f.cand.ensure(vb);
+        b.seen = f.seen.ensure(2 * vb);
         b.inter = f.inter.ensure(vb);
         b.ox = f.sox.ensure(ob); b.oy = f.soy.ensure(ob); b.osrc = f.sosrc.ensure(ob);
         b.ring_off = f.sroff.ensure(rb);
         b.result = f.sres.ensure(3 * int64_t(nseq));
-        k_sequential<<<nseq, 1, 0, s>>>(b);
+        // the sequential jobs run beside the crossing jobs' labelling
+        c.ensure_side_stream();
+        SC_HIP(hipEventRecord(c.ev_fork, s));
+        SC_HIP(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
+        k_sequential<<<nseq, BLK, 0, c.stream2>>>(b);
         SC_HIP(hipGetLastError());
+        SC_HIP(hipEventRecord(c.ev_join, c.stream2));
     }
     // cycles of the crossing jobs: minimum by pointer jumping, then list ranking
     uint32_t *cmin = nullptr, *dist = nullptr, *inode = nullptr;
@@ -1723,6 +1750,7 @@ void label_and_traverse(Ctx &c) {
     ca.ridx = ridx;
     ca.jv = jv;
     ca.jr = jr;
+    if (nseq > 0) SC_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
     k_jcounts<<<grid_for(J), BLK, 0, s>>>(ca);
     SC_HIP(hipGetLastError());
     SC_HIP(hipMemsetAsync(jv + J, 0, sizeof(uint32_t), s));
