@@ -60,6 +60,7 @@ struct Ctx {
     DBuf<uint32_t> lend_e;               // entries: edge index | 0x80000000 for the "after" entry,
                                          // sorted by (class, leaf), edge order inside a leaf
     DBuf<uint64_t> lend_key;             // (class, leaf) key of each entry
+    DBuf<int> lend_bl;                   // the "before" leaf of each entry's edge
     int64_t n_lend = 0;
     // materialised end lists (parity / drop-in only)
     DBuf<uint32_t> mend_off;

@@ -111,10 +111,11 @@ __global__ void k_unit_heads(int T, const uint32_t *__restrict__ uoff, uint32_t 
 }
 
 constexpr int STG = 256; // hits staged per wave in LDS before a flush
+constexpr uint32_t XCHUNK = 16; // consecutive blocks per XCD turn
 
 struct StabArgs {
     const uint32_t *cov_off, *cov_split, *cov_e, *lend_off, *lend_e, *uoff, *unode;
-    const int *before, *etype, *eoff;
+    const int *lend_bl, *etype, *eoff;
     const double *x0, *y0, *x1, *y1;          // endpoints (input coordinates)
     int nl, D, T;
     uint32_t nunits;
@@ -153,6 +154,33 @@ __device__ inline Box box_of(double ax, double ay, double bx, double by) {
     return r;
 }
 
+// a box in single precision rounded outward: it contains the double box, so
+// two such boxes that do not overlap come from double boxes that do not overlap
+struct BoxF {
+    float minx, maxx, miny, maxy;
+};
+__device__ inline float f_down(double d) {
+    float f = float(d);
+    if (double(f) > d) f = nextafterf(f, -INFINITY);
+    return f;
+}
+__device__ inline float f_up(double d) {
+    float f = float(d);
+    if (double(f) < d) f = nextafterf(f, INFINITY);
+    return f;
+}
+__device__ inline BoxF boxf_of(const Box &b) {
+    BoxF r;
+    r.minx = f_down(b.minx);
+    r.maxx = f_up(b.maxx);
+    r.miny = f_down(b.miny);
+    r.maxy = f_up(b.maxy);
+    return r;
+}
+__device__ inline float bcastf(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
 __device__ inline void flush(const StabArgs &a, const uint64_t *stage, uint32_t cnt, int lane) {
     unsigned long long base = 0;
     if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)cnt);
@@ -170,8 +198,15 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     __shared__ uint64_t stage_all[BLK / WAVE][STG];
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x / WAVE;
-    const uint32_t wg = blockIdx.x * (BLK / WAVE) + wv;
-    if (wg >= a.nunits) return; // whole wave; no block barrier below
+    // blocks are dealt round-robin to the 8 XCDs: hand each XCD chunks of
+    // XCHUNK consecutive logical blocks, so that neighbouring units (mostly of
+    // one node) share an L2 while the chunks stay interleaved for balance.
+    // The grid is a multiple of 8*XCHUNK blocks: this is a permutation.
+    const uint32_t nbk = gridDim.x;
+    const uint32_t r = blockIdx.x / 8, xcd = blockIdx.x % 8;
+    const uint32_t lb = (r / XCHUNK) * (8 * XCHUNK) + xcd * XCHUNK + r % XCHUNK; // logical block
+    const uint32_t wg = lb * (BLK / WAVE) + wv;
+    if (lb >= nbk || wg >= a.nunits) return; // whole wave; no block barrier below
     uint64_t *stage = stage_all[wv];
     const int v = int(a.unode[wg]);
     NodeShape s = node_shape(v, a.cov_off, a.cov_split, a.lend_off, a.nl, a.D);
@@ -218,7 +253,7 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
         ry1 = a.y1[re_l];
         rk = a.etype[re_l];
     }
-    const Box rb = box_of(rx0, ry0, rx1, ry1);
+    const BoxF rbf = boxf_of(box_of(rx0, ry0, rx1, ry1));
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
     const int sb = a.bits_b + a.bits_c;
     uint32_t cnt = 0;
@@ -228,11 +263,11 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
         bool valid = uint32_t(lane) < ncol;
         uint32_t ce = 0;
         if (valid) {
-            uint32_t raw = colsrc[col_begin + cb + lane];
+            const uint32_t raw = colsrc[col_begin + cb + lane];
             ce = raw & 0x7fffffffu;
             if (kind != 1 && (raw & 0x80000000u)) {
                 // "after" copy of an edge whose "before" leaf is under v as well
-                int bl = a.before[ce];
+                const int bl = a.lend_bl[col_begin + cb + lane];
                 if (bl >= a.nl + s.lo && bl < a.nl + s.hi) valid = false;
             }
         }
@@ -246,27 +281,30 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
             ck = a.etype[ce];
         }
         const Box cbx = box_of(cx0, cy0, cx1, cy1);
+        const BoxF cbf = boxf_of(cbx);
         for (uint32_t r = 0; r < nrows; ++r) {
-            const double qminx = bcast(rb.minx, int(r)), qmaxx = bcast(rb.maxx, int(r));
-            const double qminy = bcast(rb.miny, int(r)), qmaxy = bcast(rb.maxy, int(r));
-            const bool box = valid && !(qminx > cbx.maxx || qmaxx < cbx.minx) &&
-                             !(qminy > cbx.maxy || qmaxy < cbx.miny);
+            // single-precision screen (exact rejection), then LSMF on the doubles
+            const float qminx = bcastf(rbf.minx, int(r)), qmaxx = bcastf(rbf.maxx, int(r));
+            const float qminy = bcastf(rbf.miny, int(r)), qmaxy = bcastf(rbf.maxy, int(r));
+            const bool near = valid && !(qminx > cbf.maxx || qmaxx < cbf.minx) && !(qminy > cbf.maxy || qmaxy < cbf.miny);
             bool hit = false;
             uint64_t key = 0;
-            if (box) {
+            if (near) {
                 const double px0 = bcast(rx0, int(r)), py0 = bcast(ry0, int(r));
                 const double px1 = bcast(rx1, int(r)), py1 = bcast(ry1, int(r));
+                const Box q = box_of(px0, py0, px1, py1);
+                const bool box = !(q.minx > cbx.maxx || q.maxx < cbx.minx) && !(q.miny > cbx.maxy || q.maxy < cbx.miny);
                 const uint32_t re = uint32_t(__builtin_amdgcn_readlane(int(re_l), int(r)));
-                int tp;
+                int tp = 0;
                 uint32_t be, qe;
                 int k;
                 if (kind == 3) { // row = overlay edge, column = base edge
-                    tp = classify_pre(cx0, cy0, cx1, cy1, px0, py0, px1, py1);
+                    if (box) tp = classify_pre(cx0, cy0, cx1, cy1, px0, py0, px1, py1);
                     be = ce;
                     qe = re;
                     k = __builtin_amdgcn_readlane(rk, int(r));
                 } else {
-                    tp = classify_pre(px0, py0, px1, py1, cx0, cy0, cx1, cy1);
+                    if (box) tp = classify_pre(px0, py0, px1, py1, cx0, cy0, cx1, cy1);
                     be = re;
                     qe = ce;
                     k = ck;
@@ -364,14 +402,17 @@ void find_candidates(Ctx &c) {
         StabArgs a;
         a.cov_off = c.cov_off; a.cov_split = c.cov_split; a.cov_e = c.cov_e;
         a.lend_off = c.lend_off; a.lend_e = c.lend_e; a.uoff = units; a.unode = unode;
-        a.before = c.before_leaf; a.etype = c.etype; a.eoff = c.d_eoff;
+        a.lend_bl = c.lend_bl; a.etype = c.etype; a.eoff = c.d_eoff;
         a.x0 = c.ex0; a.y0 = c.ey0; a.x1 = c.ex1; a.y1 = c.ey1;
         a.nl = c.nleaves; a.D = c.depth; a.T = T; a.nunits = nunits;
         a.bits_b = c.bits_b; a.bits_c = c.bits_c;
         a.out = c.cand; a.cap = uint64_t(c.cand_cap); a.counter = counter;
         c.mark(8);
         if (nunits > 0) {
-            k_stab<<<grid_for(nunits, BLK / WAVE), BLK, 0, s>>>(a);
+            // a multiple of 8*XCHUNK blocks: the XCD-aware block order is then a bijection
+            const unsigned q = 8u * XCHUNK;
+            const unsigned nblk = (grid_for(nunits, BLK / WAVE) + q - 1u) / q * q;
+            k_stab<<<nblk, BLK, 0, s>>>(a);
             SC_HIP(hipGetLastError());
         }
         c.mark(9);

@@ -331,22 +331,24 @@ __global__ void k_cover(const double *__restrict__ es, const double *__restrict_
     if (!EMIT) cnt[e] = n;
 }
 
-// per node: first entry and count of base entries in the (node, class)-sorted keys
-__global__ void k_cover_bounds(const uint64_t *__restrict__ keys, int64_t n, int T, uint32_t *off, uint32_t *split) {
+// per node: end of its entries and of its base entries in the (node,
+// class)-sorted keys, marked at the last entry of each run
+__global__ void k_cover_ends(const uint64_t *__restrict__ keys, int64_t n, uint32_t *end, uint32_t *bend) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    const bool last = i + 1 == n || keys[i + 1] != k;
+    if (!last) return;
+    const uint32_t v = uint32_t(k >> 1);
+    if (i + 1 == n || (keys[i + 1] >> 1) != v) end[v] = uint32_t(i + 1);
+    if ((k & 1ull) == 0) bend[v] = uint32_t(i + 1);
+}
+
+__global__ void k_cover_split(int T, const uint32_t *__restrict__ off, const uint32_t *__restrict__ bend,
+                              uint32_t *split) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v > T) return;
-    auto lb = [&](uint64_t k) {
-        int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            int64_t mid = (lo + hi) >> 1;
-            if (keys[mid] < k) lo = mid + 1;
-            else hi = mid;
-        }
-        return uint32_t(lo);
-    };
-    const uint32_t a = lb(uint64_t(uint32_t(v)) << 1);
-    off[v] = a;
-    if (v < T) split[v] = lb((uint64_t(uint32_t(v)) << 1) | 1ull) - a;
+    if (v >= T) return;
+    split[v] = bend[v] ? bend[v] - off[v] : 0u;
 }
 
 
@@ -578,6 +580,13 @@ void build_node_intervals(Ctx &c) {
     SC_HIP(hipGetLastError());
 }
 
+namespace {
+__global__ void k_entry_before(const uint32_t *__restrict__ e, int64_t n, const int *__restrict__ before, int *bl) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) bl[i] = before[e[i] & 0x7fffffffu];
+}
+} // namespace
+
 void build_leaf_end_entries(Ctx &c) {
     SC_REQUIRE(c.T > 0, SC_ESTATE, "elementary intervals not built");
     hipStream_t s = c.stream;
@@ -599,6 +608,12 @@ void build_leaf_end_entries(Ctx &c) {
     uint32_t *tot = c.d_u32_total.ensure(1);
     scan_exclusive_u32(off, off, 2 * int64_t(nl) + 1, tot, s);
     c.n_lend = read_u32(c, tot);
+    // beside every entry, its edge's "before" leaf (the stab's dedupe test)
+    int *bl = c.lend_bl.ensure(size_t(c.n_lend > 0 ? c.n_lend : 1));
+    if (c.n_lend > 0) {
+        k_entry_before<<<grid_for(c.n_lend), BLK, 0, s>>>(vals, c.n_lend, c.before_leaf, bl);
+        SC_HIP(hipGetLastError());
+    }
 }
 
 void build_cover_lists(Ctx &c) {
@@ -629,7 +644,16 @@ void build_cover_lists(Ctx &c) {
     }
     uint32_t *off = c.cov_off.ensure(size_t(T) + 1);
     uint32_t *split = c.cov_split.ensure(T);
-    k_cover_bounds<<<grid_for(int64_t(T) + 1), BLK, 0, s>>>(keys, L, T, off, split);
+    // off[v+1] = largest run end among nodes <= v; base runs end at bend
+    uint32_t *bend = reinterpret_cast<uint32_t *>(c.u64b.ensure(size_t(T) / 2 + 1)); // T u32 (lr is done)
+    SC_HIP(hipMemsetAsync(off, 0, sizeof(uint32_t) * (size_t(T) + 1), s));
+    SC_HIP(hipMemsetAsync(bend, 0, sizeof(uint32_t) * size_t(T), s));
+    if (L > 0) {
+        k_cover_ends<<<grid_for(L), BLK, 0, s>>>(keys, L, off + 1, bend);
+        SC_HIP(hipGetLastError());
+    }
+    scan_inclusive_max_u32(off + 1, off + 1, T, s);
+    k_cover_split<<<grid_for(T), BLK, 0, s>>>(T, off, bend, split);
     SC_HIP(hipGetLastError());
 }
 
