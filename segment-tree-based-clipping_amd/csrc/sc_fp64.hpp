@@ -137,6 +137,67 @@ __host__ __device__ inline void intersection_point(int t, double al, double b1x,
     }
 }
 
+// LSMF's box of one segment: min/max of its endpoints formed as LSMF forms them
+struct Box {
+    double minx, maxx, miny, maxy;
+};
+__device__ inline Box box_of(double ax, double ay, double bx, double by) {
+    Box r;
+    r.minx = ax;
+    r.maxx = bx;
+    if (r.minx > bx) {
+        r.minx = bx;
+        r.maxx = ax;
+    }
+    r.miny = ay;
+    r.maxy = by;
+    if (r.miny > by) {
+        r.miny = by;
+        r.maxy = ay;
+    }
+    return r;
+}
+__device__ inline bool box_overlap(const Box &p, const Box &q) {
+    return !(p.minx > q.maxx || p.maxx < q.minx) && !(p.miny > q.maxy || p.maxy < q.miny);
+}
+
+// The box in single precision rounded outward: it contains the double box, so
+// two such boxes that do not overlap come from double boxes that do not
+// overlap.  The empty box (+inf, -inf, +inf, -inf) overlaps no finite box.
+struct __align__(16) BoxF {
+    float minx, maxx, miny, maxy;
+};
+__device__ inline float f_down(double d) {
+    float f = float(d);
+    if (double(f) > d) f = nextafterf(f, -INFINITY);
+    return f;
+}
+__device__ inline float f_up(double d) {
+    float f = float(d);
+    if (double(f) < d) f = nextafterf(f, INFINITY);
+    return f;
+}
+__device__ inline BoxF boxf_of(const Box &b) {
+    BoxF r;
+    r.minx = f_down(b.minx);
+    r.maxx = f_up(b.maxx);
+    r.miny = f_down(b.miny);
+    r.maxy = f_up(b.maxy);
+    return r;
+}
+__device__ inline BoxF boxf_empty() {
+    BoxF r;
+    r.minx = INFINITY;
+    r.maxx = -INFINITY;
+    r.miny = INFINITY;
+    r.maxy = -INFINITY;
+    return r;
+}
+// the same test as LSMF's, evaluated without branches
+__device__ inline bool boxf_overlap(const BoxF &p, const BoxF &q) {
+    return !(p.minx > q.maxx) & !(p.maxx < q.minx) & !(p.miny > q.maxy) & !(p.maxy < q.miny);
+}
+
 __host__ __device__ inline int alpha_to_int(double a) {
     if (a < EPS) return 0;
     return int(1000000000.0 * a);

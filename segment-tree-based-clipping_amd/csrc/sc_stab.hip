@@ -115,6 +115,7 @@ constexpr uint32_t XCHUNK = 16; // consecutive blocks per XCD turn
 
 struct StabArgs {
     const uint32_t *cov_off, *cov_split, *cov_e, *lend_off, *lend_e, *uoff, *unode;
+    const BoxF *cov_box, *lend_box;           // single-precision boxes beside the entries
     const int *lend_bl, *etype, *eoff;
     const double *x0, *y0, *x1, *y1;          // endpoints (input coordinates)
     int nl, D, T;
@@ -132,54 +133,7 @@ __device__ inline double bcast(double v, int src) {
     int hi = __builtin_amdgcn_readlane(int(b >> 32), src);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
-
-// LSMF's box of one segment: min/max of its endpoints formed as LSMF forms them
-struct Box {
-    double minx, maxx, miny, maxy;
-};
-__device__ inline Box box_of(double ax, double ay, double bx, double by) {
-    Box r;
-    r.minx = ax;
-    r.maxx = bx;
-    if (r.minx > bx) {
-        r.minx = bx;
-        r.maxx = ax;
-    }
-    r.miny = ay;
-    r.maxy = by;
-    if (r.miny > by) {
-        r.miny = by;
-        r.maxy = ay;
-    }
-    return r;
-}
-
-// a box in single precision rounded outward: it contains the double box, so
-// two such boxes that do not overlap come from double boxes that do not overlap
-struct BoxF {
-    float minx, maxx, miny, maxy;
-};
-__device__ inline float f_down(double d) {
-    float f = float(d);
-    if (double(f) > d) f = nextafterf(f, -INFINITY);
-    return f;
-}
-__device__ inline float f_up(double d) {
-    float f = float(d);
-    if (double(f) < d) f = nextafterf(f, INFINITY);
-    return f;
-}
-__device__ inline BoxF boxf_of(const Box &b) {
-    BoxF r;
-    r.minx = f_down(b.minx);
-    r.maxx = f_up(b.maxx);
-    r.miny = f_down(b.miny);
-    r.maxy = f_up(b.maxy);
-    return r;
-}
-__device__ inline float bcastf(float v, int src) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
+__device__ inline uint32_t uni(uint32_t v) { return uint32_t(__builtin_amdgcn_readfirstlane(int(v))); }
 
 __device__ inline void flush(const StabArgs &a, const uint64_t *stage, uint32_t cnt, int lane) {
     unsigned long long base = 0;
@@ -189,32 +143,38 @@ __device__ inline void flush(const StabArgs &a, const uint64_t *stage, uint32_t 
         if (base + i < a.cap) a.out[base + i] = stage[i];
 }
 
-// One wave per unit: up to 64 rows (one per lane, broadcast in turn) against
-// one or more 64-column tiles (one column per lane).  LSMF is the bounding-box
-// overlap test; pairs that pass are classified with lineSegmentIntersection2
-// (exact division-free prefilter first).  Hits are staged in LDS and written
-// with one atomic per flush.
+// One wave per unit: up to 64 rows against one or more 64-column tiles (one
+// column per lane).  The rows' single-precision boxes sit in LDS and are read
+// as broadcasts, four rows at a time; a row that overlaps no column costs
+// four compares.  Only when some lane passes that screen are the double
+// endpoints fetched (once per unit for the rows, once per tile for the
+// columns) and the exact LSMF box test and lineSegmentIntersection2's
+// classification (exact division-free prefilter first) run.  Hits are staged
+// in LDS and written with one atomic per flush.
 __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     __shared__ uint64_t stage_all[BLK / WAVE][STG];
+    __shared__ BoxF rbox_all[BLK / WAVE][RCH];
     const int lane = threadIdx.x & (WAVE - 1);
-    const int wv = threadIdx.x / WAVE;
+    const uint32_t wv = uni(threadIdx.x / WAVE);
     // blocks are dealt round-robin to the 8 XCDs: hand each XCD chunks of
     // XCHUNK consecutive logical blocks, so that neighbouring units (mostly of
     // one node) share an L2 while the chunks stay interleaved for balance.
     // The grid is a multiple of 8*XCHUNK blocks: this is a permutation.
     const uint32_t nbk = gridDim.x;
-    const uint32_t r = blockIdx.x / 8, xcd = blockIdx.x % 8;
-    const uint32_t lb = (r / XCHUNK) * (8 * XCHUNK) + xcd * XCHUNK + r % XCHUNK; // logical block
+    const uint32_t r8 = blockIdx.x / 8, xcd = blockIdx.x % 8;
+    const uint32_t lb = (r8 / XCHUNK) * (8 * XCHUNK) + xcd * XCHUNK + r8 % XCHUNK; // logical block
     const uint32_t wg = lb * (BLK / WAVE) + wv;
     if (lb >= nbk || wg >= a.nunits) return; // whole wave; no block barrier below
     uint64_t *stage = stage_all[wv];
-    const int v = int(a.unode[wg]);
+    BoxF *rbox = rbox_all[wv];
+    const int v = int(uni(a.unode[wg]));
     NodeShape s = node_shape(v, a.cov_off, a.cov_split, a.lend_off, a.nl, a.D);
-    uint32_t u = wg - a.uoff[v];
+    uint32_t u = wg - uni(a.uoff[v]);
     const uint32_t n1 = rect_units(s.nbase, s.nc), n2 = rect_units(s.nbase, s.ecn);
     int kind;
     uint32_t rows, cols, row_begin, col_begin;
     const uint32_t *colsrc;
+    const BoxF *colbox;
     if (u < n1) {
         kind = 1; // base cover x overlay cover
         rows = s.nbase;
@@ -222,6 +182,7 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
         row_begin = s.off;
         col_begin = s.off + s.nbase;
         colsrc = a.cov_e;
+        colbox = a.cov_box;
     } else if (u < n1 + n2) {
         kind = 2; // base cover x overlay end entries
         u -= n1;
@@ -230,6 +191,7 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
         row_begin = s.off;
         col_begin = s.ec0;
         colsrc = a.lend_e;
+        colbox = a.lend_box;
     } else {
         kind = 3; // overlay cover x base end entries
         u -= n1 + n2;
@@ -238,92 +200,120 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
         row_begin = s.off + s.nbase;
         col_begin = s.eb0;
         colsrc = a.lend_e;
+        colbox = a.lend_box;
     }
     uint32_t r0, nrows, t0, t1;
     rect_unit(rows, cols, u, r0, nrows, t0, t1);
-    // this lane's row edge
+    // this lane's row edge and box (the empty box past the last row)
     uint32_t re_l = 0;
-    int rk = 0;
-    double rx0 = 0, ry0 = 0, rx1 = 0, ry1 = 0;
+    BoxF rb = boxf_empty();
     if (uint32_t(lane) < nrows) {
         re_l = a.cov_e[row_begin + r0 + lane];
-        rx0 = a.x0[re_l];
-        ry0 = a.y0[re_l];
-        rx1 = a.x1[re_l];
-        ry1 = a.y1[re_l];
-        rk = a.etype[re_l];
+        rb = a.cov_box[row_begin + r0 + lane];
     }
-    const BoxF rbf = boxf_of(box_of(rx0, ry0, rx1, ry1));
+    rbox[lane] = rb;
+    __builtin_amdgcn_wave_barrier();
+    bool rows_fetched = false;
+    int rk = 0;
+    double rx0 = 0, ry0 = 0, rx1 = 0, ry1 = 0;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
     const int sb = a.bits_b + a.bits_c;
     uint32_t cnt = 0;
+    const uint32_t nr4 = (nrows + 3u) & ~3u; // rows past nrows hold the empty box
     for (uint32_t t = t0; t < t1; ++t) {
         const uint32_t cb = t * WAVE;
         const uint32_t ncol = min(uint32_t(WAVE), cols - cb);
         bool valid = uint32_t(lane) < ncol;
         uint32_t ce = 0;
+        BoxF cbf = boxf_empty();
         if (valid) {
-            const uint32_t raw = colsrc[col_begin + cb + lane];
+            const uint32_t ci = col_begin + cb + lane;
+            const uint32_t raw = colsrc[ci];
             ce = raw & 0x7fffffffu;
+            cbf = colbox[ci];
             if (kind != 1 && (raw & 0x80000000u)) {
                 // "after" copy of an edge whose "before" leaf is under v as well
-                const int bl = a.lend_bl[col_begin + cb + lane];
+                const int bl = a.lend_bl[ci];
                 if (bl >= a.nl + s.lo && bl < a.nl + s.hi) valid = false;
             }
         }
-        double cx0 = 0, cy0 = 0, cx1 = 0, cy1 = 0;
+        if (!valid) cbf = boxf_empty();
+        bool cols_fetched = false;
         int ck = 0;
-        if (valid) {
-            cx0 = a.x0[ce];
-            cy0 = a.y0[ce];
-            cx1 = a.x1[ce];
-            cy1 = a.y1[ce];
-            ck = a.etype[ce];
-        }
-        const Box cbx = box_of(cx0, cy0, cx1, cy1);
-        const BoxF cbf = boxf_of(cbx);
-        for (uint32_t r = 0; r < nrows; ++r) {
-            // single-precision screen (exact rejection), then LSMF on the doubles
-            const float qminx = bcastf(rbf.minx, int(r)), qmaxx = bcastf(rbf.maxx, int(r));
-            const float qminy = bcastf(rbf.miny, int(r)), qmaxy = bcastf(rbf.maxy, int(r));
-            const bool near = valid && !(qminx > cbf.maxx || qmaxx < cbf.minx) && !(qminy > cbf.maxy || qmaxy < cbf.miny);
-            bool hit = false;
-            uint64_t key = 0;
-            if (near) {
-                const double px0 = bcast(rx0, int(r)), py0 = bcast(ry0, int(r));
-                const double px1 = bcast(rx1, int(r)), py1 = bcast(ry1, int(r));
-                const Box q = box_of(px0, py0, px1, py1);
-                const bool box = !(q.minx > cbx.maxx || q.maxx < cbx.minx) && !(q.miny > cbx.maxy || q.maxy < cbx.miny);
-                const uint32_t re = uint32_t(__builtin_amdgcn_readlane(int(re_l), int(r)));
-                int tp = 0;
-                uint32_t be, qe;
-                int k;
-                if (kind == 3) { // row = overlay edge, column = base edge
-                    if (box) tp = classify_pre(cx0, cy0, cx1, cy1, px0, py0, px1, py1);
-                    be = ce;
-                    qe = re;
-                    k = __builtin_amdgcn_readlane(rk, int(r));
-                } else {
-                    if (box) tp = classify_pre(px0, py0, px1, py1, cx0, cy0, cx1, cy1);
-                    be = re;
-                    qe = ce;
-                    k = ck;
-                }
-                if (tp != 0) {
-                    hit = true;
-                    uint32_t local = qe - uint32_t(a.eoff[k - 1]);
-                    key = (uint64_t(k) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
-                }
+        double cx0 = 0, cy0 = 0, cx1 = 0, cy1 = 0;
+        for (uint32_t r = 0; r < nr4; r += 4) {
+            // single-precision screen (exact rejection) of four rows
+            uint64_t m[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const BoxF q = rbox[r + k];
+                m[k] = __ballot(boxf_overlap(q, cbf));
             }
-            const uint64_t m = __ballot(hit);
-            if (m) {
-                const uint32_t nh = uint32_t(__popcll(m));
-                if (cnt + nh > uint32_t(STG)) {
-                    flush(a, stage, cnt, lane);
-                    cnt = 0;
+            if ((m[0] | m[1] | m[2] | m[3]) == 0) continue;
+            if (!cols_fetched) {
+                if (valid) {
+                    cx0 = a.x0[ce];
+                    cy0 = a.y0[ce];
+                    cx1 = a.x1[ce];
+                    cy1 = a.y1[ce];
+                    ck = a.etype[ce];
                 }
-                if (hit) stage[cnt + uint32_t(__popcll(m & lt))] = key;
-                cnt += nh;
+                cols_fetched = true;
+            }
+            if (!rows_fetched) {
+                if (uint32_t(lane) < nrows) {
+                    rx0 = a.x0[re_l];
+                    ry0 = a.y0[re_l];
+                    rx1 = a.x1[re_l];
+                    ry1 = a.y1[re_l];
+                    rk = a.etype[re_l];
+                }
+                rows_fetched = true;
+            }
+            const Box cbx = box_of(cx0, cy0, cx1, cy1);
+#pragma unroll 1
+            for (int k = 0; k < 4; ++k) {
+                if (m[k] == 0) continue;
+                const int rr = int(r) + k;
+                const bool near = (m[k] >> lane) & 1ull;
+                bool hit = false;
+                uint64_t key = 0;
+                if (near) {
+                    // LSMF on the doubles, then the classification
+                    const double px0 = bcast(rx0, rr), py0 = bcast(ry0, rr);
+                    const double px1 = bcast(rx1, rr), py1 = bcast(ry1, rr);
+                    const bool box = box_overlap(box_of(px0, py0, px1, py1), cbx);
+                    const uint32_t re = uint32_t(__builtin_amdgcn_readlane(int(re_l), rr));
+                    int tp = 0;
+                    uint32_t be, qe;
+                    int kk;
+                    if (kind == 3) { // row = overlay edge, column = base edge
+                        if (box) tp = classify_pre(cx0, cy0, cx1, cy1, px0, py0, px1, py1);
+                        be = ce;
+                        qe = re;
+                        kk = __builtin_amdgcn_readlane(rk, rr);
+                    } else {
+                        if (box) tp = classify_pre(px0, py0, px1, py1, cx0, cy0, cx1, cy1);
+                        be = re;
+                        qe = ce;
+                        kk = ck;
+                    }
+                    if (tp != 0) {
+                        hit = true;
+                        uint32_t local = qe - uint32_t(a.eoff[kk - 1]);
+                        key = (uint64_t(kk) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
+                    }
+                }
+                const uint64_t hm = __ballot(hit);
+                if (hm) {
+                    const uint32_t nh = uint32_t(__popcll(hm));
+                    if (cnt + nh > uint32_t(STG)) {
+                        flush(a, stage, cnt, lane);
+                        cnt = 0;
+                    }
+                    if (hit) stage[cnt + uint32_t(__popcll(hm & lt))] = key;
+                    cnt += nh;
+                }
             }
         }
     }
@@ -403,6 +393,7 @@ void find_candidates(Ctx &c) {
         a.cov_off = c.cov_off; a.cov_split = c.cov_split; a.cov_e = c.cov_e;
         a.lend_off = c.lend_off; a.lend_e = c.lend_e; a.uoff = units; a.unode = unode;
         a.lend_bl = c.lend_bl; a.etype = c.etype; a.eoff = c.d_eoff;
+        a.cov_box = c.cov_box; a.lend_box = c.lend_box;
         a.x0 = c.ex0; a.y0 = c.ey0; a.x1 = c.ex1; a.y1 = c.ey1;
         a.nl = c.nleaves; a.D = c.depth; a.T = T; a.nunits = nunits;
         a.bits_b = c.bits_b; a.bits_c = c.bits_c;

@@ -71,7 +71,7 @@ __global__ void k_edges(const double *__restrict__ bpoly, const double *__restri
                         const int *__restrict__ beoff, const int *__restrict__ bvoff, int nbr,
                         const double *__restrict__ shift, double *x0, double *y0, double *x1, double *y1,
                         double *es, double *ee, double *eyl, double *eyh, int *ety, int *eid, double *uxs,
-                        double *uxe) {
+                        double *uxe, BoxF *ebox) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     const double *P;
@@ -111,6 +111,7 @@ __global__ void k_edges(const double *__restrict__ bpoly, const double *__restri
     eyh[e] = b < d ? d : b;
     ety[e] = ty;
     eid[e] = id;
+    ebox[e] = boxf_of(box_of(a, b, c, d));
 }
 
 // ---- A3 ----
@@ -518,7 +519,7 @@ void build_edges(Ctx &c) {
     int ne = c.ne;
     c.ex0.ensure(ne); c.ey0.ensure(ne); c.ex1.ensure(ne); c.ey1.ensure(ne);
     c.estart.ensure(ne); c.eend.ensure(ne); c.etype.ensure(ne); c.eid.ensure(ne);
-    c.eymin.ensure(ne); c.eymax.ensure(ne);
+    c.eymin.ensure(ne); c.eymax.ensure(ne); c.ebox.ensure(ne);
     if (c.paired) {
         c.uxs.ensure(ne);
         c.uxe.ensure(ne);
@@ -526,7 +527,8 @@ void build_edges(Ctx &c) {
     k_edges<<<grid_for(ne), BLK, 0, c.stream>>>(c.bpoly, c.cpoly, c.nb, ne, c.d_eoff, c.d_coff, c.K, c.d_beoff,
                                                 c.d_bvoff, c.nbr, c.paired ? c.d_shift.p : nullptr, c.ex0, c.ey0,
                                                 c.ex1, c.ey1, c.estart, c.eend, c.eymin, c.eymax, c.etype, c.eid,
-                                                c.paired ? c.uxs.p : nullptr, c.paired ? c.uxe.p : nullptr);
+                                                c.paired ? c.uxs.p : nullptr, c.paired ? c.uxe.p : nullptr,
+                                                c.ebox);
     SC_HIP(hipGetLastError());
 }
 
@@ -581,9 +583,19 @@ void build_node_intervals(Ctx &c) {
 }
 
 namespace {
-__global__ void k_entry_before(const uint32_t *__restrict__ e, int64_t n, const int *__restrict__ before, int *bl) {
+// beside every leaf end entry: its edge's "before" leaf and box
+__global__ void k_entry_before(const uint32_t *__restrict__ e, int64_t n, const int *__restrict__ before,
+                               const BoxF *__restrict__ ebox, int *bl, BoxF *box) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) bl[i] = before[e[i] & 0x7fffffffu];
+    if (i >= n) return;
+    const uint32_t k = e[i] & 0x7fffffffu;
+    bl[i] = before[k];
+    box[i] = ebox[k];
+}
+// beside every cover entry: its edge's box
+__global__ void k_entry_boxes(const uint32_t *__restrict__ e, int64_t n, const BoxF *__restrict__ ebox, BoxF *box) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) box[i] = ebox[e[i]];
 }
 } // namespace
 
@@ -608,10 +620,11 @@ void build_leaf_end_entries(Ctx &c) {
     uint32_t *tot = c.d_u32_total.ensure(1);
     scan_exclusive_u32(off, off, 2 * int64_t(nl) + 1, tot, s);
     c.n_lend = read_u32(c, tot);
-    // beside every entry, its edge's "before" leaf (the stab's dedupe test)
+    // beside every entry, its edge's "before" leaf (the stab's dedupe test) and box
     int *bl = c.lend_bl.ensure(size_t(c.n_lend > 0 ? c.n_lend : 1));
+    BoxF *bx = c.lend_box.ensure(size_t(c.n_lend > 0 ? c.n_lend : 1));
     if (c.n_lend > 0) {
-        k_entry_before<<<grid_for(c.n_lend), BLK, 0, s>>>(vals, c.n_lend, c.before_leaf, bl);
+        k_entry_before<<<grid_for(c.n_lend), BLK, 0, s>>>(vals, c.n_lend, c.before_leaf, c.ebox, bl, bx);
         SC_HIP(hipGetLastError());
     }
 }
@@ -641,6 +654,8 @@ void build_cover_lists(Ctx &c) {
                                                    keys, vals, lr);
         SC_HIP(hipGetLastError());
         radix_sort_u64(keys, vals, L, 0, ilog2_exact(T) + 1, c.rs, s);
+        k_entry_boxes<<<grid_for(L), BLK, 0, s>>>(vals, L, c.ebox, c.cov_box.ensure(size_t(L)));
+        SC_HIP(hipGetLastError());
     }
     uint32_t *off = c.cov_off.ensure(size_t(T) + 1);
     uint32_t *split = c.cov_split.ensure(T);
