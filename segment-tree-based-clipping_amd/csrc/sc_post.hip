@@ -53,44 +53,59 @@ __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, in
                               const double *__restrict__ bpoly, const double *__restrict__ cpoly, int *pj, int *pb,
                               int *pc, int *pt, double *px, double *py, int *qa, int *qb, int *all_b, int *nd_b,
                               int *all_c, int *nd_c, int *flags) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= I) return;
-    int j = seg_of(ja.p0, ja.J, p);
-    uint64_t key = pairs[p];
-    int b = int((key >> bits_c) & ((1ull << bits_b) - 1ull)) - ja.beoff[j];
-    int cl = int(key & ((1ull << bits_c) - 1ull));
-    pj[p] = j;
-    if (b < 0 || b >= ja.nb[j] || cl >= ja.nc[j]) { // a pair outside the job's rings
-        atomicOr(&flags[j], UB_UNASSIGNED);
-        pb[p] = 0;
-        pc[p] = 0;
-        pt[p] = 0;
-        px[p] = 0.0;
-        py[p] = 0.0;
-        qa[p] = 0;
-        qb[p] = 0;
-        return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (WAVE - 1);
+    bool counted = false; // a pair inside its job's rings
+    int gb = -1, t = 0;
+    if (p < I) {
+        const int j = seg_of(ja.p0, ja.J, p);
+        const uint64_t key = pairs[p];
+        const int b = int((key >> bits_c) & ((1ull << bits_b) - 1ull)) - ja.beoff[j];
+        const int cl = int(key & ((1ull << bits_c) - 1ull));
+        pj[p] = j;
+        if (b < 0 || b >= ja.nb[j] || cl >= ja.nc[j]) { // a pair outside the job's rings
+            atomicOr(&flags[j], UB_UNASSIGNED);
+            pb[p] = 0;
+            pc[p] = 0;
+            pt[p] = 0;
+            px[p] = 0.0;
+            py[p] = 0.0;
+            qa[p] = 0;
+            qb[p] = 0;
+        } else {
+            const int64_t bId = 2 * int64_t(ja.bvoff[j] + b), cId = 2 * int64_t(ja.cvoff[j] + cl);
+            const double b1x = bpoly[bId], b1y = bpoly[bId + 1], b2x = bpoly[bId + 2], b2y = bpoly[bId + 3];
+            const double c1x = cpoly[cId], c1y = cpoly[cId + 1], c2x = cpoly[cId + 2], c2y = cpoly[cId + 3];
+            double al, be;
+            t = classify(b1x, b1y, b2x, b2y, c1x, c1y, c2x, c2y, al, be);
+            double x, y;
+            intersection_point(t, al, b1x, b1y, b2x, b2y, c1x, c1y, x, y);
+            pb[p] = b;
+            pc[p] = cl;
+            pt[p] = t;
+            px[p] = x;
+            py[p] = y;
+            qa[p] = alpha_to_int(al);
+            qb[p] = alpha_to_int(be);
+            atomicAdd(&all_c[ja.JC[j] + cl], 1);
+            if (nondeg_c(t)) atomicAdd(&nd_c[ja.JC[j] + cl], 1);
+            gb = ja.JB[j] + b;
+            counted = true;
+        }
     }
-    int64_t bId = 2 * int64_t(ja.bvoff[j] + b), cId = 2 * int64_t(ja.cvoff[j] + cl);
-    double b1x = bpoly[bId], b1y = bpoly[bId + 1], b2x = bpoly[bId + 2], b2y = bpoly[bId + 3];
-    double c1x = cpoly[cId], c1y = cpoly[cId + 1], c2x = cpoly[cId + 2], c2y = cpoly[cId + 3];
-    double al, be;
-    int t = classify(b1x, b1y, b2x, b2y, c1x, c1y, c2x, c2y, al, be);
-    double x, y;
-    intersection_point(t, al, b1x, b1y, b2x, b2y, c1x, c1y, x, y);
-    pb[p] = b;
-    pc[p] = cl;
-    pt[p] = t;
-    px[p] = x;
-    py[p] = y;
-    qa[p] = alpha_to_int(al);
-    qb[p] = alpha_to_int(be);
-    if (all_b) {
-        atomicAdd(&all_b[ja.JB[j] + b], 1);
-        if (nondeg_b(t)) atomicAdd(&nd_b[ja.JB[j] + b], 1);
-    }
-    atomicAdd(&all_c[ja.JC[j] + cl], 1);
-    if (nondeg_c(t)) atomicAdd(&nd_c[ja.JC[j] + cl], 1);
+    if (!all_b) return; // uniform
+    // pairs are sorted by (job, base edge): each run of one base edge inside
+    // the wave adds its counts with one atomic per counter
+    const int prev = __shfl_up(gb, 1, WAVE);
+    const bool head = counted && (lane == 0 || prev != gb);
+    const uint64_t hm = __ballot(head), cm = __ballot(counted), ndm = __ballot(counted && nondeg_b(t));
+    if (!head) return;
+    const uint64_t above = lane == WAVE - 1 ? 0ull : hm & (~0ull << (lane + 1));
+    const int nxt = above ? __ffsll((unsigned long long)above) - 1 : WAVE;
+    const uint64_t run = (nxt == WAVE ? ~0ull : ((1ull << nxt) - 1ull)) & (~0ull << lane) & cm;
+    atomicAdd(&all_b[gb], __popcll(run));
+    const int n_nd = __popcll(run & ndm);
+    if (n_nd) atomicAdd(&nd_b[gb], n_nd);
 }
 
 // keys for the per-edge alpha order: quantised alpha modulo 1e9 (the nine
@@ -167,8 +182,9 @@ __global__ void k_aug_sources(int J, const int *__restrict__ JX, const int *__re
     if (sv) sv[pos] = v;
 }
 
-// one thread per edge with intersections: a leading degenerate intersection
-// marks the source vertex, the non-degenerate ones fill the slots after it
+// one thread per position of the per-edge alpha order: a leading degenerate
+// intersection marks its edge's source vertex, the non-degenerate ones fill
+// the slots after it in order
 template <bool SIDE_B>
 __global__ void k_aug_segments(const int *__restrict__ edge_of, const int *__restrict__ pj, int I,
                                const int *__restrict__ ord, const int *__restrict__ JX, const int *__restrict__ AX,
@@ -178,39 +194,30 @@ __global__ void k_aug_segments(const int *__restrict__ edge_of, const int *__res
                                int *av, int *slot, int *flags, int *sp) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= I) return;
-    int p0 = ord[i];
-    int j = pj[p0];
-    int e = edge_of[p0];
-    int gi = JX[j] + e;
-    int s = all_scan[gi];
-    if (i != s) return; // not the first position of this edge's segment
-    int t = all_scan[gi + 1];
-    int nd = nd_scan[gi + 1] - nd_scan[gi];
-    int src = AX[j] + e + (nd_scan[gi] - nd_scan[JX[j]]);
-    int k0 = 0;
-    int first = ord[s];
-    bool lead_degen = SIDE_B ? degen_b_first(pt[first]) : degen_c_first(pt[first]);
-    if (lead_degen) {
-        av[src] = q[first];
-        slot[first] = src;
-        k0 = 1;
+    const int p = ord[i];
+    const int j = pj[p];
+    const int e = edge_of[p];
+    const int gi = JX[j] + e;
+    const int s = all_scan[gi], t = all_scan[gi + 1];
+    const int nd = nd_scan[gi + 1] - nd_scan[gi];
+    const int src = AX[j] + e + (nd_scan[gi] - nd_scan[JX[j]]);
+    const int first = i == s ? p : ord[s];
+    const int k0 = (SIDE_B ? degen_b_first(pt[first]) : degen_c_first(pt[first])) ? 1 : 0;
+    if (i == s && k0 + nd != t - s) atomicOr(&flags[j], UB_UNASSIGNED);
+    if (i == s && k0) {
+        av[src] = q[p];
+        slot[p] = src;
+        return;
     }
-    for (int k = 0; k < nd; ++k) {
-        if (s + k0 + k >= t) {
-            atomicOr(&flags[j], UB_UNASSIGNED);
-            break;
-        }
-        int p = ord[s + k0 + k];
-        bool ok = SIDE_B ? nondeg_b(pt[p]) : nondeg_c(pt[p]);
-        if (!ok) atomicOr(&flags[j], UB_UNASSIGNED);
-        int sl = src + 1 + k;
-        ax[sl] = px[p];
-        ay[sl] = py[p];
-        av[sl] = q[p];
-        slot[p] = sl;
-        if (sp) sp[sl] = p;
-    }
-    if (k0 + nd != t - s) atomicOr(&flags[j], UB_UNASSIGNED);
+    const int k = i - s - k0;
+    if (k >= nd) return; // more positions than non-degenerate slots: flagged above
+    if (!(SIDE_B ? nondeg_b(pt[p]) : nondeg_c(pt[p]))) atomicOr(&flags[j], UB_UNASSIGNED);
+    const int sl = src + 1 + k;
+    ax[sl] = px[p];
+    ay[sl] = py[p];
+    av[sl] = q[p];
+    slot[p] = sl;
+    if (sp) sp[sl] = p;
 }
 
 // neighbour links, 1-based and local to the job's rings (as the reference's arrays)

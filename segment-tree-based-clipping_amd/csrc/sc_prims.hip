@@ -189,54 +189,103 @@ __global__ __launch_bounds__(RS_BLOCK) void k_upsweep(const uint64_t *keys, int6
     tile_hist[int64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// One tile of RS_TILE keys per block; wave w owns the RS_TILE/RS_WAVES
+// consecutive keys starting at w*RS_TILE/RS_WAVES, 64 at a time.  Keys are
+// ranked within their wave and digit (peers found with eight ballots, a
+// per-wave digit counter in LDS), a block scan over (digit, wave) counters
+// gives every key its place in the tile's digit order, the tile is sorted in
+// LDS, and each digit's run is written out contiguously at the tile's global
+// offset for that digit.  Within a digit, keys keep their input order.
+static_assert(RS_BLOCK == 256, "the tile scan gives one thread per 8-bit digit");
+constexpr int RS_PER_WAVE = RS_TILE / RS_WAVES;
+constexpr int RS_IT = RS_PER_WAVE / WAVE; // = RS_ITEMS
+
 __global__ __launch_bounds__(RS_BLOCK) void k_downsweep(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                         uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                         int64_t n, int shift, uint32_t mask,
                                                         const uint32_t *__restrict__ tile_off, int ntiles) {
-    __shared__ uint32_t s_base[256];
-    __shared__ uint32_t s_cnt[RS_WAVES][256];
-    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    __shared__ uint64_t s_k[RS_TILE];
+    __shared__ uint32_t s_v[RS_TILE];
+    __shared__ uint32_t s_cnt[256 * RS_WAVES]; // index digit*RS_WAVES + wave
+    __shared__ uint32_t s_gbase[256];          // global offset of the digit minus its tile-local start
+    __shared__ uint32_t s_wsum[RS_WAVES];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
-    s_base[threadIdx.x] = tile_off[int64_t(threadIdx.x) * ntiles + blockIdx.x];
-    for (int w = 0; w < RS_WAVES; ++w) s_cnt[w][threadIdx.x] = 0;
+    const int64_t base = int64_t(blockIdx.x) * RS_TILE;
+    for (int i = tid; i < 256 * RS_WAVES; i += RS_BLOCK) s_cnt[i] = 0;
+    uint64_t k[RS_IT];
+    uint32_t v[RS_IT], rk[RS_IT];
+#pragma unroll
+    for (int it = 0; it < RS_IT; ++it) {
+        const int64_t i = base + w * RS_PER_WAVE + it * WAVE + lane;
+        k[it] = i < n ? kin[i] : 0ull;
+        v[it] = (i < n && vin) ? vin[i] : 0u;
+    }
     __syncthreads();
-    int64_t base = int64_t(blockIdx.x) * RS_TILE;
-    for (int it = 0; it < RS_ITEMS; ++it) {
-        int64_t i = base + int64_t(it) * RS_BLOCK + threadIdx.x;
-        bool valid = i < n;
-        uint64_t k = valid ? kin[i] : 0ull;
-        uint32_t v = (valid && vin) ? vin[i] : 0u;
-        uint32_t d = uint32_t(k >> shift) & mask;
+#pragma unroll
+    for (int it = 0; it < RS_IT; ++it) {
+        const int64_t i = base + w * RS_PER_WAVE + it * WAVE + lane;
+        const bool valid = i < n;
+        const uint32_t d = uint32_t(k[it] >> shift) & mask;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(bit);
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
             m &= bit ? bb : ~bb;
         }
-        uint32_t r = __popcll(m & lt);
-        if (valid && r == 0) s_cnt[wid][d] = __popcll(m);
-        __syncthreads();
-        uint32_t pos = 0;
-        if (valid) {
-            pos = s_base[d] + r;
-            for (int w = 0; w < wid; ++w) pos += s_cnt[w][d];
-        }
-        __syncthreads();
-        {
-            uint32_t add = 0;
+        const uint32_t r = __popcll(m & lt);
+        uint32_t before = 0;
+        if (valid) before = s_cnt[d * RS_WAVES + w];
+        rk[it] = before + r;
+        if (valid && r == 0) s_cnt[d * RS_WAVES + w] = before + uint32_t(__popcll(m));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // exclusive scan over the 256*RS_WAVES counters; thread d holds digit d's
+    uint32_t c[RS_WAVES], sum = 0;
 #pragma unroll
-            for (int w = 0; w < RS_WAVES; ++w) {
-                add += s_cnt[w][threadIdx.x];
-                s_cnt[w][threadIdx.x] = 0;
-            }
-            s_base[threadIdx.x] += add;
+    for (int q = 0; q < RS_WAVES; ++q) {
+        c[q] = s_cnt[tid * RS_WAVES + q];
+        sum += c[q];
+    }
+    uint32_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc += y;
+    }
+    if (lane == WAVE - 1) s_wsum[w] = inc;
+    __syncthreads();
+    uint32_t ex = inc - sum;
+    for (int q = 0; q < w; ++q) ex += s_wsum[q];
+    {
+        uint32_t run = ex;
+#pragma unroll
+        for (int q = 0; q < RS_WAVES; ++q) {
+            s_cnt[tid * RS_WAVES + q] = run;
+            run += c[q];
         }
-        __syncthreads();
-        if (valid) {
-            kout[pos] = k;
-            if (vout) vout[pos] = v;
+    }
+    s_gbase[tid] = tile_off[int64_t(tid) * ntiles + blockIdx.x] - ex;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < RS_IT; ++it) {
+        const int64_t i = base + w * RS_PER_WAVE + it * WAVE + lane;
+        if (i < n) {
+            const uint32_t d = uint32_t(k[it] >> shift) & mask;
+            const uint32_t pos = s_cnt[d * RS_WAVES + w] + rk[it];
+            s_k[pos] = k[it];
+            s_v[pos] = v[it];
         }
+    }
+    __syncthreads();
+    const int m = int(n - base < RS_TILE ? n - base : RS_TILE);
+    for (int i = tid; i < m; i += RS_BLOCK) {
+        const uint64_t key = s_k[i];
+        const uint32_t o = s_gbase[uint32_t(key >> shift) & mask] + uint32_t(i);
+        kout[o] = key;
+        if (vout) vout[o] = s_v[i];
     }
 }
 

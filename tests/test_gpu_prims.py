@@ -40,7 +40,7 @@ def test_stable_sort_f64(gpu, n):
     assert np.array_equal(k, keys[order])
 
 
-@pytest.mark.parametrize("n", [0, 1, 5000, 200003])
+@pytest.mark.parametrize("n", [0, 1, 5000, 200003, (1 << 21) + 7])
 def test_stable_sort_i32_i64(gpu, n):
     rng = np.random.default_rng(100 + n)
     k32 = rng.integers(-1000, 1000, n).astype(np.int32)
