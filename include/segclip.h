@@ -35,6 +35,7 @@
  */
 #ifndef SEGCLIP_H
 #define SEGCLIP_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -145,6 +146,17 @@ int segclip_batch_area_backward_device(segclip_ctx *ctx, const double *g_area, c
  * the library (all contexts) and their peak since the last reset. */
 int segclip_release(segclip_ctx *ctx);
 int segclip_memory(int64_t *current, int64_t *peak, int reset_peak);
+/* Device allocator for the library's buffers (process-wide), for programs
+ * that pool device memory (a framework's caching allocator).  Buffers
+ * allocated while set come from alloc (null: out of memory) and go back
+ * through free, which is called only after the library's queued device work
+ * has finished.  Null alloc restores hipMalloc/hipFree for later buffers;
+ * buffers keep the allocator they came from, so the hooks must stay callable
+ * until those are released.  No counterpart in the reference (its buffers
+ * are host arrays and OpenACC copies). */
+typedef void *(*segclip_alloc_fn)(size_t bytes, void *user);
+typedef void (*segclip_free_fn)(void *ptr, void *user);
+int segclip_set_allocator(segclip_alloc_fn alloc, segclip_free_fn free_fn, void *user);
 /* per job of the last batch (or the single job of segclip_clip): ring
  * offsets (jobs+1 entries) and status bits (the 'undefined' bits per job) */
 int segclip_copy_batch_rings(segclip_ctx *ctx, int32_t *job_ring_off, int64_t cap, int64_t *need);

@@ -199,6 +199,14 @@ int segclip_set_provenance(segclip_ctx *h, int on) {
 int segclip_release(segclip_ctx *h) {
     return guarded(h, [&](sc::Ctx &x) { sc::release_derived(x); });
 }
+int segclip_set_allocator(segclip_alloc_fn alloc, segclip_free_fn free_fn, void *user) {
+    if ((alloc == nullptr) != (free_fn == nullptr)) return SEGCLIP_EINVAL;
+    sc::AllocHooks &h = sc::alloc_hooks();
+    h.alloc = alloc;
+    h.free = free_fn;
+    h.user = user;
+    return SEGCLIP_OK;
+}
 int segclip_memory(int64_t *current, int64_t *peak, int reset_peak) {
     if (current) *current = sc::dev_bytes().load();
     if (peak) *peak = sc::dev_peak().load();

@@ -56,17 +56,37 @@ inline void dev_account(int64_t delta) {
     }
 }
 
+// Optional device allocator supplied by the embedding program (for instance
+// a framework's caching allocator), used for every buffer allocated while it
+// is set.  The library waits for all of its device work before handing a
+// block back, so the allocator may reuse it at once on any stream.
+struct AllocHooks {
+    void *(*alloc)(size_t bytes, void *user) = nullptr;
+    void (*free)(void *ptr, void *user) = nullptr;
+    void *user = nullptr;
+};
+inline AllocHooks &alloc_hooks() {
+    static AllocHooks h;
+    return h;
+}
+
 // Growable device buffer; contents are not preserved across growth.
 template <class T> struct DBuf {
     T *p = nullptr;
     size_t cap = 0;
+    AllocHooks from; // allocator of p (alloc == nullptr: hipMalloc)
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
     void release() {
         if (p) {
-            (void)hipFree(p);
+            if (from.alloc) {
+                (void)hipDeviceSynchronize(); // work still reading p may be queued
+                from.free(p, from.user);
+            } else {
+                (void)hipFree(p);
+            }
             dev_account(-int64_t(cap * sizeof(T)));
         }
         p = nullptr;
@@ -77,11 +97,19 @@ template <class T> struct DBuf {
         if (n > cap) {
             release();
             size_t want = n + n / 8;
-            hipError_t e = hipMalloc(&p, want * sizeof(T));
-            if (e != hipSuccess) {
-                p = nullptr;
-                (void)hipGetLastError();
-                throw Error(SC_ENOMEM, "hipMalloc of " + std::to_string(want * sizeof(T)) + " bytes failed");
+            const AllocHooks &h = alloc_hooks();
+            if (h.alloc) {
+                p = static_cast<T *>(h.alloc(want * sizeof(T), h.user));
+                if (!p) throw Error(SC_ENOMEM, "allocation of " + std::to_string(want * sizeof(T)) + " bytes failed");
+                from = h;
+            } else {
+                hipError_t e = hipMalloc(&p, want * sizeof(T));
+                if (e != hipSuccess) {
+                    p = nullptr;
+                    (void)hipGetLastError();
+                    throw Error(SC_ENOMEM, "hipMalloc of " + std::to_string(want * sizeof(T)) + " bytes failed");
+                }
+                from = AllocHooks();
             }
             cap = want;
             dev_account(int64_t(want * sizeof(T)));

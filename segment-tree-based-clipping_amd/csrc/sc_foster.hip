@@ -1065,13 +1065,15 @@ __device__ void seq_run(SeqArgs &a) {
                 I = a.nx[I];
             }
         }
+        // membership of the Q lists, marked in the traversal's (still zero)
+        // state words: -1 in slot 2V for the split list, in 2V+1 for the crossing list
+        const int nmark = n;
+        for (int j = 0; j < nsplit[1]; ++j) a.seen[2 * splitL[1][j]] = -1;
+        for (int j = 0; j < ncross[1]; ++j) a.seen[2 * crossL[1][j] + 1] = -1;
         // 5) split interior bouncing pairs
         for (int i = 0; i < nsplit[0]; ++i) {
             int IP = splitL[0][i], IQ = a.nb[IP];
-            bool found = false;
-            for (int j = 0; j < nsplit[1]; ++j)
-                if (splitL[1][j] == IQ) found = true;
-            if (!found) continue;
+            if (!(IQ >= 0 && IQ < nmark && a.seen[2 * IQ] == -1)) continue;
             int VP = seq_new(a, n, a.vx[IP], a.vy[IP], seq_src(a, IP));
             int VQ = seq_new(a, n, a.vx[IQ], a.vy[IQ], seq_src(a, IQ));
             if (VP < 0 || VQ < 0) { status |= UB_TRAVERSAL; goto done; }
@@ -1092,12 +1094,13 @@ __device__ void seq_run(SeqArgs &a) {
         // 6) crossing candidates
         for (int i = 0; i < ncross[0]; ++i) {
             int IP = crossL[0][i], IQ = a.nb[IP];
-            for (int j = 0; j < ncross[1]; ++j)
-                if (crossL[1][j] == IQ) {
-                    a.lab[IP] = L_CROSSING;
-                    a.lab[IQ] = L_CROSSING;
-                }
+            if (IQ >= 0 && IQ < nmark && a.seen[2 * IQ + 1] == -1) {
+                a.lab[IP] = L_CROSSING;
+                a.lab[IQ] = L_CROSSING;
+            }
         }
+        for (int j = 0; j < nsplit[1]; ++j) a.seen[2 * splitL[1][j]] = 0;
+        for (int j = 0; j < ncross[1]; ++j) a.seen[2 * crossL[1][j] + 1] = 0;
         // createResult2
         {
             int root = rootP, I = root;

@@ -12,7 +12,7 @@ import numpy as np
 from ._lib import (Counts, SegclipError, as_ptr, lib, OK, ESPACE)  # noqa: F401
 from . import synth  # noqa: F401
 
-__all__ = ["Clipper", "SegclipError", "synth", "lib", "memory"]
+__all__ = ["Clipper", "SegclipError", "synth", "lib", "memory", "use_torch_allocator", "torch_allocator"]
 
 
 def memory(reset_peak=False):
@@ -20,6 +20,62 @@ def memory(reset_peak=False):
     cur, pk = C.c_int64(0), C.c_int64(0)
     lib().segclip_memory(C.byref(cur), C.byref(pk), 1 if reset_peak else 0)
     return cur.value, pk.value
+
+
+_ALLOC_FN = C.CFUNCTYPE(C.c_void_p, C.c_size_t, C.c_void_p)
+_FREE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
+_hooks = None
+_torch_alloc_on = False
+
+
+def use_torch_allocator(on=True):
+    """Allocate libsegclip's device buffers from torch's caching allocator
+    (on=True) or with hipMalloc (on=False), for buffers allocated from now on.
+    Released buffers then return to torch's pool, so a clip after a release
+    (for instance the recomputing backward) reuses them without driver calls,
+    and torch can use them in between."""
+    global _hooks, _torch_alloc_on
+    import torch
+    if not on:
+        _check_rc(lib().segclip_set_allocator(None, None, None))
+        _torch_alloc_on = False
+        return  # the old callbacks stay referenced: buffers from them may still be live
+
+    def _alloc(nbytes, _user):
+        try:
+            return torch.cuda.caching_allocator_alloc(int(nbytes), torch.cuda.current_device())
+        except Exception:  # reported to the library as out of memory
+            return None
+
+    def _free(ptr, _user):
+        try:
+            torch.cuda.caching_allocator_delete(ptr)
+        except Exception:  # interpreter shutdown
+            pass
+
+    if _hooks is None:
+        _hooks = (_ALLOC_FN(_alloc), _FREE_FN(_free))
+    _check_rc(lib().segclip_set_allocator(C.cast(_hooks[0], C.c_void_p), C.cast(_hooks[1], C.c_void_p), None))
+    _torch_alloc_on = True
+
+
+class torch_allocator:
+    """Context manager: libsegclip buffers allocated inside come from torch's
+    caching allocator; the previous setting is restored on exit."""
+
+    def __enter__(self):
+        self._was = _torch_alloc_on
+        use_torch_allocator(True)
+        return self
+
+    def __exit__(self, *exc):
+        use_torch_allocator(self._was)
+        return False
+
+
+def _check_rc(rc):
+    if rc != OK:
+        raise SegclipError(rc, "segclip_set_allocator failed")
 
 
 class Clipper:

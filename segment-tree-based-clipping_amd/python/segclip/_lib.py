@@ -26,7 +26,7 @@ SYMBOLS = [
     "segclip_synth_uniform", "segclip_count_vertical", "segclip_set_timing", "segclip_get_stage_times",
     "segclip_set_pairs", "segclip_set_pairs_device", "segclip_clip_batch", "segclip_copy_batch_rings",
     "segclip_copy_batch_status", "segclip_set_provenance", "segclip_batch_area", "segclip_batch_area_device",
-    "segclip_batch_area_backward_device", "segclip_release", "segclip_memory", "segclip_save_intersections_ids",
+    "segclip_batch_area_backward_device", "segclip_release", "segclip_memory", "segclip_set_allocator", "segclip_save_intersections_ids",
     "segclip_sort_intersections_at_edges", "segclip_copy_intersections_arrays", "segclip_init_labels_arrays",
     "segclip_copy_batch_routes",
 ]
@@ -92,6 +92,7 @@ def lib():
         "segclip_batch_area_device": (C.c_int, [_vp, _vp]),
         "segclip_batch_area_backward_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "segclip_memory": (C.c_int, [_i64p, _i64p, C.c_int]),
+        "segclip_set_allocator": (C.c_int, [_vp, _vp, _vp]),
         "segclip_save_intersections_ids": (C.c_int, [_vp, _ip, _ip, C.c_int, _dp, _ip, _ip, _ip]),
         "segclip_sort_intersections_at_edges": (C.c_int, [_vp, _ip, _ip, C.c_int, _ip, _ip, C.c_int, _ip, C.c_int,
                                                           C.c_int, _ip]),

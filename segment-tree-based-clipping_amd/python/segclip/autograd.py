@@ -12,12 +12,13 @@ forward held fixed).  recompute=True keeps nothing but the input rings
 between forward and backward and clips again in backward.
 
 Tensors cross the C ABI as raw device pointers; the library works on its own
-stream, so both directions synchronise the device around the call.
+stream, so both directions synchronise the device around the call.  Inside
+forward and backward the library allocates from torch's caching allocator.
 """
 import numpy as np
 import torch
 
-from . import Clipper
+from . import Clipper, torch_allocator
 
 
 def _sync():
@@ -35,6 +36,13 @@ class AreaIoU(torch.autograd.Function):
         ov_xy = ov_xy.contiguous()
         bsz = np.asarray(base_sizes, dtype=np.int32)
         csz = np.asarray(ov_sizes, dtype=np.int32)
+        # the library's buffers come from torch's pool: the release after the
+        # forward and the rebuild in the backward then cost no driver calls
+        with torch_allocator():
+            return AreaIoU._forward(ctx, base_xy, ov_xy, bsz, csz, paired, recompute)
+
+    @staticmethod
+    def _forward(ctx, base_xy, ov_xy, bsz, csz, paired, recompute):
         cl = Clipper(device=base_xy.device.index or 0)
         _sync()
         if paired:
@@ -57,6 +65,11 @@ class AreaIoU(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_area, g_iou):
+        with torch_allocator():
+            return AreaIoU._backward(ctx, g_area, g_iou)
+
+    @staticmethod
+    def _backward(ctx, g_area, g_iou):
         cl = ctx.cl
         bshape, cshape, dev = ctx.shapes
         if ctx.recompute:

@@ -84,6 +84,35 @@ def test_pairs_batch_equals_each_pair_alone(gpu):
         assert err is None, "%s: batch vs oracle: %s" % (name, err)
 
 
+def test_pooled_allocator_release_and_rebuild(gpu):
+    """Buffers from torch's caching allocator: a batch, a release, the same
+    batch again in the same context, all equal to the hipMalloc path."""
+    import segclip
+    bases, ovs, names = _pairs()
+    print("[gpu-batch] pooled allocator", len(bases), flush=True)
+    ref = segclip.Clipper().set_pairs(bases, ovs)
+    ref.clip_batch()
+    want = ref.batch_rings()
+    ref.release()
+    assert not segclip._torch_alloc_on
+    with segclip.torch_allocator():
+        assert segclip._torch_alloc_on
+        cl = segclip.Clipper().set_pairs(bases, ovs)
+        for rep in range(2):
+            cl.clip_batch()
+            got = cl.batch_rings()
+            for i, name in enumerate(names):
+                err = rings_equal(got[i], want[i])
+                assert err is None, "%s (pass %d): %s" % (name, rep, err)
+            cl.release()
+    assert not segclip._torch_alloc_on
+    cl.clip_batch()  # buffers of this pass come from hipMalloc again
+    got = cl.batch_rings()
+    for i, name in enumerate(names):
+        assert rings_equal(got[i], want[i]) is None, name
+    cl.close()
+
+
 def test_small_pairs_batch_vs_oracle(gpu):
     import segclip
     bases, ovs = S.small_pairs(512, 4000)
