@@ -23,7 +23,10 @@ GPU loop on a sample of the same pairs):
 roofline: the stab kernel (the dominant kernel), FP64 work = W pair tests x 8
 FP64 comparisons (the LSMF bounding-box test every pair runs) per launch,
 divided by its launch time from HIP events on the library's stream, against
-the 78.6 TFLOP/s FP64 peak.
+the 78.6 TFLOP/s FP64 peak.  traffic: the kernel's HBM bytes per launch
+(FETCH_SIZE + WRITE_SIZE, scaled by a plain copy) from the rocprofv3 counter
+runs summarised in profiles/clip_harmonic_200k_1m_hbm_traffic.md, given for
+the default workload only.
 
 cpu_baseline: the CPU oracle (single-thread restatement of the reference's
 multicore path) on a bounded sample of the same construction, timed on this
@@ -44,8 +47,23 @@ FP64_PEAK_TFLOPS = 78.6
 FLOPS_PER_TEST = 8.0
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "clip_harmonic_200k_1m_hbm_traffic.md")
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith(kernel + " traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(nb, nc):
@@ -256,8 +274,10 @@ def main():
     stab = sum(stab_ms) / len(stab_ms)
     flops = FLOPS_PER_TEST * cnt["tests"]
     achieved = flops / (stab * 1e-3) / 1e12 if stab > 0 else 0.0
+    traffic = measured_traffic("k_stab") if (nb, nc) == (200000, 1000000) else None
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
+            "traffic_GBps": round(traffic / (stab * 1e-3) / 1e9, 1) if traffic and stab > 0 else None,
             "kernel": "k_stab", "kernel_ms": round(stab, 4),
             "work": "W=%d pair tests x %d FP64 ops (LSMF box test), FP64 VALU (no MFMA); the peak is the "
                     "FP64 vector rate" % (cnt["tests"], FLOPS_PER_TEST)}
