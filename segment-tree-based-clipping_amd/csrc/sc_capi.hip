@@ -55,8 +55,8 @@ void release_derived(Ctx &c) {
                               &c.d_pair_off, &c.u32a, &c.u32b, &c.u32c, &c.u32d})
         b->release();
     for (DBuf<uint64_t> *b : {&c.lend_key, &c.cand, &c.pairs, &c.k64a, &c.k64b, &c.u64a, &c.u64b}) b->release();
-    c.rs.tile_hist.release();
     c.rs.global_hist.release();
+    c.rs.tile_status.release();
     c.rs.keys_alt.release();
     c.rs.vals_alt.release();
     release_foster_scratch();
@@ -75,6 +75,7 @@ template <class F> int guarded(segclip_ctx *h, F &&f) {
     try {
         SC_HIP(hipSetDevice(h->c.device));
         f(h->c);
+        sc::radix_check(h->c.rs, h->c.stream); // a sort that gave up invalidates the call
         h->c.last_error.clear();
         return SEGCLIP_OK;
     } catch (const sc::Error &e) {

@@ -174,59 +174,90 @@ __global__ __launch_bounds__(RS_BLOCK) void k_global_hist(const uint64_t *keys, 
     }
 }
 
-__global__ __launch_bounds__(RS_BLOCK) void k_upsweep(const uint64_t *keys, int64_t n, int shift, uint32_t mask,
-                                                      uint32_t *tile_hist, int ntiles) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    int64_t base = int64_t(blockIdx.x) * RS_TILE;
-#pragma unroll 4
-    for (int it = 0; it < RS_ITEMS; ++it) {
-        int64_t i = base + int64_t(it) * RS_BLOCK + threadIdx.x;
-        if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & mask], 1u);
-    }
-    __syncthreads();
-    tile_hist[int64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// One tile of RS_TILE keys per block; wave w owns the RS_TILE/RS_WAVES
-// consecutive keys starting at w*RS_TILE/RS_WAVES, 64 at a time.  Keys are
-// ranked within their wave and digit (peers found with eight ballots, a
-// per-wave digit counter in LDS), a block scan over (digit, wave) counters
-// gives every key its place in the tile's digit order, the tile is sorted in
-// LDS, and each digit's run is written out contiguously at the tile's global
-// offset for that digit.  Within a digit, keys keep their input order.
-static_assert(RS_BLOCK == 256, "the tile scan gives one thread per 8-bit digit");
+// One pass of the LSD radix sort in one launch (a chained scan over tiles).
+// A block takes the next tile number from a counter, so every tile before
+// it belongs to a block that is already running.  Wave w of the block owns
+// the RS_TILE/RS_WAVES consecutive keys starting at w*RS_TILE/RS_WAVES, 64 at
+// a time; keys are ranked within their wave and digit (peers found with
+// eight ballots, a per-wave digit counter in LDS) and a block scan over
+// (digit, wave) counters gives every key its place in the tile's digit
+// order.  The tile publishes its per-digit counts, then thread d walks back
+// over earlier tiles' published words until one carries an inclusive
+// prefix, and publishes its own.  The tile is sorted in LDS and each digit's
+// run is written out contiguously at global digit base + prefix.  Within a
+// digit keys keep their input order (the sort is stable).
 constexpr int RS_PER_WAVE = RS_TILE / RS_WAVES;
 constexpr int RS_IT = RS_PER_WAVE / WAVE; // = RS_ITEMS
+static_assert(RS_BLOCK == 256, "one thread per 8-bit digit");
+// status word: epoch (8 bits) | flag (2 bits: aggregate, inclusive prefix) | count (54 bits);
+// a word of another epoch is not yet written for this pass
+constexpr int ST_EPOCH_SHIFT = 56;
+constexpr unsigned long long ST_AGG = 1ull << 54, ST_PRE = 2ull << 54, ST_VAL = (1ull << 54) - 1;
+constexpr int RS_SPIN_LIMIT = 1 << 20; // polls of one predecessor word before giving up
+constexpr int RS_LOOK = 8;             // predecessor words read per round trip
 
-__global__ __launch_bounds__(RS_BLOCK) void k_downsweep(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-                                                        uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                        int64_t n, int shift, uint32_t mask,
-                                                        const uint32_t *__restrict__ tile_off, int ntiles) {
+struct SweepArgs {
+    const uint64_t *kin;
+    const uint32_t *vin;
+    uint64_t *kout;
+    uint32_t *vout;
+    int64_t n;
+    int shift;
+    uint32_t mask;
+    const uint32_t *ghist;       // this pass's 256 digit counts over all keys
+    unsigned long long *status;  // ntiles * 256 words, none of them tagged with this epoch
+    unsigned long long epoch;    // 1..255, shifted into place
+    uint32_t *tile_counter;      // zero before the launch
+    uint32_t *err;               // set when a wait gives up (the sort is then invalid)
+};
+
+__device__ inline uint32_t block_excl_sum256(uint32_t x, uint32_t *s_wsum, uint32_t &total) {
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
+    uint32_t inc = x;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc += y;
+    }
+    if (lane == WAVE - 1) s_wsum[w] = inc;
+    __syncthreads();
+    uint32_t ex = inc - x;
+    total = 0;
+    for (int q = 0; q < RS_WAVES; ++q) {
+        if (q < w) ex += s_wsum[q];
+        total += s_wsum[q];
+    }
+    __syncthreads();
+    return ex;
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
     __shared__ uint64_t s_k[RS_TILE];
     __shared__ uint32_t s_v[RS_TILE];
     __shared__ uint32_t s_cnt[256 * RS_WAVES]; // index digit*RS_WAVES + wave
-    __shared__ uint32_t s_gbase[256];          // global offset of the digit minus its tile-local start
+    __shared__ uint32_t s_gbase[256];          // output offset of the digit minus its tile-local start
     __shared__ uint32_t s_wsum[RS_WAVES];
+    __shared__ uint32_t s_tile;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
-    const int64_t base = int64_t(blockIdx.x) * RS_TILE;
+    if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
     for (int i = tid; i < 256 * RS_WAVES; i += RS_BLOCK) s_cnt[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int64_t base = int64_t(tile) * RS_TILE;
     uint64_t k[RS_IT];
     uint32_t v[RS_IT], rk[RS_IT];
 #pragma unroll
     for (int it = 0; it < RS_IT; ++it) {
         const int64_t i = base + w * RS_PER_WAVE + it * WAVE + lane;
-        k[it] = i < n ? kin[i] : 0ull;
-        v[it] = (i < n && vin) ? vin[i] : 0u;
+        k[it] = i < a.n ? a.kin[i] : 0ull;
+        v[it] = (i < a.n && a.vin) ? a.vin[i] : 0u;
     }
-    __syncthreads();
 #pragma unroll
     for (int it = 0; it < RS_IT; ++it) {
         const int64_t i = base + w * RS_PER_WAVE + it * WAVE + lane;
-        const bool valid = i < n;
-        const uint32_t d = uint32_t(k[it] >> shift) & mask;
+        const bool valid = i < a.n;
+        const uint32_t d = uint32_t(k[it] >> a.shift) & a.mask;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -242,23 +273,18 @@ __global__ __launch_bounds__(RS_BLOCK) void k_downsweep(const uint64_t *__restri
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // exclusive scan over the 256*RS_WAVES counters; thread d holds digit d's
-    uint32_t c[RS_WAVES], sum = 0;
+    // thread d: digit d's counts per wave and in the tile
+    uint32_t c[RS_WAVES], cnt = 0;
 #pragma unroll
     for (int q = 0; q < RS_WAVES; ++q) {
         c[q] = s_cnt[tid * RS_WAVES + q];
-        sum += c[q];
+        cnt += c[q];
     }
-    uint32_t inc = sum;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        const uint32_t y = __shfl_up(inc, d, WAVE);
-        if (lane >= d) inc += y;
-    }
-    if (lane == WAVE - 1) s_wsum[w] = inc;
-    __syncthreads();
-    uint32_t ex = inc - sum;
-    for (int q = 0; q < w; ++q) ex += s_wsum[q];
+    unsigned long long *my = a.status + size_t(tile) * 256 + tid;
+    __hip_atomic_store(my, a.epoch | (tile == 0 ? ST_PRE : ST_AGG) | cnt, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t tot;
+    const uint32_t ex = block_excl_sum256(cnt, s_wsum, tot);
     {
         uint32_t run = ex;
 #pragma unroll
@@ -267,25 +293,73 @@ __global__ __launch_bounds__(RS_BLOCK) void k_downsweep(const uint64_t *__restri
             run += c[q];
         }
     }
-    s_gbase[tid] = tile_off[int64_t(tid) * ntiles + blockIdx.x] - ex;
-    __syncthreads();
+    // global base of the digit: exclusive scan of the pass histogram
+    const uint32_t gx = block_excl_sum256(a.ghist[tid], s_wsum, tot);
+    // look-back over earlier tiles
+    unsigned long long prefix = 0;
+    if (tile > 0) {
+        const unsigned long long EM = 0xffull << ST_EPOCH_SHIFT;
+        int64_t t = int64_t(tile) - 1; // next predecessor to account for
+        int spins = 0;
+        bool done = false;
+        while (!done) {
+            // RS_LOOK predecessors per round trip: t, t-1, ...
+            unsigned long long word[RS_LOOK];
+#pragma unroll
+            for (int j = 0; j < RS_LOOK; ++j)
+                word[j] = t - j >= 0 ? __hip_atomic_load(a.status + size_t(t - j) * 256 + tid, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)
+                                     : 0ull;
+            int used = 0;
+            bool stalled = false;
+#pragma unroll
+            for (int j = 0; j < RS_LOOK; ++j) {
+                if (done || stalled || t - j < 0) continue;
+                const unsigned long long w = word[j];
+                if ((w & EM) != a.epoch) {
+                    stalled = true; // not yet published: wait for it
+                    continue;
+                }
+                prefix += w & ST_VAL;
+                ++used;
+                if (w & ST_PRE) done = true;
+            }
+            t -= used;
+            if (done) break;
+            if (t < 0) { // a tile 0 word always carries a prefix: a malformed chain
+                atomicOr(a.err, 1u);
+                break;
+            }
+            if (used == 0) {
+                if (++spins > RS_SPIN_LIMIT) {
+                    atomicOr(a.err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            } else {
+                spins = 0;
+            }
+        }
+        __hip_atomic_store(my, a.epoch | ST_PRE | (prefix + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_gbase[tid] = gx + uint32_t(prefix) - ex;
 #pragma unroll
     for (int it = 0; it < RS_IT; ++it) {
         const int64_t i = base + w * RS_PER_WAVE + it * WAVE + lane;
-        if (i < n) {
-            const uint32_t d = uint32_t(k[it] >> shift) & mask;
+        if (i < a.n) {
+            const uint32_t d = uint32_t(k[it] >> a.shift) & a.mask;
             const uint32_t pos = s_cnt[d * RS_WAVES + w] + rk[it];
             s_k[pos] = k[it];
             s_v[pos] = v[it];
         }
     }
     __syncthreads();
-    const int m = int(n - base < RS_TILE ? n - base : RS_TILE);
+    const int m = int(a.n - base < RS_TILE ? a.n - base : RS_TILE);
     for (int i = tid; i < m; i += RS_BLOCK) {
         const uint64_t key = s_k[i];
-        const uint32_t o = s_gbase[uint32_t(key >> shift) & mask] + uint32_t(i);
-        kout[o] = key;
-        if (vout) vout[o] = s_v[i];
+        const uint32_t o = s_gbase[uint32_t(key >> a.shift) & a.mask] + uint32_t(i);
+        a.kout[o] = key;
+        if (a.vout) a.vout[o] = s_v[i];
     }
 }
 
@@ -416,6 +490,19 @@ void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStr
     SC_HIP(hipGetLastError());
 }
 
+void radix_check(RadixScratch &rs, hipStream_t s) {
+    if (!rs.global_hist.p || !rs.unchecked) return;
+    rs.unchecked = false;
+    const size_t GH = size_t(RS_MAXPASS) * 256;
+    uint32_t err = 0;
+    SC_HIP(hipMemcpyAsync(&err, rs.global_hist.p + GH + RS_MAXPASS, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipStreamSynchronize(s));
+    if (err) {
+        SC_HIP(hipMemsetAsync(rs.global_hist.p + GH + RS_MAXPASS, 0, sizeof(uint32_t), s));
+        throw Error(SC_EHIP, "radix sort: a tile waited too long for its predecessor");
+    }
+}
+
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs,
                     hipStream_t s) {
     if (n <= 1 || bit_hi <= bit_lo) return;
@@ -423,18 +510,33 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int b
     SC_REQUIRE(bit_lo >= 0 && bit_hi <= 64, SC_EINVAL, "radix bit range");
     int npass = int(ceil_div(bit_hi - bit_lo, 8));
     SC_REQUIRE(npass <= RS_MAXPASS, SC_EINVAL, "radix passes");
-    uint32_t *gh = rs.global_hist.ensure(RS_MAXPASS * 256);
-    SC_HIP(hipMemsetAsync(gh, 0, sizeof(uint32_t) * RS_MAXPASS * 256, s));
+    // [0, GH): per pass 256 digit counts; [GH, GH+RS_MAXPASS): per pass tile
+    // counter; GH+RS_MAXPASS: the error word (cleared only when reported)
+    const size_t GH = size_t(RS_MAXPASS) * 256;
+    const bool fresh = rs.global_hist.p == nullptr || rs.global_hist.cap < GH + RS_MAXPASS + 1;
+    uint32_t *gh = rs.global_hist.ensure(GH + RS_MAXPASS + 1);
+    SC_HIP(hipMemsetAsync(gh, 0, sizeof(uint32_t) * (GH + RS_MAXPASS + (fresh ? 1 : 0)), s));
     int ghb = int(ceil_div(n, RS_BLOCK * 16));
     if (ghb > 1024) ghb = 1024;
     k_global_hist<<<ghb, RS_BLOCK, 0, s>>>(keys, n, bit_lo, bit_hi, npass, gh);
     SC_HIP(hipGetLastError());
     std::vector<uint32_t> hh(size_t(npass) * 256);
+    uint32_t err = 0;
     SC_HIP(hipMemcpyAsync(hh.data(), gh, sizeof(uint32_t) * hh.size(), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipMemcpyAsync(&err, gh + GH + RS_MAXPASS, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     SC_HIP(hipStreamSynchronize(s));
+    rs.unchecked = false;
+    if (err) { // an earlier sort on this scratch gave up waiting
+        SC_HIP(hipMemsetAsync(gh + GH + RS_MAXPASS, 0, sizeof(uint32_t), s));
+        throw Error(SC_EHIP, "radix sort: a tile waited too long for its predecessor");
+    }
 
-    int ntiles = int(ceil_div(n, RS_TILE));
-    uint32_t *th = rs.tile_hist.ensure(size_t(ntiles) * 256);
+    const int64_t ntiles = ceil_div(n, RS_TILE);
+    SC_REQUIRE(ntiles < (int64_t(1) << 31), SC_ERANGE, "radix sort tiles");
+    const size_t nst = size_t(ntiles) * 256;
+    unsigned long long *const old_st = rs.tile_status.p;
+    unsigned long long *st = rs.tile_status.ensure(nst);
+    if (st != old_st) rs.status_clean = 0; // new memory: contents unknown
     uint64_t *kalt = rs.keys_alt.ensure(size_t(n));
     uint32_t *valt = vals ? rs.vals_alt.ensure(size_t(n)) : nullptr;
     uint64_t *kc = keys, *ko = kalt;
@@ -444,14 +546,32 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int b
         for (int d = 0; d < 256; ++d)
             if (hh[size_t(p) * 256 + d] == uint32_t(n)) trivial = true;
         if (trivial) continue;
-        int sh = bit_lo + 8 * p;
-        int bits = bit_hi - sh < 8 ? bit_hi - sh : 8;
-        uint32_t mask = (1u << bits) - 1u;
-        k_upsweep<<<ntiles, RS_BLOCK, 0, s>>>(kc, n, sh, mask, th, ntiles);
+        // a fresh epoch tag for this pass; zero the words when the tags wrap
+        // or when the words in use were never cleared
+        if (rs.epoch >= 255 || rs.status_clean < nst) {
+            SC_HIP(hipMemsetAsync(st, 0, sizeof(unsigned long long) * rs.tile_status.cap, s));
+            rs.status_clean = rs.tile_status.cap;
+            rs.epoch = 0;
+        }
+        ++rs.epoch;
+        const int sh = bit_lo + 8 * p;
+        const int bits = bit_hi - sh < 8 ? bit_hi - sh : 8;
+        SweepArgs a;
+        a.kin = kc;
+        a.vin = vc;
+        a.kout = ko;
+        a.vout = vo;
+        a.n = n;
+        a.shift = sh;
+        a.mask = (1u << bits) - 1u;
+        a.ghist = gh + size_t(p) * 256;
+        a.status = st;
+        a.epoch = (unsigned long long)rs.epoch << ST_EPOCH_SHIFT;
+        a.tile_counter = gh + GH + p;
+        a.err = gh + GH + RS_MAXPASS;
+        k_onesweep<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
         SC_HIP(hipGetLastError());
-        scan_exclusive_u32(th, th, int64_t(ntiles) * 256, nullptr, s);
-        k_downsweep<<<ntiles, RS_BLOCK, 0, s>>>(kc, vc, ko, vo, n, sh, mask, th, ntiles);
-        SC_HIP(hipGetLastError());
+        rs.unchecked = true;
         std::swap(kc, ko);
         std::swap(vc, vo);
     }

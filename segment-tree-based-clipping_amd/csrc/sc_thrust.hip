@@ -77,6 +77,7 @@ int sort_by_key(K *keys, int32_t *vals, int64_t n, void *stream, KeyK keyk, int 
         SC_HIP(hipMemcpyAsync(keys, tk, sizeof(K) * size_t(n), hipMemcpyDeviceToDevice, s));
         SC_HIP(hipMemcpyAsync(vals, tv, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToDevice, s));
         SC_HIP(hipStreamSynchronize(s));
+        sc::radix_check(sc_.rs, s);
         return SEGCLIP_OK;
     } catch (const sc::Error &e) {
         return e.code;
