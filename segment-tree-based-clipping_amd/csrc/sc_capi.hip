@@ -54,7 +54,7 @@ void release_derived(Ctx &c) {
     for (DBuf<uint32_t> *b : {&c.lend_off, &c.lend_e, &c.mend_off, &c.mend_e, &c.cov_off, &c.cov_e, &c.cov_split,
                               &c.d_pair_off, &c.u32a, &c.u32b, &c.u32c, &c.u32d})
         b->release();
-    for (DBuf<uint64_t> *b : {&c.lend_key, &c.cand, &c.pairs, &c.k64a, &c.k64b, &c.u64a, &c.u64b}) b->release();
+    for (DBuf<uint64_t> *b : {&c.lend_key, &c.unit_desc, &c.cand, &c.pairs, &c.k64a, &c.k64b, &c.u64a, &c.u64b}) b->release();
     c.rs.global_hist.release();
     c.rs.tile_status.release();
     c.rs.keys_alt.release();

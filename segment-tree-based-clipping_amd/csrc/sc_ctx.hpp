@@ -75,6 +75,7 @@ struct Ctx {
     DBuf<BoxF> cov_box;                  // the edge's single-precision box, beside each cover entry
     int64_t n_cov = 0;
     // ---- A8/A10 candidates and unique pairs ----
+    DBuf<uint64_t> unit_desc;            // stab work units: 32-byte records
     DBuf<uint64_t> cand;                 // packed (k, b, c) keys
     int64_t n_cand = 0, cand_cap = 0;
     int64_t n_tests = 0;                 // W

@@ -110,11 +110,68 @@ __global__ void k_unit_heads(int T, const uint32_t *__restrict__ uoff, uint32_t 
     if (uoff[v + 1] > uoff[v]) head[uoff[v]] = uint32_t(v);
 }
 
+// A work unit as the stab reads it: one uniform 32-byte record.
+struct __align__(16) UnitDesc {
+    uint32_t row0;   // first row entry (cover list position)
+    uint32_t col0;   // first column entry of the rectangle
+    uint32_t cols;   // columns of the rectangle
+    uint32_t t0, t1; // 64-column tiles [t0, t1)
+    uint32_t nk;     // rows | kind << 8
+    uint32_t blo, bhi; // "before" leaves [blo, bhi) of the node's subtree (end-entry dedupe)
+};
+
+// descriptor of every unit from its node's shape (one thread per unit)
+__global__ void k_unit_desc(uint32_t nunits, const uint32_t *__restrict__ unode, const uint32_t *__restrict__ uoff,
+                            const uint32_t *__restrict__ cov_off, const uint32_t *__restrict__ cov_split,
+                            const uint32_t *__restrict__ lend_off, int nl, int D, UnitDesc *desc) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nunits) return;
+    const int v = int(unode[w]);
+    const NodeShape s = node_shape(v, cov_off, cov_split, lend_off, nl, D);
+    uint32_t u = w - uoff[v];
+    const uint32_t n1 = rect_units(s.nbase, s.nc), n2 = rect_units(s.nbase, s.ecn);
+    uint32_t kind, rows, cols, row_begin, col_begin;
+    if (u < n1) {
+        kind = 1; // base cover x overlay cover
+        rows = s.nbase;
+        cols = s.nc;
+        row_begin = s.off;
+        col_begin = s.off + s.nbase;
+    } else if (u < n1 + n2) {
+        kind = 2; // base cover x overlay end entries
+        u -= n1;
+        rows = s.nbase;
+        cols = s.ecn;
+        row_begin = s.off;
+        col_begin = s.ec0;
+    } else {
+        kind = 3; // overlay cover x base end entries
+        u -= n1 + n2;
+        rows = s.nc;
+        cols = s.ebn;
+        row_begin = s.off + s.nbase;
+        col_begin = s.eb0;
+    }
+    uint32_t r0, nrows, t0, t1;
+    rect_unit(rows, cols, u, r0, nrows, t0, t1);
+    UnitDesc d;
+    d.row0 = row_begin + r0;
+    d.col0 = col_begin;
+    d.cols = cols;
+    d.t0 = t0;
+    d.t1 = t1;
+    d.nk = nrows | (kind << 8);
+    d.blo = uint32_t(nl + s.lo);
+    d.bhi = uint32_t(nl + s.hi);
+    desc[w] = d;
+}
+
 constexpr int STG = 256; // hits staged per wave in LDS before a flush
 constexpr uint32_t XCHUNK = 16; // consecutive blocks per XCD turn
 
 struct StabArgs {
-    const uint32_t *cov_off, *cov_split, *cov_e, *lend_off, *lend_e, *uoff, *unode;
+    const UnitDesc *desc;
+    const uint32_t *cov_e, *lend_e;
     const BoxF *cov_box, *lend_box;           // single-precision boxes beside the entries
     const int *lend_bl, *etype, *eoff;
     const double *x0, *y0, *x1, *y1;          // endpoints (input coordinates)
@@ -154,6 +211,9 @@ __device__ inline void flush(const StabArgs &a, const uint64_t *stage, uint32_t 
 __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     __shared__ uint64_t stage_all[BLK / WAVE][STG];
     __shared__ BoxF rbox_all[BLK / WAVE][RCH];
+    __shared__ uint32_t queue_all[BLK / WAVE][WAVE]; // near pairs: row << 6 | column lane
+    __shared__ double4 rxy_all[BLK / WAVE][WAVE];    // double endpoints of the rows
+    __shared__ double4 cxy_all[BLK / WAVE][WAVE];    // double endpoints of the tile's columns
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t wv = uni(threadIdx.x / WAVE);
     // blocks are dealt round-robin to the 8 XCDs: hand each XCD chunks of
@@ -167,155 +227,140 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     if (lb >= nbk || wg >= a.nunits) return; // whole wave; no block barrier below
     uint64_t *stage = stage_all[wv];
     BoxF *rbox = rbox_all[wv];
-    const int v = int(uni(a.unode[wg]));
-    NodeShape s = node_shape(v, a.cov_off, a.cov_split, a.lend_off, a.nl, a.D);
-    uint32_t u = wg - uni(a.uoff[v]);
-    const uint32_t n1 = rect_units(s.nbase, s.nc), n2 = rect_units(s.nbase, s.ecn);
-    int kind;
-    uint32_t rows, cols, row_begin, col_begin;
-    const uint32_t *colsrc;
-    const BoxF *colbox;
-    if (u < n1) {
-        kind = 1; // base cover x overlay cover
-        rows = s.nbase;
-        cols = s.nc;
-        row_begin = s.off;
-        col_begin = s.off + s.nbase;
-        colsrc = a.cov_e;
-        colbox = a.cov_box;
-    } else if (u < n1 + n2) {
-        kind = 2; // base cover x overlay end entries
-        u -= n1;
-        rows = s.nbase;
-        cols = s.ecn;
-        row_begin = s.off;
-        col_begin = s.ec0;
-        colsrc = a.lend_e;
-        colbox = a.lend_box;
-    } else {
-        kind = 3; // overlay cover x base end entries
-        u -= n1 + n2;
-        rows = s.nc;
-        cols = s.ebn;
-        row_begin = s.off + s.nbase;
-        col_begin = s.eb0;
-        colsrc = a.lend_e;
-        colbox = a.lend_box;
-    }
-    uint32_t r0, nrows, t0, t1;
-    rect_unit(rows, cols, u, r0, nrows, t0, t1);
+    uint32_t *queue = queue_all[wv];
+    double4 *rxy = rxy_all[wv], *cxy = cxy_all[wv];
+    // the unit's record: a uniform load
+    const uint4 *dp = reinterpret_cast<const uint4 *>(a.desc + wg);
+    const uint4 d0 = dp[0], d1 = dp[1];
+    const uint32_t row0 = uni(d0.x), col_begin = uni(d0.y), cols = uni(d0.z), t0 = uni(d0.w);
+    const uint32_t t1 = uni(d1.x), nk = uni(d1.y), blo = uni(d1.z), bhi = uni(d1.w);
+    const uint32_t nrows = nk & 0xffu;
+    const int kind = int(nk >> 8);
+    const uint32_t *colsrc = kind == 1 ? a.cov_e : a.lend_e;
+    const BoxF *colbox = kind == 1 ? a.cov_box : a.lend_box;
+    // a tile's columns: entry, box and (end entries) the edge's "before" leaf
+    auto load_tile = [&](uint32_t t, uint32_t &raw, BoxF &bx, uint32_t &bl) {
+        const uint32_t cb = t * WAVE;
+        raw = 0xffffffffu;
+        bl = 0;
+        if (uint32_t(lane) < min(uint32_t(WAVE), cols - cb)) {
+            const uint32_t ci = col_begin + cb + lane;
+            raw = colsrc[ci];
+            bx = colbox[ci];
+            if (kind != 1) bl = uint32_t(a.lend_bl[ci]);
+        }
+    };
+    uint32_t n_raw, n_bl;
+    BoxF n_box;
+    load_tile(t0, n_raw, n_box, n_bl); // issued before the rows are waited for
     // this lane's row edge and box (the empty box past the last row)
     uint32_t re_l = 0;
     BoxF rb = boxf_empty();
     if (uint32_t(lane) < nrows) {
-        re_l = a.cov_e[row_begin + r0 + lane];
-        rb = a.cov_box[row_begin + r0 + lane];
+        re_l = a.cov_e[row0 + lane];
+        rb = a.cov_box[row0 + lane];
     }
     rbox[lane] = rb;
     __builtin_amdgcn_wave_barrier();
     bool rows_fetched = false;
     int rk = 0;
-    double rx0 = 0, ry0 = 0, rx1 = 0, ry1 = 0;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
     const int sb = a.bits_b + a.bits_c;
     uint32_t cnt = 0;
     const uint32_t nr4 = (nrows + 3u) & ~3u; // rows past nrows hold the empty box
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t cb = t * WAVE;
-        const uint32_t ncol = min(uint32_t(WAVE), cols - cb);
-        bool valid = uint32_t(lane) < ncol;
-        uint32_t ce = 0;
-        BoxF cbf = boxf_empty();
-        if (valid) {
-            const uint32_t ci = col_begin + cb + lane;
-            const uint32_t raw = colsrc[ci];
-            ce = raw & 0x7fffffffu;
-            cbf = colbox[ci];
-            if (kind != 1 && (raw & 0x80000000u)) {
-                // "after" copy of an edge whose "before" leaf is under v as well
-                const int bl = a.lend_bl[ci];
-                if (bl >= a.nl + s.lo && bl < a.nl + s.hi) valid = false;
-            }
-        }
+        if (t != t0) load_tile(t, n_raw, n_box, n_bl);
+        const uint32_t raw = n_raw, bl = n_bl;
+        BoxF cbf = n_box;
+        bool valid = raw != 0xffffffffu;
+        const uint32_t ce = raw & 0x7fffffffu;
+        // "after" copy of an edge whose "before" leaf is under v as well
+        if (valid && kind != 1 && (raw & 0x80000000u) && bl >= blo && bl < bhi) valid = false;
         if (!valid) cbf = boxf_empty();
         bool cols_fetched = false;
         int ck = 0;
-        double cx0 = 0, cy0 = 0, cx1 = 0, cy1 = 0;
+        uint32_t qn = 0; // queued near pairs of this tile
+        // LSMF on the doubles and the classification of the queued pairs, one per lane
+        auto run_queue = [&]() {
+            __builtin_amdgcn_wave_barrier();
+            const bool act = uint32_t(lane) < qn;
+            const uint32_t e = act ? queue[lane] : 0u;
+            const int rr = int(e >> 6), cc = int(e & 63u);
+            const double4 P = rxy[rr], Q = cxy[cc];
+            const double px0 = P.x, py0 = P.y, px1 = P.z, py1 = P.w;
+            const double qx0 = Q.x, qy0 = Q.y, qx1 = Q.z, qy1 = Q.w;
+            const uint32_t re = uint32_t(__shfl(int(re_l), rr, WAVE)), qce = uint32_t(__shfl(int(ce), cc, WAVE));
+            const int rkk = __shfl(rk, rr, WAVE), ckk = __shfl(ck, cc, WAVE);
+            bool hit = false;
+            uint64_t key = 0;
+            if (act && box_overlap(box_of(px0, py0, px1, py1), box_of(qx0, qy0, qx1, qy1))) {
+                int tp;
+                uint32_t be, qe;
+                int kk;
+                if (kind == 3) { // row = overlay edge, column = base edge
+                    tp = classify_pre(qx0, qy0, qx1, qy1, px0, py0, px1, py1);
+                    be = qce;
+                    qe = re;
+                    kk = rkk;
+                } else {
+                    tp = classify_pre(px0, py0, px1, py1, qx0, qy0, qx1, qy1);
+                    be = re;
+                    qe = qce;
+                    kk = ckk;
+                }
+                if (tp != 0) {
+                    hit = true;
+                    const uint32_t local = qe - uint32_t(a.eoff[kk - 1]);
+                    key = (uint64_t(kk) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
+                }
+            }
+            const uint64_t hm = __ballot(hit);
+            if (hm) {
+                const uint32_t nh = uint32_t(__popcll(hm));
+                if (cnt + nh > uint32_t(STG)) {
+                    flush(a, stage, cnt, lane);
+                    cnt = 0;
+                }
+                if (hit) stage[cnt + uint32_t(__popcll(hm & lt))] = key;
+                cnt += nh;
+            }
+            qn = 0;
+            __builtin_amdgcn_wave_barrier();
+        };
         for (uint32_t r = 0; r < nr4; r += 4) {
             // single-precision screen (exact rejection) of four rows
             uint64_t m[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const BoxF q = rbox[r + k];
-                m[k] = __ballot(boxf_overlap(q, cbf));
+                m[k] = __builtin_amdgcn_ballot_w64(boxf_overlap(q, cbf));
             }
             if ((m[0] | m[1] | m[2] | m[3]) == 0) continue;
             if (!cols_fetched) {
                 if (valid) {
-                    cx0 = a.x0[ce];
-                    cy0 = a.y0[ce];
-                    cx1 = a.x1[ce];
-                    cy1 = a.y1[ce];
+                    cxy[lane] = make_double4(a.x0[ce], a.y0[ce], a.x1[ce], a.y1[ce]);
                     ck = a.etype[ce];
                 }
                 cols_fetched = true;
             }
             if (!rows_fetched) {
                 if (uint32_t(lane) < nrows) {
-                    rx0 = a.x0[re_l];
-                    ry0 = a.y0[re_l];
-                    rx1 = a.x1[re_l];
-                    ry1 = a.y1[re_l];
+                    rxy[lane] = make_double4(a.x0[re_l], a.y0[re_l], a.x1[re_l], a.y1[re_l]);
                     rk = a.etype[re_l];
                 }
                 rows_fetched = true;
             }
-            const Box cbx = box_of(cx0, cy0, cx1, cy1);
 #pragma unroll 1
             for (int k = 0; k < 4; ++k) {
-                if (m[k] == 0) continue;
-                const int rr = int(r) + k;
-                const bool near = (m[k] >> lane) & 1ull;
-                bool hit = false;
-                uint64_t key = 0;
-                if (near) {
-                    // LSMF on the doubles, then the classification
-                    const double px0 = bcast(rx0, rr), py0 = bcast(ry0, rr);
-                    const double px1 = bcast(rx1, rr), py1 = bcast(ry1, rr);
-                    const bool box = box_overlap(box_of(px0, py0, px1, py1), cbx);
-                    const uint32_t re = uint32_t(__builtin_amdgcn_readlane(int(re_l), rr));
-                    int tp = 0;
-                    uint32_t be, qe;
-                    int kk;
-                    if (kind == 3) { // row = overlay edge, column = base edge
-                        if (box) tp = classify_pre(cx0, cy0, cx1, cy1, px0, py0, px1, py1);
-                        be = ce;
-                        qe = re;
-                        kk = __builtin_amdgcn_readlane(rk, rr);
-                    } else {
-                        if (box) tp = classify_pre(px0, py0, px1, py1, cx0, cy0, cx1, cy1);
-                        be = re;
-                        qe = ce;
-                        kk = ck;
-                    }
-                    if (tp != 0) {
-                        hit = true;
-                        uint32_t local = qe - uint32_t(a.eoff[kk - 1]);
-                        key = (uint64_t(kk) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
-                    }
-                }
-                const uint64_t hm = __ballot(hit);
-                if (hm) {
-                    const uint32_t nh = uint32_t(__popcll(hm));
-                    if (cnt + nh > uint32_t(STG)) {
-                        flush(a, stage, cnt, lane);
-                        cnt = 0;
-                    }
-                    if (hit) stage[cnt + uint32_t(__popcll(hm & lt))] = key;
-                    cnt += nh;
-                }
+                const uint64_t mk = m[k];
+                if (mk == 0) continue;
+                const uint32_t nh = uint32_t(__popcll(mk));
+                if (qn + nh > uint32_t(WAVE)) run_queue();
+                if ((mk >> lane) & 1ull) queue[qn + uint32_t(__popcll(mk & lt))] = ((r + uint32_t(k)) << 6) | uint32_t(lane);
+                qn += nh;
             }
         }
+        if (qn) run_queue();
     }
     if (cnt) flush(a, stage, cnt, lane);
 }
@@ -383,6 +428,12 @@ void find_candidates(Ctx &c) {
         SC_HIP(hipGetLastError());
         scan_inclusive_max_u32(unode, unode, nunits, s);
     }
+    UnitDesc *desc = reinterpret_cast<UnitDesc *>(c.unit_desc.ensure(size_t(nunits > 0 ? nunits : 1) * 4));
+    if (nunits > 0) {
+        k_unit_desc<<<grid_for(nunits), BLK, 0, s>>>(nunits, unode, units, c.cov_off, c.cov_split, c.lend_off,
+                                                     c.nleaves, c.depth, desc);
+        SC_HIP(hipGetLastError());
+    }
 
     if (c.cand_cap == 0) c.cand_cap = std::max<int64_t>(1 << 20, 4 * int64_t(c.ne));
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(c.d_u64_total.p);
@@ -390,8 +441,7 @@ void find_candidates(Ctx &c) {
         c.cand.ensure(size_t(c.cand_cap));
         SC_HIP(hipMemsetAsync(counter, 0, sizeof(uint64_t), s));
         StabArgs a;
-        a.cov_off = c.cov_off; a.cov_split = c.cov_split; a.cov_e = c.cov_e;
-        a.lend_off = c.lend_off; a.lend_e = c.lend_e; a.uoff = units; a.unode = unode;
+        a.desc = desc; a.cov_e = c.cov_e; a.lend_e = c.lend_e;
         a.lend_bl = c.lend_bl; a.etype = c.etype; a.eoff = c.d_eoff;
         a.cov_box = c.cov_box; a.lend_box = c.lend_box;
         a.x0 = c.ex0; a.y0 = c.ey0; a.x1 = c.ex1; a.y1 = c.ey1;
