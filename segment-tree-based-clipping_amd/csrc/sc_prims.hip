@@ -242,9 +242,17 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
     if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
     for (int i = tid; i < 256 * RS_WAVES; i += RS_BLOCK) s_cnt[i] = 0;
-    __syncthreads();
+    // a pass in which every key has the same digit keeps the order: copy the tile
+    const bool trivial = __syncthreads_or(uint64_t(a.ghist[tid]) == uint64_t(a.n));
     const uint32_t tile = s_tile;
     const int64_t base = int64_t(tile) * RS_TILE;
+    if (trivial) {
+        for (int64_t i = base + tid; i < a.n && i < base + RS_TILE; i += RS_BLOCK) {
+            a.kout[i] = a.kin[i];
+            if (a.vout) a.vout[i] = a.vin[i];
+        }
+        return;
+    }
     uint64_t k[RS_IT];
     uint32_t v[RS_IT], rk[RS_IT];
 #pragma unroll
@@ -520,16 +528,6 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int b
     if (ghb > 1024) ghb = 1024;
     k_global_hist<<<ghb, RS_BLOCK, 0, s>>>(keys, n, bit_lo, bit_hi, npass, gh);
     SC_HIP(hipGetLastError());
-    std::vector<uint32_t> hh(size_t(npass) * 256);
-    uint32_t err = 0;
-    SC_HIP(hipMemcpyAsync(hh.data(), gh, sizeof(uint32_t) * hh.size(), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipMemcpyAsync(&err, gh + GH + RS_MAXPASS, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipStreamSynchronize(s));
-    rs.unchecked = false;
-    if (err) { // an earlier sort on this scratch gave up waiting
-        SC_HIP(hipMemsetAsync(gh + GH + RS_MAXPASS, 0, sizeof(uint32_t), s));
-        throw Error(SC_EHIP, "radix sort: a tile waited too long for its predecessor");
-    }
 
     const int64_t ntiles = ceil_div(n, RS_TILE);
     SC_REQUIRE(ntiles < (int64_t(1) << 31), SC_ERANGE, "radix sort tiles");
@@ -542,10 +540,6 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int b
     uint64_t *kc = keys, *ko = kalt;
     uint32_t *vc = vals, *vo = valt;
     for (int p = 0; p < npass; ++p) {
-        bool trivial = false;
-        for (int d = 0; d < 256; ++d)
-            if (hh[size_t(p) * 256 + d] == uint32_t(n)) trivial = true;
-        if (trivial) continue;
         // a fresh epoch tag for this pass; zero the words when the tags wrap
         // or when the words in use were never cleared
         if (rs.epoch >= 255 || rs.status_clean < nst) {
