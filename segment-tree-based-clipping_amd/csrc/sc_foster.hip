@@ -772,32 +772,6 @@ struct SeqArgs {
     int *result; // [0] rings, [1] vertices, [2] status bits
 };
 
-__device__ int seq_new(SeqArgs &a, int &n, double x, double y, int src) {
-    if (n >= a.cap) return -1;
-    int v = n++;
-    a.vx[v] = x;
-    a.vy[v] = y;
-    a.vx1[v] = x;
-    a.vy1[v] = y;
-    a.vlen[v] = 1;
-    a.vslot[v] = -1;
-    a.vsrc[v] = src;
-    a.nx[v] = a.pv[v] = v;
-    a.nb[v] = -1;
-    a.lab[v] = L_NONE;
-    a.enex[v] = NEITHER;
-    a.inter[v] = 0;
-    return v;
-}
-
-__device__ void seq_insert_after(SeqArgs &a, int V, int curr) {
-    int c = a.nx[curr];
-    a.nx[a.pv[c]] = V;
-    a.pv[V] = a.pv[c];
-    a.nx[V] = c;
-    a.pv[c] = V;
-}
-
 // vertex t of node V in ring order
 __device__ inline void seq_vertex(const SeqArgs &a, int V, int t, double &x, double &y) {
     if (a.vslot[V] >= 0 && a.vlen[V] > 1) {
@@ -842,28 +816,6 @@ __device__ inline int seq_src(const SeqArgs &a, int V) {
     return V < a.nP ? a.P0 + a.vslot[V] : ~(a.Q0 + a.vslot[V]);
 }
 
-// output the vertices of node V, walking forward (next) or backward (prev)
-__device__ bool seq_emit(SeqArgs &a, int &outn, int V, bool forward) {
-    const int len = a.vlen[V];
-    if (outn + len > a.out_cap) return false;
-    for (int t = 0; t < len; ++t) {
-        const int u = forward ? t : len - 1 - t;
-        double x, y;
-        seq_vertex(a, V, u, x, y);
-        a.ox[outn] = x;
-        a.oy[outn] = y;
-        if (a.vslot[V] >= 0) {
-            const bool onP = V < a.nP;
-            const int sl = a.vslot[V] + u;
-            a.osrc[outn] = onP ? a.P0 + sl : ~(a.Q0 + sl);
-        } else {
-            a.osrc[outn] = a.vsrc[V];
-        }
-        ++outn;
-    }
-    return true;
-}
-
 __device__ int seq_toggle(int s) { return s == ENTRY ? EXIT : (s == EXIT ? ENTRY : s); }
 
 // node records -> the job's linked rings (all threads of the block)
@@ -889,33 +841,116 @@ __device__ void seq_init(SeqArgs &a) {
     }
 }
 
-// the reference's sequential algorithm on the job's rings (one thread)
+// The reference's sequential algorithm on the job's rings, run by one wave
+// with uniform control flow: every lane holds the same walk state.  Lane i
+// caches the hot fields of node w0 + i (a window of 64 consecutive nodes,
+// w0 a multiple of 64), so a walk in ring order reads registers and reloads
+// one coalesced window every 64 nodes.  Every write goes to memory (from all
+// lanes, the same value) and to the window.  Output runs are written one
+// vertex per lane.
 __device__ void seq_run(SeqArgs &a) {
+    const int lane = threadIdx.x & (WAVE - 1);
     int status = 0;
     // ring nodes already laid out by seq_init: P = [0,nP), Q = [nP,nP+nQ)
     int n = a.nP + a.nQ;
+    int w0 = -(1 << 30);
+    int c_nx = 0, c_pv = 0, c_nb = 0, c_lab = 0, c_enex = 0, c_inter = 0;
+    auto fill = [&](int I) {
+        w0 = I & ~(WAVE - 1);
+        const int j = w0 + lane;
+        if (j < n) {
+            c_nx = a.nx[j];
+            c_pv = a.pv[j];
+            c_nb = a.nb[j];
+            c_lab = a.lab[j];
+            c_enex = a.enex[j];
+            c_inter = a.inter[j];
+        }
+    };
+    auto slot = [&](int I) {
+        if (unsigned(I - w0) >= unsigned(WAVE)) fill(I);
+        return I - w0;
+    };
+    auto NX_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_nx, k); };
+    auto PV_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_pv, k); };
+    auto NB_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_nb, k); };
+    auto LAB_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_lab, k); };
+    auto ENEX_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_enex, k); };
+    auto INTER_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_inter, k); };
+    auto set_nx = [&](int I, int v) { a.nx[I] = v; if (I - w0 == lane) c_nx = v; };
+    auto set_pv = [&](int I, int v) { a.pv[I] = v; if (I - w0 == lane) c_pv = v; };
+    auto set_nb = [&](int I, int v) { a.nb[I] = v; if (I - w0 == lane) c_nb = v; };
+    auto set_lab = [&](int I, int v) { a.lab[I] = v; if (I - w0 == lane) c_lab = v; };
+    auto set_enex = [&](int I, int v) { a.enex[I] = v; if (I - w0 == lane) c_enex = v; };
+    auto set_inter = [&](int I, int v) { a.inter[I] = uint8_t(v); if (I - w0 == lane) c_inter = v; };
+    // a new single-vertex node (its own ring until inserted), -1 when full
+    auto snew = [&](double x, double y, int src) -> int {
+        if (n >= a.cap) return -1;
+        const int v = n++;
+        a.vx[v] = x;
+        a.vy[v] = y;
+        a.vx1[v] = x;
+        a.vy1[v] = y;
+        a.vlen[v] = 1;
+        a.vslot[v] = -1;
+        a.vsrc[v] = src;
+        set_nx(v, v);
+        set_pv(v, v);
+        set_nb(v, -1);
+        set_lab(v, L_NONE);
+        set_enex(v, NEITHER);
+        set_inter(v, 0);
+        return v;
+    };
+    auto sinsert = [&](int V, int curr) {
+        const int c = NX_(curr);
+        set_nx(PV_(c), V);
+        set_pv(V, PV_(c));
+        set_nx(V, c);
+        set_pv(c, V);
+    };
+    // the vertices of node V, walking forward or backward, one per lane
+    auto semit = [&](int &outn, int V, bool forward) -> bool {
+        const int len = a.vlen[V];
+        if (outn + len > a.out_cap) return false;
+        const int vs = a.vslot[V];
+        const bool onP = V < a.nP;
+        for (int t0 = 0; t0 < len; t0 += WAVE) {
+            const int t = t0 + lane;
+            if (t < len) {
+                const int u = forward ? t : len - 1 - t;
+                double x, y;
+                seq_vertex(a, V, u, x, y);
+                a.ox[outn + t] = x;
+                a.oy[outn + t] = y;
+                a.osrc[outn + t] = vs >= 0 ? (onP ? a.P0 + vs + u : ~(a.Q0 + vs + u)) : a.vsrc[V];
+            }
+        }
+        outn += len;
+        return true;
+    };
     const int rootP = 0, rootQ = a.nP;
     // 2) intersection chains
     {
         int I = rootP;
         do {
-            if (a.inter[I] && (a.lab[I] == L_LEFT_ON || a.lab[I] == L_RIGHT_ON)) {
-                int x = a.lab[I] == L_LEFT_ON ? 0 : 1;
+            if (INTER_(I) && (LAB_(I) == L_LEFT_ON || LAB_(I) == L_RIGHT_ON)) {
+                int x = LAB_(I) == L_LEFT_ON ? 0 : 1;
                 int X = I, J = I, guard = 0;
                 do {
-                    a.lab[J] = L_NONE;
-                    if (a.nb[J] >= 0) a.lab[a.nb[J]] = L_NONE;
-                    J = a.nx[J];
+                    set_lab(J, L_NONE);
+                    if (NB_(J) >= 0) set_lab(NB_(J), L_NONE);
+                    J = NX_(J);
                     if (++guard > n) { status |= UB_TRAVERSAL; goto done; }
-                } while (a.lab[J] == L_ON_ON);
-                int y = a.lab[J] == L_ON_LEFT ? 0 : 1;
+                } while (LAB_(J) == L_ON_ON);
+                int y = LAB_(J) == L_ON_LEFT ? 0 : 1;
                 int ct = x != y ? L_DCROSS : L_DBOUNCE;
-                a.lab[X] = ct;
-                a.lab[J] = ct;
-                if (a.nb[X] >= 0) a.lab[a.nb[X]] = ct;
-                if (a.nb[J] >= 0) a.lab[a.nb[J]] = ct;
+                set_lab(X, ct);
+                set_lab(J, ct);
+                if (NB_(X) >= 0) set_lab(NB_(X), ct);
+                if (NB_(J) >= 0) set_lab(NB_(J), ct);
             }
-            I = a.nx[I];
+            I = NX_(I);
         } while (I != rootP);
     }
     {
@@ -928,9 +963,9 @@ __device__ void seq_run(SeqArgs &a) {
             bool nocross = true, allon = true;
             int V = A;
             do {
-                if (a.inter[V] && (a.lab[V] == L_CROSSING || a.lab[V] == L_DCROSS)) nocross = false;
-                if (a.lab[V] != L_ON_ON) allon = false;
-                V = a.nx[V];
+                if (INTER_(V) && (LAB_(V) == L_CROSSING || LAB_(V) == L_DCROSS)) nocross = false;
+                if (LAB_(V) != L_ON_ON) allon = false;
+                V = NX_(V);
             } while (V != A);
             if (!nocross) continue;
             noInter[s] = true;
@@ -942,25 +977,25 @@ __device__ void seq_run(SeqArgs &a) {
             bool found = false, plain = false;
             V = A;
             do {
-                if (!a.inter[V]) {
+                if (!INTER_(V)) {
                     qx = a.vx[V];
                     qy = a.vy[V];
                     found = plain = true;
                     break;
                 }
-                V = a.nx[V];
+                V = NX_(V);
             } while (V != A);
             if (!found) {
                 V = A;
                 do {
-                    int nxv = a.nx[V], nbv = a.nb[V];
-                    if (nbv >= 0 && a.nb[nxv] != a.pv[nbv] && a.nb[nxv] != a.nx[nbv]) {
+                    int nxv = NX_(V), nbv = NB_(V);
+                    if (nbv >= 0 && NB_(nxv) != PV_(nbv) && NB_(nxv) != NX_(nbv)) {
                         qx = 0.5 * (a.vx[V] + a.vx[nxv]);
                         qy = 0.5 * (a.vy[V] + a.vy[nxv]);
                         found = true;
                         break;
                     }
-                    V = a.nx[V];
+                    V = NX_(V);
                 } while (V != A);
             }
             if (!found) { status |= UB_TRAVERSAL; goto done; }
@@ -971,25 +1006,25 @@ __device__ void seq_run(SeqArgs &a) {
                 a.ring_off[rings++] = outn;
                 int W = A;
                 do {
-                    if (!seq_emit(a, outn, W, true)) { status |= UB_TRAVERSAL; goto done; }
-                    W = a.nx[W];
+                    if (!semit(outn, W, true)) { status |= UB_TRAVERSAL; goto done; }
+                    W = NX_(W);
                 } while (W != A);
             }
         }
         if (ident[0] && ident[1]) {
-            int target = a.nb[rootP], V = rootQ;
+            int target = NB_(rootP), V = rootQ;
             bool found = false;
             do {
                 if (V == target) found = true;
-                V = a.nx[V];
+                V = NX_(V);
             } while (V != rootQ);
             if (found) {
                 if (rings >= a.ring_cap) { status |= UB_TRAVERSAL; goto done; }
                 a.ring_off[rings++] = outn;
                 int W = rootP;
                 do {
-                    if (!seq_emit(a, outn, W, true)) { status |= UB_TRAVERSAL; goto done; }
-                    W = a.nx[W];
+                    if (!semit(outn, W, true)) { status |= UB_TRAVERSAL; goto done; }
+                    W = NX_(W);
                 } while (W != rootP);
             }
         }
@@ -1004,22 +1039,22 @@ __device__ void seq_run(SeqArgs &a) {
             int A = roots[s], B = roots[1 - s];
             int V = -1, W = A;
             do {
-                if (!a.inter[W]) { V = W; break; }
-                W = a.nx[W];
+                if (!INTER_(W)) { V = W; break; }
+                W = NX_(W);
             } while (W != A);
             if (V < 0) {
                 W = A;
                 do {
-                    int nxv = a.nx[W], nbv = a.nb[W];
-                    if (nbv >= 0 && a.nb[nxv] != a.pv[nbv] && a.nb[nxv] != a.nx[nbv]) {
-                        int T = seq_new(a, n, 0.5 * (a.vx[W] + a.vx[nxv]), 0.5 * (a.vy[W] + a.vy[nxv]), SRC_NONE);
+                    int nxv = NX_(W), nbv = NB_(W);
+                    if (nbv >= 0 && NB_(nxv) != PV_(nbv) && NB_(nxv) != NX_(nbv)) {
+                        int T = snew(0.5 * (a.vx[W] + a.vx[nxv]), 0.5 * (a.vy[W] + a.vy[nxv]), SRC_NONE);
                         if (T < 0) { status |= UB_TRAVERSAL; goto done; }
-                        seq_insert_after(a, T, W);
+                        sinsert(T, W);
                         inserted[s] = true;
                         V = T;
                         break;
                     }
-                    W = a.nx[W];
+                    W = NX_(W);
                 } while (W != A);
             }
             if (V < 0) { status |= UB_TRAVERSAL; goto done; }
@@ -1030,28 +1065,28 @@ __device__ void seq_run(SeqArgs &a) {
             const bool in = known ? ((s == 0 ? a.windP : a.windQ) % 2) != 0 : seq_point_in(a, B, a.vx[V], a.vy[V]);
             if (in) st = seq_toggle(st);
             bool first = true;
-            int I = a.nx[V];
+            int I = NX_(V);
             while (I != V) {
-                if (a.inter[I]) {
-                    int lab = a.lab[I];
+                if (INTER_(I)) {
+                    int lab = LAB_(I);
                     if (lab == L_CROSSING) {
-                        a.enex[I] = st;
+                        set_enex(I, st);
                         st = seq_toggle(st);
                     }
                     if (lab == L_BOUNCING && st == EXIT && nsplit[s] < half / 2) splitL[s][nsplit[s]++] = I;
                     if (lab == L_DCROSS) {
-                        a.enex[I] = st;
+                        set_enex(I, st);
                         if (first) {
-                            if (st == EXIT) a.lab[I] = L_CROSSING;
+                            if (st == EXIT) set_lab(I, L_CROSSING);
                             first = false;
                         } else {
-                            if (st == ENTRY) a.lab[I] = L_CROSSING;
+                            if (st == ENTRY) set_lab(I, L_CROSSING);
                             first = true;
                             st = seq_toggle(st);
                         }
                     }
                     if (lab == L_DBOUNCE) {
-                        a.enex[I] = st;
+                        set_enex(I, st);
                         if (first) {
                             if (st == EXIT && ncross[s] < half / 2) crossL[s][ncross[s]++] = I;
                             first = false;
@@ -1062,7 +1097,7 @@ __device__ void seq_run(SeqArgs &a) {
                         st = seq_toggle(st);
                     }
                 }
-                I = a.nx[I];
+                I = NX_(I);
             }
         }
         // membership of the Q lists, marked in the traversal's (still zero)
@@ -1072,31 +1107,31 @@ __device__ void seq_run(SeqArgs &a) {
         for (int j = 0; j < ncross[1]; ++j) a.seen[2 * crossL[1][j] + 1] = -1;
         // 5) split interior bouncing pairs
         for (int i = 0; i < nsplit[0]; ++i) {
-            int IP = splitL[0][i], IQ = a.nb[IP];
+            int IP = splitL[0][i], IQ = NB_(IP);
             if (!(IQ >= 0 && IQ < nmark && a.seen[2 * IQ] == -1)) continue;
-            int VP = seq_new(a, n, a.vx[IP], a.vy[IP], seq_src(a, IP));
-            int VQ = seq_new(a, n, a.vx[IQ], a.vy[IQ], seq_src(a, IQ));
+            int VP = snew(a.vx[IP], a.vy[IP], seq_src(a, IP));
+            int VQ = snew(a.vx[IQ], a.vy[IQ], seq_src(a, IQ));
             if (VP < 0 || VQ < 0) { status |= UB_TRAVERSAL; goto done; }
             // the vertex before IP is the last vertex of its previous node, the one after the first of the next
-            double sP = areaA(a.vx1[a.pv[IP]], a.vy1[a.pv[IP]], a.vx[IP], a.vy[IP], a.vx[a.nx[IP]], a.vy[a.nx[IP]]);
-            double sQ = areaA(a.vx1[a.pv[IQ]], a.vy1[a.pv[IQ]], a.vx[IQ], a.vy[IQ], a.vx[a.nx[IQ]], a.vy[a.nx[IQ]]);
+            double sP = areaA(a.vx1[PV_(IP)], a.vy1[PV_(IP)], a.vx[IP], a.vy[IP], a.vx[NX_(IP)], a.vy[NX_(IP)]);
+            double sQ = areaA(a.vx1[PV_(IQ)], a.vy1[PV_(IQ)], a.vx[IQ], a.vy[IQ], a.vx[NX_(IQ)], a.vy[NX_(IQ)]);
             if (sP * sQ > 0) {
-                a.nb[IP] = VQ; a.nb[VQ] = IP; a.inter[IP] = a.inter[VQ] = 1;
-                a.nb[IQ] = VP; a.nb[VP] = IQ; a.inter[IQ] = a.inter[VP] = 1;
+                set_nb(IP, VQ); set_nb(VQ, IP); set_inter(IP, 1); set_inter(VQ, 1);
+                set_nb(IQ, VP); set_nb(VP, IQ); set_inter(IQ, 1); set_inter(VP, 1);
             } else {
-                a.nb[VP] = VQ; a.nb[VQ] = VP; a.inter[VP] = a.inter[VQ] = 1;
+                set_nb(VP, VQ); set_nb(VQ, VP); set_inter(VP, 1); set_inter(VQ, 1);
             }
-            seq_insert_after(a, VP, IP);
-            seq_insert_after(a, VQ, IQ);
-            a.enex[IP] = EXIT; a.enex[VP] = ENTRY; a.enex[IQ] = EXIT; a.enex[VQ] = ENTRY;
-            a.lab[IP] = a.lab[VP] = a.lab[IQ] = a.lab[VQ] = L_CROSSING;
+            sinsert(VP, IP);
+            sinsert(VQ, IQ);
+            set_enex(IP, EXIT); set_enex(VP, ENTRY); set_enex(IQ, EXIT); set_enex(VQ, ENTRY);
+            set_lab(IP, L_CROSSING); set_lab(VP, L_CROSSING); set_lab(IQ, L_CROSSING); set_lab(VQ, L_CROSSING);
         }
         // 6) crossing candidates
         for (int i = 0; i < ncross[0]; ++i) {
-            int IP = crossL[0][i], IQ = a.nb[IP];
+            int IP = crossL[0][i], IQ = NB_(IP);
             if (IQ >= 0 && IQ < nmark && a.seen[2 * IQ + 1] == -1) {
-                a.lab[IP] = L_CROSSING;
-                a.lab[IQ] = L_CROSSING;
+                set_lab(IP, L_CROSSING);
+                set_lab(IQ, L_CROSSING);
             }
         }
         for (int j = 0; j < nsplit[1]; ++j) a.seen[2 * splitL[1][j]] = 0;
@@ -1104,9 +1139,9 @@ __device__ void seq_run(SeqArgs &a) {
         // createResult2
         {
             int root = rootP, I = root;
-            bool have = a.inter[I] && a.lab[I] == L_CROSSING;
+            bool have = INTER_(I) && LAB_(I) == L_CROSSING;
             if (!have) {
-                do { I = a.nx[I]; } while (!(a.inter[I] && a.lab[I] == L_CROSSING) && I != root);
+                do { I = NX_(I); } while (!(INTER_(I) && LAB_(I) == L_CROSSING) && I != root);
                 have = I != root;
             }
             long bound = 4L * n + 16;
@@ -1115,28 +1150,28 @@ __device__ void seq_run(SeqArgs &a) {
                 if (rings >= a.ring_cap) { status |= UB_TRAVERSAL; goto done; }
                 a.ring_off[rings++] = outn;
                 int V = I;
-                a.inter[V] = 0;
+                set_inter(V, 0);
                 long steps = 0;
                 ++comp;
                 do {
-                    int st = seq_toggle(a.enex[V]);
+                    int st = seq_toggle(ENEX_(V));
                     do {
-                        V = st == EXIT ? a.nx[V] : a.pv[V];
-                        a.inter[V] = 0;
+                        V = st == EXIT ? NX_(V) : PV_(V);
+                        set_inter(V, 0);
                         // within one component the walk is determined by (vertex, direction):
                         // a repeated state can never reach I again (the reference loops forever)
                         int &seen = a.seen[2 * V + (st == EXIT ? 1 : 0)];
                         if (seen == comp) { status |= UB_TRAVERSAL; goto done; }
                         seen = comp;
-                        if (++steps > bound || !seq_emit(a, outn, V, st == EXIT)) { status |= UB_TRAVERSAL; goto done; }
-                    } while (!(a.enex[V] == st) && V != I);
+                        if (++steps > bound || !semit(outn, V, st == EXIT)) { status |= UB_TRAVERSAL; goto done; }
+                    } while (!(ENEX_(V) == st) && V != I);
                     if (V != I) {
-                        V = a.nb[V];
+                        V = NB_(V);
                         if (V < 0) { status |= UB_TRAVERSAL; goto done; }
-                        a.inter[V] = 0;
+                        set_inter(V, 0);
                     }
                 } while (V != I);
-                do { I = a.nx[I]; } while (!(a.inter[I] && a.lab[I] == L_CROSSING) && I != root);
+                do { I = NX_(I); } while (!(INTER_(I) && LAB_(I) == L_CROSSING) && I != root);
                 have = I != root;
             }
         }
@@ -1154,7 +1189,7 @@ done:
 }
 
 
-// sequential route for several jobs: one block per job, its thread 0 runs seq_run
+// sequential route for several jobs: one block per job, its first wave runs seq_run
 struct SeqJob {
     int j;                     // job
     int nP, nQ, P0, Q0;        // ring sizes (slots) and first slots in the dense ring arrays
@@ -1176,8 +1211,8 @@ struct SeqBatch {
     int *osrc, *ring_off, *result;
 };
 
-// one block per sequential job: its threads lay out the rings, thread 0 runs
-// the algorithm (no barrier after that point)
+// one block per sequential job: its threads lay out the rings, its first wave
+// runs the algorithm (no barrier after that point)
 __global__ __launch_bounds__(BLK) void k_sequential(SeqBatch b) {
     const SeqJob jb = b.jobs[blockIdx.x];
     SeqArgs a;
@@ -1212,7 +1247,7 @@ __global__ __launch_bounds__(BLK) void k_sequential(SeqBatch b) {
     }
     seq_init(a);
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x >= WAVE) return; // wave 0 runs the walk
     seq_run(a);
 }
 
