@@ -855,6 +855,7 @@ __device__ void seq_run(SeqArgs &a) {
     int n = a.nP + a.nQ;
     int w0 = -(1 << 30);
     int c_nx = 0, c_pv = 0, c_nb = 0, c_lab = 0, c_enex = 0, c_inter = 0;
+    int c_seen0 = 0, c_seen1 = 0, c_vlen = 0, c_vslot = 0;
     auto fill = [&](int I) {
         w0 = I & ~(WAVE - 1);
         const int j = w0 + lane;
@@ -865,7 +866,15 @@ __device__ void seq_run(SeqArgs &a) {
             c_lab = a.lab[j];
             c_enex = a.enex[j];
             c_inter = a.inter[j];
+            const int2 sn = reinterpret_cast<const int2 *>(a.seen)[j];
+            c_seen0 = sn.x;
+            c_seen1 = sn.y;
+            c_vlen = a.vlen[j];
+            c_vslot = a.vslot[j];
         }
+        // wait for the window here, so that its later reads wait for nothing
+        // (the walk's own stores share the same counter)
+        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0), other counters free
     };
     auto slot = [&](int I) {
         if (unsigned(I - w0) >= unsigned(WAVE)) fill(I);
@@ -877,6 +886,20 @@ __device__ void seq_run(SeqArgs &a) {
     auto LAB_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_lab, k); };
     auto ENEX_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_enex, k); };
     auto INTER_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_inter, k); };
+    auto VLEN_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_vlen, k); };
+    auto VSLOT_ = [&](int I) { const int k = slot(I); return __builtin_amdgcn_readlane(c_vslot, k); };
+    // the traversal's state words, 2 per node (direction d)
+    auto SEEN_ = [&](int I, int d) {
+        const int k = slot(I);
+        return d ? __builtin_amdgcn_readlane(c_seen1, k) : __builtin_amdgcn_readlane(c_seen0, k);
+    };
+    auto set_seen = [&](int I, int d, int v) {
+        a.seen[2 * I + d] = v;
+        if (I - w0 == lane) {
+            if (d) c_seen1 = v;
+            else c_seen0 = v;
+        }
+    };
     auto set_nx = [&](int I, int v) { a.nx[I] = v; if (I - w0 == lane) c_nx = v; };
     auto set_pv = [&](int I, int v) { a.pv[I] = v; if (I - w0 == lane) c_pv = v; };
     auto set_nb = [&](int I, int v) { a.nb[I] = v; if (I - w0 == lane) c_nb = v; };
@@ -894,6 +917,11 @@ __device__ void seq_run(SeqArgs &a) {
         a.vlen[v] = 1;
         a.vslot[v] = -1;
         a.vsrc[v] = src;
+        if (v - w0 == lane) {
+            c_vlen = 1;
+            c_vslot = -1;
+            c_seen0 = c_seen1 = 0; // zeroed with the rest at set-up
+        }
         set_nx(v, v);
         set_pv(v, v);
         set_nb(v, -1);
@@ -911,9 +939,9 @@ __device__ void seq_run(SeqArgs &a) {
     };
     // the vertices of node V, walking forward or backward, one per lane
     auto semit = [&](int &outn, int V, bool forward) -> bool {
-        const int len = a.vlen[V];
+        const int len = VLEN_(V);
         if (outn + len > a.out_cap) return false;
-        const int vs = a.vslot[V];
+        const int vs = VSLOT_(V);
         const bool onP = V < a.nP;
         for (int t0 = 0; t0 < len; t0 += WAVE) {
             const int t = t0 + lane;
@@ -1103,12 +1131,12 @@ __device__ void seq_run(SeqArgs &a) {
         // membership of the Q lists, marked in the traversal's (still zero)
         // state words: -1 in slot 2V for the split list, in 2V+1 for the crossing list
         const int nmark = n;
-        for (int j = 0; j < nsplit[1]; ++j) a.seen[2 * splitL[1][j]] = -1;
-        for (int j = 0; j < ncross[1]; ++j) a.seen[2 * crossL[1][j] + 1] = -1;
+        for (int j = 0; j < nsplit[1]; ++j) set_seen(splitL[1][j], 0, -1);
+        for (int j = 0; j < ncross[1]; ++j) set_seen(crossL[1][j], 1, -1);
         // 5) split interior bouncing pairs
         for (int i = 0; i < nsplit[0]; ++i) {
             int IP = splitL[0][i], IQ = NB_(IP);
-            if (!(IQ >= 0 && IQ < nmark && a.seen[2 * IQ] == -1)) continue;
+            if (!(IQ >= 0 && IQ < nmark && SEEN_(IQ, 0) == -1)) continue;
             int VP = snew(a.vx[IP], a.vy[IP], seq_src(a, IP));
             int VQ = snew(a.vx[IQ], a.vy[IQ], seq_src(a, IQ));
             if (VP < 0 || VQ < 0) { status |= UB_TRAVERSAL; goto done; }
@@ -1129,13 +1157,13 @@ __device__ void seq_run(SeqArgs &a) {
         // 6) crossing candidates
         for (int i = 0; i < ncross[0]; ++i) {
             int IP = crossL[0][i], IQ = NB_(IP);
-            if (IQ >= 0 && IQ < nmark && a.seen[2 * IQ + 1] == -1) {
+            if (IQ >= 0 && IQ < nmark && SEEN_(IQ, 1) == -1) {
                 set_lab(IP, L_CROSSING);
                 set_lab(IQ, L_CROSSING);
             }
         }
-        for (int j = 0; j < nsplit[1]; ++j) a.seen[2 * splitL[1][j]] = 0;
-        for (int j = 0; j < ncross[1]; ++j) a.seen[2 * crossL[1][j] + 1] = 0;
+        for (int j = 0; j < nsplit[1]; ++j) set_seen(splitL[1][j], 0, 0);
+        for (int j = 0; j < ncross[1]; ++j) set_seen(crossL[1][j], 1, 0);
         // createResult2
         {
             int root = rootP, I = root;
@@ -1160,9 +1188,9 @@ __device__ void seq_run(SeqArgs &a) {
                         set_inter(V, 0);
                         // within one component the walk is determined by (vertex, direction):
                         // a repeated state can never reach I again (the reference loops forever)
-                        int &seen = a.seen[2 * V + (st == EXIT ? 1 : 0)];
-                        if (seen == comp) { status |= UB_TRAVERSAL; goto done; }
-                        seen = comp;
+                        const int dir = st == EXIT ? 1 : 0;
+                        if (SEEN_(V, dir) == comp) { status |= UB_TRAVERSAL; goto done; }
+                        set_seen(V, dir, comp);
                         if (++steps > bound || !semit(outn, V, st == EXIT)) { status |= UB_TRAVERSAL; goto done; }
                     } while (!(ENEX_(V) == st) && V != I);
                     if (V != I) {
