@@ -13,8 +13,10 @@
  *    the tree-stage calls before it only record the edges;
  *  - there is no `lock` array: the host locks of the reference's cover-list
  *    insertion do not exist, so the driver's omp_destroy_lock loop goes;
- *  - saveIntersectionsIdsMulticore returns every intersecting pair once, in
+ *  - saveIntersectionsIdsMulticore (and countIntersectionsParallel +
+ *    polyClipArrayGPUParallel) return every intersecting pair once, in
  *    (bId, cId) order; the driver's radix sort and duplicate removal keep it;
+ *    iCount is a host array and holds the total, not per-node counts;
  *  - errors throw segclip::Error (code + message) instead of printing and
  *    exiting.
  * REAL is double.  Edge is any type with the reference Edge's fields
@@ -82,6 +84,15 @@ class SegTree {
     void saveIntersectionsIdsMulticore(double *bPoly, double *cPoly, int bSize, int cSize, int bType, int cType,
                                        std::vector<int> &bPolLineIds, std::vector<int> &cPolLineIds,
                                        std::vector<int> &intersectTypesDup);
+    // hybrid driver (clip:1024-1090): iCount (treeSize ints, host) then the
+    // driver's exclusive scan sizes the pair arrays; here iCount[0] carries
+    // the pair total of overlay cType and every other entry is 0, so the
+    // scanned iCount[treeSize-1] is that total, and polyClipArrayGPUParallel
+    // writes every pair once, in (bId, cId) order
+    void countIntersectionsParallel(double *bPoly, double *cPoly, int bSize, int cSize, int cPolyCount, int bType,
+                                    int cType, int *iCount);
+    void polyClipArrayGPUParallel(double *bPoly, double *cPoly, int bSize, int cSize, int bType, int cType,
+                                  int *iCount, int *bPolLineIds, int *cPolLineIds, int *intersectTypes);
     void saveIntersectionsParallel(double *bPol, double *cPol, int *bPolyLineIds, int *cPolyLineIds, int polySize,
                                    double *intersections, int *intersectTypes, int *intersectAlphaValuesB,
                                    int *intersectAlphaValuesC);
@@ -116,6 +127,8 @@ class SegTree {
         set_ring_sizes(counts);
     }
     void set_ring_sizes(const std::vector<int> &counts);
+    int64_t pair_count(double *bPoly, double *cPoly, int bSize, int bType, int cType, const char *what);
+    void fill_pairs(int cType, int64_t n, int *b, int *c, int *t, const char *what);
     void load(const double *bPoly, const double *cPoly, int bSize);
     void build();
     void check(int rc, const char *what);

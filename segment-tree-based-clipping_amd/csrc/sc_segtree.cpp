@@ -68,27 +68,52 @@ void SegTree::saveIntersectionsIdsMulticore(double *bPoly, double *cPoly, int bS
                                             int cType, std::vector<int> &bPolLineIds,
                                             std::vector<int> &cPolLineIds, std::vector<int> &intersectTypesDup) {
     (void)cSize;
+    const int64_t need = pair_count(bPoly, cPoly, bSize, bType, cType, "saveIntersectionsIdsMulticore");
+    const size_t n = size_t(need);
+    bPolLineIds.resize(n);
+    cPolLineIds.resize(n);
+    intersectTypesDup.resize(n);
+    fill_pairs(cType, need, bPolLineIds.data(), cPolLineIds.data(), intersectTypesDup.data(),
+               "saveIntersectionsIdsMulticore");
+}
+
+int64_t SegTree::pair_count(double *bPoly, double *cPoly, int bSize, int bType, int cType, const char *what) {
     if (bType != 0) throw Error(SEGCLIP_EINVAL, "SegTree: the base type is 0");
     if (!loaded_) load(bPoly, cPoly, bSize);
     build();
     int64_t need = 0;
     int rc = segclip_copy_pairs(ctx_, cType, nullptr, nullptr, nullptr, 0, &need);
-    if (rc != SEGCLIP_OK && rc != SEGCLIP_ESPACE) check(rc, "saveIntersectionsIdsMulticore");
-    size_t n = size_t(need);
-    bPolLineIds.resize(n);
-    cPolLineIds.resize(n);
-    intersectTypesDup.resize(n);
-    std::vector<int32_t> b(n ? n : 1), c(n ? n : 1), t(n ? n : 1);
-    check(segclip_copy_pairs(ctx_, cType, b.data(), c.data(), nullptr, need, nullptr), "copy_pairs");
-    // the pair types as the stab computed them
-    std::vector<double> xy(2 * (n ? n : 1));
-    std::vector<int32_t> qa(n ? n : 1), qb(n ? n : 1);
-    if (n) check(segclip_save_intersections_ids(ctx_, b.data(), c.data(), int(n), xy.data(), t.data(), qa.data(),
-                                                qb.data()),
-                 "saveIntersectionsIdsMulticore");
-    std::copy(b.begin(), b.begin() + n, bPolLineIds.begin());
-    std::copy(c.begin(), c.begin() + n, cPolLineIds.begin());
-    std::copy(t.begin(), t.begin() + n, intersectTypesDup.begin());
+    if (rc != SEGCLIP_OK && rc != SEGCLIP_ESPACE) check(rc, what);
+    return need;
+}
+
+void SegTree::countIntersectionsParallel(double *bPoly, double *cPoly, int bSize, int cSize, int cPolyCount,
+                                         int bType, int cType, int *iCount) {
+    (void)cSize;
+    (void)cPolyCount;
+    const int64_t n = pair_count(bPoly, cPoly, bSize, bType, cType, "countIntersectionsParallel");
+    if (n > int64_t(0x7fffffff)) throw Error(SEGCLIP_ERANGE, "SegTree: pair count exceeds int");
+    std::fill(iCount, iCount + treeSize, 0);
+    iCount[0] = int(n);
+}
+
+void SegTree::polyClipArrayGPUParallel(double *bPoly, double *cPoly, int bSize, int cSize, int bType, int cType,
+                                       int *iCount, int *bPolLineIds, int *cPolLineIds, int *intersectTypes) {
+    (void)cSize;
+    const int64_t n = pair_count(bPoly, cPoly, bSize, bType, cType, "polyClipArrayGPUParallel");
+    if (iCount && treeSize > 0 && int64_t(iCount[treeSize - 1]) != n)
+        throw Error(SEGCLIP_EINVAL, "SegTree: iCount does not hold the scanned pair total");
+    fill_pairs(cType, n, bPolLineIds, cPolLineIds, intersectTypes, "polyClipArrayGPUParallel");
+}
+
+// the n pairs of overlay cType with their types (lineSegmentIntersection, as
+// the stab classified them)
+void SegTree::fill_pairs(int cType, int64_t n, int *b, int *c, int *t, const char *what) {
+    if (n <= 0) return;
+    check(segclip_copy_pairs(ctx_, cType, b, c, nullptr, n, nullptr), what);
+    std::vector<double> xy(2 * size_t(n));
+    std::vector<int32_t> qa(static_cast<size_t>(n)), qb(static_cast<size_t>(n));
+    check(segclip_save_intersections_ids(ctx_, b, c, int(n), xy.data(), t, qa.data(), qb.data()), what);
 }
 
 void SegTree::saveIntersectionsParallel(double *, double *, int *bPolyLineIds, int *cPolyLineIds, int polySize,

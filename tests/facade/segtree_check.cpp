@@ -110,6 +110,29 @@ int main(int argc, char **argv) {
             std::vector<int> bIds, cIds, types;
             sTree.saveIntersectionsIdsMulticore(bPoly.data(), cPoly.data(), nb, cSize, 0, cType, bIds, cIds, types);
             const int n = int(bIds.size());
+            {   // the hybrid driver's sequence (clip:1024-1090) gives the same pairs
+                std::vector<int> iCount(sTree.treeSize);
+                sTree.countIntersectionsParallel(bPoly.data(), cPoly.data(), nb, cSize, K, 0, cType, iCount.data());
+                int run = 0;
+                for (int &v : iCount) {  // the driver's exclusive scan
+                    const int x = v;
+                    v = run;
+                    run += x;
+                }
+                const int dup = iCount.back();
+                std::vector<int> hb(std::max(dup, 1)), hc(std::max(dup, 1)), ht(std::max(dup, 1));
+                sTree.polyClipArrayGPUParallel(bPoly.data(), cPoly.data(), nb, cSize, 0, cType, iCount.data(),
+                                               hb.data(), hc.data(), ht.data());
+                bool same = dup == n;
+                for (int i = 0; same && i < n; ++i)
+                    same = hb[i] == bIds[i] && hc[i] == cIds[i] && ht[i] == types[i];
+                if (!same) {
+                    std::fprintf(stderr, "segtree_check: hybrid pairs differ from the multicore pairs (overlay %d)\n",
+                                 cType);
+                    std::fclose(o);
+                    return 1;
+                }
+            }
             std::vector<int> bSorted(n), cSorted(n);
             std::iota(bSorted.begin(), bSorted.end(), 0);
             std::iota(cSorted.begin(), cSorted.end(), 0);
