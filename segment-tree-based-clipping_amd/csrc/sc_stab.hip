@@ -332,8 +332,10 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
             uint64_t m[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
+                // LSMF's test, one ballot per comparison (the masks are ANDed in scalar registers)
                 const BoxF q = rbox[r + k];
-                m[k] = __builtin_amdgcn_ballot_w64(boxf_overlap(q, cbf));
+                m[k] = __builtin_amdgcn_ballot_w64(!(q.minx > cbf.maxx)) & __builtin_amdgcn_ballot_w64(!(q.maxx < cbf.minx)) &
+                       __builtin_amdgcn_ballot_w64(!(q.miny > cbf.maxy)) & __builtin_amdgcn_ballot_w64(!(q.maxy < cbf.miny));
             }
             if ((m[0] | m[1] | m[2] | m[3]) == 0) continue;
             if (!cols_fetched) {
