@@ -748,6 +748,17 @@ __global__ void k_ring_off_final(int nr, const uint32_t *cnt, const uint32_t *rs
     if (cnt[r]) off2[nonempty[r]] = int(kscan[off[r]]);
 }
 
+__global__ void k_nonempty(int nr, const uint32_t *cnt, uint32_t *flag) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nr) flag[r] = cnt[r] ? 1u : 0u;
+}
+
+// per job: its first ring after clean-up (jr = first ring before it)
+__global__ void k_job_rings(int J, const uint32_t *jr, const uint32_t *nonempty, uint32_t *out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j <= J) out[j] = nonempty[jr[j]];
+}
+
 // ---------------------------------------------------------------------------
 // sequential path: the reference's algorithm on index-linked vertices, one thread
 // ---------------------------------------------------------------------------
@@ -1386,6 +1397,7 @@ __global__ void k_jseq_copy(const SeqJob *jobs, const int *result, const uint32_
 // scratch owned by this stage
 struct FosterScratch {
     DBuf<uint32_t> crossP, crossQ, ccP, ccQ;
+    DBuf<uint32_t> jring; // per job first ring after clean-up, then the output totals
     DBuf<int> enexP, enexQ, cposP, cposQ;
     DBuf<uint8_t> splitP, splitQ;
     DBuf<int> ppos, plen, pdir, pj;
@@ -1419,10 +1431,13 @@ template <class T> void d2h_sync(Ctx &c, T *h, const T *d, int64_t n) {
 
 // clean-up of nr rings laid out at off[0..nr] in (ox, oy, osrc); result into
 // Ctx; ring_idx[r] = index of ring r among the non-empty rings (nr+1 entries)
-void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, std::vector<uint32_t> &ring_idx) {
+// clean-up of the rings; job_ring gets, per job, its first ring after it (J+1
+// entries, from the jobs' first rings before it, jr on the device)
+void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uint32_t *jr,
+                   std::vector<uint32_t> &job_ring) {
     hipStream_t s = c.stream;
     if (nr == 0 || n == 0) {
-        ring_idx.assign(size_t(nr) + 1, 0u);
+        job_ring.assign(size_t(J) + 1, 0u);
         c.nrings = 0;
         c.n_out = 0;
         c.d_ring_off.ensure(1);
@@ -1466,15 +1481,21 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, std::vector<uint
     uint32_t *rcnt = f.rcnt.ensure(nr + 1), *nonempty = f.nonempty.ensure(nr + 1);
     k_ring_counts<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, kscan, rcnt);
     SC_HIP(hipGetLastError());
-    {
-        std::vector<uint32_t> h(static_cast<size_t>(nr));
-        d2h_sync(c, h.data(), rcnt, nr);
-        ring_idx.assign(size_t(nr) + 1, 0u);
-        for (int r = 0; r < nr; ++r) ring_idx[r + 1] = ring_idx[r] + (h[r] ? 1u : 0u);
-        SC_HIP(hipMemcpyAsync(nonempty, ring_idx.data(), sizeof(uint32_t) * (nr + 1), hipMemcpyHostToDevice, s));
-        c.nrings = int(ring_idx[nr]);
-    }
-    uint32_t nout = read_u32(c, tot);
+    // surviving rings: exclusive scan of the non-empty flags; both totals and
+    // the jobs' first rings come back in one copy
+    k_nonempty<<<grid_for(nr), BLK, 0, s>>>(nr, rcnt, nonempty);
+    SC_HIP(hipGetLastError());
+    SC_HIP(hipMemsetAsync(nonempty + nr, 0, sizeof(uint32_t), s));
+    scan_exclusive_u32(nonempty, nonempty, int64_t(nr) + 1, tot + 1, s);
+    uint32_t *jring = f.jring.ensure(size_t(J) + 3);
+    k_job_rings<<<grid_for(int64_t(J) + 1), BLK, 0, s>>>(J, jr, nonempty, jring);
+    SC_HIP(hipGetLastError());
+    SC_HIP(hipMemcpyAsync(jring + J + 1, tot, sizeof(uint32_t) * 2, hipMemcpyDeviceToDevice, s));
+    job_ring.resize(size_t(J) + 3);
+    d2h_sync(c, job_ring.data(), jring, int64_t(J) + 3);
+    const uint32_t nout = job_ring[size_t(J) + 1];
+    c.nrings = int(job_ring[size_t(J) + 2]);
+    job_ring.resize(size_t(J) + 1);
     c.n_out = nout;
     c.d_ring_x.ensure(nout);
     c.d_ring_y.ensure(nout);
@@ -1493,7 +1514,7 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, std::vector<uint
 
 void release_foster_scratch() {
     FosterScratch &f = fscratch();
-    for (DBuf<uint32_t> *b : {&f.crossP, &f.crossQ, &f.ccP, &f.ccQ, &f.succ, &f.val, &f.val2, &f.jmp, &f.jmp2,
+    for (DBuf<uint32_t> *b : {&f.jring, &f.crossP, &f.crossQ, &f.ccP, &f.ccQ, &f.succ, &f.val, &f.val2, &f.jmp, &f.jmp2,
                               &f.inode, &f.dist, &f.dist2, &f.tot, &f.isstart, &f.ringidx, &f.head, &f.piece_at,
                               &f.jv, &f.jr, &f.keep, &f.mark, &f.kscan, &f.rcnt, &f.nonempty})
         b->release();
@@ -1823,8 +1844,6 @@ void label_and_traverse(Ctx &c) {
     SC_HIP(hipMemsetAsync(jr + J, 0, sizeof(uint32_t), s));
     scan_exclusive_u32(jv, jv, int64_t(J) + 1, tot, s);
     scan_exclusive_u32(jr, jr, int64_t(J) + 1, tot + 1, s);
-    std::vector<uint32_t> hJR(size_t(J) + 1);
-    SC_HIP(hipMemcpyAsync(hJR.data(), jr, sizeof(uint32_t) * (J + 1), hipMemcpyDeviceToHost, s));
     d2h_sync(c, htot, tot, 2);
     const int64_t NV = htot[0];
     const int NR = int(htot[1]);
@@ -1885,10 +1904,10 @@ void label_and_traverse(Ctx &c) {
         js.flags[sj[q].j] |= hres[3 * q + 2];
         c.undefined |= hres[3 * q + 2];
     }
-    std::vector<uint32_t> ring_idx;
-    cleanup_rings(c, f, NR, NV, ring_idx);
+    std::vector<uint32_t> job_ring;
+    cleanup_rings(c, f, NR, NV, J, jr, job_ring);
     js.ring_off.resize(size_t(J) + 1);
-    for (int j = 0; j <= J; ++j) js.ring_off[j] = int(ring_idx[hJR[j]]);
+    for (int j = 0; j <= J; ++j) js.ring_off[j] = int(job_ring[j]);
 }
 
 void clip_jobs(Ctx &c, int k0, int k1) {
