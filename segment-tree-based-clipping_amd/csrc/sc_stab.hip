@@ -420,8 +420,12 @@ void find_candidates(Ctx &c) {
     uint64_t *tot64 = c.d_u64_total.ensure(1);
     scan_exclusive_u32(units, units, int64_t(T) + 1, tot, s);
     scan_exclusive_u64(tests, tests, int64_t(T) + 1, tot64, s);
-    uint32_t nunits = read_u32(c, tot);
-    c.n_tests = int64_t(read_u64(c, tot64));
+    uint32_t nunits = 0;
+    uint64_t ntests = 0;
+    SC_HIP(hipMemcpyAsync(&nunits, tot, sizeof(nunits), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipMemcpyAsync(&ntests, tot64, sizeof(ntests), hipMemcpyDeviceToHost, s));
+    SC_HIP(hipStreamSynchronize(s)); // one wait for both totals
+    c.n_tests = int64_t(ntests);
     c.n_units = nunits;
     uint32_t *unode = c.u32c.ensure(size_t(nunits) + 1);
     if (nunits > 0) {
