@@ -42,7 +42,7 @@ void release_derived(Ctx &c) {
         b->release();
     for (DBuf<int> *b : {&c.sp_slot, &c.sp_ndx, &c.sp_edge, &c.sp_av, &c.sp_nbr, &c.sp_lab, &c.sp_pos, &c.sp_ndf})
         b->release();
-    for (DBuf<BoxF> *b : {&c.ebox, &c.lend_box, &c.cov_box}) b->release();
+    for (DBuf<YRangeF> *b : {&c.eyr, &c.lend_yr, &c.cov_yr}) b->release();
     c.lend_bl.release();
     c.sp_x.release();
     c.sp_y.release();

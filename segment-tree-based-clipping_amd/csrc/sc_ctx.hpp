@@ -48,7 +48,7 @@ struct Ctx {
     DBuf<double> uxs, uxe;           // paired mode: input x-extent for the bounding-box test
     DBuf<double> eymin, eymax;       // y-extent
     DBuf<int> etype, eid;
-    DBuf<BoxF> ebox;                 // per edge: LSMF box rounded outward to single precision
+    DBuf<YRangeF> eyr;               // per edge: y-extent of LSMF's box, rounded outward to single precision
 
     // ---- A3 elementary intervals ----
     DBuf<double> ea;
@@ -62,7 +62,7 @@ struct Ctx {
                                          // sorted by (class, leaf), edge order inside a leaf
     DBuf<uint64_t> lend_key;             // (class, leaf) key of each entry
     DBuf<int> lend_bl;                   // the "before" leaf of each entry's edge
-    DBuf<BoxF> lend_box;                 // the edge's single-precision box, beside each entry
+    DBuf<YRangeF> lend_yr;               // the edge's single-precision y-extent, beside each entry
     int64_t n_lend = 0;
     // materialised end lists (parity / drop-in only)
     DBuf<uint32_t> mend_off;
@@ -72,7 +72,7 @@ struct Ctx {
     DBuf<uint32_t> cov_off;              // T+1
     DBuf<uint32_t> cov_e;                // edge indices: base run then overlay run, order within a run unspecified
     DBuf<uint32_t> cov_split;            // per node: count of base entries
-    DBuf<BoxF> cov_box;                  // the edge's single-precision box, beside each cover entry
+    DBuf<YRangeF> cov_yr;                // the edge's single-precision y-extent, beside each cover entry
     int64_t n_cov = 0;
     // ---- A8/A10 candidates and unique pairs ----
     DBuf<uint64_t> unit_desc;            // stab work units: 32-byte records

@@ -198,6 +198,26 @@ __device__ inline bool boxf_overlap(const BoxF &p, const BoxF &q) {
     return !(p.minx > q.maxx) & !(p.maxx < q.minx) & !(p.miny > q.maxy) & !(p.maxy < q.miny);
 }
 
+// The y-extent of LSMF's box rounded outward to single precision.  Two
+// segments whose single-precision y-extents are disjoint have disjoint LSMF
+// boxes; the empty range (+inf, -inf) meets no finite range.
+struct __align__(8) YRangeF {
+    float lo, hi;
+};
+__device__ inline YRangeF yrange_of(double ay, double by) {
+    const Box b = box_of(0.0, ay, 0.0, by);
+    YRangeF r;
+    r.lo = f_down(b.miny);
+    r.hi = f_up(b.maxy);
+    return r;
+}
+__device__ inline YRangeF yrange_empty() {
+    YRangeF r;
+    r.lo = INFINITY;
+    r.hi = -INFINITY;
+    return r;
+}
+
 __host__ __device__ inline int alpha_to_int(double a) {
     if (a < EPS) return 0;
     return int(1000000000.0 * a);

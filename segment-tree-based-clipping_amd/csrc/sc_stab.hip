@@ -59,32 +59,38 @@ __device__ inline NodeShape node_shape(int v, const uint32_t *cov_off, const uin
 __host__ __device__ inline uint32_t cdiv32(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 // Work units of one rectangle (rows x cols): row blocks of RCH rows; a unit
-// of a full row block covers one 64-column tile, a unit of the last (partial)
-// row block covers enough tiles that it runs about RCH row steps.
-__host__ __device__ inline uint32_t tiles_per_unit(uint32_t rows_last) { return cdiv32(RCH, rows_last); }
+// of a full row block covers FTPU 64-column tiles, a unit of the last
+// (partial) row block covers enough tiles that it runs about FTPU*RCH row
+// steps.  Several tiles per unit spread the unit's set-up (its record, its
+// rows) over more tests.
+constexpr uint32_t FTPU = 2;
+__host__ __device__ inline uint32_t tiles_per_unit(uint32_t rows_last) { return FTPU * cdiv32(RCH, rows_last); }
 __host__ __device__ inline uint32_t rect_units(uint32_t rows, uint32_t cols) {
     if (!rows || !cols) return 0;
     const uint32_t rb = cdiv32(rows, RCH), ct = cdiv32(cols, WAVE);
     const uint32_t last = rows - (rb - 1) * RCH;
-    return (rb - 1) * ct + cdiv32(ct, tiles_per_unit(last));
+    return (rb - 1) * cdiv32(ct, FTPU) + cdiv32(ct, tiles_per_unit(last));
 }
 // unit u of a rectangle -> first row, row count, tile range [t0,t1)
 __host__ __device__ inline void rect_unit(uint32_t rows, uint32_t cols, uint32_t u, uint32_t &r0, uint32_t &nr,
                                           uint32_t &t0, uint32_t &t1) {
     const uint32_t rb = cdiv32(rows, RCH), ct = cdiv32(cols, WAVE);
-    const uint32_t full = (rb - 1) * ct;
+    const uint32_t upb = cdiv32(ct, FTPU); // units per full row block
+    const uint32_t full = (rb - 1) * upb;
+    uint32_t g;
     if (u < full) {
-        r0 = (u / ct) * RCH;
+        r0 = (u / upb) * RCH;
         nr = RCH;
-        t0 = u % ct;
-        t1 = t0 + 1;
+        t0 = (u % upb) * FTPU;
+        g = FTPU;
     } else {
-        const uint32_t last = rows - (rb - 1) * RCH, g = tiles_per_unit(last);
+        const uint32_t last = rows - (rb - 1) * RCH;
+        g = tiles_per_unit(last);
         r0 = (rb - 1) * RCH;
         nr = last;
         t0 = (u - full) * g;
-        t1 = t0 + g < ct ? t0 + g : ct;
     }
+    t1 = t0 + g < ct ? t0 + g : ct;
 }
 
 __global__ void k_units(const uint32_t *__restrict__ cov_off, const uint32_t *__restrict__ cov_split,
@@ -172,7 +178,7 @@ constexpr uint32_t XCHUNK = 16; // consecutive blocks per XCD turn
 struct StabArgs {
     const UnitDesc *desc;
     const uint32_t *cov_e, *lend_e;
-    const BoxF *cov_box, *lend_box;           // single-precision boxes beside the entries
+    const YRangeF *cov_yr, *lend_yr;          // single-precision y-extents beside the entries
     const int *lend_bl, *etype, *eoff;
     const double *x0, *y0, *x1, *y1;          // endpoints (input coordinates)
     int nl, D, T;
@@ -201,16 +207,17 @@ __device__ inline void flush(const StabArgs &a, const uint64_t *stage, uint32_t 
 }
 
 // One wave per unit: up to 64 rows against one or more 64-column tiles (one
-// column per lane).  The rows' single-precision boxes sit in LDS and are read
-// as broadcasts, four rows at a time; a row that overlaps no column costs
-// four compares.  Only when some lane passes that screen are the double
-// endpoints fetched (once per unit for the rows, once per tile for the
-// columns) and the exact LSMF box test and lineSegmentIntersection2's
-// classification (exact division-free prefilter first) run.  Hits are staged
-// in LDS and written with one atomic per flush.
+// column per lane).  The rows' single-precision y-extents sit in LDS and are
+// read as broadcasts, four rows at a time; a row that meets no column costs
+// two compares.  Pairs that pass are queued per wave; a full queue (or the
+// end of a tile) runs the exact LSMF box test and lineSegmentIntersection2's
+// classification (exact division-free prefilter first) on 64 pairs at once,
+// with the double endpoints fetched into LDS once per unit (rows) and per
+// tile (columns).  Hits are staged in LDS and written with one atomic per
+// flush.
 __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     __shared__ uint64_t stage_all[BLK / WAVE][STG];
-    __shared__ BoxF rbox_all[BLK / WAVE][RCH];
+    __shared__ YRangeF ry_all[BLK / WAVE][RCH];
     __shared__ uint32_t queue_all[BLK / WAVE][WAVE]; // near pairs: row << 6 | column lane
     __shared__ double4 rxy_all[BLK / WAVE][WAVE];    // double endpoints of the rows
     __shared__ double4 cxy_all[BLK / WAVE][WAVE];    // double endpoints of the tile's columns
@@ -226,7 +233,7 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     const uint32_t wg = lb * (BLK / WAVE) + wv;
     if (lb >= nbk || wg >= a.nunits) return; // whole wave; no block barrier below
     uint64_t *stage = stage_all[wv];
-    BoxF *rbox = rbox_all[wv];
+    YRangeF *ry = ry_all[wv];
     uint32_t *queue = queue_all[wv];
     double4 *rxy = rxy_all[wv], *cxy = cxy_all[wv];
     // the unit's record: a uniform load
@@ -237,30 +244,30 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     const uint32_t nrows = nk & 0xffu;
     const int kind = int(nk >> 8);
     const uint32_t *colsrc = kind == 1 ? a.cov_e : a.lend_e;
-    const BoxF *colbox = kind == 1 ? a.cov_box : a.lend_box;
+    const YRangeF *colyr = kind == 1 ? a.cov_yr : a.lend_yr;
     // a tile's columns: entry, box and (end entries) the edge's "before" leaf
-    auto load_tile = [&](uint32_t t, uint32_t &raw, BoxF &bx, uint32_t &bl) {
+    auto load_tile = [&](uint32_t t, uint32_t &raw, YRangeF &bx, uint32_t &bl) {
         const uint32_t cb = t * WAVE;
         raw = 0xffffffffu;
         bl = 0;
         if (uint32_t(lane) < min(uint32_t(WAVE), cols - cb)) {
             const uint32_t ci = col_begin + cb + lane;
             raw = colsrc[ci];
-            bx = colbox[ci];
+            bx = colyr[ci];
             if (kind != 1) bl = uint32_t(a.lend_bl[ci]);
         }
     };
     uint32_t n_raw, n_bl;
-    BoxF n_box;
+    YRangeF n_box;
     load_tile(t0, n_raw, n_box, n_bl); // issued before the rows are waited for
     // this lane's row edge and box (the empty box past the last row)
     uint32_t re_l = 0;
-    BoxF rb = boxf_empty();
+    YRangeF rb = yrange_empty();
     if (uint32_t(lane) < nrows) {
         re_l = a.cov_e[row0 + lane];
-        rb = a.cov_box[row0 + lane];
+        rb = a.cov_yr[row0 + lane];
     }
-    rbox[lane] = rb;
+    ry[lane] = rb;
     __builtin_amdgcn_wave_barrier();
     bool rows_fetched = false;
     int rk = 0;
@@ -271,12 +278,12 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     for (uint32_t t = t0; t < t1; ++t) {
         if (t != t0) load_tile(t, n_raw, n_box, n_bl);
         const uint32_t raw = n_raw, bl = n_bl;
-        BoxF cbf = n_box;
+        YRangeF cbf = n_box;
         bool valid = raw != 0xffffffffu;
         const uint32_t ce = raw & 0x7fffffffu;
         // "after" copy of an edge whose "before" leaf is under v as well
         if (valid && kind != 1 && (raw & 0x80000000u) && bl >= blo && bl < bhi) valid = false;
-        if (!valid) cbf = boxf_empty();
+        if (!valid) cbf = yrange_empty();
         bool cols_fetched = false;
         int ck = 0;
         uint32_t qn = 0; // queued near pairs of this tile
@@ -332,10 +339,12 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
             uint64_t m[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                // LSMF's test, one ballot per comparison (the masks are ANDed in scalar registers)
-                const BoxF q = rbox[r + k];
-                m[k] = __builtin_amdgcn_ballot_w64(!(q.minx > cbf.maxx)) & __builtin_amdgcn_ballot_w64(!(q.maxx < cbf.minx)) &
-                       __builtin_amdgcn_ballot_w64(!(q.miny > cbf.maxy)) & __builtin_amdgcn_ballot_w64(!(q.maxy < cbf.miny));
+                // LSMF's y test, one ballot per comparison (the masks are ANDed in scalar
+                // registers).  Every tested pair overlaps in x by construction (a cover
+                // entry spans the node, an end entry has an endpoint in it), and the
+                // screen only has to keep every pair LSMF keeps.
+                const YRangeF q = ry[r + k];
+                m[k] = __builtin_amdgcn_ballot_w64(!(q.lo > cbf.hi)) & __builtin_amdgcn_ballot_w64(!(q.hi < cbf.lo));
             }
             if ((m[0] | m[1] | m[2] | m[3]) == 0) continue;
             if (!cols_fetched) {
@@ -449,7 +458,7 @@ void find_candidates(Ctx &c) {
         StabArgs a;
         a.desc = desc; a.cov_e = c.cov_e; a.lend_e = c.lend_e;
         a.lend_bl = c.lend_bl; a.etype = c.etype; a.eoff = c.d_eoff;
-        a.cov_box = c.cov_box; a.lend_box = c.lend_box;
+        a.cov_yr = c.cov_yr; a.lend_yr = c.lend_yr;
         a.x0 = c.ex0; a.y0 = c.ey0; a.x1 = c.ex1; a.y1 = c.ey1;
         a.nl = c.nleaves; a.D = c.depth; a.T = T; a.nunits = nunits;
         a.bits_b = c.bits_b; a.bits_c = c.bits_c;

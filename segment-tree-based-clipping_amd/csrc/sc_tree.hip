@@ -71,7 +71,7 @@ __global__ void k_edges(const double *__restrict__ bpoly, const double *__restri
                         const int *__restrict__ beoff, const int *__restrict__ bvoff, int nbr,
                         const double *__restrict__ shift, double *x0, double *y0, double *x1, double *y1,
                         double *es, double *ee, double *eyl, double *eyh, int *ety, int *eid, double *uxs,
-                        double *uxe, BoxF *ebox) {
+                        double *uxe, YRangeF *eyr) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     const double *P;
@@ -111,7 +111,7 @@ __global__ void k_edges(const double *__restrict__ bpoly, const double *__restri
     eyh[e] = b < d ? d : b;
     ety[e] = ty;
     eid[e] = id;
-    ebox[e] = boxf_of(box_of(a, b, c, d));
+    eyr[e] = yrange_of(b, d);
 }
 
 // ---- A3 ----
@@ -519,7 +519,7 @@ void build_edges(Ctx &c) {
     int ne = c.ne;
     c.ex0.ensure(ne); c.ey0.ensure(ne); c.ex1.ensure(ne); c.ey1.ensure(ne);
     c.estart.ensure(ne); c.eend.ensure(ne); c.etype.ensure(ne); c.eid.ensure(ne);
-    c.eymin.ensure(ne); c.eymax.ensure(ne); c.ebox.ensure(ne);
+    c.eymin.ensure(ne); c.eymax.ensure(ne); c.eyr.ensure(ne);
     if (c.paired) {
         c.uxs.ensure(ne);
         c.uxe.ensure(ne);
@@ -528,7 +528,7 @@ void build_edges(Ctx &c) {
                                                 c.d_bvoff, c.nbr, c.paired ? c.d_shift.p : nullptr, c.ex0, c.ey0,
                                                 c.ex1, c.ey1, c.estart, c.eend, c.eymin, c.eymax, c.etype, c.eid,
                                                 c.paired ? c.uxs.p : nullptr, c.paired ? c.uxe.p : nullptr,
-                                                c.ebox);
+                                                c.eyr);
     SC_HIP(hipGetLastError());
 }
 
@@ -583,19 +583,20 @@ void build_node_intervals(Ctx &c) {
 }
 
 namespace {
-// beside every leaf end entry: its edge's "before" leaf and box
+// beside every leaf end entry: its edge's "before" leaf and y-extent
 __global__ void k_entry_before(const uint32_t *__restrict__ e, int64_t n, const int *__restrict__ before,
-                               const BoxF *__restrict__ ebox, int *bl, BoxF *box) {
+                               const YRangeF *__restrict__ eyr, int *bl, YRangeF *yr) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = e[i] & 0x7fffffffu;
     bl[i] = before[k];
-    box[i] = ebox[k];
+    yr[i] = eyr[k];
 }
-// beside every cover entry: its edge's box
-__global__ void k_entry_boxes(const uint32_t *__restrict__ e, int64_t n, const BoxF *__restrict__ ebox, BoxF *box) {
+// beside every cover entry: its edge's y-extent
+__global__ void k_entry_boxes(const uint32_t *__restrict__ e, int64_t n, const YRangeF *__restrict__ eyr,
+                              YRangeF *yr) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) box[i] = ebox[e[i]];
+    if (i < n) yr[i] = eyr[e[i]];
 }
 } // namespace
 
@@ -620,11 +621,11 @@ void build_leaf_end_entries(Ctx &c) {
     uint32_t *tot = c.d_u32_total.ensure(1);
     scan_exclusive_u32(off, off, 2 * int64_t(nl) + 1, tot, s);
     c.n_lend = read_u32(c, tot);
-    // beside every entry, its edge's "before" leaf (the stab's dedupe test) and box
+    // beside every entry, its edge's "before" leaf (the stab's dedupe test) and y-extent
     int *bl = c.lend_bl.ensure(size_t(c.n_lend > 0 ? c.n_lend : 1));
-    BoxF *bx = c.lend_box.ensure(size_t(c.n_lend > 0 ? c.n_lend : 1));
+    YRangeF *bx = c.lend_yr.ensure(size_t(c.n_lend > 0 ? c.n_lend : 1));
     if (c.n_lend > 0) {
-        k_entry_before<<<grid_for(c.n_lend), BLK, 0, s>>>(vals, c.n_lend, c.before_leaf, c.ebox, bl, bx);
+        k_entry_before<<<grid_for(c.n_lend), BLK, 0, s>>>(vals, c.n_lend, c.before_leaf, c.eyr, bl, bx);
         SC_HIP(hipGetLastError());
     }
 }
@@ -654,7 +655,7 @@ void build_cover_lists(Ctx &c) {
                                                    keys, vals, lr);
         SC_HIP(hipGetLastError());
         radix_sort_u64(keys, vals, L, 0, ilog2_exact(T) + 1, c.rs, s);
-        k_entry_boxes<<<grid_for(L), BLK, 0, s>>>(vals, L, c.ebox, c.cov_box.ensure(size_t(L)));
+        k_entry_boxes<<<grid_for(L), BLK, 0, s>>>(vals, L, c.eyr, c.cov_yr.ensure(size_t(L)));
         SC_HIP(hipGetLastError());
     }
     uint32_t *off = c.cov_off.ensure(size_t(T) + 1);
