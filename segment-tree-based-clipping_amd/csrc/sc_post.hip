@@ -108,38 +108,22 @@ __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, in
     if (n_nd) atomicAdd(&nd_b[gb], n_nd);
 }
 
-// keys for the per-edge alpha order: quantised alpha modulo 1e9 (the nine
-// decimal digits the reference's LSD sort looks at) in the low 30 bits
-__global__ void k_alpha_keys(const int *__restrict__ q, const uint32_t *__restrict__ order, int I, uint64_t *keys,
-                             uint32_t *vals) {
+// keys for the per-edge alpha order: the edge's slot over the batch above
+// the quantised alpha modulo 1e9 (the nine decimal digits the reference's
+// LSD sort looks at, sortIntersectionsAtEdgesParallel oacc:7436-7473) in the
+// low 30 bits; values are pair indices
+__global__ void k_edge_alpha_keys(const int *__restrict__ edge_of, const int *__restrict__ pj,
+                                  const int *__restrict__ off, const int *__restrict__ q, int I, uint64_t *keys,
+                                  uint32_t *vals) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= I) return;
-    uint32_t p = order ? order[i] : uint32_t(i);
-    keys[i] = uint64_t(uint32_t(q[p] % 1000000000));
-    vals[i] = p;
-}
-
-// edge key unique over the batch: offset of the edge's count slot
-__global__ void k_edge_keys(const int *__restrict__ edge_of, const int *__restrict__ pj, const int *__restrict__ off,
-                            const uint32_t *__restrict__ vals, int I, uint64_t *keys) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= I) return;
-    uint32_t p = vals[i];
-    keys[i] = uint64_t(uint32_t(off[pj[p]] + edge_of[p]));
+    keys[i] = (uint64_t(uint32_t(off[pj[i]] + edge_of[i])) << 30) | uint64_t(uint32_t(q[i] % 1000000000));
+    vals[i] = uint32_t(i);
 }
 
 __global__ void k_copy_order(const uint32_t *__restrict__ vals, int I, int *ord) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < I) ord[i] = int(vals[i]);
-}
-
-// (overlay edge slot, base edge) keys: pairs grouped by overlay edge, base order inside
-__global__ void k_ckeys(const int *__restrict__ pj, const int *__restrict__ pb, const int *__restrict__ pc,
-                        const int *__restrict__ JC, int I, int bits_bl, uint64_t *keys, uint32_t *vals) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= I) return;
-    keys[p] = (uint64_t(uint32_t(JC[pj[p]] + pc[p])) << bits_bl) | uint64_t(uint32_t(pb[p]));
-    vals[p] = uint32_t(p);
 }
 
 // per job: augmented sizes nb + (non-degenerate intersections on the base), likewise overlay
@@ -477,31 +461,21 @@ void post_jobs(Ctx &c, int k0, int k1) {
     scan(nd_c, nJC + 1);
 
     if (I > 0) {
-        // base side: pairs are in (job, base edge, overlay edge) order; stable
-        // sort by alpha, then stably by edge -> per edge alpha order
+        // Per edge alpha order on each side: one stable sort of the pairs (in
+        // pair order) by (edge slot, quantised alpha).  Pairs are in (job,
+        // base edge, overlay edge) order, so ties keep the base side's pair
+        // order and, on the overlay side, base-edge order.
         uint64_t *ak = c.k64a.ensure(Ia);
         uint32_t *av = c.u32a.ensure(Ia);
         const int bits_eb = bits_for(int64_t(nJB) + 1), bits_ec = bits_for(int64_t(nJC) + 1);
-        k_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_qa, nullptr, I, ak, av);
+        SC_REQUIRE(bits_eb + 30 <= 64 && bits_ec + 30 <= 64, SC_ERANGE, "edge order key too wide");
+        k_edge_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, ja.JB, c.p_qa, I, ak, av);
         SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, av, I, 0, 30, c.rs, s);
-        k_edge_keys<<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, ja.JB, av, I, ak);
-        SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, av, I, 0, bits_eb, c.rs, s);
+        radix_sort_u64(ak, av, I, 0, bits_eb + 30, c.rs, s);
         k_copy_order<<<grid_for(I), BLK, 0, s>>>(av, I, c.ordB);
-        // overlay side: (overlay edge, base edge) order, then stably by alpha, then by edge
-        uint32_t *cv = c.u32b.ensure(Ia);
-        const int bits_bl = bits_for(c.maxbsz);
-        SC_REQUIRE(bits_ec + bits_bl <= 64, SC_ERANGE, "overlay order key too wide");
-        k_ckeys<<<grid_for(I), BLK, 0, s>>>(c.p_j, c.p_b, c.p_c, ja.JC, I, bits_bl, ak, cv);
+        k_edge_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_c, c.p_j, ja.JC, c.p_qb, I, ak, av);
         SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, cv, I, 0, bits_ec + bits_bl, c.rs, s);
-        k_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_qb, cv, I, ak, av);
-        SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, av, I, 0, 30, c.rs, s);
-        k_edge_keys<<<grid_for(I), BLK, 0, s>>>(c.p_c, c.p_j, ja.JC, av, I, ak);
-        SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, av, I, 0, bits_ec, c.rs, s);
+        radix_sort_u64(ak, av, I, 0, bits_ec + 30, c.rs, s);
         k_copy_order<<<grid_for(I), BLK, 0, s>>>(av, I, c.ordC);
         SC_HIP(hipGetLastError());
     }
