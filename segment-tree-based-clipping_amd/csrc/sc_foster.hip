@@ -428,22 +428,45 @@ __global__ void k_jmask(int M, const int *pj, const int *kind, uint32_t *succ, u
     val[x] = NIL;
 }
 
-__global__ void k_jump_min(int M, const uint32_t *val, const uint32_t *jmp, uint32_t *val2, uint32_t *jmp2) {
+// Cycles of the crossing graph by doubling.  After r rounds the record of x
+// describes the window of 2^r nodes x, succ(x), succ(succ(x)), ... (wrapping
+// around x's cycle): the window's minimum value, the summed piece lengths of
+// the nodes before that minimum's first occurrence, the summed lengths of the
+// whole window and the node after the window.  A window sum is read only when
+// the window holds no occurrence of the cycle's minimum, so does not wrap and
+// stays below the cycle's total; the 32-bit wrap-around of longer windows is
+// never read.
+__global__ void k_jump_init(int M, const uint32_t *val, const uint32_t *succ, const int *plen, uint4 *rec) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M) return;
-    uint32_t j = jmp[x];
-    uint32_t v = val[x], w = val[j];
-    val2[x] = v < w ? v : w;
-    jmp2[x] = jmp[j];
+    rec[x] = make_uint4(val[x], 0u, uint32_t(plen[x]), succ[x]);
 }
 
-// list ranking of the traversed cycles, broken at their start (a P crossing)
-__global__ void k_rank_init(int M, const uint32_t *cmin, const uint32_t *succ, const int *pj, const int *AB,
-                            int sparse, SparseP sp, const uint32_t *ccountP, const int *plen, uint32_t *inode,
-                            uint32_t *nxt, uint32_t *dist) {
+__global__ void k_jump_rank(int M, const uint4 *__restrict__ rec, uint4 *__restrict__ rec2) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M) return;
-    uint32_t m = cmin[x];
+    const uint4 a = rec[x];
+    const uint4 b = rec[a.w];
+    uint4 o;
+    o.x = a.x <= b.x ? a.x : b.x;
+    o.y = a.x <= b.x ? a.y : a.z + b.y;
+    o.z = a.z + b.z;
+    o.w = b.w;
+    rec2[x] = o;
+}
+
+// Per node: its cycle's minimum, the list start of a traversed cycle (the P
+// crossing whose even value 2*slot is the minimum; every other value on the
+// cycle is larger) and the summed piece lengths from the node up to that
+// start, the whole cycle's total at the start itself.  Equivalent to the
+// list ranking of the cycle broken before its start.
+__global__ void k_rank_final(int M, const uint4 *__restrict__ rec, const uint32_t *succ, const int *pj,
+                             const int *AB, int sparse, SparseP sp, const uint32_t *ccountP, const int *plen,
+                             uint32_t *cmin, uint32_t *inode, uint32_t *dist) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= M) return;
+    const uint4 r = rec[x];
+    const uint32_t m = r.x;
     bool trav = m != NIL && (m & 1u) == 0u;
     uint32_t I = NIL;
     if (trav) {
@@ -451,22 +474,11 @@ __global__ void k_rank_init(int M, const uint32_t *cmin, const uint32_t *succ, c
         I = k >= 0 ? ccountP[k] : NIL;
         trav = k >= 0;
     }
+    cmin[x] = m;
     inode[x] = I;
-    nxt[x] = trav ? (succ[x] == I ? NIL : succ[x]) : NIL;
-    dist[x] = trav ? uint32_t(plen[x]) : 0u;
-}
-
-__global__ void k_rank_step(int M, const uint32_t *dist, const uint32_t *nxt, uint32_t *dist2, uint32_t *nxt2) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= M) return;
-    uint32_t n = nxt[x];
-    if (n == NIL) {
-        dist2[x] = dist[x];
-        nxt2[x] = NIL;
-    } else {
-        dist2[x] = dist[x] + dist[n];
-        nxt2[x] = nxt[n];
-    }
+    uint32_t d = 0u;
+    if (trav) d = uint32_t(x) == I ? uint32_t(plen[x]) + rec[succ[x]].y : r.y;
+    dist[x] = d;
 }
 
 // component totals at their starts (base-ring order)
@@ -1401,7 +1413,8 @@ struct FosterScratch {
     DBuf<int> enexP, enexQ, cposP, cposQ;
     DBuf<uint8_t> splitP, splitQ;
     DBuf<int> ppos, plen, pdir, pj;
-    DBuf<uint32_t> succ, val, val2, jmp, jmp2, inode, dist, dist2, tot, isstart, ringidx, head, piece_at;
+    DBuf<uint4> jrec, jrec2; // doubling records of the crossing cycles
+    DBuf<uint32_t> succ, val, val2, inode, dist, tot, isstart, ringidx, head, piece_at;
     DBuf<int> jst;
     DBuf<uint32_t> jv, jr;
     DBuf<int> st;
@@ -1514,10 +1527,12 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
 
 void release_foster_scratch() {
     FosterScratch &f = fscratch();
-    for (DBuf<uint32_t> *b : {&f.jring, &f.crossP, &f.crossQ, &f.ccP, &f.ccQ, &f.succ, &f.val, &f.val2, &f.jmp, &f.jmp2,
-                              &f.inode, &f.dist, &f.dist2, &f.tot, &f.isstart, &f.ringidx, &f.head, &f.piece_at,
+    for (DBuf<uint32_t> *b : {&f.jring, &f.crossP, &f.crossQ, &f.ccP, &f.ccQ, &f.succ, &f.val, &f.val2,
+                              &f.inode, &f.dist, &f.tot, &f.isstart, &f.ringidx, &f.head, &f.piece_at,
                               &f.jv, &f.jr, &f.keep, &f.mark, &f.kscan, &f.rcnt, &f.nonempty})
         b->release();
+    f.jrec.release();
+    f.jrec2.release();
     for (DBuf<int> *b : {&f.enexP, &f.enexQ, &f.cposP, &f.cposQ, &f.ppos, &f.plen, &f.pdir, &f.pj, &f.jst, &f.st,
                          &f.osrc, &f.off, &f.rstart, &f.ringid, &f.sres, &f.sroff, &f.sosrc, &f.nx, &f.pv, &f.nb,
                          &f.lab, &f.enex, &f.vsrc, &f.cand})
@@ -1791,26 +1806,18 @@ void label_and_traverse(Ctx &c) {
     uint32_t *cmin = nullptr, *dist = nullptr, *inode = nullptr;
     uint32_t *base = f.tot.ensure(int64_t(mP) + 1), *ridx = f.ringidx.ensure(int64_t(mP) + 1);
     if (maxm > 0) {
-        uint32_t *val = pa.val, *val2 = f.val2.ensure(M), *jmp = f.jmp.ensure(M), *jmp2 = f.jmp2.ensure(M);
-        SC_HIP(hipMemcpyAsync(jmp, pa.succ, sizeof(uint32_t) * M, hipMemcpyDeviceToDevice, s));
-        const int rounds = ilog2_exact(maxm) + 1;
+        uint4 *rec = f.jrec.ensure(M), *rec2 = f.jrec2.ensure(M);
+        k_jump_init<<<grid_for(M), BLK, 0, s>>>(M, pa.val, pa.succ, pa.plen, rec);
+        const int rounds = ilog2_exact(maxm); // windows of 2^rounds >= maxm nodes cover every cycle
         for (int r = 0; r < rounds; ++r) {
-            k_jump_min<<<grid_for(M), BLK, 0, s>>>(M, val, jmp, val2, jmp2);
-            std::swap(val, val2);
-            std::swap(jmp, jmp2);
+            k_jump_rank<<<grid_for(M), BLK, 0, s>>>(M, rec, rec2);
+            std::swap(rec, rec2);
         }
-        SC_HIP(hipGetLastError());
-        cmin = val;
+        cmin = f.val2.ensure(M);
         inode = f.inode.ensure(M);
-        uint32_t *nxt = jmp2, *nxt2 = jmp, *dist2 = f.dist2.ensure(M);
         dist = f.dist.ensure(M);
-        k_rank_init<<<grid_for(M), BLK, 0, s>>>(M, cmin, pa.succ, pa.pj, AB, sparse ? 1 : 0, sp, ccP, pa.plen, inode,
-                                                nxt, dist);
-        for (int r = 0; r < rounds; ++r) {
-            k_rank_step<<<grid_for(M), BLK, 0, s>>>(M, dist, nxt, dist2, nxt2);
-            std::swap(dist, dist2);
-            std::swap(nxt, nxt2);
-        }
+        k_rank_final<<<grid_for(M), BLK, 0, s>>>(M, rec, pa.succ, pa.pj, AB, sparse ? 1 : 0, sp, ccP, pa.plen, cmin,
+                                                 inode, dist);
         SC_HIP(hipGetLastError());
         uint32_t *isstart = f.isstart.ensure(int64_t(mP) + 1);
         k_start_totals<<<grid_for(mP), BLK, 0, s>>>(mP, pa.cposP, pa.pj, AB, sparse ? c.sp_slot.p : nullptr, cmin,
