@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace sc {
@@ -116,6 +117,12 @@ template <class T> struct DBuf {
         }
         return p;
     }
+    // exchange storage (pointer, capacity, allocator) with another buffer
+    void swap(DBuf &o) {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+        std::swap(from, o.from);
+    }
     operator T *() const { return p; }
 };
 
@@ -171,6 +178,14 @@ struct RadixScratch {
 };
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs,
                     hipStream_t s);
+// The same sort on keys/vals held in buffers, with alternate buffers of the
+// caller's own (ensured here to n entries): when the pass count is odd the
+// sorted data ends in the alternates, and each is exchanged with its partner
+// instead of copied back.  Raw pointers taken from any of the four before the
+// call are stale after it.  A call site that ensures keys beyond n ensures
+// keys_alt to the same size, so neither partner has to grow again.
+void radix_sort_u64_buf(DBuf<uint64_t> &keys, DBuf<uint64_t> &keys_alt, DBuf<uint32_t> *vals,
+                        DBuf<uint32_t> *vals_alt, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs, hipStream_t s);
 // throws if a sort since the last check gave up waiting (its output is invalid)
 void radix_check(RadixScratch &rs, hipStream_t s);
 

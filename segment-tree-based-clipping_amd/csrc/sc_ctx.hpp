@@ -61,6 +61,8 @@ struct Ctx {
     DBuf<uint32_t> lend_e;               // entries: edge index | 0x80000000 for the "after" entry,
                                          // sorted by (class, leaf), edge order inside a leaf
     DBuf<uint64_t> lend_key;             // (class, leaf) key of each entry
+    DBuf<uint64_t> lend_key_alt;         // sort partners of lend_key / lend_e
+    DBuf<uint32_t> lend_e_alt;
     DBuf<int> lend_bl;                   // the "before" leaf of each entry's edge
     DBuf<YRangeF> lend_yr;               // the edge's single-precision y-extent, beside each entry
     int64_t n_lend = 0;
@@ -73,10 +75,13 @@ struct Ctx {
     DBuf<uint32_t> cov_e;                // edge indices: base run then overlay run, order within a run unspecified
     DBuf<uint32_t> cov_split;            // per node: count of base entries
     DBuf<YRangeF> cov_yr;                // the edge's single-precision y-extent, beside each cover entry
+    DBuf<uint64_t> cov_key, cov_key_alt; // (node, class) sort keys of the cover entries, and sort partners
+    DBuf<uint32_t> cov_e_alt;
     int64_t n_cov = 0;
     // ---- A8/A10 candidates and unique pairs ----
     DBuf<uint64_t> unit_desc;            // stab work units: 32-byte records
     DBuf<uint64_t> cand;                 // packed (k, b, c) keys
+    DBuf<uint64_t> cand_alt;             // its sort partner, same capacity
     int64_t n_cand = 0, cand_cap = 0;
     int64_t n_tests = 0;                 // W
     int64_t n_units = 0;                 // stab work units (waves)
@@ -94,6 +99,8 @@ struct Ctx {
     DBuf<double> p_x, p_y;               // intersection point
     DBuf<int> p_qa, p_qb;                // quantised alpha (B side), beta (C side)
     DBuf<int> ordB, ordC;                // per-edge alpha order (positions into the pair list)
+    DBuf<uint64_t> ord_key, ord_key_alt; // (edge slot, alpha) sort keys and their sort partners
+    DBuf<uint32_t> ord_val, ord_val_alt;
     DBuf<int> nd_b, nd_c;                // non-degenerate counts -> exclusive scans (n+1 per job)
     DBuf<int> all_b, all_c;
     int nbaug = 0, ncaug = 0;

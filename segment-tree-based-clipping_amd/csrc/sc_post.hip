@@ -465,18 +465,17 @@ void post_jobs(Ctx &c, int k0, int k1) {
         // pair order) by (edge slot, quantised alpha).  Pairs are in (job,
         // base edge, overlay edge) order, so ties keep the base side's pair
         // order and, on the overlay side, base-edge order.
-        uint64_t *ak = c.k64a.ensure(Ia);
-        uint32_t *av = c.u32a.ensure(Ia);
         const int bits_eb = bits_for(int64_t(nJB) + 1), bits_ec = bits_for(int64_t(nJC) + 1);
         SC_REQUIRE(bits_eb + 30 <= 64 && bits_ec + 30 <= 64, SC_ERANGE, "edge order key too wide");
-        k_edge_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, ja.JB, c.p_qa, I, ak, av);
+        k_edge_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, ja.JB, c.p_qa, I, c.ord_key.ensure(Ia),
+                                                      c.ord_val.ensure(Ia));
         SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, av, I, 0, bits_eb + 30, c.rs, s);
-        k_copy_order<<<grid_for(I), BLK, 0, s>>>(av, I, c.ordB);
-        k_edge_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_c, c.p_j, ja.JC, c.p_qb, I, ak, av);
+        radix_sort_u64_buf(c.ord_key, c.ord_key_alt, &c.ord_val, &c.ord_val_alt, I, 0, bits_eb + 30, c.rs, s);
+        k_copy_order<<<grid_for(I), BLK, 0, s>>>(c.ord_val, I, c.ordB);
+        k_edge_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_c, c.p_j, ja.JC, c.p_qb, I, c.ord_key, c.ord_val);
         SC_HIP(hipGetLastError());
-        radix_sort_u64(ak, av, I, 0, bits_ec + 30, c.rs, s);
-        k_copy_order<<<grid_for(I), BLK, 0, s>>>(av, I, c.ordC);
+        radix_sort_u64_buf(c.ord_key, c.ord_key_alt, &c.ord_val, &c.ord_val_alt, I, 0, bits_ec + 30, c.rs, s);
+        k_copy_order<<<grid_for(I), BLK, 0, s>>>(c.ord_val, I, c.ordC);
         SC_HIP(hipGetLastError());
     }
     // augmented ring sizes and offsets per job

@@ -511,9 +511,10 @@ void radix_check(RadixScratch &rs, hipStream_t s) {
     }
 }
 
-void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs,
-                    hipStream_t s) {
-    if (n <= 1 || bit_hi <= bit_lo) return;
+namespace {
+// the passes of the radix sort; true when the sorted data ends in kalt/valt
+bool radix_passes(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, int64_t n, int bit_lo,
+                  int bit_hi, RadixScratch &rs, hipStream_t s) {
     SC_REQUIRE(n < (int64_t(1) << 32), SC_ERANGE, "radix sort longer than 2^32 keys");
     SC_REQUIRE(bit_lo >= 0 && bit_hi <= 64, SC_EINVAL, "radix bit range");
     int npass = int(ceil_div(bit_hi - bit_lo, 8));
@@ -535,8 +536,6 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int b
     unsigned long long *const old_st = rs.tile_status.p;
     unsigned long long *st = rs.tile_status.ensure(nst);
     if (st != old_st) rs.status_clean = 0; // new memory: contents unknown
-    uint64_t *kalt = rs.keys_alt.ensure(size_t(n));
-    uint32_t *valt = vals ? rs.vals_alt.ensure(size_t(n)) : nullptr;
     uint64_t *kc = keys, *ko = kalt;
     uint32_t *vc = vals, *vo = valt;
     for (int p = 0; p < npass; ++p) {
@@ -569,9 +568,31 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int b
         std::swap(kc, ko);
         std::swap(vc, vo);
     }
-    if (kc != keys) {
-        SC_HIP(hipMemcpyAsync(keys, kc, sizeof(uint64_t) * size_t(n), hipMemcpyDeviceToDevice, s));
-        if (vals) SC_HIP(hipMemcpyAsync(vals, vc, sizeof(uint32_t) * size_t(n), hipMemcpyDeviceToDevice, s));
+    return kc != keys;
+}
+} // namespace
+
+void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs,
+                    hipStream_t s) {
+    if (n <= 1 || bit_hi <= bit_lo) return;
+    uint64_t *kalt = rs.keys_alt.ensure(size_t(n));
+    uint32_t *valt = vals ? rs.vals_alt.ensure(size_t(n)) : nullptr;
+    if (radix_passes(keys, vals, kalt, valt, n, bit_lo, bit_hi, rs, s)) {
+        SC_HIP(hipMemcpyAsync(keys, kalt, sizeof(uint64_t) * size_t(n), hipMemcpyDeviceToDevice, s));
+        if (vals) SC_HIP(hipMemcpyAsync(vals, valt, sizeof(uint32_t) * size_t(n), hipMemcpyDeviceToDevice, s));
+    }
+}
+
+void radix_sort_u64_buf(DBuf<uint64_t> &keys, DBuf<uint64_t> &keys_alt, DBuf<uint32_t> *vals,
+                        DBuf<uint32_t> *vals_alt, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs, hipStream_t s) {
+    if (n <= 1 || bit_hi <= bit_lo) return;
+    SC_REQUIRE(keys.cap >= size_t(n) && (!vals || vals->cap >= size_t(n)) && (!vals == !vals_alt), SC_EINVAL,
+               "radix sort buffers");
+    uint64_t *kalt = keys_alt.ensure(size_t(n));
+    uint32_t *valt = vals ? vals_alt->ensure(size_t(n)) : nullptr;
+    if (radix_passes(keys.p, vals ? vals->p : nullptr, kalt, valt, n, bit_lo, bit_hi, rs, s)) {
+        keys.swap(keys_alt);
+        if (vals) vals->swap(*vals_alt);
     }
 }
 

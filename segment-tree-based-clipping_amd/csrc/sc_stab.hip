@@ -454,6 +454,7 @@ void find_candidates(Ctx &c) {
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(c.d_u64_total.p);
     for (int attempt = 0; attempt < 2; ++attempt) {
         c.cand.ensure(size_t(c.cand_cap));
+        c.cand_alt.ensure(size_t(c.cand_cap));
         SC_HIP(hipMemsetAsync(counter, 0, sizeof(uint64_t), s));
         StabArgs a;
         a.desc = desc; a.cov_e = c.cov_e; a.lend_e = c.lend_e;
@@ -481,7 +482,7 @@ void find_candidates(Ctx &c) {
     // unique pairs sorted by (overlay, base edge, overlay-local edge)
     const int kb = c.bits_k + c.bits_b + c.bits_c;
     const int64_t n = c.n_cand;
-    radix_sort_u64(c.cand, nullptr, n, 0, kb, c.rs, s);
+    radix_sort_u64_buf(c.cand, c.cand_alt, nullptr, nullptr, n, 0, kb, c.rs, s);
     uint32_t *flag = c.u32b.ensure(size_t(n) + 1);
     SC_HIP(hipMemsetAsync(flag, 0, sizeof(uint32_t) * (size_t(n) + 1), s));
     if (n > 0) {

@@ -613,7 +613,9 @@ void build_leaf_end_entries(Ctx &c) {
                                                 c.before_leaf, keys, vals);
     SC_HIP(hipGetLastError());
     int kb = ilog2_exact(2 * int64_t(nl) + 1);
-    radix_sort_u64(keys, vals, m, 0, kb, c.rs, s);
+    radix_sort_u64_buf(c.lend_key, c.lend_key_alt, &c.lend_e, &c.lend_e_alt, m, 0, kb, c.rs, s);
+    keys = c.lend_key.p;
+    vals = c.lend_e.p;
     uint32_t *off = c.lend_off.ensure(2 * size_t(nl) + 1);
     SC_HIP(hipMemsetAsync(off, 0, sizeof(uint32_t) * (2 * size_t(nl) + 1), s));
     k_key_hist<<<grid_for(m), BLK, 0, s>>>(keys, m, 2ull * uint64_t(nl), off);
@@ -648,13 +650,15 @@ void build_cover_lists(Ctx &c) {
     const int64_t L = c.n_cov;
     // (node, class) keys with the edge as value, edge-major, then a stable
     // radix sort: within a node base entries come first, each run in edge order
-    uint64_t *keys = c.k64a.ensure(size_t(L > 0 ? L : 1));
+    uint64_t *keys = c.cov_key.ensure(size_t(L > 0 ? L : 1));
     uint32_t *vals = c.cov_e.ensure(size_t(L > 0 ? L : 1));
     if (L > 0) {
         k_cover<true><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, nullptr, cnt,
                                                    keys, vals, lr);
         SC_HIP(hipGetLastError());
-        radix_sort_u64(keys, vals, L, 0, ilog2_exact(T) + 1, c.rs, s);
+        radix_sort_u64_buf(c.cov_key, c.cov_key_alt, &c.cov_e, &c.cov_e_alt, L, 0, ilog2_exact(T) + 1, c.rs, s);
+        keys = c.cov_key.p;
+        vals = c.cov_e.p;
         k_entry_boxes<<<grid_for(L), BLK, 0, s>>>(vals, L, c.eyr, c.cov_yr.ensure(size_t(L)));
         SC_HIP(hipGetLastError());
     }
@@ -682,7 +686,7 @@ void materialize_end_lists(Ctx &c) {
     int tb = ilog2_exact(int64_t(c.K) + 1);
     SC_REQUIRE(D + idb + tb <= 64, SC_ERANGE, "end-list key does not fit in 64 bits");
     uint64_t *keys = c.u64a.ensure(size_t(n > 0 ? n : 1));
-    // leaf keys/vals are still sorted in k64b / lend_e (first n_lend valid entries)
+    // leaf keys/vals are still sorted in lend_key / lend_e (first n_lend valid entries)
     k_mend_keys<<<grid_for(c.n_lend), BLK, 0, s>>>(c.lend_key, c.lend_e, c.n_lend, c.nleaves, D, c.eid, c.etype, idb, tb,
                                                    keys);
     SC_HIP(hipGetLastError());
