@@ -689,25 +689,148 @@ __global__ void k_keep0(int64_t n, const int *ringid, const int *off, const int 
     keep[i] = k;
 }
 
-__global__ void k_keep_marks(int64_t n, const uint32_t *keep, uint32_t *mark) {
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) mark[i] = keep[i] ? uint32_t(i) + 1u : 0u;
-}
+// One clean-up round in one launch.  A vertex after its ring's root stays
+// unless it is collinear with the last vertex kept before it (under the
+// previous round's flags) and its successor (the root, for the ring's last
+// vertex); the rounds repeat until no flag changes.
+// Tiles of KEEP_TILE vertices are taken in launch order from a counter (every
+// earlier tile belongs to a block already running).  A tile publishes the
+// largest mark (kept index + 1) among its vertices under the previous round's
+// flags; wave 0 then walks back over earlier tiles' words, 64 per round trip,
+// until one carries an inclusive prefix.  Each vertex so learns the last
+// vertex kept before it and its new flag is written in place: a tile reads
+// only its own flags, other tiles' only through the published words.
+constexpr int KEEP_ITEMS = 8;
+constexpr int KEEP_TILE = BLK * KEEP_ITEMS;
+constexpr int KEEP_ROUNDS = 48;                // rounds before the exact sequential pass
+constexpr int KEEP_BATCH = 3;                  // rounds launched per convergence check
+constexpr int KEEP_SPIN_LIMIT = 1 << 20;       // polls of a predecessor window before giving up
+constexpr int KS_EPOCH_SHIFT = 56;             // status word: epoch | flag | value
+constexpr unsigned long long KS_PRE = 1ull << 54, KS_VAL = 0xffffffffull;
 
-__global__ void k_keep_step(int64_t n, const int *ringid, const int *off, const int *rstart, const double *x,
-                            const double *y, const uint32_t *lastk, uint32_t *keep, int *changed) {
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int r = ringid[i];
-    int rs = rstart[r], last = off[r + 1] - 1;
-    if (i <= rs) return;
-    int pv = int(lastk[i - 1]) - 1; // last kept index before i (>= rs)
-    int nx = i == last ? rs : int(i) + 1;
-    uint32_t k = collinear(x, y, pv, int(i), nx) ? 0u : 1u;
-    if (k != keep[i]) {
-        keep[i] = k;
-        atomicOr(changed, 1);
+struct KeepRound {
+    int64_t n;
+    const int *ringid, *off, *rstart;
+    const double *x, *y;
+    uint32_t *keep;
+    unsigned long long *status; // per tile; no word carries this round's epoch before the launch
+    unsigned long long epoch;   // 1..KEEP_ROUNDS, shifted into place
+    uint32_t *tile_counter;     // zero before the launch
+    uint32_t *changed;          // set when a flag changes
+    uint32_t *err;              // set when a wait gives up (the result is then invalid)
+};
+
+__global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
+    __shared__ uint32_t s_tile, s_prefix;
+    __shared__ uint32_t s_wmax[BLK / WAVE];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
+    if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int64_t i0 = int64_t(tile) * KEEP_TILE + int64_t(tid) * KEEP_ITEMS;
+    uint32_t kp[KEEP_ITEMS];
+    uint32_t mine = 0; // marks grow with the index: the thread's largest is its last kept vertex
+#pragma unroll
+    for (int it = 0; it < KEEP_ITEMS; ++it) {
+        const int64_t i = i0 + it;
+        kp[it] = i < a.n ? a.keep[i] : 0u;
+        if (kp[it]) mine = uint32_t(i) + 1u;
     }
+    // exclusive max over the threads before this one, and the tile's maximum
+    uint32_t inc = mine;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc = inc > y ? inc : y;
+    }
+    uint32_t ex = __shfl_up(inc, 1, WAVE);
+    if (lane == 0) ex = 0;
+    if (lane == WAVE - 1) s_wmax[w] = inc;
+    __syncthreads();
+    uint32_t agg = 0;
+#pragma unroll
+    for (int q = 0; q < BLK / WAVE; ++q) {
+        const uint32_t m = s_wmax[q];
+        if (q < w) ex = ex > m ? ex : m;
+        agg = agg > m ? agg : m;
+    }
+    unsigned long long *my = a.status + tile;
+    if (tid == 0)
+        __hip_atomic_store(my, a.epoch | (tile == 0 ? KS_PRE : 0ull) | agg, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    if (w == 0) {
+        uint32_t prefix = 0;
+        if (tile > 0) {
+            const unsigned long long EM = 0xffull << KS_EPOCH_SHIFT;
+            int64_t t = int64_t(tile) - 1; // nearest predecessor not yet accounted for
+            int spins = 0;
+            while (true) {
+                const int64_t q = t - lane;
+                const unsigned long long word =
+                    q >= 0 ? __hip_atomic_load(a.status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : (a.epoch | KS_PRE); // before tile 0: an empty prefix
+                const bool ready = (word & EM) == a.epoch;
+                const uint64_t notready = __ballot(!ready);
+                const uint64_t pre = __ballot(ready && (word & KS_PRE));
+                int upto = -1; // lanes 0..upto are summed
+                bool finished = false;
+                if (pre) {
+                    const int p = __builtin_ctzll(pre);
+                    const uint64_t need = p == WAVE - 1 ? ~0ull : ((2ull << p) - 1ull);
+                    if ((notready & need) == 0) {
+                        upto = p;
+                        finished = true;
+                    }
+                } else if (notready == 0) {
+                    upto = WAVE - 1;
+                }
+                if (upto >= 0) {
+                    uint32_t v = lane <= upto ? uint32_t(word & KS_VAL) : 0u;
+#pragma unroll
+                    for (int d = 1; d < WAVE; d <<= 1) {
+                        const uint32_t y = __shfl_xor(v, d, WAVE);
+                        v = v > y ? v : y;
+                    }
+                    prefix = prefix > v ? prefix : v;
+                    if (finished) break;
+                    t -= WAVE;
+                    spins = 0;
+                } else {
+                    if (++spins > KEEP_SPIN_LIMIT) {
+                        if (lane == 0) atomicOr(a.err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            if (lane == 0)
+                __hip_atomic_store(my, a.epoch | KS_PRE | (prefix > agg ? prefix : agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    uint32_t cur = s_prefix > ex ? s_prefix : ex; // mark of the last vertex kept before i
+    bool ch = false;
+#pragma unroll
+    for (int it = 0; it < KEEP_ITEMS; ++it) {
+        const int64_t i = i0 + it;
+        if (i < a.n) {
+            const int r = a.ringid[i];
+            const int rs = a.rstart[r], last = a.off[r + 1] - 1;
+            if (i > rs) {
+                const int pv = int(cur) - 1; // >= rs: the ring's root is always kept
+                const int nx = i == last ? rs : int(i) + 1;
+                const uint32_t k = collinear(a.x, a.y, pv, int(i), nx) ? 0u : 1u;
+                if (k != kp[it]) {
+                    a.keep[i] = k;
+                    ch = true;
+                }
+            }
+        }
+        if (kp[it]) cur = uint32_t(i) + 1u;
+    }
+    if (__ballot(ch) != 0ull && lane == 0) atomicOr(a.changed, 1u);
 }
 
 // exact sequential pass per ring (used when the fixed point needs too many rounds)
@@ -1417,13 +1540,13 @@ struct FosterScratch {
     DBuf<uint32_t> succ, val, val2, inode, dist, tot, isstart, ringidx, head, piece_at;
     DBuf<int> jst;
     DBuf<uint32_t> jv, jr;
-    DBuf<int> st;
     DBuf<double> ox, oy, qpts, mx, my, nx0, ny0, nx1, ny1, vx1, vy1;
     DBuf<int> osrc, qoks, mav, mnbr, mlab, mjob, nslot, ninter, nlab, nnb, nlen, vlen, vslot, seen;
     DBuf<int64_t> moff, segoff, nodeoff;
     DBuf<uint32_t> nflag, nidx;
     DBuf<int> off, rstart, ringid;
-    DBuf<uint32_t> keep, mark, kscan, rcnt, nonempty;
+    DBuf<uint32_t> keep, kscan, rcnt, nonempty, kctl;
+    DBuf<unsigned long long> kstatus; // clean-up rounds: per tile status words
     // sequential route
     DBuf<SeqJob> sjobs;
     DBuf<int> sres, sroff, sosrc;
@@ -1464,23 +1587,40 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
     k_ring_heads<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, reinterpret_cast<uint32_t *>(ringid));
     scan_inclusive_max_u32(reinterpret_cast<uint32_t *>(ringid), reinterpret_cast<uint32_t *>(ringid), n, s);
     uint32_t *keep = f.keep.ensure(size_t(n) + 1);
-    uint32_t *mark = f.mark.ensure(size_t(n));
     k_keep0<<<grid_for(n), BLK, 0, s>>>(n, ringid, f.off, rstart, f.ox, f.oy, keep);
     SC_HIP(hipGetLastError());
-    int *st = f.st.ensure(1);
+    // clean-up rounds, KEEP_BATCH per look at the per-round change flags; a
+    // round after one without change changes nothing either
+    const int64_t ntiles = (n + KEEP_TILE - 1) / KEEP_TILE;
+    uint32_t *ctl = f.kctl.ensure(2 * KEEP_ROUNDS + 1); // changed flags, error word, tile counters
+    unsigned long long *kst = f.kstatus.ensure(size_t(ntiles));
+    SC_HIP(hipMemsetAsync(ctl, 0, sizeof(uint32_t) * (2 * KEEP_ROUNDS + 1), s));
+    SC_HIP(hipMemsetAsync(kst, 0, sizeof(unsigned long long) * size_t(ntiles), s));
+    KeepRound kr;
+    kr.n = n;
+    kr.ringid = ringid;
+    kr.off = f.off;
+    kr.rstart = rstart;
+    kr.x = f.ox;
+    kr.y = f.oy;
+    kr.keep = keep;
+    kr.status = kst;
+    kr.err = ctl + KEEP_ROUNDS;
     bool converged = false;
-    for (int it = 0; it < 48; ++it) {
-        k_keep_marks<<<grid_for(n), BLK, 0, s>>>(n, keep, mark);
-        scan_inclusive_max_u32(mark, mark, n, s);
-        SC_HIP(hipMemsetAsync(st, 0, sizeof(int), s));
-        k_keep_step<<<grid_for(n), BLK, 0, s>>>(n, ringid, f.off, rstart, f.ox, f.oy, mark, keep, st);
-        SC_HIP(hipGetLastError());
-        int ch = 0;
-        d2h_sync(c, &ch, st, 1);
-        if (!ch) {
-            converged = true;
-            break;
+    for (int it = 0; it < KEEP_ROUNDS && !converged;) {
+        const int end = std::min(it + KEEP_BATCH, KEEP_ROUNDS);
+        for (; it < end; ++it) {
+            kr.epoch = (unsigned long long)(it + 1) << KS_EPOCH_SHIFT;
+            kr.tile_counter = ctl + KEEP_ROUNDS + 1 + it;
+            kr.changed = ctl + it;
+            k_keep_round<<<unsigned(ntiles), BLK, 0, s>>>(kr);
+            SC_HIP(hipGetLastError());
         }
+        // one copy from the batch's last change flag up to the error word
+        uint32_t h[KEEP_ROUNDS + 1];
+        d2h_sync(c, h, ctl + it - 1, int64_t(KEEP_ROUNDS - it + 2));
+        SC_REQUIRE(h[KEEP_ROUNDS - it + 1] == 0, SC_EHIP, "clean-up scan gave up waiting for an earlier tile");
+        converged = h[0] == 0;
     }
     if (!converged) {
         k_cleanup_seq<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, rstart, f.ox, f.oy, keep);
@@ -1529,11 +1669,12 @@ void release_foster_scratch() {
     FosterScratch &f = fscratch();
     for (DBuf<uint32_t> *b : {&f.jring, &f.crossP, &f.crossQ, &f.ccP, &f.ccQ, &f.succ, &f.val, &f.val2,
                               &f.inode, &f.dist, &f.tot, &f.isstart, &f.ringidx, &f.head, &f.piece_at,
-                              &f.jv, &f.jr, &f.keep, &f.mark, &f.kscan, &f.rcnt, &f.nonempty})
+                              &f.jv, &f.jr, &f.keep, &f.kscan, &f.rcnt, &f.nonempty, &f.kctl})
         b->release();
     f.jrec.release();
+    f.kstatus.release();
     f.jrec2.release();
-    for (DBuf<int> *b : {&f.enexP, &f.enexQ, &f.cposP, &f.cposQ, &f.ppos, &f.plen, &f.pdir, &f.pj, &f.jst, &f.st,
+    for (DBuf<int> *b : {&f.enexP, &f.enexQ, &f.cposP, &f.cposQ, &f.ppos, &f.plen, &f.pdir, &f.pj, &f.jst,
                          &f.osrc, &f.off, &f.rstart, &f.ringid, &f.sres, &f.sroff, &f.sosrc, &f.nx, &f.pv, &f.nb,
                          &f.lab, &f.enex, &f.vsrc, &f.cand})
         b->release();
