@@ -189,6 +189,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_global_hist(const uint64_t *keys, 
 constexpr int RS_PER_WAVE = RS_TILE / RS_WAVES;
 constexpr int RS_IT = RS_PER_WAVE / WAVE; // = RS_ITEMS
 static_assert(RS_BLOCK == 256, "one thread per 8-bit digit");
+static_assert(RS_TILE <= 65535, "16-bit tile offsets");
 // status word: epoch (8 bits) | flag (2 bits: aggregate, inclusive prefix) | count (54 bits);
 // a word of another epoch is not yet written for this pass
 constexpr int ST_EPOCH_SHIFT = 56;
@@ -234,7 +235,9 @@ __device__ inline uint32_t block_excl_sum256(uint32_t x, uint32_t *s_wsum, uint3
 __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
     __shared__ uint64_t s_k[RS_TILE];
     __shared__ uint32_t s_v[RS_TILE];
-    __shared__ uint32_t s_cnt[256 * RS_WAVES]; // index digit*RS_WAVES + wave
+    // index digit*RS_WAVES + wave; counts and tile offsets stay below RS_TILE, so
+    // 16 bits suffice (the smaller footprint lets three blocks share a CU)
+    __shared__ uint16_t s_cnt[256 * RS_WAVES];
     __shared__ uint32_t s_gbase[256];          // output offset of the digit minus its tile-local start
     __shared__ uint32_t s_wsum[RS_WAVES];
     __shared__ uint32_t s_tile;
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
         uint32_t before = 0;
         if (valid) before = s_cnt[d * RS_WAVES + w];
         rk[it] = before + r;
-        if (valid && r == 0) s_cnt[d * RS_WAVES + w] = before + uint32_t(__popcll(m));
+        if (valid && r == 0) s_cnt[d * RS_WAVES + w] = uint16_t(before + uint32_t(__popcll(m)));
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
@@ -297,7 +300,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
         uint32_t run = ex;
 #pragma unroll
         for (int q = 0; q < RS_WAVES; ++q) {
-            s_cnt[tid * RS_WAVES + q] = run;
+            s_cnt[tid * RS_WAVES + q] = uint16_t(run);
             run += c[q];
         }
     }
