@@ -53,7 +53,7 @@ void release_derived(Ctx &c) {
         b->release();
     for (DBuf<uint32_t> *b : {&c.lend_off, &c.lend_e, &c.mend_off, &c.mend_e, &c.cov_off, &c.cov_e, &c.cov_split,
                               &c.d_pair_off, &c.u32a, &c.u32b, &c.u32c, &c.u32d, &c.lend_e_alt,
-                              &c.cov_e_alt, &c.ord_val, &c.ord_val_alt})
+                              &c.ord_val, &c.ord_val_alt})
         b->release();
     for (DBuf<uint64_t> *b : {&c.lend_key, &c.lend_key_alt, &c.cov_key, &c.cov_key_alt, &c.unit_desc, &c.cand,
                               &c.cand_alt, &c.pairs, &c.k64a, &c.k64b, &c.u64a, &c.u64b, &c.ord_key, &c.ord_key_alt})

@@ -75,8 +75,7 @@ struct Ctx {
     DBuf<uint32_t> cov_e;                // edge indices: base run then overlay run, order within a run unspecified
     DBuf<uint32_t> cov_split;            // per node: count of base entries
     DBuf<YRangeF> cov_yr;                // the edge's single-precision y-extent, beside each cover entry
-    DBuf<uint64_t> cov_key, cov_key_alt; // (node, class) sort keys of the cover entries, and sort partners
-    DBuf<uint32_t> cov_e_alt;
+    DBuf<uint64_t> cov_key, cov_key_alt; // (node, class) << 32 | edge sort keys, and their sort partner
     int64_t n_cov = 0;
     // ---- A8/A10 candidates and unique pairs ----
     DBuf<uint64_t> unit_desc;            // stab work units: 32-byte records

@@ -290,8 +290,7 @@ __device__ inline int upper_bound_d(const double *a, int n, double q) {
 template <bool EMIT>
 __global__ void k_cover(const double *__restrict__ es, const double *__restrict__ ee, int ne, int nb,
                         const double *__restrict__ ea, int epc, int nleaves, uint32_t *__restrict__ cnt,
-                        const uint32_t *__restrict__ off, uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                        int2 *__restrict__ lr) {
+                        const uint32_t *__restrict__ off, uint64_t *__restrict__ keys, int2 *__restrict__ lr) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     int L, R;
@@ -310,19 +309,13 @@ __global__ void k_cover(const double *__restrict__ es, const double *__restrict_
         int l = L + nleaves, r = R + nleaves;
         while (l < r) {
             if (l & 1) {
-                if (EMIT) {
-                    keys[at + n] = (uint64_t(uint32_t(l)) << 1) | cls;
-                    vals[at + n] = uint32_t(e);
-                }
+                if (EMIT) keys[at + n] = (((uint64_t(uint32_t(l)) << 1) | cls) << 32) | uint32_t(e);
                 ++n;
                 ++l;
             }
             if (r & 1) {
                 --r;
-                if (EMIT) {
-                    keys[at + n] = (uint64_t(uint32_t(r)) << 1) | cls;
-                    vals[at + n] = uint32_t(e);
-                }
+                if (EMIT) keys[at + n] = (((uint64_t(uint32_t(r)) << 1) | cls) << 32) | uint32_t(e);
                 ++n;
             }
             l >>= 1;
@@ -337,11 +330,11 @@ __global__ void k_cover(const double *__restrict__ es, const double *__restrict_
 __global__ void k_cover_ends(const uint64_t *__restrict__ keys, int64_t n, uint32_t *end, uint32_t *bend) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint64_t k = keys[i];
-    const bool last = i + 1 == n || keys[i + 1] != k;
+    const uint64_t k = keys[i] >> 32;
+    const bool last = i + 1 == n || (keys[i + 1] >> 32) != k;
     if (!last) return;
     const uint32_t v = uint32_t(k >> 1);
-    if (i + 1 == n || (keys[i + 1] >> 1) != v) end[v] = uint32_t(i + 1);
+    if (i + 1 == n || (keys[i + 1] >> 33) != v) end[v] = uint32_t(i + 1);
     if ((k & 1ull) == 0) bend[v] = uint32_t(i + 1);
 }
 
@@ -593,10 +586,14 @@ __global__ void k_entry_before(const uint32_t *__restrict__ e, int64_t n, const 
     yr[i] = eyr[k];
 }
 // beside every cover entry: its edge's y-extent
-__global__ void k_entry_boxes(const uint32_t *__restrict__ e, int64_t n, const YRangeF *__restrict__ eyr,
-                              YRangeF *yr) {
+// cover entries from their sorted packed keys: the edge (low word) and its y-extent
+__global__ void k_cover_entries(const uint64_t *__restrict__ keys, int64_t n, const YRangeF *__restrict__ eyr,
+                                uint32_t *ce, YRangeF *yr) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) yr[i] = eyr[e[i]];
+    if (i >= n) return;
+    const uint32_t e = uint32_t(keys[i]);
+    ce[i] = e;
+    yr[i] = eyr[e];
 }
 } // namespace
 
@@ -641,25 +638,27 @@ void build_cover_lists(Ctx &c) {
     uint32_t *cnt = c.u32b.ensure(size_t(ne) + 1);
     int2 *lr = reinterpret_cast<int2 *>(c.u64b.ensure(ne));
     k_cover<false><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, cnt, nullptr,
-                                                nullptr, nullptr, lr);
+                                                nullptr, lr);
     SC_HIP(hipGetLastError());
     SC_HIP(hipMemsetAsync(cnt + ne, 0, sizeof(uint32_t), s));
     uint32_t *tot = c.d_u32_total.ensure(1);
     scan_exclusive_u32(cnt, cnt, int64_t(ne) + 1, tot, s);
     c.n_cov = read_u32(c, tot);
     const int64_t L = c.n_cov;
-    // (node, class) keys with the edge as value, edge-major, then a stable
-    // radix sort: within a node base entries come first, each run in edge order
+    // (node, class) << 32 | edge keys, edge-major, then a stable radix sort on
+    // the (node, class) bits: within a node base entries come first, each run
+    // in edge order
     uint64_t *keys = c.cov_key.ensure(size_t(L > 0 ? L : 1));
-    uint32_t *vals = c.cov_e.ensure(size_t(L > 0 ? L : 1));
+    c.cov_e.ensure(size_t(L > 0 ? L : 1));
     if (L > 0) {
         k_cover<true><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, nullptr, cnt,
-                                                   keys, vals, lr);
+                                                   keys, lr);
         SC_HIP(hipGetLastError());
-        radix_sort_u64_buf(c.cov_key, c.cov_key_alt, &c.cov_e, &c.cov_e_alt, L, 0, ilog2_exact(T) + 1, c.rs, s);
+        const int kb = ilog2_exact(T) + 1;
+        SC_REQUIRE(kb <= 32, SC_ERANGE, "cover key too wide");
+        radix_sort_u64_buf(c.cov_key, c.cov_key_alt, nullptr, nullptr, L, 32, 32 + kb, c.rs, s);
         keys = c.cov_key.p;
-        vals = c.cov_e.p;
-        k_entry_boxes<<<grid_for(L), BLK, 0, s>>>(vals, L, c.eyr, c.cov_yr.ensure(size_t(L)));
+        k_cover_entries<<<grid_for(L), BLK, 0, s>>>(keys, L, c.eyr, c.cov_e, c.cov_yr.ensure(size_t(L)));
         SC_HIP(hipGetLastError());
     }
     uint32_t *off = c.cov_off.ensure(size_t(T) + 1);
