@@ -585,7 +585,6 @@ __global__ void k_entry_before(const uint32_t *__restrict__ e, int64_t n, const 
     bl[i] = before[k];
     yr[i] = eyr[k];
 }
-// beside every cover entry: its edge's y-extent
 // cover entries from their sorted packed keys: the edge (low word) and its y-extent
 __global__ void k_cover_entries(const uint64_t *__restrict__ keys, int64_t n, const YRangeF *__restrict__ eyr,
                                 uint32_t *ce, YRangeF *yr) {
