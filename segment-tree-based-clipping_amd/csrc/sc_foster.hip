@@ -428,23 +428,29 @@ __global__ void k_jmask(int M, const int *pj, const int *kind, uint32_t *succ, u
     val[x] = NIL;
 }
 
-// Cycles of the crossing graph by doubling.  After r rounds the record of x
-// describes the window of 2^r nodes x, succ(x), succ(succ(x)), ... (wrapping
-// around x's cycle): the window's minimum value, the summed piece lengths of
-// the nodes before that minimum's first occurrence, the summed lengths of the
-// whole window and the node after the window.  A window sum is read only when
-// the window holds no occurrence of the cycle's minimum, so does not wrap and
-// stays below the cycle's total; the 32-bit wrap-around of longer windows is
-// never read.
-__global__ void k_jump_init(int M, const uint32_t *val, const uint32_t *succ, const int *plen, uint4 *rec) {
+// Cycles of the crossing graph by doubling.  On a crossing job successors
+// alternate between P and Q crossings and only P crossings carry values, so
+// the doubling runs over the P crossings alone, stepping two pieces at a time
+// (a P piece and the Q piece after it).  After r rounds the record of P
+// crossing x describes the window of 2^r such steps from x (wrapping around
+// x's cycle): the window's minimum value, the summed piece lengths before
+// that minimum's first occurrence, the summed lengths of the whole window and
+// the P crossing after the window.  A window sum is read only when the window
+// holds no occurrence of the cycle's minimum, so does not wrap and stays below
+// the cycle's total; the 32-bit wrap-around of longer windows is never read.
+__global__ void k_jump_init(int mP, int M, const uint32_t *val, const uint32_t *succ, const int *plen, uint4 *rec) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= M) return;
-    rec[x] = make_uint4(val[x], 0u, uint32_t(plen[x]), succ[x]);
+    if (x >= mP) return;
+    const uint32_t q = succ[x];
+    uint32_t n2 = q < uint32_t(M) ? succ[q] : uint32_t(x);
+    if (n2 >= uint32_t(mP)) n2 = uint32_t(x); // not a P-Q-P step: a crossing outside the traversed jobs
+    const uint32_t w = uint32_t(plen[x]) + (q < uint32_t(M) && q != uint32_t(x) ? uint32_t(plen[q]) : 0u);
+    rec[x] = make_uint4(val[x], 0u, w, n2);
 }
 
-__global__ void k_jump_rank(int M, const uint4 *__restrict__ rec, uint4 *__restrict__ rec2) {
+__global__ void k_jump_rank(int mP, const uint4 *__restrict__ rec, uint4 *__restrict__ rec2) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= M) return;
+    if (x >= mP) return;
     const uint4 a = rec[x];
     const uint4 b = rec[a.w];
     uint4 o;
@@ -455,18 +461,24 @@ __global__ void k_jump_rank(int M, const uint4 *__restrict__ rec, uint4 *__restr
     rec2[x] = o;
 }
 
-// Per node: its cycle's minimum, the list start of a traversed cycle (the P
-// crossing whose even value 2*slot is the minimum; every other value on the
-// cycle is larger) and the summed piece lengths from the node up to that
-// start, the whole cycle's total at the start itself.  Equivalent to the
-// list ranking of the cycle broken before its start.
-__global__ void k_rank_final(int M, const uint4 *__restrict__ rec, const uint32_t *succ, const int *pj,
+// Per crossing (P and Q): its cycle's minimum, the list start of a traversed
+// cycle (the P crossing whose even value 2*slot is the minimum; every other
+// value on the cycle is larger) and the summed piece lengths from the
+// crossing up to that start, the whole cycle's total at the start itself.
+// Equivalent to the list ranking of the cycle broken before its start.
+__global__ void k_rank_final(int M, int mP, const uint4 *__restrict__ rec, const uint32_t *succ, const int *pj,
                              const int *AB, int sparse, SparseP sp, const uint32_t *ccountP, const int *plen,
                              uint32_t *cmin, uint32_t *inode, uint32_t *dist) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M) return;
-    const uint4 r = rec[x];
-    const uint32_t m = r.x;
+    // the P crossing whose record describes x's cycle: x itself, or a Q crossing's successor
+    const uint32_t p = x < mP ? uint32_t(x) : succ[x];
+    uint32_t m = NIL, dp = 0u;
+    if (p < uint32_t(mP)) {
+        const uint4 r = rec[p];
+        m = r.x;
+        dp = r.y;
+    }
     bool trav = m != NIL && (m & 1u) == 0u;
     uint32_t I = NIL;
     if (trav) {
@@ -477,7 +489,16 @@ __global__ void k_rank_final(int M, const uint4 *__restrict__ rec, const uint32_
     cmin[x] = m;
     inode[x] = I;
     uint32_t d = 0u;
-    if (trav) d = uint32_t(x) == I ? uint32_t(plen[x]) + rec[succ[x]].y : r.y;
+    if (trav) {
+        if (x >= mP) {
+            d = uint32_t(plen[x]) + (p == I ? 0u : dp);
+        } else if (uint32_t(x) != I) {
+            d = dp;
+        } else { // the start: the whole cycle, its own piece then the Q piece after it and on
+            const uint32_t q = succ[x], n2 = succ[q];
+            d = uint32_t(plen[x]) + uint32_t(plen[q]) + (n2 == I || n2 >= uint32_t(mP) ? 0u : rec[n2].y);
+        }
+    }
     dist[x] = d;
 }
 
@@ -754,9 +775,12 @@ __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
         if (q < w) ex = ex > m ? ex : m;
         agg = agg > m ? agg : m;
     }
+    // marks grow with the index, so a tile holding a kept vertex knows its
+    // inclusive prefix already (its own largest mark) and publishes it as one;
+    // a look-back then normally stops at the nearest predecessor
     unsigned long long *my = a.status + tile;
     if (tid == 0)
-        __hip_atomic_store(my, a.epoch | (tile == 0 ? KS_PRE : 0ull) | agg, __ATOMIC_RELAXED,
+        __hip_atomic_store(my, a.epoch | (tile == 0 || agg != 0 ? KS_PRE : 0ull) | agg, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     if (w == 0) {
         uint32_t prefix = 0;
@@ -803,9 +827,8 @@ __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
                     __builtin_amdgcn_s_sleep(1);
                 }
             }
-            if (lane == 0)
-                __hip_atomic_store(my, a.epoch | KS_PRE | (prefix > agg ? prefix : agg), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0 && agg == 0)
+                __hip_atomic_store(my, a.epoch | KS_PRE | prefix, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) s_prefix = prefix;
     }
@@ -1817,11 +1840,14 @@ void label_and_traverse(Ctx &c) {
     js.route.assign(hkind, hkind + J);
     js.reason.resize(size_t(J));
     SC_HIP(hipMemcpyAsync(js.reason.data(), st.slow, sizeof(int) * size_t(J), hipMemcpyDeviceToHost, s));
-    int maxm = 0;
+    int maxm = 0, maxmP = 0; // crossings of the largest traversed job, and its P crossings
     std::vector<int> seq_jobs;
     bool any_nocross = false;
     for (int j = 0; j < J; ++j) {
-        if (hkind[j] == K_CROSS) maxm = std::max(maxm, hmP[j] + hmQ[j]);
+        if (hkind[j] == K_CROSS) {
+            maxm = std::max(maxm, hmP[j] + hmQ[j]);
+            maxmP = std::max(maxmP, hmP[j]);
+        }
         else if (hkind[j] == K_SEQ) seq_jobs.push_back(j);
         else any_nocross = true;
     }
@@ -1947,17 +1973,20 @@ void label_and_traverse(Ctx &c) {
     uint32_t *cmin = nullptr, *dist = nullptr, *inode = nullptr;
     uint32_t *base = f.tot.ensure(int64_t(mP) + 1), *ridx = f.ringidx.ensure(int64_t(mP) + 1);
     if (maxm > 0) {
-        uint4 *rec = f.jrec.ensure(M), *rec2 = f.jrec2.ensure(M);
-        k_jump_init<<<grid_for(M), BLK, 0, s>>>(M, pa.val, pa.succ, pa.plen, rec);
-        const int rounds = ilog2_exact(maxm); // windows of 2^rounds >= maxm nodes cover every cycle
-        for (int r = 0; r < rounds; ++r) {
-            k_jump_rank<<<grid_for(M), BLK, 0, s>>>(M, rec, rec2);
-            std::swap(rec, rec2);
+        uint4 *rec = f.jrec.ensure(mP > 0 ? mP : 1), *rec2 = f.jrec2.ensure(mP > 0 ? mP : 1);
+        if (mP > 0) {
+            k_jump_init<<<grid_for(mP), BLK, 0, s>>>(mP, M, pa.val, pa.succ, pa.plen, rec);
+            // a cycle of a traversed job holds at most maxmP P crossings
+            const int rounds = ilog2_exact(maxmP);
+            for (int r = 0; r < rounds; ++r) {
+                k_jump_rank<<<grid_for(mP), BLK, 0, s>>>(mP, rec, rec2);
+                std::swap(rec, rec2);
+            }
         }
         cmin = f.val2.ensure(M);
         inode = f.inode.ensure(M);
         dist = f.dist.ensure(M);
-        k_rank_final<<<grid_for(M), BLK, 0, s>>>(M, rec, pa.succ, pa.pj, AB, sparse ? 1 : 0, sp, ccP, pa.plen, cmin,
+        k_rank_final<<<grid_for(M), BLK, 0, s>>>(M, mP, rec, pa.succ, pa.pj, AB, sparse ? 1 : 0, sp, ccP, pa.plen, cmin,
                                                  inode, dist);
         SC_HIP(hipGetLastError());
         uint32_t *isstart = f.isstart.ensure(int64_t(mP) + 1);
