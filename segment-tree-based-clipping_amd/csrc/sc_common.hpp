@@ -164,6 +164,27 @@ void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *
 // Inclusive running maximum of n values (in == out allowed).
 void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s);
 
+// Several 32-bit spans set to a value each, in one launch: add() queues a
+// span and launches when FILL_MAX are queued, flush() launches the rest (the
+// destructor does not: call flush() before the spans are read).
+constexpr int FILL_MAX = 8;
+struct FillBatch {
+    hipStream_t s;
+    int count = 0;
+    void *p[FILL_MAX];
+    int64_t n[FILL_MAX];
+    uint32_t v[FILL_MAX];
+    explicit FillBatch(hipStream_t stream) : s(stream) {}
+    void add(void *ptr, int64_t words, uint32_t value) {
+        if (words <= 0) return;
+        p[count] = ptr;
+        n[count] = words;
+        v[count] = value;
+        if (++count == FILL_MAX) flush();
+    }
+    void flush();
+};
+
 // Stable LSD radix sort of (keys, vals) by key bits [bit_lo, bit_hi).
 // vals may be null (keys only).  The sorted data ends in keys/vals; the
 // alternate buffers are scratch of the same length.

@@ -21,7 +21,13 @@ struct JobSet {
     std::vector<int> ring_off; // J+1: rings of job j are [ring_off[j], ring_off[j+1]) of the output
     std::vector<int> cin;      // first input vertex (no closing vertices) of each job's overlay
     std::vector<int> route, reason; // labelling route per job (0 crossing, 1 no crossing, 2 sequential), why sequential
-    DBuf<int> d_p0, d_bvoff, d_beoff, d_nb, d_nc, d_cvoff, d_JB, d_JC, d_AB, d_AC, d_cin, d_ring;
+    DBuf<int> d_AB, d_AC, d_ring;
+    // device copies of p0, bvoff, beoff, nb, nc, cvoff, JB, JC, cin: views into
+    // one buffer filled by one host-to-device copy
+    DBuf<int> d_pack;
+    std::vector<int> h_pack;
+    int *d_p0 = nullptr, *d_bvoff = nullptr, *d_beoff = nullptr, *d_nb = nullptr, *d_nc = nullptr,
+        *d_cvoff = nullptr, *d_JB = nullptr, *d_JC = nullptr, *d_cin = nullptr;
 };
 
 struct Ctx {

@@ -1617,8 +1617,12 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
     const int64_t ntiles = (n + KEEP_TILE - 1) / KEEP_TILE;
     uint32_t *ctl = f.kctl.ensure(2 * KEEP_ROUNDS + 1); // changed flags, error word, tile counters
     unsigned long long *kst = f.kstatus.ensure(size_t(ntiles));
-    SC_HIP(hipMemsetAsync(ctl, 0, sizeof(uint32_t) * (2 * KEEP_ROUNDS + 1), s));
-    SC_HIP(hipMemsetAsync(kst, 0, sizeof(unsigned long long) * size_t(ntiles), s));
+    {
+        FillBatch fb(s);
+        fb.add(ctl, 2 * KEEP_ROUNDS + 1, 0u);
+        fb.add(kst, 2 * ntiles, 0u);
+        fb.flush();
+    }
     KeepRound kr;
     kr.n = n;
     kr.ringid = ringid;
@@ -1729,7 +1733,7 @@ void label_and_traverse(Ctx &c) {
     // the base side is either its dense rings (slots) or, in the sparse layout,
     // its intersections (positions); the overlay side is always dense
     const SparseP sp = sparse ? sparse_view(c) : SparseP{};
-    const int *PO = sparse ? js.d_p0.p : AB; // offsets of the base-side arrays
+    const int *PO = sparse ? js.d_p0 : AB; // offsets of the base-side arrays
     const int64_t nP = sparse ? js.p0[J] : js.AB[J], nQ = js.AC[J];
     const Side P = sparse ? Side{PO, AC, c.sp_x, c.sp_y, c.sp_lab, c.sp_nbr, c.sp_av}
                           : Side{AB, AC, c.aug_bx, c.aug_by, c.lab_b, c.nbr_b, c.av_b};
@@ -1759,8 +1763,12 @@ void label_and_traverse(Ctx &c) {
     else k_first_plain<<<grid_for(int64_t(J) * WAVE), BLK, 0, s>>>(P, J, st.firstP);
     k_first_plain<<<grid_for(int64_t(J) * WAVE), BLK, 0, s>>>(Q, J, st.firstQ);
     SC_HIP(hipGetLastError());
-    SC_HIP(hipMemsetAsync(crossP + nP, 0, sizeof(uint32_t), s));
-    SC_HIP(hipMemsetAsync(crossQ + nQ, 0, sizeof(uint32_t), s));
+    {
+        FillBatch fb(s);
+        fb.add(crossP + nP, 1, 0u);
+        fb.add(crossQ + nQ, 1, 0u);
+        fb.flush();
+    }
     uint32_t *ccP = f.ccP.ensure(nP + 1), *ccQ = f.ccQ.ensure(nQ + 1);
     uint32_t *tot = c.d_u32_total.ensure(2);
     scan_exclusive_u32(crossP, ccP, nP + 1, tot, s);
@@ -1992,8 +2000,12 @@ void label_and_traverse(Ctx &c) {
         uint32_t *isstart = f.isstart.ensure(int64_t(mP) + 1);
         k_start_totals<<<grid_for(mP), BLK, 0, s>>>(mP, pa.cposP, pa.pj, AB, sparse ? c.sp_slot.p : nullptr, cmin,
                                                     dist, base, isstart);
-        SC_HIP(hipMemsetAsync(base + mP, 0, sizeof(uint32_t), s));
-        SC_HIP(hipMemsetAsync(isstart + mP, 0, sizeof(uint32_t), s));
+        {
+            FillBatch fb(s);
+            fb.add(base + mP, 1, 0u);
+            fb.add(isstart + mP, 1, 0u);
+            fb.flush();
+        }
         SC_HIP(hipGetLastError());
         scan_exclusive_u32(base, base, int64_t(mP) + 1, nullptr, s);
         scan_exclusive_u32(isstart, ridx, int64_t(mP) + 1, nullptr, s);
@@ -2017,8 +2029,12 @@ void label_and_traverse(Ctx &c) {
     if (nseq > 0) SC_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
     k_jcounts<<<grid_for(J), BLK, 0, s>>>(ca);
     SC_HIP(hipGetLastError());
-    SC_HIP(hipMemsetAsync(jv + J, 0, sizeof(uint32_t), s));
-    SC_HIP(hipMemsetAsync(jr + J, 0, sizeof(uint32_t), s));
+    {
+        FillBatch fb(s);
+        fb.add(jv + J, 1, 0u);
+        fb.add(jr + J, 1, 0u);
+        fb.flush();
+    }
     scan_exclusive_u32(jv, jv, int64_t(J) + 1, tot, s);
     scan_exclusive_u32(jr, jr, int64_t(J) + 1, tot + 1, s);
     d2h_sync(c, htot, tot, 2);

@@ -369,12 +369,6 @@ int bits_for(int64_t n) {
     return b < 1 ? 1 : b;
 }
 
-template <class T> T *upload(Ctx &c, DBuf<T> &d, const std::vector<T> &h) {
-    d.ensure(h.size());
-    SC_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c.stream));
-    return d.p;
-}
-
 } // namespace
 
 void post_jobs(Ctx &c, int k0, int k1) {
@@ -410,18 +404,35 @@ void post_jobs(Ctx &c, int k0, int k1) {
     const int I = js.p0[J];
     c.cur_k = J == 1 ? k0 : 0;
     c.I = I;
+    js.cin[J] = c.eoff[k1] - c.nb;
+    {
+        // the job arrays back to back, one copy to the device
+        const std::vector<int> *parts[] = {&js.p0, &js.bvoff, &js.beoff, &js.nb, &js.nc,
+                                           &js.cvoff, &js.JB, &js.JC, &js.cin};
+        int **views[] = {&js.d_p0, &js.d_bvoff, &js.d_beoff, &js.d_nb, &js.d_nc,
+                         &js.d_cvoff, &js.d_JB, &js.d_JC, &js.d_cin};
+        size_t tot = 0;
+        for (const std::vector<int> *v : parts) tot += v->size();
+        js.h_pack.resize(tot);
+        int *dp = js.d_pack.ensure(tot);
+        size_t at = 0;
+        for (int q = 0; q < 9; ++q) {
+            std::copy(parts[q]->begin(), parts[q]->end(), js.h_pack.begin() + at);
+            *views[q] = dp + at;
+            at += parts[q]->size();
+        }
+        SC_HIP(hipMemcpyAsync(dp, js.h_pack.data(), sizeof(int) * tot, hipMemcpyHostToDevice, s));
+    }
     JobArgs ja;
     ja.J = J;
-    ja.p0 = upload(c, js.d_p0, js.p0);
-    ja.bvoff = upload(c, js.d_bvoff, js.bvoff);
-    ja.beoff = upload(c, js.d_beoff, js.beoff);
-    ja.nb = upload(c, js.d_nb, js.nb);
-    ja.nc = upload(c, js.d_nc, js.nc);
-    ja.cvoff = upload(c, js.d_cvoff, js.cvoff);
-    ja.JB = upload(c, js.d_JB, js.JB);
-    ja.JC = upload(c, js.d_JC, js.JC);
-    js.cin[J] = c.eoff[k1] - c.nb;
-    upload(c, js.d_cin, js.cin);
+    ja.p0 = js.d_p0;
+    ja.bvoff = js.d_bvoff;
+    ja.beoff = js.d_beoff;
+    ja.nb = js.d_nb;
+    ja.nc = js.d_nc;
+    ja.cvoff = js.d_cvoff;
+    ja.JB = js.d_JB;
+    ja.JC = js.d_JC;
     const int nJB = js.JB[J], nJC = js.JC[J];
     const int Ia = I > 0 ? I : 1;
     // one-to-many batches keep the shared base ring sparse (no dense copy per job)
@@ -430,20 +441,22 @@ void post_jobs(Ctx &c, int k0, int k1) {
     c.p_j.ensure(Ia); c.p_b.ensure(Ia); c.p_c.ensure(Ia); c.p_type.ensure(Ia); c.p_x.ensure(Ia); c.p_y.ensure(Ia);
     c.p_qa.ensure(Ia); c.p_qb.ensure(Ia); c.ordB.ensure(Ia); c.ordC.ensure(Ia);
     c.slot_b.ensure(Ia); c.slot_c.ensure(Ia);
+    FillBatch fb(s);
     int *all_b = nullptr, *nd_b = nullptr;
     if (!sparse) {
         all_b = c.all_b.ensure(nJB + 1);
         nd_b = c.nd_b.ensure(nJB + 1);
-        SC_HIP(hipMemsetAsync(all_b, 0, sizeof(int) * (nJB + 1), s));
-        SC_HIP(hipMemsetAsync(nd_b, 0, sizeof(int) * (nJB + 1), s));
+        fb.add(all_b, nJB + 1, 0u);
+        fb.add(nd_b, nJB + 1, 0u);
     }
     int *all_c = c.all_c.ensure(nJC + 1), *nd_c = c.nd_c.ensure(nJC + 1);
     int *flags = c.flags.ensure(J);
-    SC_HIP(hipMemsetAsync(flags, 0, sizeof(int) * J, s));
-    SC_HIP(hipMemsetAsync(all_c, 0, sizeof(int) * (nJC + 1), s));
-    SC_HIP(hipMemsetAsync(nd_c, 0, sizeof(int) * (nJC + 1), s));
-    SC_HIP(hipMemsetAsync(c.slot_b.p, 0xff, sizeof(int) * Ia, s));
-    SC_HIP(hipMemsetAsync(c.slot_c.p, 0xff, sizeof(int) * Ia, s));
+    fb.add(flags, J, 0u);
+    fb.add(all_c, nJC + 1, 0u);
+    fb.add(nd_c, nJC + 1, 0u);
+    fb.add(c.slot_b.p, Ia, ~0u);
+    fb.add(c.slot_c.p, Ia, ~0u);
+    fb.flush();
     if (I > 0) {
         k_pair_decode<<<grid_for(I), BLK, 0, s>>>(ja, c.pairs.p + P0, I, c.bits_b, c.bits_c, c.bpoly, c.cpoly, c.p_j,
                                                   c.p_b, c.p_c, c.p_type, c.p_x, c.p_y, c.p_qa, c.p_qb, all_b, nd_b,
@@ -505,8 +518,12 @@ void post_jobs(Ctx &c, int k0, int k1) {
     }
     k_job_sizes<<<grid_for(J), BLK, 0, s>>>(ja, nd_b, nd_c, nba, nca);
     SC_HIP(hipGetLastError());
-    SC_HIP(hipMemsetAsync(nba + J, 0, sizeof(int), s));
-    SC_HIP(hipMemsetAsync(nca + J, 0, sizeof(int), s));
+    {
+        FillBatch fb(s);
+        fb.add(nba + J, 1, 0u);
+        fb.add(nca + J, 1, 0u);
+        fb.flush();
+    }
     uint32_t *tot = c.d_u32_total.ensure(2);
     scan_exclusive_u32(reinterpret_cast<uint32_t *>(nba), reinterpret_cast<uint32_t *>(nba), J + 1, tot, s);
     scan_exclusive_u32(reinterpret_cast<uint32_t *>(nca), reinterpret_cast<uint32_t *>(nca), J + 1, tot + 1, s);
@@ -594,8 +611,12 @@ void init_labels_one(Ctx &c, int nba, int nca, const double *bx, const double *b
     thread_local DBuf<int> *dd = new DBuf<int>(); // scratch, kept for the thread's lifetime
     int *d = dd->ensure(4);
     SC_HIP(hipMemcpyAsync(d, off.data(), sizeof(int) * 4, hipMemcpyHostToDevice, s));
-    SC_HIP(hipMemsetAsync(labb, 0, sizeof(int) * size_t(nba), s));
-    SC_HIP(hipMemsetAsync(labc, 0, sizeof(int) * size_t(nca), s));
+    {
+        FillBatch fb(s);
+        fb.add(labb, nba, 0u);
+        fb.add(labc, nca, 0u);
+        fb.flush();
+    }
     k_init_labels<<<grid_for(nba), BLK, 0, s>>>(1, d, d + 2, bx, by, avb, nbrb, cx, cy, labb, labc, flags);
     SC_HIP(hipGetLastError());
     SC_HIP(hipStreamSynchronize(s)); // off is a stack vector

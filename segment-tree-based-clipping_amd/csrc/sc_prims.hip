@@ -485,6 +485,40 @@ void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *
     scan_impl<uint64_t>(in, out, n, d_total, s, scan_scratch<uint64_t>());
 }
 
+namespace {
+struct FillArgs {
+    uint32_t *p[FILL_MAX];
+    int64_t n[FILL_MAX];
+    uint32_t v[FILL_MAX];
+};
+// blockIdx.y picks the span
+__global__ void k_fill_spans(FillArgs a) {
+    const int q = blockIdx.y;
+    uint32_t *p = a.p[q];
+    const int64_t n = a.n[q];
+    const uint32_t v = a.v[q];
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+} // namespace
+
+void FillBatch::flush() {
+    if (count == 0) return;
+    FillArgs a;
+    int64_t mx = 0;
+    for (int q = 0; q < count; ++q) {
+        a.p[q] = static_cast<uint32_t *>(p[q]);
+        a.n[q] = n[q];
+        a.v[q] = v[q];
+        mx = n[q] > mx ? n[q] : mx;
+    }
+    int64_t gx = ceil_div(mx, int64_t(256));
+    if (gx > 2048) gx = 2048;
+    k_fill_spans<<<dim3(unsigned(gx), unsigned(count)), 256, 0, s>>>(a);
+    SC_HIP(hipGetLastError());
+    count = 0;
+}
+
 void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s) {
     if (n <= 0) return;
     int64_t nb = ceil_div(n, SCAN_TILE);

@@ -423,8 +423,12 @@ void find_candidates(Ctx &c) {
     uint64_t *tests = c.u64b.ensure(size_t(T) + 1);
     k_units<<<grid_for(T), BLK, 0, s>>>(c.cov_off, c.cov_split, c.lend_off, c.nleaves, c.depth, T, units, tests);
     SC_HIP(hipGetLastError());
-    SC_HIP(hipMemsetAsync(units + T, 0, sizeof(uint32_t), s));
-    SC_HIP(hipMemsetAsync(tests + T, 0, sizeof(uint64_t), s));
+    {
+        FillBatch fb(s);
+        fb.add(units + T, 1, 0u);
+        fb.add(tests + T, 2, 0u);
+        fb.flush();
+    }
     uint32_t *tot = c.d_u32_total.ensure(1);
     uint64_t *tot64 = c.d_u64_total.ensure(1);
     scan_exclusive_u32(units, units, int64_t(T) + 1, tot, s);

@@ -664,8 +664,12 @@ void build_cover_lists(Ctx &c) {
     uint32_t *split = c.cov_split.ensure(T);
     // off[v+1] = largest run end among nodes <= v; base runs end at bend
     uint32_t *bend = reinterpret_cast<uint32_t *>(c.u64b.ensure(size_t(T) / 2 + 1)); // T u32 (lr is done)
-    SC_HIP(hipMemsetAsync(off, 0, sizeof(uint32_t) * (size_t(T) + 1), s));
-    SC_HIP(hipMemsetAsync(bend, 0, sizeof(uint32_t) * size_t(T), s));
+    {
+        FillBatch fb(s);
+        fb.add(off, int64_t(T) + 1, 0u);
+        fb.add(bend, T, 0u);
+        fb.flush();
+    }
     if (L > 0) {
         k_cover_ends<<<grid_for(L), BLK, 0, s>>>(keys, L, off + 1, bend);
         SC_HIP(hipGetLastError());
