@@ -49,14 +49,16 @@ void release_derived(Ctx &c) {
     for (DBuf<int> *b : {&c.etype, &c.eid, &c.after_leaf, &c.before_leaf, &c.p_j, &c.p_b, &c.p_c, &c.p_type,
                          &c.p_qa, &c.p_qb, &c.ordB, &c.ordC, &c.nd_b, &c.nd_c, &c.all_b, &c.all_c, &c.nbr_b,
                          &c.nbr_c, &c.av_b, &c.av_c, &c.lab_b, &c.lab_c, &c.slot_b, &c.slot_c, &c.sv_b, &c.sp_b,
-                         &c.sv_c, &c.sp_c})
+                         &c.sv_c, &c.sp_c, &c.arr_b, &c.arr_c})
         b->release();
     for (DBuf<uint32_t> *b : {&c.lend_off, &c.lend_e, &c.mend_off, &c.mend_e, &c.cov_off, &c.cov_e, &c.cov_split,
                               &c.d_pair_off, &c.u32a, &c.u32b, &c.u32c, &c.u32d, &c.lend_e_alt,
-                              &c.ord_val, &c.ord_val_alt})
+                              &c.ord_val, &c.ord_val_alt, &c.seg_wb, &c.seg_wc, &c.seg_bb,
+                              &c.seg_bc, &c.seg_ctl})
         b->release();
     for (DBuf<uint64_t> *b : {&c.lend_key, &c.lend_key_alt, &c.cov_key, &c.cov_key_alt, &c.unit_desc, &c.cand,
-                              &c.cand_alt, &c.pairs, &c.k64a, &c.k64b, &c.u64a, &c.u64b, &c.ord_key, &c.ord_key_alt})
+                              &c.cand_alt, &c.pairs, &c.k64a, &c.k64b, &c.u64a, &c.u64b, &c.ord_key, &c.ord_key_alt,
+                              &c.seg_kb, &c.seg_kc})
         b->release();
     c.rs.global_hist.release();
     c.rs.tile_status.release();

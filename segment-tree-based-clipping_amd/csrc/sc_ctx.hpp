@@ -106,6 +106,9 @@ struct Ctx {
     DBuf<int> ordB, ordC;                // per-edge alpha order (positions into the pair list)
     DBuf<uint64_t> ord_key, ord_key_alt; // (edge slot, alpha) sort keys and their sort partners
     DBuf<uint32_t> ord_val, ord_val_alt;
+    DBuf<int> arr_b, arr_c;              // arrival of each pair inside its edge group (-1: not counted)
+    DBuf<uint64_t> seg_kb, seg_kc;       // (alpha, pair) keys at their edge groups' positions
+    DBuf<uint32_t> seg_wb, seg_wc, seg_bb, seg_bc, seg_ctl; // groups sorted by a wave / a block; list counts
     DBuf<int> nd_b, nd_c;                // non-degenerate counts -> exclusive scans (n+1 per job)
     DBuf<int> all_b, all_c;
     int nbaug = 0, ncaug = 0;

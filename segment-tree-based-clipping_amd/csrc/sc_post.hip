@@ -52,7 +52,7 @@ struct JobArgs {
 __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, int I, int bits_b, int bits_c,
                               const double *__restrict__ bpoly, const double *__restrict__ cpoly, int *pj, int *pb,
                               int *pc, int *pt, double *px, double *py, int *qa, int *qb, int *all_b, int *nd_b,
-                              int *all_c, int *nd_c, int *flags) {
+                              int *all_c, int *nd_c, int *flags, int *arr_b, int *arr_c) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & (WAVE - 1);
     bool counted = false; // a pair inside its job's rings
@@ -65,6 +65,8 @@ __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, in
         pj[p] = j;
         if (b < 0 || b >= ja.nb[j] || cl >= ja.nc[j]) { // a pair outside the job's rings
             atomicOr(&flags[j], UB_UNASSIGNED);
+            arr_c[p] = -1;
+            if (arr_b) arr_b[p] = -1;
             pb[p] = 0;
             pc[p] = 0;
             pt[p] = 0;
@@ -87,7 +89,7 @@ __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, in
             py[p] = y;
             qa[p] = alpha_to_int(al);
             qb[p] = alpha_to_int(be);
-            atomicAdd(&all_c[ja.JC[j] + cl], 1);
+            arr_c[p] = atomicAdd(&all_c[ja.JC[j] + cl], 1); // arrival inside the overlay edge's group
             if (nondeg_c(t)) atomicAdd(&nd_c[ja.JC[j] + cl], 1);
             gb = ja.JB[j] + b;
             counted = true;
@@ -99,13 +101,147 @@ __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, in
     const int prev = __shfl_up(gb, 1, WAVE);
     const bool head = counted && (lane == 0 || prev != gb);
     const uint64_t hm = __ballot(head), cm = __ballot(counted), ndm = __ballot(counted && nondeg_b(t));
-    if (!head) return;
-    const uint64_t above = lane == WAVE - 1 ? 0ull : hm & (~0ull << (lane + 1));
-    const int nxt = above ? __ffsll((unsigned long long)above) - 1 : WAVE;
-    const uint64_t run = (nxt == WAVE ? ~0ull : ((1ull << nxt) - 1ull)) & (~0ull << lane) & cm;
-    atomicAdd(&all_b[gb], __popcll(run));
-    const int n_nd = __popcll(run & ndm);
-    if (n_nd) atomicAdd(&nd_b[gb], n_nd);
+    int base = 0;
+    if (head) {
+        const uint64_t above = lane == WAVE - 1 ? 0ull : hm & (~0ull << (lane + 1));
+        const int nxt = above ? __ffsll((unsigned long long)above) - 1 : WAVE;
+        const uint64_t run = (nxt == WAVE ? ~0ull : ((1ull << nxt) - 1ull)) & (~0ull << lane) & cm;
+        base = atomicAdd(&all_b[gb], __popcll(run));
+        const int n_nd = __popcll(run & ndm);
+        if (n_nd) atomicAdd(&nd_b[gb], n_nd);
+    }
+    // arrival inside the base edge's group: the run's base plus the counted lanes before this one
+    const uint64_t upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1ull);
+    const int h = (hm & upto) ? 63 - __clzll(hm & upto) : 0; // this lane's run head
+    const int hb = __shfl(base, h, WAVE);
+    if (counted) arr_b[p] = hb + __popcll(cm & ((1ull << lane) - 1ull) & ~((1ull << h) - 1ull));
+}
+
+// Per-edge alpha order without a global sort.  Every counted pair has an
+// arrival index inside its edge's group (taken from the counting atomics), so
+// the groups are filled at their exclusive count offsets; inside a group the
+// pairs are then ranked by (quantised alpha, pair index), keys that are all
+// distinct.  The result equals a stable sort of the pairs (in pair order) by
+// (edge slot, quantised alpha): ties keep pair order.
+constexpr int SEG_THREAD = 8;      // groups up to this size: a register sorting network per thread
+constexpr int SEG_WAVE = 64;       // up to this size: a bitonic sort across one wave
+constexpr int SEG_LDS_MAX = 4096;  // up to this size: a bitonic sort in one block's LDS
+constexpr int SEG_CTL = 3;         // per side: wave-list count, block-list count, too-large flag
+
+__global__ void k_seg_scatter(int I, const int *__restrict__ edge_of, const int *__restrict__ pj,
+                              const int *__restrict__ JX, const int *__restrict__ scan, const int *__restrict__ arr,
+                              const int *__restrict__ q, uint64_t *K) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= I) return;
+    const int a = arr[p];
+    if (a < 0) return; // not counted (flagged job)
+    const int pos = scan[JX[pj[p]] + edge_of[p]] + a;
+    K[pos] = (uint64_t(uint32_t(q[p] % 1000000000)) << 32) | uint32_t(p);
+}
+
+// ascending bitonic network on N register values (indices fixed at compile time)
+template <int N> __device__ inline void reg_bitonic(uint64_t (&v)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint64_t x = v[i], y = v[l];
+                    const bool sw = (i & k) == 0 ? x > y : x < y;
+                    v[i] = sw ? y : x;
+                    v[l] = sw ? x : y;
+                }
+            }
+}
+
+// one thread per edge group: groups of up to SEG_THREAD sorted in registers,
+// larger ones listed for a wave or a block, groups beyond SEG_LDS_MAX reported
+__global__ void k_seg_small(int nseg, const int *__restrict__ scan, const uint64_t *__restrict__ K, int *ord,
+                            uint32_t *wlist, uint32_t *blist, uint32_t *ctl) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nseg) return;
+    const int s = scan[g], n = scan[g + 1] - s;
+    if (n <= 0) return;
+    if (n == 1) {
+        ord[s] = int(uint32_t(K[s]));
+        return;
+    }
+    if (n > SEG_THREAD) {
+        if (n <= SEG_WAVE) wlist[atomicAdd(&ctl[0], 1u)] = uint32_t(g);
+        else if (n <= SEG_LDS_MAX) blist[atomicAdd(&ctl[1], 1u)] = uint32_t(g);
+        else atomicOr(&ctl[2], 1u);
+        return;
+    }
+    uint64_t v[SEG_THREAD];
+#pragma unroll
+    for (int i = 0; i < SEG_THREAD; ++i) v[i] = i < n ? K[s + i] : ~0ull; // keys are below ~0
+    reg_bitonic<SEG_THREAD>(v);
+#pragma unroll
+    for (int i = 0; i < SEG_THREAD; ++i)
+        if (i < n) ord[s + i] = int(uint32_t(v[i]));
+}
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const int lo = __shfl_xor(int(uint32_t(v)), m, WAVE), hi = __shfl_xor(int(uint32_t(v >> 32)), m, WAVE);
+    return (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo);
+}
+
+// one wave per listed group of SEG_THREAD+1..SEG_WAVE pairs, one key per lane
+__global__ __launch_bounds__(BLK) void k_seg_wave(const int *__restrict__ scan, const uint64_t *__restrict__ K,
+                                                  int *ord, const uint32_t *__restrict__ wlist,
+                                                  const uint32_t *__restrict__ ctl) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t nw = ctl[0];
+    const uint32_t stride = gridDim.x * (BLK / WAVE);
+    for (uint32_t w = blockIdx.x * (BLK / WAVE) + threadIdx.x / WAVE; w < nw; w += stride) {
+        const int g = int(wlist[w]);
+        const int s = scan[g], n = scan[g + 1] - s;
+        uint64_t v = lane < n ? K[s + lane] : ~0ull;
+#pragma unroll
+        for (int k = 2; k <= WAVE; k <<= 1)
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const uint64_t o = shfl_xor_u64(v, j);
+                const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+                v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
+            }
+        if (lane < n) ord[s + lane] = int(uint32_t(v));
+    }
+}
+
+// one block per listed group of SEG_WAVE+1..SEG_LDS_MAX pairs: bitonic sort in LDS
+__global__ __launch_bounds__(BLK) void k_seg_block(const int *__restrict__ scan, const uint64_t *__restrict__ K,
+                                                   int *ord, const uint32_t *__restrict__ blist,
+                                                   const uint32_t *__restrict__ ctl) {
+    __shared__ uint64_t sk[SEG_LDS_MAX];
+    const uint32_t nb = ctl[1];
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int g = int(blist[b]);
+        const int s = scan[g], n = scan[g + 1] - s;
+        int m = SEG_WAVE * 2;
+        while (m < n) m <<= 1;
+        for (int i = threadIdx.x; i < m; i += BLK) sk[i] = i < n ? K[s + i] : ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= m; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = threadIdx.x; i < m; i += BLK) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const uint64_t x = sk[i], y = sk[l];
+                        if ((i & k) == 0 ? x > y : x < y) {
+                            sk[i] = y;
+                            sk[l] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int i = threadIdx.x; i < n; i += BLK) ord[s + i] = int(uint32_t(sk[i]));
+        __syncthreads();
+    }
 }
 
 // keys for the per-edge alpha order: the edge's slot over the batch above
@@ -460,7 +596,8 @@ void post_jobs(Ctx &c, int k0, int k1) {
     if (I > 0) {
         k_pair_decode<<<grid_for(I), BLK, 0, s>>>(ja, c.pairs.p + P0, I, c.bits_b, c.bits_c, c.bpoly, c.cpoly, c.p_j,
                                                   c.p_b, c.p_c, c.p_type, c.p_x, c.p_y, c.p_qa, c.p_qb, all_b, nd_b,
-                                                  all_c, nd_c, flags);
+                                                  all_c, nd_c, flags, sparse ? nullptr : c.arr_b.ensure(Ia),
+                                                  c.arr_c.ensure(Ia));
         SC_HIP(hipGetLastError());
     }
     auto scan = [&](int *a, int64_t n) {
@@ -473,23 +610,48 @@ void post_jobs(Ctx &c, int k0, int k1) {
     scan(all_c, nJC + 1);
     scan(nd_c, nJC + 1);
 
+    // Per edge alpha order on each side (positions = the edge groups' count
+    // offsets): by ranking inside the groups, or (a group beyond SEG_LDS_MAX,
+    // or a sparse base side) by one stable radix sort of the pairs (in pair
+    // order) by (edge slot, quantised alpha).  Pairs are in (job, base edge,
+    // overlay edge) order, so ties keep the base side's pair order and, on the
+    // overlay side, base-edge order.
+    const int bits_eb = bits_for(int64_t(nJB) + 1), bits_ec = bits_for(int64_t(nJC) + 1);
+    SC_REQUIRE(bits_eb + 30 <= 64 && bits_ec + 30 <= 64, SC_ERANGE, "edge order key too wide");
+    auto radix_order = [&](bool side_b) {
+        const int *edge_of = side_b ? c.p_b.p : c.p_c.p;
+        const int *JX = side_b ? ja.JB : ja.JC;
+        k_edge_alpha_keys<<<grid_for(I), BLK, 0, s>>>(edge_of, c.p_j, JX, side_b ? c.p_qa : c.p_qb, I,
+                                                      c.ord_key.ensure(Ia), c.ord_val.ensure(Ia));
+        SC_HIP(hipGetLastError());
+        radix_sort_u64_buf(c.ord_key, c.ord_key_alt, &c.ord_val, &c.ord_val_alt, I, 0,
+                           (side_b ? bits_eb : bits_ec) + 30, c.rs, s);
+        k_copy_order<<<grid_for(I), BLK, 0, s>>>(c.ord_val, I, side_b ? c.ordB : c.ordC);
+        SC_HIP(hipGetLastError());
+    };
+    uint32_t *segctl = c.seg_ctl.ensure(2 * SEG_CTL); // per side: list counts, too-large flag
     if (I > 0) {
-        // Per edge alpha order on each side: one stable sort of the pairs (in
-        // pair order) by (edge slot, quantised alpha).  Pairs are in (job,
-        // base edge, overlay edge) order, so ties keep the base side's pair
-        // order and, on the overlay side, base-edge order.
-        const int bits_eb = bits_for(int64_t(nJB) + 1), bits_ec = bits_for(int64_t(nJC) + 1);
-        SC_REQUIRE(bits_eb + 30 <= 64 && bits_ec + 30 <= 64, SC_ERANGE, "edge order key too wide");
-        k_edge_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_b, c.p_j, ja.JB, c.p_qa, I, c.ord_key.ensure(Ia),
-                                                      c.ord_val.ensure(Ia));
-        SC_HIP(hipGetLastError());
-        radix_sort_u64_buf(c.ord_key, c.ord_key_alt, &c.ord_val, &c.ord_val_alt, I, 0, bits_eb + 30, c.rs, s);
-        k_copy_order<<<grid_for(I), BLK, 0, s>>>(c.ord_val, I, c.ordB);
-        k_edge_alpha_keys<<<grid_for(I), BLK, 0, s>>>(c.p_c, c.p_j, ja.JC, c.p_qb, I, c.ord_key, c.ord_val);
-        SC_HIP(hipGetLastError());
-        radix_sort_u64_buf(c.ord_key, c.ord_key_alt, &c.ord_val, &c.ord_val_alt, I, 0, bits_ec + 30, c.rs, s);
-        k_copy_order<<<grid_for(I), BLK, 0, s>>>(c.ord_val, I, c.ordC);
-        SC_HIP(hipGetLastError());
+        {
+            FillBatch fb(s);
+            fb.add(segctl, 2 * SEG_CTL, 0u);
+            fb.flush();
+        }
+        auto seg_order = [&](const int *edge_of, const int *JX, const int *scan, const int *arr, const int *q,
+                             int nseg, DBuf<uint64_t> &K, DBuf<uint32_t> &wl, DBuf<uint32_t> &bl, uint32_t *ctl,
+                             int *ord) {
+            k_seg_scatter<<<grid_for(I), BLK, 0, s>>>(I, edge_of, c.p_j, JX, scan, arr, q, K.ensure(Ia));
+            // listed groups hold more than SEG_THREAD (wave list) or SEG_WAVE (block list) pairs each
+            const int64_t nw = std::min<int64_t>(nseg, int64_t(I) / (SEG_THREAD + 1)) + 1;
+            const int64_t nb = std::min<int64_t>(nseg, int64_t(I) / (SEG_WAVE + 1)) + 1;
+            k_seg_small<<<grid_for(nseg), BLK, 0, s>>>(nseg, scan, K, ord, wl.ensure(size_t(nw)),
+                                                       bl.ensure(size_t(nb)), ctl);
+            k_seg_wave<<<1024, BLK, 0, s>>>(scan, K, ord, wl, ctl);
+            k_seg_block<<<512, BLK, 0, s>>>(scan, K, ord, bl, ctl);
+            SC_HIP(hipGetLastError());
+        };
+        if (sparse) radix_order(true);
+        else seg_order(c.p_b, ja.JB, all_b, c.arr_b, c.p_qa, nJB, c.seg_kb, c.seg_wb, c.seg_bb, segctl, c.ordB);
+        seg_order(c.p_c, ja.JC, all_c, c.arr_c, c.p_qb, nJC, c.seg_kc, c.seg_wc, c.seg_bc, segctl + SEG_CTL, c.ordC);
     }
     // augmented ring sizes and offsets per job
     int *nba = js.d_AB.ensure(J + 1), *nca = js.d_AC.ensure(J + 1);
@@ -531,7 +693,12 @@ void post_jobs(Ctx &c, int k0, int k1) {
     js.AC.resize(J + 1);
     SC_HIP(hipMemcpyAsync(js.AB.data(), nba, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
     SC_HIP(hipMemcpyAsync(js.AC.data(), nca, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
+    uint32_t hseg[2 * SEG_CTL] = {};
+    if (I > 0) SC_HIP(hipMemcpyAsync(hseg, segctl, sizeof(hseg), hipMemcpyDeviceToHost, s));
     SC_HIP(hipStreamSynchronize(s));
+    // a group too large for LDS: that side's order by the radix sort instead
+    if (hseg[2] && !sparse) radix_order(true);
+    if (hseg[SEG_CTL + 2]) radix_order(false);
     const int64_t nba_tot = js.AB[J], nca_tot = js.AC[J];
     c.nbaug = int(nba_tot);
     c.ncaug = int(nca_tot);

@@ -54,3 +54,35 @@ def test_one_to_many_matches_per_overlay_oracle(gpu):
         assert np.array_equal(pb, r.pair_b) and np.array_equal(pc, r.pair_c) and np.array_equal(pt, r.pair_type)
         err = _rings_equal(cl.rings(), r.rings)
         assert err is None, "overlay %d: %s" % (k, err)
+
+
+def _sawtooth(T):
+    """Counter-clockwise sawtooth: a spine at y=-2 and T teeth reaching y=1, valleys at y=-1."""
+    pts = [(0.0, -2.0), (1.0, -2.0)]
+    for i in range(T, 0, -1):
+        pts.append(((i - 0.5) / T, 1.0))
+        pts.append(((i - 1.0) / T, -1.0))
+    return np.array(pts, dtype=np.float64)
+
+
+@pytest.mark.parametrize("T", [40, 2500])
+@pytest.mark.parametrize("swap", [False, True])
+def test_many_intersections_on_one_edge(gpu, T, swap):
+    """2T intersections on each horizontal edge of a rectangle: per-edge groups
+    ranked by one block (80 pairs) and groups beyond the LDS limit (5000
+    pairs), on the base side and (swapped) on the overlay side."""
+    import segclip
+    rect = np.array([(-0.5, 0.0), (1.5, 0.0), (1.5, 0.5), (-0.5, 0.5)], dtype=np.float64)
+    saw = _sawtooth(T)
+    b, ov = (saw, rect) if swap else (rect, saw)
+    print("[gpu-clip] sawtooth", T, "swap", swap, flush=True)
+    tr = O.Tree(b, [ov])
+    r = tr.clip(1, canonical=True)
+    cl = segclip.Clipper().set_polygons(b, [ov])
+    cl.clip(1)
+    pb, pc, pt = cl.pairs(1)
+    assert len(pb) == 4 * T
+    assert np.array_equal(pb, r.pair_b) and np.array_equal(pc, r.pair_c) and np.array_equal(pt, r.pair_type)
+    err = _rings_equal(cl.rings(), r.rings)
+    assert err is None, err
+    assert cl.counts()["rings"] == len(r.rings) == T
