@@ -739,10 +739,12 @@ struct KeepRound {
     uint32_t *tile_counter;     // zero before the launch
     uint32_t *changed;          // set when a flag changes
     uint32_t *err;              // set when a wait gives up (the result is then invalid)
+    uint32_t *tstate;           // per tile: flags changed in its last round, the prefix that round saw
+    int first;                  // the first round: every tile decides
 };
 
 __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
-    __shared__ uint32_t s_tile, s_prefix;
+    __shared__ uint32_t s_tile, s_prefix, s_skip, s_ch;
     __shared__ uint32_t s_wmax[BLK / WAVE];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
     if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
@@ -830,9 +832,20 @@ __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
             if (lane == 0 && agg == 0)
                 __hip_atomic_store(my, a.epoch | KS_PRE | prefix, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (lane == 0) s_prefix = prefix;
+        if (lane == 0) {
+            s_prefix = prefix;
+            // a tile whose flags did not change in its last round and whose
+            // incoming prefix is the same sees the same last-kept vertices:
+            // its decisions, and so its flags, stay as they are
+            s_skip = !a.first && a.tstate[2 * tile] == 0u && a.tstate[2 * tile + 1] == prefix ? 1u : 0u;
+            s_ch = 0u;
+        }
     }
     __syncthreads();
+    if (s_skip) {
+        if (tid == 0) a.tstate[2 * tile] = 0u;
+        return;
+    }
     uint32_t cur = s_prefix > ex ? s_prefix : ex; // mark of the last vertex kept before i
     bool ch = false;
 #pragma unroll
@@ -853,7 +866,15 @@ __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
         }
         if (kp[it]) cur = uint32_t(i) + 1u;
     }
-    if (__ballot(ch) != 0ull && lane == 0) atomicOr(a.changed, 1u);
+    if (__ballot(ch) != 0ull && lane == 0) {
+        atomicOr(a.changed, 1u);
+        s_ch = 1u;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        a.tstate[2 * tile] = s_ch;
+        a.tstate[2 * tile + 1] = s_prefix;
+    }
 }
 
 // exact sequential pass per ring (used when the fixed point needs too many rounds)
@@ -1568,7 +1589,7 @@ struct FosterScratch {
     DBuf<int64_t> moff, segoff, nodeoff;
     DBuf<uint32_t> nflag, nidx;
     DBuf<int> off, rstart, ringid;
-    DBuf<uint32_t> keep, kscan, rcnt, nonempty, kctl;
+    DBuf<uint32_t> keep, kscan, rcnt, nonempty, kctl, ktstate;
     DBuf<unsigned long long> kstatus; // clean-up rounds: per tile status words
     // sequential route
     DBuf<SeqJob> sjobs;
@@ -1633,6 +1654,7 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
     kr.keep = keep;
     kr.status = kst;
     kr.err = ctl + KEEP_ROUNDS;
+    kr.tstate = f.ktstate.ensure(2 * size_t(ntiles)); // written by every tile in the first round
     bool converged = false;
     for (int it = 0; it < KEEP_ROUNDS && !converged;) {
         const int end = std::min(it + KEEP_BATCH, KEEP_ROUNDS);
@@ -1640,6 +1662,7 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
             kr.epoch = (unsigned long long)(it + 1) << KS_EPOCH_SHIFT;
             kr.tile_counter = ctl + KEEP_ROUNDS + 1 + it;
             kr.changed = ctl + it;
+            kr.first = it == 0 ? 1 : 0;
             k_keep_round<<<unsigned(ntiles), BLK, 0, s>>>(kr);
             SC_HIP(hipGetLastError());
         }
@@ -1696,7 +1719,7 @@ void release_foster_scratch() {
     FosterScratch &f = fscratch();
     for (DBuf<uint32_t> *b : {&f.jring, &f.crossP, &f.crossQ, &f.ccP, &f.ccQ, &f.succ, &f.val, &f.val2,
                               &f.inode, &f.dist, &f.tot, &f.isstart, &f.ringidx, &f.head, &f.piece_at,
-                              &f.jv, &f.jr, &f.keep, &f.kscan, &f.rcnt, &f.nonempty, &f.kctl})
+                              &f.jv, &f.jr, &f.keep, &f.kscan, &f.rcnt, &f.nonempty, &f.kctl, &f.ktstate})
         b->release();
     f.jrec.release();
     f.kstatus.release();

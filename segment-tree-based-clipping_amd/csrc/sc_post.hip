@@ -227,14 +227,14 @@ __global__ __launch_bounds__(BLK) void k_seg_block(const int *__restrict__ scan,
         __syncthreads();
         for (int k = 2; k <= m; k <<= 1)
             for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int i = threadIdx.x; i < m; i += BLK) {
-                    const int l = i ^ j;
-                    if (l > i) {
-                        const uint64_t x = sk[i], y = sk[l];
-                        if ((i & k) == 0 ? x > y : x < y) {
-                            sk[i] = y;
-                            sk[l] = x;
-                        }
+                // the m/2 compare-exchange pairs (i, i + j), i with bit j clear
+                for (int h = threadIdx.x; h < (m >> 1); h += BLK) {
+                    const int i = ((h & ~(j - 1)) << 1) | (h & (j - 1));
+                    const int l = i + j;
+                    const uint64_t x = sk[i], y = sk[l];
+                    if ((i & k) == 0 ? x > y : x < y) {
+                        sk[i] = y;
+                        sk[l] = x;
                     }
                 }
                 __syncthreads();
@@ -646,7 +646,7 @@ void post_jobs(Ctx &c, int k0, int k1) {
             k_seg_small<<<grid_for(nseg), BLK, 0, s>>>(nseg, scan, K, ord, wl.ensure(size_t(nw)),
                                                        bl.ensure(size_t(nb)), ctl);
             k_seg_wave<<<1024, BLK, 0, s>>>(scan, K, ord, wl, ctl);
-            k_seg_block<<<512, BLK, 0, s>>>(scan, K, ord, bl, ctl);
+            k_seg_block<<<1280, BLK, 0, s>>>(scan, K, ord, bl, ctl); // five 32 KB blocks per CU
             SC_HIP(hipGetLastError());
         };
         if (sparse) radix_order(true);
