@@ -24,6 +24,7 @@
 // cases where it is not (an intersection never given a slot) are reported
 // through the undefined-behaviour flags instead of being guessed.
 #include "sc_ctx.hpp"
+#include "sc_seg.hpp"
 #include "sc_fp64.hpp"
 
 namespace sc {
@@ -123,11 +124,6 @@ __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, in
 // pairs are then ranked by (quantised alpha, pair index), keys that are all
 // distinct.  The result equals a stable sort of the pairs (in pair order) by
 // (edge slot, quantised alpha): ties keep pair order.
-constexpr int SEG_THREAD = 8;      // groups up to this size: a register sorting network per thread
-constexpr int SEG_WAVE = 64;       // up to this size: a bitonic sort across one wave
-constexpr int SEG_LDS_MAX = 4096;  // up to this size: a bitonic sort in one block's LDS
-constexpr int SEG_CTL = 3;         // per side: wave-list count, block-list count, too-large flag
-
 __global__ void k_seg_scatter(int I, const int *__restrict__ edge_of, const int *__restrict__ pj,
                               const int *__restrict__ JX, const int *__restrict__ scan, const int *__restrict__ arr,
                               const int *__restrict__ q, uint64_t *K) {
@@ -137,111 +133,6 @@ __global__ void k_seg_scatter(int I, const int *__restrict__ edge_of, const int 
     if (a < 0) return; // not counted (flagged job)
     const int pos = scan[JX[pj[p]] + edge_of[p]] + a;
     K[pos] = (uint64_t(uint32_t(q[p] % 1000000000)) << 32) | uint32_t(p);
-}
-
-// ascending bitonic network on N register values (indices fixed at compile time)
-template <int N> __device__ inline void reg_bitonic(uint64_t (&v)[N]) {
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1)
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1)
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const uint64_t x = v[i], y = v[l];
-                    const bool sw = (i & k) == 0 ? x > y : x < y;
-                    v[i] = sw ? y : x;
-                    v[l] = sw ? x : y;
-                }
-            }
-}
-
-// one thread per edge group: groups of up to SEG_THREAD sorted in registers,
-// larger ones listed for a wave or a block, groups beyond SEG_LDS_MAX reported
-__global__ void k_seg_small(int nseg, const int *__restrict__ scan, const uint64_t *__restrict__ K, int *ord,
-                            uint32_t *wlist, uint32_t *blist, uint32_t *ctl) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nseg) return;
-    const int s = scan[g], n = scan[g + 1] - s;
-    if (n <= 0) return;
-    if (n == 1) {
-        ord[s] = int(uint32_t(K[s]));
-        return;
-    }
-    if (n > SEG_THREAD) {
-        if (n <= SEG_WAVE) wlist[atomicAdd(&ctl[0], 1u)] = uint32_t(g);
-        else if (n <= SEG_LDS_MAX) blist[atomicAdd(&ctl[1], 1u)] = uint32_t(g);
-        else atomicOr(&ctl[2], 1u);
-        return;
-    }
-    uint64_t v[SEG_THREAD];
-#pragma unroll
-    for (int i = 0; i < SEG_THREAD; ++i) v[i] = i < n ? K[s + i] : ~0ull; // keys are below ~0
-    reg_bitonic<SEG_THREAD>(v);
-#pragma unroll
-    for (int i = 0; i < SEG_THREAD; ++i)
-        if (i < n) ord[s + i] = int(uint32_t(v[i]));
-}
-
-__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m) {
-    const int lo = __shfl_xor(int(uint32_t(v)), m, WAVE), hi = __shfl_xor(int(uint32_t(v >> 32)), m, WAVE);
-    return (uint64_t(uint32_t(hi)) << 32) | uint32_t(lo);
-}
-
-// one wave per listed group of SEG_THREAD+1..SEG_WAVE pairs, one key per lane
-__global__ __launch_bounds__(BLK) void k_seg_wave(const int *__restrict__ scan, const uint64_t *__restrict__ K,
-                                                  int *ord, const uint32_t *__restrict__ wlist,
-                                                  const uint32_t *__restrict__ ctl) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint32_t nw = ctl[0];
-    const uint32_t stride = gridDim.x * (BLK / WAVE);
-    for (uint32_t w = blockIdx.x * (BLK / WAVE) + threadIdx.x / WAVE; w < nw; w += stride) {
-        const int g = int(wlist[w]);
-        const int s = scan[g], n = scan[g + 1] - s;
-        uint64_t v = lane < n ? K[s + lane] : ~0ull;
-#pragma unroll
-        for (int k = 2; k <= WAVE; k <<= 1)
-#pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                const uint64_t o = shfl_xor_u64(v, j);
-                const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-                v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
-            }
-        if (lane < n) ord[s + lane] = int(uint32_t(v));
-    }
-}
-
-// one block per listed group of SEG_WAVE+1..SEG_LDS_MAX pairs: bitonic sort in LDS
-__global__ __launch_bounds__(BLK) void k_seg_block(const int *__restrict__ scan, const uint64_t *__restrict__ K,
-                                                   int *ord, const uint32_t *__restrict__ blist,
-                                                   const uint32_t *__restrict__ ctl) {
-    __shared__ uint64_t sk[SEG_LDS_MAX];
-    const uint32_t nb = ctl[1];
-    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
-        const int g = int(blist[b]);
-        const int s = scan[g], n = scan[g + 1] - s;
-        int m = SEG_WAVE * 2;
-        while (m < n) m <<= 1;
-        for (int i = threadIdx.x; i < m; i += BLK) sk[i] = i < n ? K[s + i] : ~0ull;
-        __syncthreads();
-        for (int k = 2; k <= m; k <<= 1)
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                // the m/2 compare-exchange pairs (i, i + j), i with bit j clear
-                for (int h = threadIdx.x; h < (m >> 1); h += BLK) {
-                    const int i = ((h & ~(j - 1)) << 1) | (h & (j - 1));
-                    const int l = i + j;
-                    const uint64_t x = sk[i], y = sk[l];
-                    if ((i & k) == 0 ? x > y : x < y) {
-                        sk[i] = y;
-                        sk[l] = x;
-                    }
-                }
-                __syncthreads();
-            }
-        for (int i = threadIdx.x; i < n; i += BLK) ord[s + i] = int(uint32_t(sk[i]));
-        __syncthreads();
-    }
 }
 
 // keys for the per-edge alpha order: the edge's slot over the batch above
@@ -643,11 +534,7 @@ void post_jobs(Ctx &c, int k0, int k1) {
             // listed groups hold more than SEG_THREAD (wave list) or SEG_WAVE (block list) pairs each
             const int64_t nw = std::min<int64_t>(nseg, int64_t(I) / (SEG_THREAD + 1)) + 1;
             const int64_t nb = std::min<int64_t>(nseg, int64_t(I) / (SEG_WAVE + 1)) + 1;
-            k_seg_small<<<grid_for(nseg), BLK, 0, s>>>(nseg, scan, K, ord, wl.ensure(size_t(nw)),
-                                                       bl.ensure(size_t(nb)), ctl);
-            k_seg_wave<<<1024, BLK, 0, s>>>(scan, K, ord, wl, ctl);
-            k_seg_block<<<1280, BLK, 0, s>>>(scan, K, ord, bl, ctl); // five 32 KB blocks per CU
-            SC_HIP(hipGetLastError());
+            seg_sort<false>(nseg, scan, K, ord, wl.ensure(size_t(nw)), bl.ensure(size_t(nb)), ctl, s);
         };
         if (sparse) radix_order(true);
         else seg_order(c.p_b, ja.JB, all_b, c.arr_b, c.p_qa, nJB, c.seg_kb, c.seg_wb, c.seg_bb, segctl, c.ordB);

@@ -211,17 +211,37 @@ __global__ void k_node_intervals(const double *__restrict__ ea, int T, int D, do
     niv[2 * v + 1] = ea[first + (1 << sh)];
 }
 
-// ---- A6 leaf end entries ----
-// first i in [1, epc-1] with ea[i] == q, or -1 (intended findBoarder, oacc:420-460)
-__device__ inline int first_point(const double *ea, int epc, double q) {
-    int lo = 1, hi = epc; // search [1, epc)
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (ea[mid] < q) lo = mid + 1;
-        else hi = mid;
+// Two binary searches over a[lo0, n) run interleaved, so that their
+// dependent loads overlap: r1 = first index with a[i] >= q1; r2 = first index
+// with a[i] >= q2, or with a[i] > q2 when UPPER2.
+template <bool UPPER2>
+__device__ inline void search2(const double *a, int lo0, int n, double q1, double q2, int &r1, int &r2) {
+    int l1 = lo0, h1 = n, l2 = lo0, h2 = n;
+    while (l1 < h1 || l2 < h2) {
+        const int m1 = (l1 + h1) >> 1, m2 = (l2 + h2) >> 1;
+        const bool a1 = l1 < h1, a2 = l2 < h2;
+        const double v1 = a1 ? a[m1] : 0.0, v2 = a2 ? a[m2] : 0.0;
+        if (a1) {
+            if (v1 < q1) l1 = m1 + 1;
+            else h1 = m1;
+        }
+        if (a2) {
+            if (UPPER2 ? v2 <= q2 : v2 < q2) l2 = m2 + 1;
+            else h2 = m2;
+        }
     }
-    if (lo < epc && ea[lo] == q) return lo;
-    return -1;
+    r1 = l1;
+    r2 = l2;
+}
+
+// ---- A6 leaf end entries ----
+// first i in [1, epc-1] with ea[i] == q, or -1 (intended findBoarder,
+// oacc:420-460), for two queries at once
+__device__ inline void first_point2(const double *ea, int epc, double q1, double q2, int &r1, int &r2) {
+    int lo1, lo2;
+    search2<false>(ea, 1, epc, q1, q2, lo1, lo2); // search [1, epc)
+    r1 = lo1 < epc && ea[lo1] == q1 ? lo1 : -1;
+    r2 = lo2 < epc && ea[lo2] == q2 ? lo2 : -1;
 }
 
 
@@ -234,13 +254,13 @@ __global__ void k_leaf_targets(const double *__restrict__ es, const double *__re
     if (e >= ne) return;
     const int half = nleaves, T = 2 * nleaves;
     int a = -1, b = -1;
-    int ie = first_point(ea, epc, ee[e]);
+    int ie, is;
+    first_point2(ea, epc, ee[e], es[e], ie, is);
     if (ie >= 0) {
         int rl = half + ie;
         if (ie != 1 && ie + 1 <= epc - 1 && ea[ie] == ea[ie + 1]) rl = half + ie + 1;
         if (rl < T && ea[rl - half] != ea[rl - half + 1]) a = rl;
     }
-    int is = first_point(ea, epc, es[e]);
     if (is >= 0) {
         int l = is - 1;
         if (ea[l] != ea[l + 1]) b = half + l;
@@ -263,25 +283,6 @@ __global__ void k_key_hist(const uint64_t *__restrict__ keys, int64_t n, uint64_
 }
 
 // ---- A5 cover lists ----
-__device__ inline int lower_bound_d(const double *a, int n, double q) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (a[mid] < q) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-__device__ inline int upper_bound_d(const double *a, int n, double q) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (a[mid] <= q) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // Cover list of an edge: the canonical decomposition of its leaf range
 // [L,R) (L = first elementary point >= start, R = last point <= end), i.e. the
 // nodes v with Int(v) inside [start,end] whose parent's interval is not
@@ -295,8 +296,8 @@ __global__ void k_cover(const double *__restrict__ es, const double *__restrict_
     if (e >= ne) return;
     int L, R;
     if (!EMIT) {
-        L = lower_bound_d(ea, epc, es[e]);
-        R = upper_bound_d(ea, epc, ee[e]) - 1; // last point <= end
+        search2<true>(ea, 0, epc, es[e], ee[e], L, R);
+        R -= 1; // last point <= end
         lr[e] = make_int2(L, R);
     } else {
         int2 v = lr[e];
@@ -323,19 +324,6 @@ __global__ void k_cover(const double *__restrict__ es, const double *__restrict_
         }
     }
     if (!EMIT) cnt[e] = n;
-}
-
-// per node: end of its entries and of its base entries in the (node,
-// class)-sorted keys, marked at the last entry of each run
-__global__ void k_cover_ends(const uint64_t *__restrict__ keys, int64_t n, uint32_t *end, uint32_t *bend) {
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t k = keys[i] >> 32;
-    const bool last = i + 1 == n || (keys[i + 1] >> 32) != k;
-    if (!last) return;
-    const uint32_t v = uint32_t(k >> 1);
-    if (i + 1 == n || (keys[i + 1] >> 33) != v) end[v] = uint32_t(i + 1);
-    if ((k & 1ull) == 0) bend[v] = uint32_t(i + 1);
 }
 
 __global__ void k_cover_split(int T, const uint32_t *__restrict__ off, const uint32_t *__restrict__ bend,
@@ -585,14 +573,23 @@ __global__ void k_entry_before(const uint32_t *__restrict__ e, int64_t n, const 
     bl[i] = before[k];
     yr[i] = eyr[k];
 }
-// cover entries from their sorted packed keys: the edge (low word) and its y-extent
+// cover entries from their sorted packed keys: the edge (low word) and its
+// y-extent; at the last entry of a (node, class) run, the node's end and (for
+// the base class) its base end
 __global__ void k_cover_entries(const uint64_t *__restrict__ keys, int64_t n, const YRangeF *__restrict__ eyr,
-                                uint32_t *ce, YRangeF *yr) {
+                                uint32_t *ce, YRangeF *yr, uint32_t *end, uint32_t *bend) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t e = uint32_t(keys[i]);
+    const uint64_t k = keys[i];
+    const uint32_t e = uint32_t(k);
     ce[i] = e;
     yr[i] = eyr[e];
+    const uint64_t hk = k >> 32;
+    const uint64_t nk = i + 1 == n ? ~0ull : keys[i + 1] >> 32;
+    if (nk == hk) return;
+    const uint32_t v = uint32_t(hk >> 1);
+    if (i + 1 == n || (nk >> 1) != v) end[v] = uint32_t(i + 1);
+    if ((hk & 1ull) == 0) bend[v] = uint32_t(i + 1);
 }
 } // namespace
 
@@ -656,9 +653,6 @@ void build_cover_lists(Ctx &c) {
         const int kb = ilog2_exact(T) + 1;
         SC_REQUIRE(kb <= 32, SC_ERANGE, "cover key too wide");
         radix_sort_u64_buf(c.cov_key, c.cov_key_alt, nullptr, nullptr, L, 32, 32 + kb, c.rs, s);
-        keys = c.cov_key.p;
-        k_cover_entries<<<grid_for(L), BLK, 0, s>>>(keys, L, c.eyr, c.cov_e, c.cov_yr.ensure(size_t(L)));
-        SC_HIP(hipGetLastError());
     }
     uint32_t *off = c.cov_off.ensure(size_t(T) + 1);
     uint32_t *split = c.cov_split.ensure(T);
@@ -671,7 +665,8 @@ void build_cover_lists(Ctx &c) {
         fb.flush();
     }
     if (L > 0) {
-        k_cover_ends<<<grid_for(L), BLK, 0, s>>>(keys, L, off + 1, bend);
+        k_cover_entries<<<grid_for(L), BLK, 0, s>>>(c.cov_key, L, c.eyr, c.cov_e, c.cov_yr.ensure(size_t(L)), off + 1,
+                                                    bend);
         SC_HIP(hipGetLastError());
     }
     scan_inclusive_max_u32(off + 1, off + 1, T, s);
