@@ -18,14 +18,15 @@
 //    crossing vertex to the next crossing vertex in the walking direction,
 //    then jumps to the neighbour of that vertex on the other ring.  Pieces
 //    form cycles; a cycle's output component is started at its base-ring
-//    vertex that comes first in base-ring order (pointer jumping for the
-//    minimum), and the pieces of a component are laid out by list ranking;
+//    vertex that comes first in base-ring order, and the pieces of a
+//    component are laid out by their distance from that start (one doubling
+//    pass finds both);
 //  - no crossing (same conditions, no crossing vertex): each ring is output
 //    when its first plain vertex lies inside the other ring;
 //  - sequential: everything else -- intersection chains (LEFT_ON ...
 //    ON_RIGHT), split bouncing pairs, rings made only of intersections,
 //    inconsistent labels -- runs the reference's sequential algorithm, one
-//    device thread per such job.
+//    device wave per such job.
 // The outputs of all jobs are laid out job after job, then cleaned up.
 //
 // Clean-up: the reference removes, in one pass from the root, every vertex
@@ -2000,7 +2001,7 @@ void label_and_traverse(Ctx &c) {
         SC_HIP(hipGetLastError());
         SC_HIP(hipEventRecord(c.ev_join, c.stream2));
     }
-    // cycles of the crossing jobs: minimum by pointer jumping, then list ranking
+    // cycles of the crossing jobs: minimum and distance to it by doubling
     uint32_t *cmin = nullptr, *dist = nullptr, *inode = nullptr;
     uint32_t *base = f.tot.ensure(int64_t(mP) + 1), *ridx = f.ringidx.ensure(int64_t(mP) + 1);
     if (maxm > 0) {
