@@ -17,6 +17,42 @@ Ctx::~Ctx() {
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
+    if (pin) (void)hipHostFree(pin);
+}
+void *Ctx::pinned(size_t bytes) {
+    if (bytes > pin_cap) {
+        if (pin) SC_HIP(hipHostFree(pin));
+        pin = nullptr;
+        pin_cap = 0;
+        const size_t want = bytes < 4096 ? 4096 : bytes + bytes / 4;
+        SC_HIP(hipHostMalloc(&pin, want, hipHostMallocDefault));
+        pin_cap = want;
+    }
+    return pin;
+}
+void Ctx::wait() {
+    for (;;) {
+        const hipError_t e = hipStreamQuery(stream);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) SC_HIP(e);
+    }
+}
+void ReadBack::run() {
+    size_t tot = 0;
+    for (int i = 0; i < n; ++i) tot += (bytes[i] + 15) & ~size_t(15);
+    char *st = static_cast<char *>(c.pinned(tot));
+    size_t at = 0;
+    for (int i = 0; i < n; ++i) {
+        if (bytes[i]) SC_HIP(hipMemcpyAsync(st + at, src[i], bytes[i], hipMemcpyDeviceToHost, c.stream));
+        at += (bytes[i] + 15) & ~size_t(15);
+    }
+    c.wait();
+    at = 0;
+    for (int i = 0; i < n; ++i) {
+        if (bytes[i]) std::memcpy(dst[i], st + at, bytes[i]);
+        at += (bytes[i] + 15) & ~size_t(15);
+    }
+    n = 0;
 }
 void Ctx::ensure_side_stream() {
     if (stream2) return;

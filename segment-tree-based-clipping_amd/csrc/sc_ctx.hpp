@@ -153,6 +153,12 @@ struct Ctx {
     hipEvent_t ev[10] = {};
     bool ev_ready = false;
     void ensure_events();
+    // pinned host staging for read-backs, and a wait that spins on the
+    // stream (no sleep between the last kernel and the next launch)
+    void *pin = nullptr;
+    size_t pin_cap = 0;
+    void *pinned(size_t bytes);
+    void wait();
     void mark(int i) {
         if (timing) SC_HIP(hipEventRecord(ev[i], stream));
     }
@@ -201,6 +207,24 @@ void init_labels_one(Ctx &c, int nba, int nca, const double *bx, const double *b
                      const double *cx, const double *cy, int *labb, int *labc, int *flags);
 void release_foster_scratch();
 
+// Device-to-host read-backs through the context's pinned staging, one wait
+// for all of them: add() up to RB_MAX pieces, then run().
+constexpr int RB_MAX = 4;
+struct ReadBack {
+    Ctx &c;
+    int n = 0;
+    void *dst[RB_MAX];
+    const void *src[RB_MAX];
+    size_t bytes[RB_MAX];
+    explicit ReadBack(Ctx &ctx) : c(ctx) {}
+    template <class T> void add(T *h, const T *d, size_t count) {
+        dst[n] = h;
+        src[n] = d;
+        bytes[n] = sizeof(T) * count;
+        ++n;
+    }
+    void run();
+};
 uint32_t read_u32(Ctx &c, const uint32_t *d);
 uint64_t read_u64(Ctx &c, const uint64_t *d);
 

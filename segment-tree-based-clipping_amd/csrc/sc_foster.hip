@@ -1606,8 +1606,9 @@ FosterScratch &fscratch() {
 }
 
 template <class T> void d2h_sync(Ctx &c, T *h, const T *d, int64_t n) {
-    SC_HIP(hipMemcpyAsync(h, d, sizeof(T) * size_t(n), hipMemcpyDeviceToHost, c.stream));
-    SC_HIP(hipStreamSynchronize(c.stream));
+    ReadBack rb(c);
+    rb.add(h, d, size_t(n));
+    rb.run();
 }
 
 // clean-up of nr rings laid out at off[0..nr] in (ox, oy, osrc); result into
@@ -1709,9 +1710,12 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
                                               c.d_ring_src);
     k_ring_off_final<<<grid_for(nr), BLK, 0, s>>>(nr, rcnt, nullptr, nonempty, f.off, kscan, off2);
     SC_HIP(hipGetLastError());
-    int last = int(nout);
-    SC_HIP(hipMemcpyAsync(off2 + c.nrings, &last, sizeof(int), hipMemcpyHostToDevice, s));
-    SC_HIP(hipStreamSynchronize(s));
+    {
+        FillBatch fb(s);
+        fb.add(off2 + c.nrings, 1, uint32_t(int(nout)));
+        fb.flush();
+    }
+    c.wait();
 }
 
 } // namespace
@@ -2112,11 +2116,17 @@ void label_and_traverse(Ctx &c) {
         k_jseq_copy<<<nseq, BLK, 0, s>>>(f.sjobs, f.sres, jv, jr, f.sox, f.soy, f.sosrc, f.sroff, f.ox, f.oy,
                                           f.osrc, off);
         SC_HIP(hipGetLastError());
-        SC_HIP(hipMemcpyAsync(hres.data(), f.sres.p, sizeof(int) * 3 * size_t(nseq), hipMemcpyDeviceToHost, s));
     }
-    int last = int(NV);
-    SC_HIP(hipMemcpyAsync(off + NR, &last, sizeof(int), hipMemcpyHostToDevice, s));
-    SC_HIP(hipStreamSynchronize(s));
+    {
+        FillBatch fb(s);
+        fb.add(off + NR, 1, uint32_t(int(NV)));
+        fb.flush();
+    }
+    if (nseq > 0) {
+        ReadBack rb(c);
+        rb.add(hres.data(), static_cast<const int *>(f.sres.p), 3 * size_t(nseq));
+        rb.run();
+    }
     for (int q = 0; q < nseq; ++q) {
         js.flags[sj[q].j] |= hres[3 * q + 2];
         c.undefined |= hres[3 * q + 2];

@@ -578,11 +578,14 @@ void post_jobs(Ctx &c, int k0, int k1) {
     scan_exclusive_u32(reinterpret_cast<uint32_t *>(nca), reinterpret_cast<uint32_t *>(nca), J + 1, tot + 1, s);
     js.AB.resize(J + 1);
     js.AC.resize(J + 1);
-    SC_HIP(hipMemcpyAsync(js.AB.data(), nba, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipMemcpyAsync(js.AC.data(), nca, sizeof(int) * (J + 1), hipMemcpyDeviceToHost, s));
     uint32_t hseg[2 * SEG_CTL] = {};
-    if (I > 0) SC_HIP(hipMemcpyAsync(hseg, segctl, sizeof(hseg), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipStreamSynchronize(s));
+    {
+        ReadBack rb(c);
+        rb.add(js.AB.data(), static_cast<const int *>(nba), size_t(J) + 1);
+        rb.add(js.AC.data(), static_cast<const int *>(nca), size_t(J) + 1);
+        if (I > 0) rb.add(hseg, static_cast<const uint32_t *>(segctl), size_t(2 * SEG_CTL));
+        rb.run();
+    }
     // a group too large for LDS: that side's order by the radix sort instead
     if (hseg[2] && !sparse) radix_order(true);
     if (hseg[SEG_CTL + 2]) radix_order(false);
@@ -634,8 +637,11 @@ void post_jobs(Ctx &c, int k0, int k1) {
         }
     }
     js.flags.resize(J);
-    SC_HIP(hipMemcpyAsync(js.flags.data(), flags, sizeof(int) * J, hipMemcpyDeviceToHost, s));
-    SC_HIP(hipStreamSynchronize(s));
+    {
+        ReadBack rb(c);
+        rb.add(js.flags.data(), static_cast<const int *>(flags), size_t(J));
+        rb.run();
+    }
     int any = 0;
     for (int f : js.flags) any |= f;
     c.undefined = any;

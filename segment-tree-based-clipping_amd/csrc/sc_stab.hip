@@ -435,9 +435,12 @@ void find_candidates(Ctx &c) {
     scan_exclusive_u64(tests, tests, int64_t(T) + 1, tot64, s);
     uint32_t nunits = 0;
     uint64_t ntests = 0;
-    SC_HIP(hipMemcpyAsync(&nunits, tot, sizeof(nunits), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipMemcpyAsync(&ntests, tot64, sizeof(ntests), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipStreamSynchronize(s)); // one wait for both totals
+    {
+        ReadBack rb(c); // one wait for both totals
+        rb.add(&nunits, tot, 1);
+        rb.add(&ntests, tot64, 1);
+        rb.run();
+    }
     c.n_tests = int64_t(ntests);
     c.n_units = nunits;
     uint32_t *unode = c.u32c.ensure(size_t(nunits) + 1);
@@ -505,8 +508,11 @@ void find_candidates(Ctx &c) {
     SC_HIP(hipGetLastError());
     // koff[k] = first pair of overlay k (k = 1..K+1): pairs of overlay k are [koff[k], koff[k+1])
     std::vector<uint32_t> h(size_t(c.K) + 2);
-    SC_HIP(hipMemcpyAsync(h.data(), koff, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipStreamSynchronize(s));
+    {
+        ReadBack rb(c);
+        rb.add(h.data(), koff, h.size());
+        rb.run();
+    }
     c.pair_off.assign(size_t(c.K) + 1, 0);
     for (int k = 1; k <= c.K; ++k) c.pair_off[k] = h[k + 1];
 }

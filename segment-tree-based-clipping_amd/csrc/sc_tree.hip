@@ -377,14 +377,16 @@ __global__ void k_mend_scatter(const uint64_t *__restrict__ k, const uint32_t *_
 
 uint32_t read_u32(Ctx &c, const uint32_t *d) {
     uint32_t h = 0;
-    SC_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c.stream));
-    SC_HIP(hipStreamSynchronize(c.stream));
+    ReadBack rb(c);
+    rb.add(&h, d, 1);
+    rb.run();
     return h;
 }
 uint64_t read_u64(Ctx &c, const uint64_t *d) {
     uint64_t h = 0;
-    SC_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c.stream));
-    SC_HIP(hipStreamSynchronize(c.stream));
+    ReadBack rb(c);
+    rb.add(&h, d, 1);
+    rb.run();
     return h;
 }
 
