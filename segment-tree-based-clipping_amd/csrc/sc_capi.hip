@@ -73,6 +73,7 @@ float Ctx::elapsed(int a, int b) {
 
 void release_derived(Ctx &c) {
     SC_HIP(hipStreamSynchronize(c.stream));
+    fetch_flags(c); // the batch status stays readable after a release
     for (DBuf<double> *b : {&c.ex0, &c.ey0, &c.ex1, &c.ey1, &c.estart, &c.eend, &c.uxs, &c.uxe, &c.eymin, &c.eymax,
                             &c.ea, &c.niv, &c.p_x, &c.p_y, &c.aug_bx, &c.aug_by, &c.aug_cx, &c.aug_cy})
         b->release();
@@ -329,6 +330,7 @@ int segclip_reset(segclip_ctx *h) {
         x.jobs.J = 0;
         x.jobs.AB.clear();
         x.jobs.ring_off.clear();
+        x.flags_pending = false;
     });
 }
 int segclip_copy_batch_rings(segclip_ctx *h, int32_t *job_ring_off, int64_t cap, int64_t *need) {
@@ -352,6 +354,8 @@ int segclip_copy_batch_routes(segclip_ctx *h, int32_t *route, int32_t *reason, i
 }
 int segclip_copy_batch_status(segclip_ctx *h, int32_t *status, int64_t cap, int64_t *need) {
     if (!h) return SEGCLIP_EINVAL;
+    int frc = guarded(h, [&](sc::Ctx &x) { sc::fetch_flags(x); });
+    if (frc != SEGCLIP_OK) return frc;
     const sc::JobSet &js = h->c.jobs;
     if (int(js.flags.size()) != js.J || js.J < 1) return SEGCLIP_ESTATE;
     int rc = need_space(js.J, cap, need);
@@ -362,6 +366,8 @@ int segclip_copy_batch_status(segclip_ctx *h, int32_t *status, int64_t cap, int6
 
 int segclip_get_counts(segclip_ctx *h, segclip_counts *o) {
     if (!h || !o) return SEGCLIP_EINVAL;
+    int frc = guarded(h, [&](sc::Ctx &x) { sc::fetch_flags(x); });
+    if (frc != SEGCLIP_OK) return frc;
     const sc::Ctx &c = h->c;
     std::memset(o, 0, sizeof(*o));
     o->num_edges = c.ne;

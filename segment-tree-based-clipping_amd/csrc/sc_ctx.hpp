@@ -155,6 +155,7 @@ struct Ctx {
     void ensure_events();
     // pinned host staging for read-backs, and a wait that spins on the
     // stream (no sleep between the last kernel and the next launch)
+    bool flags_pending = false; // jobs.flags / undefined still on the device
     void *pin = nullptr;
     size_t pin_cap = 0;
     void *pinned(size_t bytes);
@@ -180,6 +181,9 @@ void materialize_end_lists(Ctx &c);
 void build_cover_lists(Ctx &c);
 void find_candidates(Ctx &c);          // all overlays at once, then unique pairs
 void post_jobs(Ctx &c, int k0, int k1); // A11-A15 for overlays k0..k1 in one batch
+// the batch's per-job flags (jobs.flags, undefined) from the device, when the
+// post stage left them there (labelling fetches them with its first read-back)
+void fetch_flags(Ctx &c);
 SparseP sparse_view(Ctx &c);           // the sparse base ring of the last batch
 void post_overlay(Ctx &c, int k);      // A11-A15 for overlay k
 void label_and_traverse(Ctx &c);       // A16-A17 for the jobs of the last post_jobs

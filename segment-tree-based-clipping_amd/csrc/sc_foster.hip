@@ -1819,7 +1819,19 @@ void label_and_traverse(Ctx &c) {
     k_jdecide<<<grid_for(J + 1), BLK, 0, s>>>(st, ccP, ccQ);
     SC_HIP(hipGetLastError());
     uint32_t htot[2];
-    d2h_sync(c, htot, tot, 2);
+    {
+        ReadBack rb(c); // with the post stage's job flags, when still on the device
+        rb.add(htot, static_cast<const uint32_t *>(tot), 2);
+        const bool fl = c.flags_pending;
+        if (fl) rb.add(js.flags.data(), static_cast<const int *>(c.flags.p), size_t(J));
+        rb.run();
+        if (fl) {
+            c.flags_pending = false;
+            int any = 0;
+            for (int v : js.flags) any |= v;
+            c.undefined = any;
+        }
+    }
     const int mP = int(htot[0]), mQ = int(htot[1]);
     const int M = mP + mQ;
     // entry/exit status, split pairs and pieces of the crossing jobs

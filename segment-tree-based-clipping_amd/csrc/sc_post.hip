@@ -636,12 +636,17 @@ void post_jobs(Ctx &c, int k0, int k1) {
             SC_HIP(hipGetLastError());
         }
     }
-    js.flags.resize(J);
-    {
-        ReadBack rb(c);
-        rb.add(js.flags.data(), static_cast<const int *>(flags), size_t(J));
-        rb.run();
-    }
+    js.flags.assign(size_t(J), 0);
+    c.flags_pending = true; // read back later, with the next read-back that needs a wait anyway
+}
+
+void fetch_flags(Ctx &c) {
+    if (!c.flags_pending) return;
+    JobSet &js = c.jobs;
+    ReadBack rb(c);
+    rb.add(js.flags.data(), static_cast<const int *>(c.flags.p), size_t(js.J));
+    rb.run();
+    c.flags_pending = false;
     int any = 0;
     for (int f : js.flags) any |= f;
     c.undefined = any;

@@ -391,10 +391,12 @@ __global__ void k_compact_unique(const uint64_t *__restrict__ k, const uint32_t 
 }
 
 // first unique pair of every overlay k (pairs are sorted by overlay first)
-__global__ void k_overlay_offsets(const uint64_t *__restrict__ pairs, int64_t n, int K, int sh, uint32_t *off) {
+// koff[k] = first pair of overlay k; the pair count is read on the device
+__global__ void k_overlay_offsets(const uint64_t *__restrict__ pairs, const uint32_t *__restrict__ npairs, int K,
+                                  int sh, uint32_t *off) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > K + 1) return;
-    int64_t lo = 0, hi = n;
+    int64_t lo = 0, hi = int64_t(*npairs);
     while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
         if (int64_t(pairs[mid] >> sh) < k) lo = mid + 1;
@@ -497,22 +499,26 @@ void find_candidates(Ctx &c) {
         SC_HIP(hipGetLastError());
     }
     scan_exclusive_u32(flag, flag, n + 1, tot, s);
-    c.n_pairs = read_u32(c, tot);
-    c.pairs.ensure(size_t(c.n_pairs));
+    // unique pairs compacted into a buffer sized for all candidates; their
+    // count and the per-overlay offsets come back in one read-back
+    c.pairs.ensure(size_t(n > 0 ? n : 1));
     uint32_t *koff = c.d_pair_off.ensure(size_t(c.K) + 2);
     if (n > 0) {
         k_compact_unique<<<grid_for(n), BLK, 0, s>>>(c.cand, flag, n, c.pairs);
         SC_HIP(hipGetLastError());
     }
-    k_overlay_offsets<<<grid_for(int64_t(c.K) + 2), BLK, 0, s>>>(c.pairs, c.n_pairs, c.K, c.bits_b + c.bits_c, koff);
+    k_overlay_offsets<<<grid_for(int64_t(c.K) + 2), BLK, 0, s>>>(c.pairs, tot, c.K, c.bits_b + c.bits_c, koff);
     SC_HIP(hipGetLastError());
     // koff[k] = first pair of overlay k (k = 1..K+1): pairs of overlay k are [koff[k], koff[k+1])
     std::vector<uint32_t> h(size_t(c.K) + 2);
+    uint32_t np = 0;
     {
         ReadBack rb(c);
-        rb.add(h.data(), koff, h.size());
+        rb.add(h.data(), static_cast<const uint32_t *>(koff), h.size());
+        rb.add(&np, static_cast<const uint32_t *>(tot), 1);
         rb.run();
     }
+    c.n_pairs = np;
     c.pair_off.assign(size_t(c.K) + 1, 0);
     for (int k = 1; k <= c.K; ++k) c.pair_off[k] = h[k + 1];
 }
