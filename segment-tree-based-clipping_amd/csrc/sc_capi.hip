@@ -17,18 +17,25 @@ Ctx::~Ctx() {
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
-    if (pin) (void)hipHostFree(pin);
 }
+namespace {
+struct PinnedStage {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+thread_local PinnedStage tl_pin; // never freed: a read-back has finished with it when it returns
+} // namespace
 void *Ctx::pinned(size_t bytes) {
-    if (bytes > pin_cap) {
-        if (pin) SC_HIP(hipHostFree(pin));
-        pin = nullptr;
-        pin_cap = 0;
-        const size_t want = bytes < 4096 ? 4096 : bytes + bytes / 4;
-        SC_HIP(hipHostMalloc(&pin, want, hipHostMallocDefault));
-        pin_cap = want;
+    PinnedStage &ps = tl_pin;
+    if (bytes > ps.cap) {
+        if (ps.p) SC_HIP(hipHostFree(ps.p));
+        ps.p = nullptr;
+        ps.cap = 0;
+        const size_t want = bytes < 65536 ? 65536 : bytes + bytes / 4;
+        SC_HIP(hipHostMalloc(&ps.p, want, hipHostMallocDefault));
+        ps.cap = want;
     }
-    return pin;
+    return ps.p;
 }
 void Ctx::wait() {
     for (;;) {

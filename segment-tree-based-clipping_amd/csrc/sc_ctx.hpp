@@ -153,12 +153,12 @@ struct Ctx {
     hipEvent_t ev[10] = {};
     bool ev_ready = false;
     void ensure_events();
-    // pinned host staging for read-backs, and a wait that spins on the
-    // stream (no sleep between the last kernel and the next launch)
     bool flags_pending = false; // jobs.flags / undefined still on the device
-    void *pin = nullptr;
-    size_t pin_cap = 0;
-    void *pinned(size_t bytes);
+    // pinned host staging for read-backs (one buffer per host thread, kept
+    // for the process: contexts come and go, pinned allocations are slow),
+    // and a wait that spins on the stream (no sleep between the last kernel
+    // and the next launch)
+    static void *pinned(size_t bytes);
     void wait();
     void mark(int i) {
         if (timing) SC_HIP(hipEventRecord(ev[i], stream));
