@@ -275,11 +275,14 @@ __global__ void k_leaf_targets(const double *__restrict__ es, const double *__re
     vals[2 * int64_t(e) + 1] = uint32_t(e);
 }
 
-__global__ void k_key_hist(const uint64_t *__restrict__ keys, int64_t n, uint64_t limit, uint32_t *cnt) {
+// sorted keys: at the last entry of each run of key k < limit, end[k] = its
+// position + 1 (an inclusive max-scan then fills the keys without entries)
+__global__ void k_key_ends(const uint64_t *__restrict__ keys, int64_t n, uint64_t limit, uint32_t *end) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint64_t k = keys[i];
-    if (k < limit) atomicAdd(&cnt[k], 1u);
+    const uint64_t k = keys[i];
+    if (k >= limit) return;
+    if (i + 1 == n || keys[i + 1] != k) end[k] = uint32_t(i + 1);
 }
 
 // ---- A5 cover lists ----
@@ -612,12 +615,16 @@ void build_leaf_end_entries(Ctx &c) {
     keys = c.lend_key.p;
     vals = c.lend_e.p;
     uint32_t *off = c.lend_off.ensure(2 * size_t(nl) + 1);
-    SC_HIP(hipMemsetAsync(off, 0, sizeof(uint32_t) * (2 * size_t(nl) + 1), s));
-    k_key_hist<<<grid_for(m), BLK, 0, s>>>(keys, m, 2ull * uint64_t(nl), off);
+    // off[k] = entries with key < k: run ends of the sorted keys, then a running maximum
+    {
+        FillBatch fb(s);
+        fb.add(off, 2 * int64_t(nl) + 1, 0u);
+        fb.flush();
+    }
+    k_key_ends<<<grid_for(m), BLK, 0, s>>>(keys, m, 2ull * uint64_t(nl), off + 1);
     SC_HIP(hipGetLastError());
-    uint32_t *tot = c.d_u32_total.ensure(1);
-    scan_exclusive_u32(off, off, 2 * int64_t(nl) + 1, tot, s);
-    c.n_lend = read_u32(c, tot);
+    scan_inclusive_max_u32(off + 1, off + 1, 2 * int64_t(nl), s);
+    c.n_lend = read_u32(c, off + 2 * size_t(nl));
     // beside every entry, its edge's "before" leaf (the stab's dedupe test) and y-extent
     int *bl = c.lend_bl.ensure(size_t(c.n_lend > 0 ? c.n_lend : 1));
     YRangeF *bx = c.lend_yr.ensure(size_t(c.n_lend > 0 ? c.n_lend : 1));
