@@ -48,6 +48,8 @@ FLOPS_PER_TEST = 8.0
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "clip_harmonic_200k_1m_hbm_traffic.md")
+# settings chosen by `bench.py --tune` on the plain 200k x 1M clip (segclip.tune)
+TUNING_FILE = os.path.join(ROOT, "segment-tree-based-clipping_amd", "tuning.json")
 
 
 def log(*a):
@@ -218,6 +220,10 @@ def main():
                     help="run only one batched workload (one_to_many, small_pairs, large_pairs) and print its figures")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 launches), to scale HBM counters")
+    ap.add_argument("--tune", action="store_true",
+                    help="time every stab_tiles candidate on the plain clip and write the fastest to "
+                         "segment-tree-based-clipping_amd/tuning.json")
+    ap.add_argument("--no-tuning", action="store_true", help="ignore tuning.json (library defaults)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -248,9 +254,25 @@ def main():
     cl.set_polygons(base, [ov])  # host -> HBM once, outside the timed region
     cl.set_timing(True)
 
-    def step():
-        cl.reset()
-        cl.clip(1)
+    def step(c=cl):
+        c.reset()
+        c.clip(1)
+
+    workload = "plain pairwise clip, harmonic star rings %d x %d (seeds 2001/2002)" % (nb, nc)
+    if args.tune:
+        import torch
+        from segclip import tune
+        best, ms = tune.autotune(cl, step, "stab_tiles", warmup=1, reps=5)
+        rec = tune.save(TUNING_FILE, {"stab_tiles": best}, workload, {"stab_tiles": ms},
+                        torch.cuda.get_device_name(0))
+        print(json.dumps(rec), flush=True)
+        return
+    tuning = None
+    if not args.no_tuning and os.path.exists(TUNING_FILE):
+        from segclip import tune
+        tuning = tune.load(TUNING_FILE)
+        cl.apply_tuning(tuning)
+        log("[bench] tuning %s" % json.dumps(tuning))
 
     for i in range(args.warmup):
         step()
@@ -309,7 +331,8 @@ def main():
         "config": {"workload": "plain pairwise clip, harmonic star rings %d x %d (seeds 2001/2002), "
                                "inputs resident in HBM" % (nb, nc),
                    "pairs_per_s": 1000.0 / ms, "intersections": cnt["overlay_pairs"], "rings": cnt["rings"],
-                   "out_vertices": cnt["out_vertices"], "stage_ms": {k: round(v, 4) for k, v in avg.items()}},
+                   "out_vertices": cnt["out_vertices"], "stage_ms": {k: round(v, 4) for k, v in avg.items()},
+                   "tuning": tuning},
         "roofline": roof, "cpu_baseline": cpu, "batched": batched,
     }
     print(json.dumps(line), flush=True)

@@ -237,6 +237,17 @@ int segclip_count_vertical(const double *xy, int n);
 int segclip_set_timing(segclip_ctx *ctx, int on);
 int segclip_get_stage_times(segclip_ctx *ctx, float *ms8);
 
+/* Tuning settings of a context, by name.  No counterpart in the reference
+ * (its stab runs one tree node per parallel-loop iteration, countIntersectionsParallel,
+ * oacc_segment_tree.cpp:6250-6526); they serve
+ * the autotuner, which times candidate values on a workload and persists the
+ * fastest as JSON (segclip.tune).  Results never depend on a setting.
+ *   "stab_tiles"  64-column tiles per stab work unit (one wave), 1..16,
+ *                 default 2.
+ * An unknown name or a value out of range returns SEGCLIP_EINVAL. */
+int segclip_set_tuning(segclip_ctx *ctx, const char *name, int value);
+int segclip_get_tuning(segclip_ctx *ctx, const char *name, int *value);
+
 #ifdef __cplusplus
 }
 #endif

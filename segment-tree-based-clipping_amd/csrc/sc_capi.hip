@@ -559,6 +559,29 @@ int segclip_set_timing(segclip_ctx *h, int on) {
     });
 }
 
+int segclip_set_tuning(segclip_ctx *h, const char *name, int value) {
+    return guarded(h, [&](sc::Ctx &c) {
+        SC_REQUIRE(name != nullptr, SEGCLIP_EINVAL, "tuning name is null");
+        if (std::strcmp(name, "stab_tiles") == 0) {
+            SC_REQUIRE(value >= 1 && value <= 16, SEGCLIP_EINVAL, "stab_tiles must be in 1..16");
+            c.stab_tiles = value;
+        } else {
+            SC_REQUIRE(false, SEGCLIP_EINVAL, std::string("unknown tuning setting: ") + name);
+        }
+    });
+}
+
+int segclip_get_tuning(segclip_ctx *h, const char *name, int *value) {
+    return guarded(h, [&](sc::Ctx &c) {
+        SC_REQUIRE(name != nullptr && value != nullptr, SEGCLIP_EINVAL, "null argument");
+        if (std::strcmp(name, "stab_tiles") == 0) {
+            *value = c.stab_tiles;
+        } else {
+            SC_REQUIRE(false, SEGCLIP_EINVAL, std::string("unknown tuning setting: ") + name);
+        }
+    });
+}
+
 int segclip_get_stage_times(segclip_ctx *h, float *ms8) {
     if (!h || !ms8) return SEGCLIP_EINVAL;
     const sc::StageTimes &t = h->c.times;

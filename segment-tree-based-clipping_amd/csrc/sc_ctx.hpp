@@ -148,6 +148,9 @@ struct Ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     void ensure_side_stream();
 
+    // tuning: 64-column tiles per stab work unit (1..16); results do not depend on it
+    int stab_tiles = 2;
+
     bool timing = false;
     StageTimes times;
     hipEvent_t ev[10] = {};

@@ -59,33 +59,37 @@ __device__ inline NodeShape node_shape(int v, const uint32_t *cov_off, const uin
 __host__ __device__ inline uint32_t cdiv32(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 // Work units of one rectangle (rows x cols): row blocks of RCH rows; a unit
-// of a full row block covers FTPU 64-column tiles, a unit of the last
-// (partial) row block covers enough tiles that it runs about FTPU*RCH row
+// of a full row block covers g = ftpu 64-column tiles, a unit of the last
+// (partial) row block covers enough tiles that it runs about ftpu*RCH row
 // steps.  Several tiles per unit spread the unit's set-up (its record, its
-// rows) over more tests.
-constexpr uint32_t FTPU = 2;
-__host__ __device__ inline uint32_t tiles_per_unit(uint32_t rows_last) { return FTPU * cdiv32(RCH, rows_last); }
-__host__ __device__ inline uint32_t rect_units(uint32_t rows, uint32_t cols) {
+// rows) over more tests; fewer give more, shorter waves.  ftpu is a tuning
+// setting of the context (1..STAB_TILES_MAX); every value enumerates the same
+// (row, column) rectangle, so the candidate set does not depend on it.
+constexpr uint32_t STAB_TILES_MAX = 16;
+__host__ __device__ inline uint32_t tiles_per_unit(uint32_t rows_last, uint32_t ftpu) {
+    return ftpu * cdiv32(RCH, rows_last);
+}
+__host__ __device__ inline uint32_t rect_units(uint32_t rows, uint32_t cols, uint32_t ftpu) {
     if (!rows || !cols) return 0;
     const uint32_t rb = cdiv32(rows, RCH), ct = cdiv32(cols, WAVE);
     const uint32_t last = rows - (rb - 1) * RCH;
-    return (rb - 1) * cdiv32(ct, FTPU) + cdiv32(ct, tiles_per_unit(last));
+    return (rb - 1) * cdiv32(ct, ftpu) + cdiv32(ct, tiles_per_unit(last, ftpu));
 }
 // unit u of a rectangle -> first row, row count, tile range [t0,t1)
-__host__ __device__ inline void rect_unit(uint32_t rows, uint32_t cols, uint32_t u, uint32_t &r0, uint32_t &nr,
-                                          uint32_t &t0, uint32_t &t1) {
+__host__ __device__ inline void rect_unit(uint32_t rows, uint32_t cols, uint32_t u, uint32_t ftpu, uint32_t &r0,
+                                          uint32_t &nr, uint32_t &t0, uint32_t &t1) {
     const uint32_t rb = cdiv32(rows, RCH), ct = cdiv32(cols, WAVE);
-    const uint32_t upb = cdiv32(ct, FTPU); // units per full row block
+    const uint32_t upb = cdiv32(ct, ftpu); // units per full row block
     const uint32_t full = (rb - 1) * upb;
     uint32_t g;
     if (u < full) {
         r0 = (u / upb) * RCH;
         nr = RCH;
-        t0 = (u % upb) * FTPU;
-        g = FTPU;
+        t0 = (u % upb) * ftpu;
+        g = ftpu;
     } else {
         const uint32_t last = rows - (rb - 1) * RCH;
-        g = tiles_per_unit(last);
+        g = tiles_per_unit(last, ftpu);
         r0 = (rb - 1) * RCH;
         nr = last;
         t0 = (u - full) * g;
@@ -94,15 +98,15 @@ __host__ __device__ inline void rect_unit(uint32_t rows, uint32_t cols, uint32_t
 }
 
 __global__ void k_units(const uint32_t *__restrict__ cov_off, const uint32_t *__restrict__ cov_split,
-                        const uint32_t *__restrict__ lend_off, int nl, int D, int T, uint32_t *units,
-                        uint64_t *tests) {
+                        const uint32_t *__restrict__ lend_off, int nl, int D, int T, uint32_t ftpu,
+                        uint32_t *units, uint64_t *tests) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= T) return;
     uint32_t u = 0;
     uint64_t w = 0;
     if (v >= 1 && cov_off[v + 1] > cov_off[v]) {
         NodeShape s = node_shape(v, cov_off, cov_split, lend_off, nl, D);
-        u = rect_units(s.nbase, s.nc) + rect_units(s.nbase, s.ecn) + rect_units(s.nc, s.ebn);
+        u = rect_units(s.nbase, s.nc, ftpu) + rect_units(s.nbase, s.ecn, ftpu) + rect_units(s.nc, s.ebn, ftpu);
         w = uint64_t(s.nbase) * s.nc + uint64_t(s.nbase) * s.ecn + uint64_t(s.nc) * s.ebn;
     }
     units[v] = u;
@@ -129,13 +133,14 @@ struct __align__(16) UnitDesc {
 // descriptor of every unit from its node's shape (one thread per unit)
 __global__ void k_unit_desc(uint32_t nunits, const uint32_t *__restrict__ unode, const uint32_t *__restrict__ uoff,
                             const uint32_t *__restrict__ cov_off, const uint32_t *__restrict__ cov_split,
-                            const uint32_t *__restrict__ lend_off, int nl, int D, UnitDesc *desc) {
+                            const uint32_t *__restrict__ lend_off, int nl, int D, uint32_t ftpu,
+                            UnitDesc *desc) {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nunits) return;
     const int v = int(unode[w]);
     const NodeShape s = node_shape(v, cov_off, cov_split, lend_off, nl, D);
     uint32_t u = w - uoff[v];
-    const uint32_t n1 = rect_units(s.nbase, s.nc), n2 = rect_units(s.nbase, s.ecn);
+    const uint32_t n1 = rect_units(s.nbase, s.nc, ftpu), n2 = rect_units(s.nbase, s.ecn, ftpu);
     uint32_t kind, rows, cols, row_begin, col_begin;
     if (u < n1) {
         kind = 1; // base cover x overlay cover
@@ -159,7 +164,7 @@ __global__ void k_unit_desc(uint32_t nunits, const uint32_t *__restrict__ unode,
         col_begin = s.eb0;
     }
     uint32_t r0, nrows, t0, t1;
-    rect_unit(rows, cols, u, r0, nrows, t0, t1);
+    rect_unit(rows, cols, u, ftpu, r0, nrows, t0, t1);
     UnitDesc d;
     d.row0 = row_begin + r0;
     d.col0 = col_begin;
@@ -423,7 +428,11 @@ void find_candidates(Ctx &c) {
     SC_REQUIRE(c.bits_b + c.bits_c + c.bits_k <= 64, SC_ERANGE, "pair key does not fit in 64 bits");
     uint32_t *units = c.u32a.ensure(size_t(T) + 1);
     uint64_t *tests = c.u64b.ensure(size_t(T) + 1);
-    k_units<<<grid_for(T), BLK, 0, s>>>(c.cov_off, c.cov_split, c.lend_off, c.nleaves, c.depth, T, units, tests);
+    SC_REQUIRE(c.stab_tiles >= 1 && uint32_t(c.stab_tiles) <= STAB_TILES_MAX, SC_EINVAL,
+               "stab_tiles out of range");
+    const uint32_t ftpu = uint32_t(c.stab_tiles);
+    k_units<<<grid_for(T), BLK, 0, s>>>(c.cov_off, c.cov_split, c.lend_off, c.nleaves, c.depth, T, ftpu, units,
+                                        tests);
     SC_HIP(hipGetLastError());
     {
         FillBatch fb(s);
@@ -455,7 +464,7 @@ void find_candidates(Ctx &c) {
     UnitDesc *desc = reinterpret_cast<UnitDesc *>(c.unit_desc.ensure(size_t(nunits > 0 ? nunits : 1) * 4));
     if (nunits > 0) {
         k_unit_desc<<<grid_for(nunits), BLK, 0, s>>>(nunits, unode, units, c.cov_off, c.cov_split, c.lend_off,
-                                                     c.nleaves, c.depth, desc);
+                                                     c.nleaves, c.depth, ftpu, desc);
         SC_HIP(hipGetLastError());
     }
 

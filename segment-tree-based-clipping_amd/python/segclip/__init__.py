@@ -220,6 +220,22 @@ class Clipper:
     def set_timing(self, on=True):
         self._check(lib().segclip_set_timing(self._h, 1 if on else 0))
 
+    def set_tuning(self, name, value):
+        """A tuning setting of this context (see segclip.tune); results do not depend on it."""
+        self._check(lib().segclip_set_tuning(self._h, name.encode(), int(value)))
+        return self
+
+    def get_tuning(self, name):
+        v = C.c_int(0)
+        self._check(lib().segclip_get_tuning(self._h, name.encode(), C.byref(v)))
+        return v.value
+
+    def apply_tuning(self, settings):
+        """Apply a dict of settings, e.g. the 'settings' of a segclip.tune JSON file."""
+        for k, v in settings.items():
+            self.set_tuning(k, v)
+        return self
+
     def stage_times(self):
         a = (C.c_float * 8)()
         self._check(lib().segclip_get_stage_times(self._h, a))

@@ -28,7 +28,7 @@ SYMBOLS = [
     "segclip_copy_batch_status", "segclip_set_provenance", "segclip_batch_area", "segclip_batch_area_device",
     "segclip_batch_area_backward_device", "segclip_release", "segclip_memory", "segclip_set_allocator", "segclip_save_intersections_ids",
     "segclip_sort_intersections_at_edges", "segclip_copy_intersections_arrays", "segclip_init_labels_arrays",
-    "segclip_copy_batch_routes",
+    "segclip_copy_batch_routes", "segclip_set_tuning", "segclip_get_tuning",
 ]
 
 
@@ -127,6 +127,8 @@ def lib():
         "segclip_count_vertical": (C.c_int, [_dp, C.c_int]),
         "segclip_set_timing": (C.c_int, [_vp, C.c_int]),
         "segclip_get_stage_times": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+        "segclip_set_tuning": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+        "segclip_get_tuning": (C.c_int, [_vp, C.c_char_p, _ip]),
     }
     for name in SYMBOLS:
         fn = getattr(L, name)  # AttributeError here = a declared symbol is not exported

@@ -1,0 +1,93 @@
+"""Autotuner for a context's tuning settings.
+
+A setting changes how the library splits its work, never its results: the
+stab's "stab_tiles" (64-column tiles per work unit, one wave each) changes the
+number and length of the waves, and the candidate list is sorted afterwards,
+so every value yields the same pairs, intersections and rings.  The tuner
+therefore measures speed only; the tests check that results agree across
+values.
+
+Usage: time each candidate value on a workload (a function that runs one clip
+on a Clipper), keep the fastest, and persist it as JSON:
+
+    best, ms = tune.autotune(clipper, step)
+    tune.save(path, {"stab_tiles": best}, workload, {"stab_tiles": ms})
+    clipper.apply_tuning(tune.load(path))
+
+The reference has no counterpart: its stab runs one tree node per
+parallel-loop iteration (oacc_segment_tree.cpp:6250-6526) with nothing to
+choose.
+"""
+
+import json
+import math
+import time
+
+# candidate values per setting, and the library's defaults
+SETTINGS = {"stab_tiles": (1, 2, 4, 8)}
+DEFAULTS = {"stab_tiles": 2}
+LIMITS = {"stab_tiles": (1, 16)}
+
+
+def autotune(clipper, step, name="stab_tiles", candidates=None, warmup=1, reps=3):
+    """Time step(clipper) under each candidate value of one setting.
+
+    Returns (best value, {value: best-of-reps ms}) and leaves the clipper set
+    to the best value.  step must run one whole workload and may return before
+    the device is done; the device is synchronised around every timing.
+    """
+    import torch
+
+    if name not in SETTINGS:
+        raise ValueError("unknown tuning setting %r" % name)
+    candidates = tuple(candidates or SETTINGS[name])
+    lo, hi = LIMITS[name]
+    for v in candidates:
+        if not lo <= int(v) <= hi:
+            raise ValueError("%s=%d is outside %d..%d" % (name, v, lo, hi))
+    ms = {}
+    for v in candidates:
+        clipper.set_tuning(name, v)
+        for _ in range(warmup):
+            step(clipper)
+        torch.cuda.synchronize()
+        best = math.inf
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            step(clipper)
+            torch.cuda.synchronize()
+            best = min(best, 1000.0 * (time.perf_counter() - t0))
+        ms[int(v)] = best
+    choice = min(ms, key=ms.get)
+    clipper.set_tuning(name, choice)
+    return choice, ms
+
+
+def save(path, settings, workload, timings_ms=None, device=None):
+    """Persist chosen settings (and the timings behind them) as JSON."""
+    check(settings)
+    rec = {"settings": {k: int(v) for k, v in settings.items()}, "workload": workload,
+           "device": device, "timings_ms": {k: {str(a): round(b, 4) for a, b in t.items()}
+                                            for k, t in (timings_ms or {}).items()}}
+    with open(path, "w") as f:
+        json.dump(rec, f, indent=1, sort_keys=True)
+        f.write("\n")
+    return rec
+
+
+def load(path):
+    """The settings dict of a tuning file written by save(); validated."""
+    with open(path) as f:
+        rec = json.load(f)
+    settings = {k: int(v) for k, v in rec.get("settings", {}).items()}
+    check(settings)
+    return settings
+
+
+def check(settings):
+    for k, v in settings.items():
+        if k not in LIMITS:
+            raise ValueError("unknown tuning setting %r" % k)
+        lo, hi = LIMITS[k]
+        if not lo <= int(v) <= hi:
+            raise ValueError("%s=%d is outside %d..%d" % (k, v, lo, hi))
