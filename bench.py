@@ -50,6 +50,15 @@ FLOPS_PER_TEST = 8.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "clip_harmonic_200k_1m_hbm_traffic.md")
 # settings chosen by `bench.py --tune` on the plain 200k x 1M clip (segclip.tune)
 TUNING_FILE = os.path.join(ROOT, "segment-tree-based-clipping_amd", "tuning.json")
+USE_TUNING = True
+
+
+def tuning_for(name):
+    """Settings that `bench.py --tune` chose for workload 'name' (None: defaults)."""
+    if not USE_TUNING or not os.path.exists(TUNING_FILE):
+        return None
+    from segclip import tune
+    return tune.load(TUNING_FILE, name) or None
 
 
 def log(*a):
@@ -117,6 +126,8 @@ def bench_one_to_many(steps):
     base, ovs = segclip.synth.one_to_many(100000, 4096, 3000)
     cl = segclip.Clipper().set_polygons(base, ovs)
     cl.set_timing(True)
+    tuning = tuning_for("one_to_many")
+    cl.apply_tuning(tuning or {})
 
     def step():
         cl.reset()
@@ -130,7 +141,8 @@ def bench_one_to_many(steps):
     return {"workload": "one-to-many: harmonic base 100,000 vertices x 4,096 star overlays (8..20,000 vertices)",
             "pairs": len(ovs), "ms_per_batch": 1e3 * dt, "pairs_per_s": len(ovs) / dt,
             "edges_per_s": edges / dt, "plain_pairs_per_s": plain, "plain_sample": n,
-            "speedup_vs_plain": (len(ovs) / dt) / plain, "rings": cnt["rings"], "out_vertices": cnt["out_vertices"]}
+            "speedup_vs_plain": (len(ovs) / dt) / plain, "rings": cnt["rings"], "out_vertices": cnt["out_vertices"],
+            "tuning": tuning}
 
 
 def bench_small_pairs(steps):
@@ -139,6 +151,8 @@ def bench_small_pairs(steps):
     from segclip.autograd import clip_area_iou
     bases, ovs = segclip.synth.small_pairs(16384, 4000)
     cl = segclip.Clipper().set_pairs(bases, ovs)
+    tuning = tuning_for("small_pairs")  # the forward batch (the autograd path keeps the defaults)
+    cl.apply_tuning(tuning or {})
     J = len(bases)
     out = torch.empty((4, J), dtype=torch.float64, device="cuda")
 
@@ -163,7 +177,7 @@ def bench_small_pairs(steps):
     plain, n = plain_loop_rate(bases, ovs, 256)
     return {"workload": "16,384 pairs, 4..64 vertices (convex / star)", "pairs": J,
             "ms_per_batch": 1e3 * dt, "pairs_per_s": J / dt, "fwd_bwd_ms": 1e3 * dtb, "fwd_bwd_pairs_per_s": J / dtb,
-            "plain_pairs_per_s": plain, "plain_sample": n, "speedup_vs_plain": (J / dt) / plain}
+            "plain_pairs_per_s": plain, "plain_sample": n, "speedup_vs_plain": (J / dt) / plain, "tuning": tuning}
 
 
 def bench_large_pairs(steps):
@@ -191,6 +205,8 @@ def bench_large_pairs(steps):
     cur, peak = segclip.memory()
     # forward only, batched, for the same pairs
     cl = segclip.Clipper().set_pairs(bases, ovs)
+    tuning = tuning_for("large_pairs")  # the forward batch (the autograd path keeps the defaults)
+    cl.apply_tuning(tuning or {})
 
     def fwd():
         cl.reset()
@@ -203,7 +219,42 @@ def bench_large_pairs(steps):
     return {"workload": "256 pairs of harmonic stars, 10k..200k vertices", "pairs": J,
             "fwd_bwd_ms": 1e3 * dt, "fwd_bwd_pairs_per_s": J / dt, "fwd_bwd_edges_per_s": edges / dt,
             "peak_hbm_bytes_fwd_bwd": peak, "fwd_ms": 1e3 * dtf, "fwd_pairs_per_s": J / dtf,
-            "plain_fwd_pairs_per_s": plain, "plain_sample": n, "speedup_fwd_vs_plain": (J / dtf) / plain}
+            "plain_fwd_pairs_per_s": plain, "plain_sample": n, "speedup_fwd_vs_plain": (J / dtf) / plain,
+            "tuning": tuning}
+
+
+def run_tuner(nb, nc):
+    """Time every stab_tiles candidate on the plain clip and the three batched
+    forward workloads; write the fastest per workload to TUNING_FILE."""
+    import torch
+    import segclip
+    from segclip import tune
+    entries = {}
+
+    def one(name, text, cl, step, reps):
+        log("[tune] %s" % name)
+        best, ms = tune.autotune(cl, lambda c: step(), "stab_tiles", warmup=1, reps=reps)
+        entries[name] = {"settings": {"stab_tiles": best}, "workload": text, "timings_ms": {"stab_tiles": ms}}
+        log("[tune] %s: %s" % (name, json.dumps(entries[name])))
+        cl.close()
+
+    base = segclip.synth.harmonic(nb, 2001)
+    ov = segclip.synth.harmonic(nc, 2002, 0.15, -0.1, 1.0, 17.0 * math.pi / 180.0)
+    cl = segclip.Clipper().set_polygons(base, [ov])
+    one("plain", "plain pairwise clip, harmonic star rings %d x %d (seeds 2001/2002)" % (nb, nc), cl,
+        lambda: (cl.reset(), cl.clip(1)), 5)
+    base, ovs = segclip.synth.one_to_many(100000, 4096, 3000)
+    cl = segclip.Clipper().set_polygons(base, ovs)
+    one("one_to_many", "harmonic base 100,000 vertices x 4,096 star overlays", cl,
+        lambda: (cl.reset(), cl.clip_batch(1, len(ovs))), 3)
+    bases, ovs = segclip.synth.small_pairs(16384, 4000)
+    cl = segclip.Clipper().set_pairs(bases, ovs)
+    one("small_pairs", "16,384 pairs, 4..64 vertices, forward batch", cl, lambda: (cl.reset(), cl.clip_batch()), 5)
+    bases, ovs = segclip.synth.large_pairs(256, 5000)
+    cl = segclip.Clipper().set_pairs(bases, ovs)
+    one("large_pairs", "256 pairs of harmonic stars, 10k..200k vertices, forward batch", cl,
+        lambda: (cl.reset(), cl.clip_batch()), 3)
+    return tune.save(TUNING_FILE, entries, torch.cuda.get_device_name(0))
 
 
 def main():
@@ -221,10 +272,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 launches), to scale HBM counters")
     ap.add_argument("--tune", action="store_true",
-                    help="time every stab_tiles candidate on the plain clip and write the fastest to "
+                    help="time every stab_tiles candidate on the plain clip and the batched workloads and "
+                         "write the fastest per workload to "
                          "segment-tree-based-clipping_amd/tuning.json")
     ap.add_argument("--no-tuning", action="store_true", help="ignore tuning.json (library defaults)")
     args = ap.parse_args()
+    global USE_TUNING
+    USE_TUNING = not args.no_tuning
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 8
@@ -245,6 +299,9 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.tune:
+        print(json.dumps(run_tuner(args.nb, args.nc)), flush=True)
+        return
     import segclip
     nb, nc = args.nb, args.nc
     log("[bench] generating harmonic rings %d x %d" % (nb, nc))
@@ -254,24 +311,13 @@ def main():
     cl.set_polygons(base, [ov])  # host -> HBM once, outside the timed region
     cl.set_timing(True)
 
-    def step(c=cl):
-        c.reset()
-        c.clip(1)
+    def step():
+        cl.reset()
+        cl.clip(1)
 
-    workload = "plain pairwise clip, harmonic star rings %d x %d (seeds 2001/2002)" % (nb, nc)
-    if args.tune:
-        import torch
-        from segclip import tune
-        best, ms = tune.autotune(cl, step, "stab_tiles", warmup=1, reps=5)
-        rec = tune.save(TUNING_FILE, {"stab_tiles": best}, workload, {"stab_tiles": ms},
-                        torch.cuda.get_device_name(0))
-        print(json.dumps(rec), flush=True)
-        return
-    tuning = None
-    if not args.no_tuning and os.path.exists(TUNING_FILE):
-        from segclip import tune
-        tuning = tune.load(TUNING_FILE)
-        cl.apply_tuning(tuning)
+    tuning = tuning_for("plain")
+    cl.apply_tuning(tuning or {})
+    if tuning:
         log("[bench] tuning %s" % json.dumps(tuning))
 
     for i in range(args.warmup):

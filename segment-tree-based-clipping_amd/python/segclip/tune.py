@@ -11,8 +11,9 @@ Usage: time each candidate value on a workload (a function that runs one clip
 on a Clipper), keep the fastest, and persist it as JSON:
 
     best, ms = tune.autotune(clipper, step)
-    tune.save(path, {"stab_tiles": best}, workload, {"stab_tiles": ms})
-    clipper.apply_tuning(tune.load(path))
+    tune.save(path, {"plain": {"settings": {"stab_tiles": best}, "workload": text,
+                               "timings_ms": {"stab_tiles": ms}}})
+    clipper.apply_tuning(tune.load(path, "plain"))
 
 The reference has no counterpart: its stab runs one tree node per
 parallel-loop iteration (oacc_segment_tree.cpp:6250-6526) with nothing to
@@ -63,23 +64,33 @@ def autotune(clipper, step, name="stab_tiles", candidates=None, warmup=1, reps=3
     return choice, ms
 
 
-def save(path, settings, workload, timings_ms=None, device=None):
-    """Persist chosen settings (and the timings behind them) as JSON."""
-    check(settings)
-    rec = {"settings": {k: int(v) for k, v in settings.items()}, "workload": workload,
-           "device": device, "timings_ms": {k: {str(a): round(b, 4) for a, b in t.items()}
-                                            for k, t in (timings_ms or {}).items()}}
+def save(path, entries, device=None):
+    """Persist chosen settings per named workload as JSON.
+
+    entries maps a workload name to {"settings": {...}, "workload": text,
+    "timings_ms": {setting: {value: ms}}}.
+    """
+    out = {}
+    for name, e in entries.items():
+        check(e["settings"])
+        out[name] = {"settings": {k: int(v) for k, v in e["settings"].items()},
+                     "workload": e.get("workload", ""),
+                     "timings_ms": {k: {str(a): round(b, 4) for a, b in t.items()}
+                                    for k, t in e.get("timings_ms", {}).items()}}
+    rec = {"device": device, "workloads": out}
     with open(path, "w") as f:
         json.dump(rec, f, indent=1, sort_keys=True)
         f.write("\n")
     return rec
 
 
-def load(path):
-    """The settings dict of a tuning file written by save(); validated."""
+def load(path, name):
+    """Settings of workload 'name' in a file written by save(), validated;
+    {} (the library's defaults) when the file has no entry for it."""
     with open(path) as f:
         rec = json.load(f)
-    settings = {k: int(v) for k, v in rec.get("settings", {}).items()}
+    e = rec.get("workloads", {}).get(name)
+    settings = {k: int(v) for k, v in (e or {}).get("settings", {}).items()}
     check(settings)
     return settings
 

@@ -12,11 +12,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_save_load_round_trip(tmp_path):
     p = str(tmp_path / "t.json")
-    rec = tune.save(p, {"stab_tiles": 4}, "w", {"stab_tiles": {1: 5.0, 2: 4.5, 4: 4.25}}, "dev")
-    assert rec["settings"] == {"stab_tiles": 4}
-    assert tune.load(p) == {"stab_tiles": 4}
+    rec = tune.save(p, {"a": {"settings": {"stab_tiles": 4}, "workload": "w",
+                              "timings_ms": {"stab_tiles": {1: 5.0, 2: 4.5, 4: 4.25}}},
+                        "b": {"settings": {"stab_tiles": 1}}}, "dev")
+    assert rec["workloads"]["a"]["settings"] == {"stab_tiles": 4}
+    assert tune.load(p, "a") == {"stab_tiles": 4}
+    assert tune.load(p, "b") == {"stab_tiles": 1}
+    assert tune.load(p, "missing") == {}
     with open(p) as f:
-        assert json.load(f)["timings_ms"]["stab_tiles"]["4"] == 4.25
+        assert json.load(f)["workloads"]["a"]["timings_ms"]["stab_tiles"]["4"] == 4.25
 
 
 @pytest.mark.parametrize("bad", [{"stab_tiles": 0}, {"stab_tiles": 17}, {"nope": 2}])
@@ -25,9 +29,9 @@ def test_invalid_settings_refused(tmp_path, bad):
         tune.check(bad)
     p = str(tmp_path / "bad.json")
     with open(p, "w") as f:
-        json.dump({"settings": bad}, f)
+        json.dump({"workloads": {"x": {"settings": bad}}}, f)
     with pytest.raises(ValueError):
-        tune.load(p)
+        tune.load(p, "x")
 
 
 def test_defaults_are_candidates():
@@ -40,5 +44,8 @@ def test_committed_tuning_file_loads():
     p = os.path.join(ROOT, "segment-tree-based-clipping_amd", "tuning.json")
     if not os.path.exists(p):
         pytest.skip("no tuning file committed")
-    s = tune.load(p)
-    assert set(s) <= set(tune.SETTINGS)
+    with open(p) as f:
+        names = list(json.load(f)["workloads"])
+    assert "plain" in names
+    for n in names:
+        assert set(tune.load(p, n)) <= set(tune.SETTINGS)
