@@ -224,7 +224,8 @@ def bench_large_pairs(steps):
 
 
 def run_tuner(nb, nc):
-    """Time every stab_tiles candidate on the plain clip and the three batched
+    """Time every tuning candidate (segclip.tune.SETTINGS, one setting after the
+    other) on the plain clip and the three batched
     forward workloads; write the fastest per workload to TUNING_FILE."""
     import torch
     import segclip
@@ -233,8 +234,8 @@ def run_tuner(nb, nc):
 
     def one(name, text, cl, step, reps):
         log("[tune] %s" % name)
-        best, ms = tune.autotune(cl, lambda c: step(), "stab_tiles", warmup=1, reps=reps)
-        entries[name] = {"settings": {"stab_tiles": best}, "workload": text, "timings_ms": {"stab_tiles": ms}}
+        chosen, ms = tune.autotune_all(cl, lambda c: step(), warmup=1, reps=reps)
+        entries[name] = {"settings": chosen, "workload": text, "timings_ms": ms}
         log("[tune] %s: %s" % (name, json.dumps(entries[name])))
         cl.close()
 
@@ -272,7 +273,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 launches), to scale HBM counters")
     ap.add_argument("--tune", action="store_true",
-                    help="time every stab_tiles candidate on the plain clip and the batched workloads and "
+                    help="time every candidate of each tuning setting on the plain clip and the batched workloads and "
                          "write the fastest per workload to "
                          "segment-tree-based-clipping_amd/tuning.json")
     ap.add_argument("--no-tuning", action="store_true", help="ignore tuning.json (library defaults)")

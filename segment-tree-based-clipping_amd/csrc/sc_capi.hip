@@ -565,6 +565,9 @@ int segclip_set_tuning(segclip_ctx *h, const char *name, int value) {
         if (std::strcmp(name, "stab_tiles") == 0) {
             SC_REQUIRE(value >= 1 && value <= 16, SEGCLIP_EINVAL, "stab_tiles must be in 1..16");
             c.stab_tiles = value;
+        } else if (std::strcmp(name, "stab_units_per_wave") == 0) {
+            SC_REQUIRE(value >= 1 && value <= 8, SEGCLIP_EINVAL, "stab_units_per_wave must be in 1..8");
+            c.stab_upw = value;
         } else {
             SC_REQUIRE(false, SEGCLIP_EINVAL, std::string("unknown tuning setting: ") + name);
         }
@@ -576,6 +579,8 @@ int segclip_get_tuning(segclip_ctx *h, const char *name, int *value) {
         SC_REQUIRE(name != nullptr && value != nullptr, SEGCLIP_EINVAL, "null argument");
         if (std::strcmp(name, "stab_tiles") == 0) {
             *value = c.stab_tiles;
+        } else if (std::strcmp(name, "stab_units_per_wave") == 0) {
+            *value = c.stab_upw;
         } else {
             SC_REQUIRE(false, SEGCLIP_EINVAL, std::string("unknown tuning setting: ") + name);
         }

@@ -188,6 +188,7 @@ struct StabArgs {
     const double *x0, *y0, *x1, *y1;          // endpoints (input coordinates)
     int nl, D, T;
     uint32_t nunits;
+    uint32_t upw;                             // consecutive units per wave
     int bits_b, bits_c;
     uint64_t *out;
     uint64_t cap;
@@ -211,7 +212,8 @@ __device__ inline void flush(const StabArgs &a, const uint64_t *stage, uint32_t 
         if (base + i < a.cap) a.out[base + i] = stage[i];
 }
 
-// One wave per unit: up to 64 rows against one or more 64-column tiles (one
+// One wave per unit (or per run of upw consecutive units, a tuning setting):
+// up to 64 rows against one or more 64-column tiles (one
 // column per lane).  The rows' single-precision y-extents sit in LDS and are
 // read as broadcasts, four rows at a time; a row that meets no column costs
 // two compares.  Pairs that pass are queued per wave; a full queue (or the
@@ -235,12 +237,19 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     const uint32_t nbk = gridDim.x;
     const uint32_t r8 = blockIdx.x / 8, xcd = blockIdx.x % 8;
     const uint32_t lb = (r8 / XCHUNK) * (8 * XCHUNK) + xcd * XCHUNK + r8 % XCHUNK; // logical block
-    const uint32_t wg = lb * (BLK / WAVE) + wv;
-    if (lb >= nbk || wg >= a.nunits) return; // whole wave; no block barrier below
+    // this wave runs units [u_first, u_last): a.upw consecutive units, hits
+    // staged across them (one flush per full stage or per wave)
+    const uint64_t u_first = uint64_t(lb * (BLK / WAVE) + wv) * a.upw;
+    if (lb >= nbk || u_first >= a.nunits) return; // whole wave; no block barrier below
+    const uint32_t u_last = uint32_t(min(u_first + a.upw, uint64_t(a.nunits)));
     uint64_t *stage = stage_all[wv];
     YRangeF *ry = ry_all[wv];
     uint32_t *queue = queue_all[wv];
     double4 *rxy = rxy_all[wv], *cxy = cxy_all[wv];
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+    const int sb = a.bits_b + a.bits_c;
+    uint32_t cnt = 0;
+    for (uint32_t wg = uint32_t(u_first); wg < u_last; ++wg) {
     // the unit's record: a uniform load
     const uint4 *dp = reinterpret_cast<const uint4 *>(a.desc + wg);
     const uint4 d0 = dp[0], d1 = dp[1];
@@ -276,9 +285,6 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     __builtin_amdgcn_wave_barrier();
     bool rows_fetched = false;
     int rk = 0;
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
-    const int sb = a.bits_b + a.bits_c;
-    uint32_t cnt = 0;
     const uint32_t nr4 = (nrows + 3u) & ~3u; // rows past nrows hold the empty box
     for (uint32_t t = t0; t < t1; ++t) {
         if (t != t0) load_tile(t, n_raw, n_box, n_bl);
@@ -378,6 +384,8 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
         }
         if (qn) run_queue();
     }
+    __builtin_amdgcn_wave_barrier(); // the unit's LDS reads before the next unit's writes
+    } // units of this wave
     if (cnt) flush(a, stage, cnt, lane);
 }
 
@@ -430,6 +438,7 @@ void find_candidates(Ctx &c) {
     uint64_t *tests = c.u64b.ensure(size_t(T) + 1);
     SC_REQUIRE(c.stab_tiles >= 1 && uint32_t(c.stab_tiles) <= STAB_TILES_MAX, SC_EINVAL,
                "stab_tiles out of range");
+    SC_REQUIRE(c.stab_upw >= 1 && c.stab_upw <= 8, SC_EINVAL, "stab_units_per_wave out of range");
     const uint32_t ftpu = uint32_t(c.stab_tiles);
     k_units<<<grid_for(T), BLK, 0, s>>>(c.cov_off, c.cov_split, c.lend_off, c.nleaves, c.depth, T, ftpu, units,
                                         tests);
@@ -480,13 +489,14 @@ void find_candidates(Ctx &c) {
         a.cov_yr = c.cov_yr; a.lend_yr = c.lend_yr;
         a.x0 = c.ex0; a.y0 = c.ey0; a.x1 = c.ex1; a.y1 = c.ey1;
         a.nl = c.nleaves; a.D = c.depth; a.T = T; a.nunits = nunits;
+        a.upw = uint32_t(c.stab_upw);
         a.bits_b = c.bits_b; a.bits_c = c.bits_c;
         a.out = c.cand; a.cap = uint64_t(c.cand_cap); a.counter = counter;
         c.mark(8);
         if (nunits > 0) {
             // a multiple of 8*XCHUNK blocks: the XCD-aware block order is then a bijection
             const unsigned q = 8u * XCHUNK;
-            const unsigned nblk = (grid_for(nunits, BLK / WAVE) + q - 1u) / q * q;
+            const unsigned nblk = (grid_for((nunits + a.upw - 1u) / a.upw, BLK / WAVE) + q - 1u) / q * q;
             k_stab<<<nblk, BLK, 0, s>>>(a);
             SC_HIP(hipGetLastError());
         }

@@ -2,7 +2,8 @@
 
 A setting changes how the library splits its work, never its results: the
 stab's "stab_tiles" (64-column tiles per work unit, one wave each) changes the
-number and length of the waves, and the candidate list is sorted afterwards,
+number and length of the waves, "stab_units_per_wave" how many consecutive
+units one wave runs, and the candidate list is sorted afterwards,
 so every value yields the same pairs, intersections and rings.  The tuner
 therefore measures speed only; the tests check that results agree across
 values.
@@ -25,9 +26,9 @@ import math
 import time
 
 # candidate values per setting, and the library's defaults
-SETTINGS = {"stab_tiles": (1, 2, 4, 8)}
-DEFAULTS = {"stab_tiles": 2}
-LIMITS = {"stab_tiles": (1, 16)}
+SETTINGS = {"stab_tiles": (1, 2, 4, 8), "stab_units_per_wave": (1, 2, 4)}
+DEFAULTS = {"stab_tiles": 2, "stab_units_per_wave": 1}
+LIMITS = {"stab_tiles": (1, 16), "stab_units_per_wave": (1, 8)}
 
 
 def autotune(clipper, step, name="stab_tiles", candidates=None, warmup=1, reps=3):
@@ -62,6 +63,15 @@ def autotune(clipper, step, name="stab_tiles", candidates=None, warmup=1, reps=3
     choice = min(ms, key=ms.get)
     clipper.set_tuning(name, choice)
     return choice, ms
+
+
+def autotune_all(clipper, step, warmup=1, reps=3):
+    """Tune every setting in turn (each with the earlier ones fixed at their
+    chosen values).  Returns ({setting: value}, {setting: {value: ms}})."""
+    chosen, timings = {}, {}
+    for name in SETTINGS:
+        chosen[name], timings[name] = autotune(clipper, step, name, warmup=warmup, reps=reps)
+    return chosen, timings
 
 
 def save(path, entries, device=None):

@@ -1,5 +1,6 @@
 """GPU checks of the tuning settings: every value of "stab_tiles" (64-column
-tiles per stab work unit) gives bit-identical pairs, intersections and rings,
+tiles per stab work unit) and of "stab_units_per_wave" (consecutive units one
+wave runs) gives bit-identical pairs, intersections and rings,
 equal to the oracle's; only the number of stab waves changes.  Out-of-range
 values and unknown names are refused with SEGCLIP_EINVAL."""
 import math
@@ -14,6 +15,8 @@ from fixtures import rings_equal
 pytestmark = pytest.mark.gpu
 
 TILES = (1, 2, 3, 4, 8, 16)
+# (stab_tiles, stab_units_per_wave) combinations beside the defaults (2, 1)
+COMBOS = ((2, 2), (2, 3), (2, 8), (4, 4), (1, 2))
 
 
 def _snapshot(cl, k=1):
@@ -40,8 +43,24 @@ def test_stab_tiles_do_not_change_results(gpu):
         cl.clip(1)
         assert cl.counts()["undefined"] & 1 == 0
         snaps[t] = _snapshot(cl)
+    cl.set_tuning("stab_tiles", 2)
+    combo = {}
+    for t, u in COMBOS:
+        print("[gpu-tuning] stab_tiles=%d stab_units_per_wave=%d" % (t, u), flush=True)
+        cl.set_tuning("stab_tiles", t)
+        cl.set_tuning("stab_units_per_wave", u)
+        assert cl.get_tuning("stab_units_per_wave") == u
+        cl.reset()
+        cl.clip(1)
+        combo[(t, u)] = _snapshot(cl)
     ref = snaps[2]
     assert len(ref["pairs"][0]) > 1000
+    for (t, u), s in combo.items():
+        for a, r in zip(s["pairs"], ref["pairs"]):
+            assert np.array_equal(a, r), "stab_tiles=%d units_per_wave=%d: pairs differ" % (t, u)
+        assert np.array_equal(s["ix"].view(np.uint64), ref["ix"].view(np.uint64)), (t, u)
+        err = rings_equal(s["rings"], ref["rings"])
+        assert err is None, "stab_tiles=%d units_per_wave=%d: %s" % (t, u, err)
     for t in TILES:
         s = snaps[t]
         for a, r in zip(s["pairs"], ref["pairs"]):
@@ -67,16 +86,17 @@ def test_stab_tiles_batched_pairs(gpu):
     ovs.append(S.star(3000, 72, 0.1, 0.05, 0.4, 0.9))
     print("[gpu-tuning] paired batch", len(bases), flush=True)
     got = {}
-    for t in (1, 2, 8):
+    for t, u in ((1, 1), (2, 1), (8, 1), (2, 4), (4, 8)):
         cl = segclip.Clipper().set_pairs(bases, ovs)
         cl.set_tuning("stab_tiles", t)
+        cl.set_tuning("stab_units_per_wave", u)
         cl.clip_batch()
-        got[t] = cl.batch_rings()
+        got[(t, u)] = cl.batch_rings()
         cl.close()
-    for t in (1, 8):
+    for t in ((1, 1), (8, 1), (2, 4), (4, 8)):
         for i in range(len(bases)):
-            err = rings_equal(got[t][i], got[2][i])
-            assert err is None, "stab_tiles=%d pair %d: %s" % (t, i, err)
+            err = rings_equal(got[t][i], got[(2, 1)][i])
+            assert err is None, "settings %s pair %d: %s" % (t, i, err)
 
 
 def test_tuning_refuses_bad_settings(gpu):
@@ -86,6 +106,10 @@ def test_tuning_refuses_bad_settings(gpu):
     for v in (0, 17, -1):
         with pytest.raises(SegclipError):
             cl.set_tuning("stab_tiles", v)
+    for v in (0, 9):
+        with pytest.raises(SegclipError):
+            cl.set_tuning("stab_units_per_wave", v)
+    assert cl.get_tuning("stab_units_per_wave") == 1
     with pytest.raises(SegclipError):
         cl.set_tuning("no_such_setting", 2)
     with pytest.raises(SegclipError):
