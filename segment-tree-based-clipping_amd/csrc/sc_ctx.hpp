@@ -151,7 +151,7 @@ struct Ctx {
     // tuning: 64-column tiles per stab work unit (1..16); results do not depend on it
     int stab_tiles = 2;
     // tuning: consecutive stab work units per wave (1..8); results do not depend on it
-    int stab_upw = 1;
+    int stab_upw = 4;
 
     bool timing = false;
     StageTimes times;

@@ -26,8 +26,8 @@ import math
 import time
 
 # candidate values per setting, and the library's defaults
-SETTINGS = {"stab_tiles": (1, 2, 4, 8), "stab_units_per_wave": (1, 2, 4)}
-DEFAULTS = {"stab_tiles": 2, "stab_units_per_wave": 1}
+SETTINGS = {"stab_tiles": (1, 2, 4, 8), "stab_units_per_wave": (1, 2, 4, 8)}
+DEFAULTS = {"stab_tiles": 2, "stab_units_per_wave": 4}
 LIMITS = {"stab_tiles": (1, 16), "stab_units_per_wave": (1, 8)}
 
 

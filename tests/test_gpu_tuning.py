@@ -15,8 +15,8 @@ from fixtures import rings_equal
 pytestmark = pytest.mark.gpu
 
 TILES = (1, 2, 3, 4, 8, 16)
-# (stab_tiles, stab_units_per_wave) combinations beside the defaults (2, 1)
-COMBOS = ((2, 2), (2, 3), (2, 8), (4, 4), (1, 2))
+# (stab_tiles, stab_units_per_wave) combinations beside (2, 1) and the defaults (2, 4)
+COMBOS = ((2, 1), (2, 2), (2, 3), (2, 8), (4, 4), (1, 2))
 
 
 def _snapshot(cl, k=1):
@@ -109,7 +109,7 @@ def test_tuning_refuses_bad_settings(gpu):
     for v in (0, 9):
         with pytest.raises(SegclipError):
             cl.set_tuning("stab_units_per_wave", v)
-    assert cl.get_tuning("stab_units_per_wave") == 1
+    assert cl.get_tuning("stab_units_per_wave") == 4
     with pytest.raises(SegclipError):
         cl.set_tuning("no_such_setting", 2)
     with pytest.raises(SegclipError):
