@@ -242,9 +242,9 @@ int segclip_get_stage_times(segclip_ctx *ctx, float *ms8);
  * oacc_segment_tree.cpp:6250-6526); they serve
  * the autotuner, which times candidate values on a workload and persists the
  * fastest as JSON (segclip.tune).  Results never depend on a setting.
- *   "stab_tiles"  64-column tiles per stab work unit, 1..16, default 2.
+ *   "stab_tiles"  64-column tiles per stab work unit, 1..16, default 1.
  *   "stab_units_per_wave"  consecutive stab work units one wave runs (hits
- *                 staged across them), 1..8, default 4.
+ *                 staged across them), 1..32, default 8.
  * An unknown name or a value out of range returns SEGCLIP_EINVAL. */
 int segclip_set_tuning(segclip_ctx *ctx, const char *name, int value);
 int segclip_get_tuning(segclip_ctx *ctx, const char *name, int *value);

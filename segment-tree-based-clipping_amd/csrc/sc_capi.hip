@@ -566,7 +566,7 @@ int segclip_set_tuning(segclip_ctx *h, const char *name, int value) {
             SC_REQUIRE(value >= 1 && value <= 16, SEGCLIP_EINVAL, "stab_tiles must be in 1..16");
             c.stab_tiles = value;
         } else if (std::strcmp(name, "stab_units_per_wave") == 0) {
-            SC_REQUIRE(value >= 1 && value <= 8, SEGCLIP_EINVAL, "stab_units_per_wave must be in 1..8");
+            SC_REQUIRE(value >= 1 && value <= 32, SEGCLIP_EINVAL, "stab_units_per_wave must be in 1..32");
             c.stab_upw = value;
         } else {
             SC_REQUIRE(false, SEGCLIP_EINVAL, std::string("unknown tuning setting: ") + name);

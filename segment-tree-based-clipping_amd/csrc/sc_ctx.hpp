@@ -149,9 +149,9 @@ struct Ctx {
     void ensure_side_stream();
 
     // tuning: 64-column tiles per stab work unit (1..16); results do not depend on it
-    int stab_tiles = 2;
-    // tuning: consecutive stab work units per wave (1..8); results do not depend on it
-    int stab_upw = 4;
+    int stab_tiles = 1;
+    // tuning: consecutive stab work units per wave (1..32); results do not depend on it
+    int stab_upw = 8;
 
     bool timing = false;
     StageTimes times;

@@ -438,7 +438,7 @@ void find_candidates(Ctx &c) {
     uint64_t *tests = c.u64b.ensure(size_t(T) + 1);
     SC_REQUIRE(c.stab_tiles >= 1 && uint32_t(c.stab_tiles) <= STAB_TILES_MAX, SC_EINVAL,
                "stab_tiles out of range");
-    SC_REQUIRE(c.stab_upw >= 1 && c.stab_upw <= 8, SC_EINVAL, "stab_units_per_wave out of range");
+    SC_REQUIRE(c.stab_upw >= 1 && c.stab_upw <= 32, SC_EINVAL, "stab_units_per_wave out of range");
     const uint32_t ftpu = uint32_t(c.stab_tiles);
     k_units<<<grid_for(T), BLK, 0, s>>>(c.cov_off, c.cov_split, c.lend_off, c.nleaves, c.depth, T, ftpu, units,
                                         tests);

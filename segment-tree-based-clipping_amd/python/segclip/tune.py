@@ -26,9 +26,9 @@ import math
 import time
 
 # candidate values per setting, and the library's defaults
-SETTINGS = {"stab_tiles": (1, 2, 4, 8), "stab_units_per_wave": (1, 2, 4, 8)}
-DEFAULTS = {"stab_tiles": 2, "stab_units_per_wave": 4}
-LIMITS = {"stab_tiles": (1, 16), "stab_units_per_wave": (1, 8)}
+SETTINGS = {"stab_tiles": (1, 2, 4, 8), "stab_units_per_wave": (1, 2, 4, 8, 16)}
+DEFAULTS = {"stab_tiles": 1, "stab_units_per_wave": 8}
+LIMITS = {"stab_tiles": (1, 16), "stab_units_per_wave": (1, 32)}
 
 
 def autotune(clipper, step, name="stab_tiles", candidates=None, warmup=1, reps=3):
