@@ -250,141 +250,141 @@ __global__ __launch_bounds__(BLK) void k_stab(StabArgs a) {
     const int sb = a.bits_b + a.bits_c;
     uint32_t cnt = 0;
     for (uint32_t wg = uint32_t(u_first); wg < u_last; ++wg) {
-    // the unit's record: a uniform load
-    const uint4 *dp = reinterpret_cast<const uint4 *>(a.desc + wg);
-    const uint4 d0 = dp[0], d1 = dp[1];
-    const uint32_t row0 = uni(d0.x), col_begin = uni(d0.y), cols = uni(d0.z), t0 = uni(d0.w);
-    const uint32_t t1 = uni(d1.x), nk = uni(d1.y), blo = uni(d1.z), bhi = uni(d1.w);
-    const uint32_t nrows = nk & 0xffu;
-    const int kind = int(nk >> 8);
-    const uint32_t *colsrc = kind == 1 ? a.cov_e : a.lend_e;
-    const YRangeF *colyr = kind == 1 ? a.cov_yr : a.lend_yr;
-    // a tile's columns: entry, box and (end entries) the edge's "before" leaf
-    auto load_tile = [&](uint32_t t, uint32_t &raw, YRangeF &bx, uint32_t &bl) {
-        const uint32_t cb = t * WAVE;
-        raw = 0xffffffffu;
-        bl = 0;
-        if (uint32_t(lane) < min(uint32_t(WAVE), cols - cb)) {
-            const uint32_t ci = col_begin + cb + lane;
-            raw = colsrc[ci];
-            bx = colyr[ci];
-            if (kind != 1) bl = uint32_t(a.lend_bl[ci]);
-        }
-    };
-    uint32_t n_raw, n_bl;
-    YRangeF n_box;
-    load_tile(t0, n_raw, n_box, n_bl); // issued before the rows are waited for
-    // this lane's row edge and box (the empty box past the last row)
-    uint32_t re_l = 0;
-    YRangeF rb = yrange_empty();
-    if (uint32_t(lane) < nrows) {
-        re_l = a.cov_e[row0 + lane];
-        rb = a.cov_yr[row0 + lane];
-    }
-    ry[lane] = rb;
-    __builtin_amdgcn_wave_barrier();
-    bool rows_fetched = false;
-    int rk = 0;
-    const uint32_t nr4 = (nrows + 3u) & ~3u; // rows past nrows hold the empty box
-    for (uint32_t t = t0; t < t1; ++t) {
-        if (t != t0) load_tile(t, n_raw, n_box, n_bl);
-        const uint32_t raw = n_raw, bl = n_bl;
-        YRangeF cbf = n_box;
-        bool valid = raw != 0xffffffffu;
-        const uint32_t ce = raw & 0x7fffffffu;
-        // "after" copy of an edge whose "before" leaf is under v as well
-        if (valid && kind != 1 && (raw & 0x80000000u) && bl >= blo && bl < bhi) valid = false;
-        if (!valid) cbf = yrange_empty();
-        bool cols_fetched = false;
-        int ck = 0;
-        uint32_t qn = 0; // queued near pairs of this tile
-        // LSMF on the doubles and the classification of the queued pairs, one per lane
-        auto run_queue = [&]() {
-            __builtin_amdgcn_wave_barrier();
-            const bool act = uint32_t(lane) < qn;
-            const uint32_t e = act ? queue[lane] : 0u;
-            const int rr = int(e >> 6), cc = int(e & 63u);
-            const double4 P = rxy[rr], Q = cxy[cc];
-            const double px0 = P.x, py0 = P.y, px1 = P.z, py1 = P.w;
-            const double qx0 = Q.x, qy0 = Q.y, qx1 = Q.z, qy1 = Q.w;
-            const uint32_t re = uint32_t(__shfl(int(re_l), rr, WAVE)), qce = uint32_t(__shfl(int(ce), cc, WAVE));
-            const int rkk = __shfl(rk, rr, WAVE), ckk = __shfl(ck, cc, WAVE);
-            bool hit = false;
-            uint64_t key = 0;
-            if (act && box_overlap(box_of(px0, py0, px1, py1), box_of(qx0, qy0, qx1, qy1))) {
-                int tp;
-                uint32_t be, qe;
-                int kk;
-                if (kind == 3) { // row = overlay edge, column = base edge
-                    tp = classify_pre(qx0, qy0, qx1, qy1, px0, py0, px1, py1);
-                    be = qce;
-                    qe = re;
-                    kk = rkk;
-                } else {
-                    tp = classify_pre(px0, py0, px1, py1, qx0, qy0, qx1, qy1);
-                    be = re;
-                    qe = qce;
-                    kk = ckk;
-                }
-                if (tp != 0) {
-                    hit = true;
-                    const uint32_t local = qe - uint32_t(a.eoff[kk - 1]);
-                    key = (uint64_t(kk) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
-                }
+        // the unit's record: a uniform load
+        const uint4 *dp = reinterpret_cast<const uint4 *>(a.desc + wg);
+        const uint4 d0 = dp[0], d1 = dp[1];
+        const uint32_t row0 = uni(d0.x), col_begin = uni(d0.y), cols = uni(d0.z), t0 = uni(d0.w);
+        const uint32_t t1 = uni(d1.x), nk = uni(d1.y), blo = uni(d1.z), bhi = uni(d1.w);
+        const uint32_t nrows = nk & 0xffu;
+        const int kind = int(nk >> 8);
+        const uint32_t *colsrc = kind == 1 ? a.cov_e : a.lend_e;
+        const YRangeF *colyr = kind == 1 ? a.cov_yr : a.lend_yr;
+        // a tile's columns: entry, box and (end entries) the edge's "before" leaf
+        auto load_tile = [&](uint32_t t, uint32_t &raw, YRangeF &bx, uint32_t &bl) {
+            const uint32_t cb = t * WAVE;
+            raw = 0xffffffffu;
+            bl = 0;
+            if (uint32_t(lane) < min(uint32_t(WAVE), cols - cb)) {
+                const uint32_t ci = col_begin + cb + lane;
+                raw = colsrc[ci];
+                bx = colyr[ci];
+                if (kind != 1) bl = uint32_t(a.lend_bl[ci]);
             }
-            const uint64_t hm = __ballot(hit);
-            if (hm) {
-                const uint32_t nh = uint32_t(__popcll(hm));
-                if (cnt + nh > uint32_t(STG)) {
-                    flush(a, stage, cnt, lane);
-                    cnt = 0;
-                }
-                if (hit) stage[cnt + uint32_t(__popcll(hm & lt))] = key;
-                cnt += nh;
-            }
-            qn = 0;
-            __builtin_amdgcn_wave_barrier();
         };
-        for (uint32_t r = 0; r < nr4; r += 4) {
-            // single-precision screen (exact rejection) of four rows
-            uint64_t m[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                // LSMF's y test, one ballot per comparison (the masks are ANDed in scalar
-                // registers).  Every tested pair overlaps in x by construction (a cover
-                // entry spans the node, an end entry has an endpoint in it), and the
-                // screen only has to keep every pair LSMF keeps.
-                const YRangeF q = ry[r + k];
-                m[k] = __builtin_amdgcn_ballot_w64(!(q.lo > cbf.hi)) & __builtin_amdgcn_ballot_w64(!(q.hi < cbf.lo));
-            }
-            if ((m[0] | m[1] | m[2] | m[3]) == 0) continue;
-            if (!cols_fetched) {
-                if (valid) {
-                    cxy[lane] = make_double4(a.x0[ce], a.y0[ce], a.x1[ce], a.y1[ce]);
-                    ck = a.etype[ce];
-                }
-                cols_fetched = true;
-            }
-            if (!rows_fetched) {
-                if (uint32_t(lane) < nrows) {
-                    rxy[lane] = make_double4(a.x0[re_l], a.y0[re_l], a.x1[re_l], a.y1[re_l]);
-                    rk = a.etype[re_l];
-                }
-                rows_fetched = true;
-            }
-#pragma unroll 1
-            for (int k = 0; k < 4; ++k) {
-                const uint64_t mk = m[k];
-                if (mk == 0) continue;
-                const uint32_t nh = uint32_t(__popcll(mk));
-                if (qn + nh > uint32_t(WAVE)) run_queue();
-                if ((mk >> lane) & 1ull) queue[qn + uint32_t(__popcll(mk & lt))] = ((r + uint32_t(k)) << 6) | uint32_t(lane);
-                qn += nh;
-            }
+        uint32_t n_raw, n_bl;
+        YRangeF n_box;
+        load_tile(t0, n_raw, n_box, n_bl); // issued before the rows are waited for
+        // this lane's row edge and box (the empty box past the last row)
+        uint32_t re_l = 0;
+        YRangeF rb = yrange_empty();
+        if (uint32_t(lane) < nrows) {
+            re_l = a.cov_e[row0 + lane];
+            rb = a.cov_yr[row0 + lane];
         }
-        if (qn) run_queue();
-    }
-    __builtin_amdgcn_wave_barrier(); // the unit's LDS reads before the next unit's writes
+        ry[lane] = rb;
+        __builtin_amdgcn_wave_barrier();
+        bool rows_fetched = false;
+        int rk = 0;
+        const uint32_t nr4 = (nrows + 3u) & ~3u; // rows past nrows hold the empty box
+        for (uint32_t t = t0; t < t1; ++t) {
+            if (t != t0) load_tile(t, n_raw, n_box, n_bl);
+            const uint32_t raw = n_raw, bl = n_bl;
+            YRangeF cbf = n_box;
+            bool valid = raw != 0xffffffffu;
+            const uint32_t ce = raw & 0x7fffffffu;
+            // "after" copy of an edge whose "before" leaf is under v as well
+            if (valid && kind != 1 && (raw & 0x80000000u) && bl >= blo && bl < bhi) valid = false;
+            if (!valid) cbf = yrange_empty();
+            bool cols_fetched = false;
+            int ck = 0;
+            uint32_t qn = 0; // queued near pairs of this tile
+            // LSMF on the doubles and the classification of the queued pairs, one per lane
+            auto run_queue = [&]() {
+                __builtin_amdgcn_wave_barrier();
+                const bool act = uint32_t(lane) < qn;
+                const uint32_t e = act ? queue[lane] : 0u;
+                const int rr = int(e >> 6), cc = int(e & 63u);
+                const double4 P = rxy[rr], Q = cxy[cc];
+                const double px0 = P.x, py0 = P.y, px1 = P.z, py1 = P.w;
+                const double qx0 = Q.x, qy0 = Q.y, qx1 = Q.z, qy1 = Q.w;
+                const uint32_t re = uint32_t(__shfl(int(re_l), rr, WAVE)), qce = uint32_t(__shfl(int(ce), cc, WAVE));
+                const int rkk = __shfl(rk, rr, WAVE), ckk = __shfl(ck, cc, WAVE);
+                bool hit = false;
+                uint64_t key = 0;
+                if (act && box_overlap(box_of(px0, py0, px1, py1), box_of(qx0, qy0, qx1, qy1))) {
+                    int tp;
+                    uint32_t be, qe;
+                    int kk;
+                    if (kind == 3) { // row = overlay edge, column = base edge
+                        tp = classify_pre(qx0, qy0, qx1, qy1, px0, py0, px1, py1);
+                        be = qce;
+                        qe = re;
+                        kk = rkk;
+                    } else {
+                        tp = classify_pre(px0, py0, px1, py1, qx0, qy0, qx1, qy1);
+                        be = re;
+                        qe = qce;
+                        kk = ckk;
+                    }
+                    if (tp != 0) {
+                        hit = true;
+                        const uint32_t local = qe - uint32_t(a.eoff[kk - 1]);
+                        key = (uint64_t(kk) << sb) | (uint64_t(be) << a.bits_c) | uint64_t(local);
+                    }
+                }
+                const uint64_t hm = __ballot(hit);
+                if (hm) {
+                    const uint32_t nh = uint32_t(__popcll(hm));
+                    if (cnt + nh > uint32_t(STG)) {
+                        flush(a, stage, cnt, lane);
+                        cnt = 0;
+                    }
+                    if (hit) stage[cnt + uint32_t(__popcll(hm & lt))] = key;
+                    cnt += nh;
+                }
+                qn = 0;
+                __builtin_amdgcn_wave_barrier();
+            };
+            for (uint32_t r = 0; r < nr4; r += 4) {
+                // single-precision screen (exact rejection) of four rows
+                uint64_t m[4];
+    #pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    // LSMF's y test, one ballot per comparison (the masks are ANDed in scalar
+                    // registers).  Every tested pair overlaps in x by construction (a cover
+                    // entry spans the node, an end entry has an endpoint in it), and the
+                    // screen only has to keep every pair LSMF keeps.
+                    const YRangeF q = ry[r + k];
+                    m[k] = __builtin_amdgcn_ballot_w64(!(q.lo > cbf.hi)) & __builtin_amdgcn_ballot_w64(!(q.hi < cbf.lo));
+                }
+                if ((m[0] | m[1] | m[2] | m[3]) == 0) continue;
+                if (!cols_fetched) {
+                    if (valid) {
+                        cxy[lane] = make_double4(a.x0[ce], a.y0[ce], a.x1[ce], a.y1[ce]);
+                        ck = a.etype[ce];
+                    }
+                    cols_fetched = true;
+                }
+                if (!rows_fetched) {
+                    if (uint32_t(lane) < nrows) {
+                        rxy[lane] = make_double4(a.x0[re_l], a.y0[re_l], a.x1[re_l], a.y1[re_l]);
+                        rk = a.etype[re_l];
+                    }
+                    rows_fetched = true;
+                }
+    #pragma unroll 1
+                for (int k = 0; k < 4; ++k) {
+                    const uint64_t mk = m[k];
+                    if (mk == 0) continue;
+                    const uint32_t nh = uint32_t(__popcll(mk));
+                    if (qn + nh > uint32_t(WAVE)) run_queue();
+                    if ((mk >> lane) & 1ull) queue[qn + uint32_t(__popcll(mk & lt))] = ((r + uint32_t(k)) << 6) | uint32_t(lane);
+                    qn += nh;
+                }
+            }
+            if (qn) run_queue();
+        }
+        __builtin_amdgcn_wave_barrier(); // the unit's LDS reads before the next unit's writes
     } // units of this wave
     if (cnt) flush(a, stage, cnt, lane);
 }
