@@ -20,13 +20,16 @@ GPU loop on a sample of the same pairs):
   large_pairs   256 pairs of 10k..200k-vertex harmonic stars: forward +
                 backward with the clip recomputed in backward, peak HBM
 
-roofline: the stab kernel (the dominant kernel), FP64 work = W pair tests x 8
-FP64 comparisons (the LSMF bounding-box test every pair runs) per launch,
-divided by its launch time from HIP events on the library's stream, against
-the 78.6 TFLOP/s FP64 peak.  traffic: the kernel's HBM bytes per launch
-(FETCH_SIZE + WRITE_SIZE, scaled by a plain copy) from the rocprofv3 counter
-runs summarised in profiles/clip_harmonic_200k_1m_hbm_traffic.md, given for
-the default workload only.
+roofline: the stab kernel (the dominant kernel), HBM bound.  Algorithmic
+bytes per launch = 36 (L + E) + 12 K: every cover entry (L) and leaf end entry
+(E; upper-level end lists are ranges of the leaf entries, not stored) read
+once with its 32-byte coordinate record, plus a 12-byte triple per emitted
+candidate (K), divided by the launch time from HIP events on the library's
+stream, against 8 TB/s.  traffic: the kernel's measured bytes per launch
+from the rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs summarised in
+profiles/clip_harmonic_200k_1m_hbm_traffic.md (default workload only).
+Those counters count L2-to-fabric requests, so Infinity Cache hits are
+included: an upper bound on DRAM bytes.
 
 cpu_baseline: the CPU oracle (single-thread restatement of the reference's
 multicore path) on a bounded sample of the same construction, timed on this
@@ -43,8 +46,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "segment-tree-based-clipping_amd", "python"))
 
 METRIC = "edges clipped/s and pairs/s on 1 GPU, batched vs plain; fwd+bwd area pairs/s"
-FP64_PEAK_TFLOPS = 78.6
-FLOPS_PER_TEST = 8.0
+HBM_PEAK_GBPS = 8000.0
+
+
+def stab_bytes(cover_entries, end_entries, candidates):
+    """Algorithmic HBM bytes of one stab launch: lists read once with a 32-byte
+    coordinate record per entry (4-byte entry + 32-byte gather), 12 bytes per
+    emitted candidate triple."""
+    return 36 * (cover_entries + end_entries) + 12 * candidates
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "clip_harmonic_200k_1m_hbm_traffic.md")
@@ -337,19 +346,31 @@ def main():
     log("[bench] %d steps, %.3f ms/step, counts %s" % (args.steps, ms, json.dumps(cnt)))
     avg = {k: sum(s[k] for s in stages) / len(stages) for k in stages[0]}
     log("[bench] stage ms: %s" % json.dumps({k: round(v, 4) for k, v in avg.items()}))
+    # E summed over every level (the reference's end-list layout), outside the timed region
+    materialized_end = None
+    try:
+        cl.materialize_end_lists()
+        materialized_end = cl.counts()["end_entries"]
+        log("[bench] materialised end-list entries (all levels): %d" % materialized_end)
+    except Exception as e:
+        log("[bench] end-list materialisation skipped: %s" % e)
     cl.release()
     del cl
 
     stab = sum(stab_ms) / len(stab_ms)
-    flops = FLOPS_PER_TEST * cnt["tests"]
-    achieved = flops / (stab * 1e-3) / 1e12 if stab > 0 else 0.0
+    algo = stab_bytes(cnt["cover_entries"], cnt["leaf_end_entries"], cnt["candidates"])
+    achieved = algo / (stab * 1e-3) / 1e9 if stab > 0 else 0.0
     traffic = measured_traffic("k_stab") if (nb, nc) == (200000, 1000000) else None
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-            "traffic_GBps": round(traffic / (stab * 1e-3) / 1e9, 1) if traffic and stab > 0 else None,
-            "kernel": "k_stab", "kernel_ms": round(stab, 4),
-            "work": "W=%d pair tests x %d FP64 ops (LSMF box test), FP64 VALU (no MFMA); the peak is the "
-                    "FP64 vector rate" % (cnt["tests"], FLOPS_PER_TEST)}
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+            "kernel": "k_stab", "kernel_ms": round(stab, 4), "algorithmic_bytes": algo,
+            "work": "36 (L + E) + 12 K bytes with L=%d cover entries, E=%d leaf end entries, K=%d candidates; "
+                    "W=%d pair tests" % (cnt["cover_entries"], cnt["leaf_end_entries"], cnt["candidates"],
+                                          cnt["tests"]),
+            "traffic_note": "FETCH_SIZE + WRITE_SIZE per launch, scaled by a 1 GiB copy: L2-to-fabric bytes, "
+                            "Infinity Cache hits included"}
+    if materialized_end is not None:
+        roof["materialized_end_entries"] = materialized_end
 
     batched = {}
     if args.batched != "none":

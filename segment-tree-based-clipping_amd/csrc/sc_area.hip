@@ -1,15 +1,19 @@
 // sc_area.hip -- differentiable intersection area and IoU of a batch of jobs
 // (row A19; the reference has no counterpart: it only writes the rings).
 //
-// Forward, per job: A_I = shoelace area of the job's cleaned output rings,
-// A_P / A_Q = shoelace areas of its base / overlay ring, IoU = A_I / (A_P +
-// A_Q - A_I).  Twice the signed area is sum_i (x_i y_{i+1} - x_{i+1} y_i);
-// each sum is reduced by one block in a fixed order, so repeated runs give
-// the same bits.
+// Forward, per job: A_I = sum of the unsigned shoelace areas of the job's
+// cleaned output rings, A_P / A_Q = unsigned shoelace areas of its base /
+// overlay ring, IoU = A_I / (A_P + A_Q - A_I).  Twice the signed area of a
+// ring is S = sum_i (x_i y_{i+1} - x_{i+1} y_i) and its area is |S| / 2: the
+// input rings may run either way round, and with a clockwise input one job's
+// output can hold rings of both orientations.  An intersection of two simple
+// rings has no holes, so every output ring counts positively.  Each sum is
+// reduced in a fixed order, so repeated runs give the same bits.
 //
 // Backward, with the combinatorics of the forward held fixed: an output
-// vertex gets d(2A)/dx_i = y_{i+1} - y_{i-1}, d(2A)/dy_i = x_{i-1} - x_{i+1},
-// routed by its provenance (the augmented slot it was emitted from):
+// vertex gets d|S|/dx_i = sgn(S) (y_{i+1} - y_{i-1}), d|S|/dy_i =
+// sgn(S) (x_{i-1} - x_{i+1}) with S its ring's signed sum, routed by its
+// provenance (the augmented slot it was emitted from):
 //  - a source vertex of either ring, or an intersection located at an input
 //    vertex (types 2-8: at b1 or c1 as the intersection stage places them),
 //    passes the gradient to that input vertex;
@@ -62,31 +66,39 @@ __global__ __launch_bounds__(BLK) void k_ring_area2(const double *__restrict__ p
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
-// twice the signed area of each job's output rings; one block per job
-__global__ __launch_bounds__(BLK) void k_job_out_area2(const int *__restrict__ jring, const int *__restrict__ roff,
-                                                       const double *__restrict__ x, const double *__restrict__ y,
-                                                       double *out) {
-    __shared__ double sh[BLK];
-    const int j = blockIdx.x;
-    const int r0 = jring[j], nr = jring[j + 1] - r0;
+// twice the signed area of every output ring; one wave per ring, lanes
+// stride the ring and a fixed shuffle tree adds their sums
+__global__ __launch_bounds__(BLK) void k_out_ring_area2(int nrings, const int *__restrict__ roff,
+                                                        const double *__restrict__ x, const double *__restrict__ y,
+                                                        double *out) {
+    const int r = int(blockIdx.x) * (BLK / WAVE) + int(threadIdx.x) / WAVE;
+    const int lane = int(threadIdx.x) & (WAVE - 1);
+    if (r >= nrings) return; // whole wave
+    const int64_t a = roff[r], b = roff[r + 1];
     double s = 0.0;
-    if (nr > 0) {
-        const int64_t v0 = roff[r0], v1 = roff[r0 + nr];
-        for (int64_t i = v0 + threadIdx.x; i < v1; i += BLK) {
-            int r = r0 + seg_of(roff + r0, nr, i);
-            int64_t nx = i + 1 == roff[r + 1] ? roff[r] : i + 1;
-            s += x[i] * y[nx] - x[nx] * y[i];
-        }
+    for (int64_t i = a + lane; i < b; i += WAVE) {
+        const int64_t nx = i + 1 == b ? a : i + 1;
+        s += x[i] * y[nx] - x[nx] * y[i];
     }
-    double t = block_sum(s, sh);
-    if (threadIdx.x == 0) out[j] = t;
+    for (int d = WAVE / 2; d > 0; d >>= 1) s += __shfl_down(s, d, WAVE);
+    if (lane == 0) out[r] = s;
+}
+
+// twice the unsigned area of each job's output: the rings' |S| in ring order
+__global__ void k_job_out_area2(int J, const int *__restrict__ jring, const double *__restrict__ ring2,
+                                double *out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= J) return;
+    double s = 0.0;
+    for (int r = jring[j]; r < jring[j + 1]; ++r) s += fabs(ring2[r]);
+    out[j] = s;
 }
 
 __global__ void k_job_final(int J, int paired, const double *out2, const double *barea2, const double *carea2,
                             double *out4) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= J) return;
-    double I = 0.5 * out2[j], P = 0.5 * barea2[paired ? j : 0], Q = 0.5 * carea2[j];
+    double I = 0.5 * out2[j], P = 0.5 * fabs(barea2[paired ? j : 0]), Q = 0.5 * fabs(carea2[j]);
     double U = P + Q - I;
     out4[j] = I;
     out4[J + j] = P;
@@ -94,9 +106,12 @@ __global__ void k_job_final(int J, int paired, const double *out2, const double 
     out4[3 * J + j] = U != 0.0 ? I / U : 0.0;
 }
 
-// per job: coefficients of d(2A_I), d(2A_P), d(2A_Q) in the loss
-__global__ void k_coef(int J, int paired, const double *out4, const double *g_area, const double *g_iou, double *cI,
-                       double *cQ, double *cPr) {
+__device__ inline double sgn1(double v) { return v < 0.0 ? -1.0 : 1.0; }
+
+// per job: coefficients of d(2A_I) (per output ring, times its orientation
+// sign in k_output_grad), d(S_P) and d(S_Q) in the loss, S the signed sums
+__global__ void k_coef(int J, int paired, const double *out4, const double *barea2, const double *carea2,
+                       const double *g_area, const double *g_iou, double *cI, double *cQ, double *cPr) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= J) return;
     double I = out4[j], P = out4[J + j], Q = out4[2 * J + j];
@@ -108,9 +123,9 @@ __global__ void k_coef(int J, int paired, const double *out4, const double *g_ar
         dP = -gi * I / (U * U);
     }
     cI[j] = 0.5 * dI;
-    cQ[j] = 0.5 * dP;
+    cQ[j] = 0.5 * dP * sgn1(carea2[j]);
     // the base ring may be shared by every job (one-to-many)
-    unsafeAtomicAdd(&cPr[paired ? j : 0], 0.5 * dP);
+    unsafeAtomicAdd(&cPr[paired ? j : 0], 0.5 * dP * sgn1(barea2[paired ? j : 0]));
 }
 
 // d(2A)/d(vertex) of the input rings: base rings (all of them, coefficient per
@@ -135,6 +150,7 @@ struct ProvArgs {
     const int *jring, *roff, *src;
     const double *ox, *oy;
     const double *cI;
+    const double *ring2; // signed 2A of every output ring
     const int *sv_b, *sp_b, *sv_c, *sp_c;
     const int *p_b, *p_c, *p_type;
     const int *bvoff, *beoff, *nb, *cvoff, *cin, *nc; // per job
@@ -199,7 +215,7 @@ __global__ void k_output_grad(ProvArgs a, int64_t v0, int64_t v1) {
     const int j = seg_of(a.jring, a.J, r);
     const int64_t rb = a.roff[r], re = a.roff[r + 1];
     const int64_t nx = i + 1 == re ? rb : i + 1, pv = i == rb ? re - 1 : i - 1;
-    const double c = a.cI[j];
+    const double c = a.cI[j] * sgn1(a.ring2[r]);
     const double gx = c * (a.oy[nx] - a.oy[pv]), gy = c * (a.ox[pv] - a.ox[nx]);
     const int s = a.src[i];
     if (s == SRC_NONE) {
@@ -221,7 +237,7 @@ __global__ void k_output_grad(ProvArgs a, int64_t v0, int64_t v1) {
 }
 
 struct AreaScratch {
-    DBuf<double> out2, barea2, carea2, out4, cI, cQ, cPr;
+    DBuf<double> out2, ring2, barea2, carea2, out4, cI, cQ, cPr;
     DBuf<int> bad, bin_off;
 };
 
@@ -247,7 +263,11 @@ void batch_area(Ctx &c, double *out4) {
     AreaScratch &a = ascratch(c);
     const int J = js.J;
     double *out2 = a.out2.ensure(J), *barea2 = a.barea2.ensure(c.nbr), *carea2 = a.carea2.ensure(J);
-    k_job_out_area2<<<J, BLK, 0, s>>>(js.d_ring, c.d_ring_off, c.d_ring_x, c.d_ring_y, out2);
+    double *ring2 = a.ring2.ensure(size_t(c.nrings > 0 ? c.nrings : 1));
+    if (c.nrings > 0)
+        k_out_ring_area2<<<grid_for(c.nrings, BLK / WAVE), BLK, 0, s>>>(c.nrings, c.d_ring_off, c.d_ring_x,
+                                                                       c.d_ring_y, ring2);
+    k_job_out_area2<<<grid_for(J), BLK, 0, s>>>(J, js.d_ring, ring2, out2);
     if (c.paired) k_ring_area2<<<J, BLK, 0, s>>>(c.bpoly, c.d_bvoff, js.k0 - 1, barea2);
     else k_ring_area2<<<1, BLK, 0, s>>>(c.bpoly, c.d_bvoff, 0, barea2);
     k_ring_area2<<<J, BLK, 0, s>>>(c.cpoly, c.d_coff, js.k0 - 1, carea2);
@@ -267,8 +287,8 @@ void batch_area_backward(Ctx &c, const double *g_area, const double *g_iou, doub
     SC_HIP(hipMemsetAsync(cPr, 0, sizeof(double) * c.nbr, s));
     SC_HIP(hipMemsetAsync(cQ, 0, sizeof(double) * c.K, s));
     // k_coef writes cQ for the batch's jobs at their overlay index
-    k_coef<<<grid_for(J), BLK, 0, s>>>(J, c.paired ? 1 : 0, out4, g_area, g_iou, cI, cQ + (js.k0 - 1),
-                                       c.paired ? cPr + (js.k0 - 1) : cPr);
+    k_coef<<<grid_for(J), BLK, 0, s>>>(J, c.paired ? 1 : 0, out4, a.barea2, a.carea2, g_area, g_iou, cI,
+                                       cQ + (js.k0 - 1), c.paired ? cPr + (js.k0 - 1) : cPr);
     SC_HIP(hipGetLastError());
     // input ring terms: base rings (input offsets beoff), all overlays (input offsets eoff - nb)
     int *boff = c.d_beoff;
@@ -289,6 +309,7 @@ void batch_area_backward(Ctx &c, const double *g_area, const double *g_iou, doub
     p.ox = c.d_ring_x;
     p.oy = c.d_ring_y;
     p.cI = cI;
+    p.ring2 = a.ring2;
     p.sv_b = c.sv_b;
     p.sp_b = c.sp_b;
     p.sv_c = c.sv_c;

@@ -189,11 +189,10 @@ struct FillBatch {
 // vals may be null (keys only).  The sorted data ends in keys/vals; the
 // alternate buffers are scratch of the same length.
 struct RadixScratch {
-    DBuf<uint32_t> global_hist;            // per pass 256 digit counts and a tile counter; the error word
+    DBuf<uint32_t> global_hist;            // per pass 256 digit counts and a tile counter
     DBuf<unsigned long long> tile_status;  // chained scan: per tile and digit, epoch | flag | count
     size_t status_clean = 0;               // tile_status entries known to hold no current epoch
     uint32_t epoch = 0;                    // tag of the last pass's status words
-    bool unchecked = false;                // a pass ran since the error word was last read
     DBuf<uint64_t> keys_alt;
     DBuf<uint32_t> vals_alt;
 };
@@ -207,8 +206,6 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int b
 // keys_alt to the same size, so neither partner has to grow again.
 void radix_sort_u64_buf(DBuf<uint64_t> &keys, DBuf<uint64_t> &keys_alt, DBuf<uint32_t> *vals,
                         DBuf<uint32_t> *vals_alt, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs, hipStream_t s);
-// throws if a sort since the last check gave up waiting (its output is invalid)
-void radix_check(RadixScratch &rs, hipStream_t s);
 
 // Order-preserving map of an IEEE double to uint64 (-0.0 and +0.0 map to the
 // same key, so a stable sort on it orders ties by original position as a

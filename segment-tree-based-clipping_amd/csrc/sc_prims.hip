@@ -209,7 +209,6 @@ struct SweepArgs {
     unsigned long long *status;  // ntiles * 256 words, none of them tagged with this epoch
     unsigned long long epoch;    // 1..255, shifted into place
     uint32_t *tile_counter;      // zero before the launch
-    uint32_t *err;               // set when a wait gives up (the sort is then invalid)
 };
 
 __device__ inline uint32_t block_excl_sum256(uint32_t x, uint32_t *s_wsum, uint32_t &total) {
@@ -337,19 +336,21 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
             }
             t -= used;
             if (done) break;
-            if (t < 0) { // a tile 0 word always carries a prefix: a malformed chain
-                atomicOr(a.err, 1u);
+            // A tile 0 word always carries a prefix, so t < 0 is a malformed
+            // chain; a predecessor that stays unpublished past the spin limit
+            // is treated the same way.  Either way this thread counts its
+            // digit in the earlier tiles' input keys itself: exact and slow,
+            // so a pass never scatters with a partial prefix (the keys are
+            // used as indices before the host could see an error).
+            const bool give_up = t < 0 || (used == 0 && ++spins > RS_SPIN_LIMIT);
+            if (give_up) {
+                uint64_t own = 0;
+                for (int64_t i = 0; i < base; ++i) own += (uint32_t(a.kin[i] >> a.shift) & a.mask) == uint32_t(tid);
+                prefix = own;
                 break;
             }
-            if (used == 0) {
-                if (++spins > RS_SPIN_LIMIT) {
-                    atomicOr(a.err, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            } else {
-                spins = 0;
-            }
+            if (used == 0) __builtin_amdgcn_s_sleep(1);
+            else spins = 0;
         }
         __hip_atomic_store(my, a.epoch | ST_PRE | (prefix + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -535,19 +536,6 @@ void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStr
     SC_HIP(hipGetLastError());
 }
 
-void radix_check(RadixScratch &rs, hipStream_t s) {
-    if (!rs.global_hist.p || !rs.unchecked) return;
-    rs.unchecked = false;
-    const size_t GH = size_t(RS_MAXPASS) * 256;
-    uint32_t err = 0;
-    SC_HIP(hipMemcpyAsync(&err, rs.global_hist.p + GH + RS_MAXPASS, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipStreamSynchronize(s));
-    if (err) {
-        SC_HIP(hipMemsetAsync(rs.global_hist.p + GH + RS_MAXPASS, 0, sizeof(uint32_t), s));
-        throw Error(SC_EHIP, "radix sort: a tile waited too long for its predecessor");
-    }
-}
-
 namespace {
 // the passes of the radix sort; true when the sorted data ends in kalt/valt
 bool radix_passes(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, int64_t n, int bit_lo,
@@ -556,12 +544,10 @@ bool radix_passes(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt
     SC_REQUIRE(bit_lo >= 0 && bit_hi <= 64, SC_EINVAL, "radix bit range");
     int npass = int(ceil_div(bit_hi - bit_lo, 8));
     SC_REQUIRE(npass <= RS_MAXPASS, SC_EINVAL, "radix passes");
-    // [0, GH): per pass 256 digit counts; [GH, GH+RS_MAXPASS): per pass tile
-    // counter; GH+RS_MAXPASS: the error word (cleared only when reported)
+    // [0, GH): per pass 256 digit counts; [GH, GH+RS_MAXPASS): per pass tile counter
     const size_t GH = size_t(RS_MAXPASS) * 256;
-    const bool fresh = rs.global_hist.p == nullptr || rs.global_hist.cap < GH + RS_MAXPASS + 1;
-    uint32_t *gh = rs.global_hist.ensure(GH + RS_MAXPASS + 1);
-    SC_HIP(hipMemsetAsync(gh, 0, sizeof(uint32_t) * (GH + RS_MAXPASS + (fresh ? 1 : 0)), s));
+    uint32_t *gh = rs.global_hist.ensure(GH + RS_MAXPASS);
+    SC_HIP(hipMemsetAsync(gh, 0, sizeof(uint32_t) * (GH + RS_MAXPASS), s));
     int ghb = int(ceil_div(n, RS_BLOCK * 16));
     if (ghb > 1024) ghb = 1024;
     k_global_hist<<<ghb, RS_BLOCK, 0, s>>>(keys, n, bit_lo, bit_hi, npass, gh);
@@ -598,10 +584,8 @@ bool radix_passes(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt
         a.status = st;
         a.epoch = (unsigned long long)rs.epoch << ST_EPOCH_SHIFT;
         a.tile_counter = gh + GH + p;
-        a.err = gh + GH + RS_MAXPASS;
         k_onesweep<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
         SC_HIP(hipGetLastError());
-        rs.unchecked = true;
         std::swap(kc, ko);
         std::swap(vc, vo);
     }

@@ -6,7 +6,13 @@
 // callers compiled against the reference header, through the reference's
 // overloaded function names.
 #include "../../include/segclip.h"
+#include "../../include/thrust_func.h"
 #include "sc_common.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
 
 namespace {
 
@@ -77,7 +83,6 @@ int sort_by_key(K *keys, int32_t *vals, int64_t n, void *stream, KeyK keyk, int 
         SC_HIP(hipMemcpyAsync(keys, tk, sizeof(K) * size_t(n), hipMemcpyDeviceToDevice, s));
         SC_HIP(hipMemcpyAsync(vals, tv, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToDevice, s));
         SC_HIP(hipStreamSynchronize(s));
-        sc::radix_check(sc_.rs, s);
         return SEGCLIP_OK;
     } catch (const sc::Error &e) {
         return e.code;
@@ -124,16 +129,46 @@ int segclip_exclusive_scan_i64(int64_t *data, int64_t n, void *stream) {
 
 // Overloads with the reference's names and signatures (REAL = double,
 // LONG = long), default stream, for callers built against thrust_func.h.
+// A failed call throws std::runtime_error (Thrust throws
+// thrust::system_error, also a std::runtime_error), so a sort that did not
+// happen is never silent.
+namespace {
+void throw_if(int rc, const char *what) {
+    if (rc != SEGCLIP_OK)
+        throw std::runtime_error(std::string(what) + " failed with libsegclip status " + std::to_string(rc));
+}
+} // namespace
+
 void thrustDeviceStableSortByKey(double *arr, int *arrIdx, int size) {
-    (void)segclip_stable_sort_by_key_f64(arr, arrIdx, size, nullptr);
+    throw_if(segclip_stable_sort_by_key_f64(arr, arrIdx, size, nullptr), "thrustDeviceStableSortByKey(REAL)");
 }
 void thrustDeviceStableSortByKey(int *arr, int *arrIdx, int size) {
-    (void)segclip_stable_sort_by_key_i32(arr, arrIdx, size, nullptr);
+    throw_if(segclip_stable_sort_by_key_i32(arr, arrIdx, size, nullptr), "thrustDeviceStableSortByKey(int)");
 }
 void thrustDeviceStableSortByKey(long *arr, int *arrIdx, int size) {
-    (void)segclip_stable_sort_by_key_i64(reinterpret_cast<int64_t *>(arr), arrIdx, size, nullptr);
+    throw_if(segclip_stable_sort_by_key_i64(reinterpret_cast<int64_t *>(arr), arrIdx, size, nullptr),
+             "thrustDeviceStableSortByKey(LONG)");
 }
-void thrustDeviceExclusiveScan(int *arr, int size) { (void)segclip_exclusive_scan_i32(arr, size, nullptr); }
+void thrustDeviceExclusiveScan(int *arr, int size) {
+    throw_if(segclip_exclusive_scan_i32(arr, size, nullptr), "thrustDeviceExclusiveScan(int)");
+}
 void thrustDeviceExclusiveScan(long *arr, int size) {
-    (void)segclip_exclusive_scan_i64(reinterpret_cast<int64_t *>(arr), size, nullptr);
+    throw_if(segclip_exclusive_scan_i64(reinterpret_cast<int64_t *>(arr), size, nullptr),
+             "thrustDeviceExclusiveScan(long)");
+}
+// Host stable sort of (REAL key, int payload), thrust_func.cu:40-42: the
+// payload follows its key, ties keep their order.
+void thrustHostStableSortByKey(double *arr, int *arrIdx, int size) {
+    if (size <= 1) return;
+    std::vector<int> perm(static_cast<size_t>(size));
+    for (int i = 0; i < size; ++i) perm[size_t(i)] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return arr[a] < arr[b]; });
+    std::vector<double> k(static_cast<size_t>(size));
+    std::vector<int> v(static_cast<size_t>(size));
+    for (int i = 0; i < size; ++i) {
+        k[size_t(i)] = arr[perm[size_t(i)]];
+        v[size_t(i)] = arrIdx[perm[size_t(i)]];
+    }
+    std::copy(k.begin(), k.end(), arr);
+    std::copy(v.begin(), v.end(), arrIdx);
 }

@@ -22,12 +22,15 @@ def clip_rings(b, o):
 
 
 def area(b, o):
-    return 0.5 * sum(shoelace2(r) for r in clip_rings(b, o).rings)
+    """Intersection area: the output rings' unsigned areas (an intersection of
+    two simple rings has no holes; with a clockwise input the oracle's output
+    can hold rings of both orientations)."""
+    return 0.5 * sum(abs(shoelace2(r)) for r in clip_rings(b, o).rings)
 
 
 def area_iou(b, o):
     i = area(b, o)
-    p, q = 0.5 * shoelace2(b), 0.5 * shoelace2(o)
+    p, q = 0.5 * abs(shoelace2(b)), 0.5 * abs(shoelace2(o))
     u = p + q - i
     return i, (i / u if u != 0 else 0.0)
 
@@ -56,9 +59,10 @@ def grad(b, o):
     unmatched = 0
     for ring in res.rings:
         n = len(ring)
+        sg = -1.0 if shoelace2(ring) < 0 else 1.0  # d|S| = sgn(S) dS
         for i in range(n):
             nx, pv = ring[(i + 1) % n], ring[(i - 1) % n]
-            g = 0.5 * np.array([nx[1] - pv[1], pv[0] - nx[0]])
+            g = 0.5 * sg * np.array([nx[1] - pv[1], pv[0] - nx[0]])
             key = (float(ring[i, 0]), float(ring[i, 1]))
             if key in vb:
                 gb[vb[key]] += g

@@ -59,3 +59,28 @@ def test_planted_sign_error_is_caught():
         R._dA = orig
     scale = max(np.max(np.abs(fb)), np.max(np.abs(fo)))
     assert max(np.max(np.abs(gb2 - fb)), np.max(np.abs(go2 - fo))) > 1e-3 * scale
+
+
+def test_clockwise_rings_same_area_and_gradient_matches_fd():
+    """Area and IoU do not depend on ring orientation; with clockwise rings the
+    output mixes orientations, and the gradient (carrying each ring's sign)
+    still equals central finite differences."""
+    b0, o0 = S.star(300, 5, 0.0, 0.0, 0.6, 1.0), S.star(200, 6, 0.3, 0.1, 0.3, 0.8)
+    want = R.area_iou(b0, o0)
+    for fb in (False, True):
+        for fo in (False, True):
+            b = b0[::-1].copy() if fb else b0
+            o = o0[::-1].copy() if fo else o0
+            got = R.area_iou(b, o)
+            assert got[0] == pytest.approx(want[0], rel=1e-12)
+            assert got[1] == pytest.approx(want[1], rel=1e-12)
+    # the signed sum differs from the area for the clockwise base: mixed orientations
+    rings = R.clip_rings(b0[::-1].copy(), o0).rings
+    assert abs(0.5 * sum(R.shoelace2(r) for r in rings) - want[0]) > 1e-3
+    for b, o in _generic_pairs(24, 4200)[:2]:
+        b, o = b[::-1].copy(), o[::-1].copy()
+        gb, go = R.grad(b, o)
+        fb, fo = R.fd_grad(b.copy(), o.copy())
+        scale = max(np.max(np.abs(fb)), np.max(np.abs(fo)))
+        assert np.max(np.abs(gb - fb)) <= 1e-6 * scale
+        assert np.max(np.abs(go - fo)) <= 1e-6 * scale

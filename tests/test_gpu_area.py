@@ -67,8 +67,8 @@ def test_batch_area_matches_cpu(gpu):
     for j, (b, o) in enumerate(zip(bases, ovs)):
         i, iou = R.area_iou(b, o)
         assert a[0, j] == pytest.approx(i, rel=1e-12, abs=1e-13), j
-        assert a[1, j] == pytest.approx(0.5 * R.shoelace2(b), rel=1e-12), j
-        assert a[2, j] == pytest.approx(0.5 * R.shoelace2(o), rel=1e-12), j
+        assert a[1, j] == pytest.approx(abs(0.5 * R.shoelace2(b)), rel=1e-12), j  # touch_inside: clockwise
+        assert a[2, j] == pytest.approx(abs(0.5 * R.shoelace2(o)), rel=1e-12), j
         assert a[3, j] == pytest.approx(iou, rel=1e-11, abs=1e-13), j
     # the same bits on a second run (fixed-order reductions)
     cl.clip_batch()
@@ -151,3 +151,49 @@ def test_autograd_binding(gpu, recompute):
     assert np.array_equal(area.detach().cpu().numpy(), a0[0])
     assert np.allclose(bx.grad.cpu().numpy(), gb, rtol=0, atol=1e-13)
     assert np.allclose(ox.grad.cpu().numpy(), gc, rtol=0, atol=1e-13)
+
+
+def test_clockwise_rings(gpu):
+    """Clockwise base and/or overlay rings: areas and IoU equal the CPU
+    reference's (orientation-free), the gradient equals the CPU reference
+    gradient and central finite differences of the GPU's own areas."""
+    import segclip
+    b0, o0 = S.star(300, 5, 0.0, 0.0, 0.6, 1.0), S.star(200, 6, 0.3, 0.1, 0.3, 0.8)
+    pairs = [(b0, o0)] + _generic(64, 4400, 6)
+    bases, ovs = [], []
+    for i, (b, o) in enumerate(pairs):
+        for fb, fo in ((True, False), (False, True), (True, True)):
+            bases.append(b[::-1].copy() if fb else b)
+            ovs.append(o[::-1].copy() if fo else o)
+    print("[gpu-area] clockwise rings", len(bases), flush=True)
+    cl = segclip.Clipper().set_pairs(bases, ovs).clip_batch()
+    a = cl.batch_area()
+    for j, (b, o) in enumerate(zip(bases, ovs)):
+        i, iou = R.area_iou(b, o)
+        assert a[0, j] == pytest.approx(i, rel=1e-12, abs=1e-13), j
+        assert a[1, j] == pytest.approx(abs(0.5 * R.shoelace2(b)), rel=1e-12), j
+        assert a[2, j] == pytest.approx(abs(0.5 * R.shoelace2(o)), rel=1e-12), j
+        assert a[3, j] == pytest.approx(iou, rel=1e-11, abs=1e-13), j
+    rng = np.random.default_rng(11)
+    gi = rng.standard_normal(len(bases))
+    _, gb, gc = _gpu_grad(bases, ovs, np.ones(len(bases)), gi)
+    gbs, gcs = _split(gb, [len(x) for x in bases]), _split(gc, [len(x) for x in ovs])
+    for j in range(3, len(bases)):  # the fd-safe pairs: CPU reference gradient of the area alone
+        _, xb, xc = _gpu_grad([bases[j]], [ovs[j]])
+        rb, rc = R.grad(bases[j], ovs[j])
+        scale = max(1.0, np.max(np.abs(rb)), np.max(np.abs(rc)))
+        assert np.max(np.abs(xb - rb)) <= 1e-12 * scale, j
+        assert np.max(np.abs(xc - rc)) <= 1e-12 * scale, j
+    db = [rng.standard_normal(b.shape) for b in bases[3:]]
+    dc = [rng.standard_normal(o.shape) for o in ovs[3:]]
+
+    def loss(sign):
+        c2 = segclip.Clipper().set_pairs([b + sign * FD_H * d for b, d in zip(bases[3:], db)],
+                                         [o + sign * FD_H * d for o, d in zip(ovs[3:], dc)]).clip_batch()
+        x = c2.batch_area()
+        return x[0] + gi[3:] * x[3]
+
+    fd = (loss(1.0) - loss(-1.0)) / (2 * FD_H)
+    for j in range(len(db)):
+        an = np.sum(gbs[j + 3] * db[j]) + np.sum(gcs[j + 3] * dc[j])
+        assert abs(an - fd[j]) <= FD_REL * max(1.0, abs(fd[j])), (j, an, fd[j])
