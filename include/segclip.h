@@ -185,7 +185,8 @@ typedef struct {
     int32_t overlay_pairs;    /* I of that overlay */
     int32_t nb_aug, nc_aug;   /* augmented ring sizes */
     int32_t rings;            /* output components after clean-up */
-    int32_t undefined;        /* bit 1: an intersection got no slot; 2: ring walk; 4: traversal */
+    int32_t undefined;        /* bit 1: an intersection got no slot; 2: ring walk; 4: traversal;
+                                 16: a quantised alpha or beta beyond the int range */
     int64_t out_vertices;
     int64_t stab_units;       /* work units (waves) of the stab */
 } segclip_counts;

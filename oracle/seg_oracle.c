@@ -29,9 +29,27 @@
  */
 #include "so_internal.h"
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Threads: the loops the reference runs with OpenMP (insertAllParallel
+ * oacc:2571-2607, insertToLeafLevelEdgeListMulticore oacc:2010-2113,
+ * buildInternalEndListsMulticore oacc:1717-1968, saveIntersectionsIdsMulticore
+ * oacc:6010-6244, saveIntersectionsParallel oacc:7257-7341,
+ * sortIntersectionsAtEdgesParallel oacc:7436-7473, calculateInitLabelParallel
+ * oacc:7925-7988) run as OpenMP loops here too.  Every parallel loop writes
+ * its results in the single-thread order (per-node lists re-sorted by edge
+ * index, stab output concatenated in node order), so the results do not
+ * depend on the thread count. */
+void so_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int so_get_threads(void) { return omp_get_max_threads(); }
+
+static int cmp_int(const void *a, const void *b) {
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
 
 void *so_xmalloc(size_t n) {
     void *p = malloc(n ? n : 1);
@@ -370,8 +388,15 @@ static int edge_intersects(const so_edge *x, double es, double ee) { /* edge.h:1
 static void insert_rec(const so_tree *t, const so_edge *x, int eidx, int v, int64_t *cnt, int *out,
                        int64_t *fill) {
     if (edge_contains(x, t->niv[2 * v], t->niv[2 * v + 1])) {
-        if (out) out[fill[v]++] = eidx;
-        else cnt[v]++;
+        if (out) {
+            int64_t at;
+#pragma omp atomic capture
+            at = fill[v]++;
+            out[at] = eidx;
+        } else {
+#pragma omp atomic
+            cnt[v]++;
+        }
         return;
     }
     int l = 2 * v, r = 2 * v + 1;
@@ -381,9 +406,14 @@ static void insert_rec(const so_tree *t, const so_edge *x, int eidx, int v, int6
         insert_rec(t, x, eidx, r, cnt, out, fill);
 }
 
+/* insertAllParallel: every edge inserted recursively, in parallel over
+ * edges (the reference locks the node, oacc:2527; here the node's counter
+ * is advanced atomically).  Each node's list is then sorted by edge index,
+ * the order of a single-thread insertion in edge order. */
 static void build_cover(so_tree *t) {
     int T = t->T;
     int64_t *cnt = so_xcalloc((size_t)T + 1, sizeof(int64_t));
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int e = 0; e < t->ne; e++) insert_rec(t, &t->edges[e], e, 1, cnt, NULL, NULL);
     t->cov_off = so_xmalloc(sizeof(int64_t) * ((size_t)T + 1));
     int64_t run = 0;
@@ -394,7 +424,13 @@ static void build_cover(so_tree *t) {
     t->cov_e = so_xmalloc(sizeof(int) * (size_t)(run > 0 ? run : 1));
     int64_t *fill = cnt;
     for (int v = 0; v < T; v++) fill[v] = t->cov_off[v];
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int e = 0; e < t->ne; e++) insert_rec(t, &t->edges[e], e, 1, NULL, t->cov_e, fill);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int v = 1; v < T; v++) {
+        const int64_t n = t->cov_off[v + 1] - t->cov_off[v];
+        if (n > 1) qsort(t->cov_e + t->cov_off[v], (size_t)n, sizeof(int), cmp_int);
+    }
     free(cnt);
 }
 
@@ -462,6 +498,7 @@ static void build_end_lists(so_tree *t) {
     t->after_leaf = so_xmalloc(sizeof(int) * (size_t)t->ne);
     t->before_leaf = so_xmalloc(sizeof(int) * (size_t)t->ne);
     int64_t *lcnt = so_xcalloc((size_t)T + 1, sizeof(int64_t));
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < t->ne; e++) {
         int en = find_boundary(t, t->edges[e].end);
         int st = find_boundary(t, t->edges[e].start);
@@ -476,8 +513,14 @@ static void build_end_lists(so_tree *t) {
         }
         t->after_leaf[e] = after;
         t->before_leaf[e] = before;
-        if (after >= 0) lcnt[after]++;
-        if (before >= 0) lcnt[before]++;
+        if (after >= 0) {
+#pragma omp atomic
+            lcnt[after]++;
+        }
+        if (before >= 0) {
+#pragma omp atomic
+            lcnt[before]++;
+        }
     }
     /* per node lists, built level by level (buildInternalEndListsMulticore) */
     int64_t *off = so_xcalloc((size_t)T + 1, sizeof(int64_t));
@@ -493,16 +536,25 @@ static void build_end_lists(so_tree *t) {
     }
     int maxId = t->nb >= t->nct + t->K ? t->nb : t->nct + t->K; /* clip:1581-1585 */
     {
+        /* (type, id) keys are distinct within a leaf, so the order after the
+         * sort does not depend on the scatter order */
         int64_t mx = 0;
         for (int v = half; v < T; v++) if (lens[v] > mx) mx = lens[v];
-        int *tmp = so_xmalloc(sizeof(int) * (size_t)(mx > 0 ? mx : 1));
-        for (int v = half; v < T; v++) leaf_radix(t, lists[v], (int)lens[v], tmp, 2 * maxId);
-        free(tmp);
+#pragma omp parallel
+        {
+            int *tmp = so_xmalloc(sizeof(int) * (size_t)(mx > 0 ? mx : 1));
+#pragma omp for schedule(dynamic, 1024)
+            for (int v = half; v < T; v++) leaf_radix(t, lists[v], (int)lens[v], tmp, 2 * maxId);
+            free(tmp);
+        }
     }
     /* internal levels: id-merge with per-type fill buffer (oacc:1758-1950) */
     int ntypes = t->K + 1; /* POLY_TYPE_COUNT = argc-1 */
-    int *fill = so_xmalloc(sizeof(int) * (size_t)ntypes);
     for (int lstart = half / 2; lstart >= 1; lstart /= 2) {
+#pragma omp parallel
+        {
+        int *fill = so_xmalloc(sizeof(int) * (size_t)ntypes);
+#pragma omp for schedule(dynamic, 256)
         for (int v = lstart; v < 2 * lstart; v++) {
             int c1 = 2 * v, c2 = 2 * v + 1;
             int64_t n1 = lens[c1], n2 = lens[c2];
@@ -553,8 +605,9 @@ static void build_end_lists(so_tree *t) {
             lists[v] = out;
             lens[v] = ne_;
         }
+        free(fill);
+        }
     }
-    free(fill);
     int64_t run = 0;
     for (int v = 0; v <= T; v++) {
         off[v] = run;
@@ -654,11 +707,12 @@ static void test_pair(const so_tree *t, int bId, int cId, trip_vec *out) {
     }
 }
 
-static void stab(const so_tree *t, int cType, trip_vec *out) {
+/* One node's pairs in the reference's loop order (oacc:6149-6241). */
+static void stab_node(const so_tree *t, int v, int cType, trip_vec *out) {
     const int bType = 0;
-    for (int v = 1; v < t->T; v++) {
+    {
         int64_t cs = t->cov_off[v], ce = t->cov_off[v + 1];
-        if (cs == ce) continue; /* no Node allocated, oacc:6149-6150 */
+        if (cs == ce) return; /* no Node allocated, oacc:6149-6150 */
         for (int64_t i = cs; i < ce; i++) {
             const so_edge *it = &t->edges[t->cov_e[i]];
             for (int64_t j = cs; j < ce; j++) {
@@ -673,6 +727,42 @@ static void stab(const so_tree *t, int cType, trip_vec *out) {
             }
         }
     }
+}
+
+/* saveIntersectionsIdsMulticore: every node, in parallel over chunks of
+ * consecutive nodes (the reference appends under omp critical,
+ * oacc:6170,6226); the chunks' pairs are concatenated in node order, which is
+ * the single-thread emission order. */
+static void stab(const so_tree *t, int cType, trip_vec *out) {
+    const int CH = 64;
+    const int nch = (t->T + CH - 1) / CH;
+    trip_vec *part = so_xcalloc((size_t)nch, sizeof(trip_vec));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < nch; q++) {
+        const int v0 = q * CH > 1 ? q * CH : 1, v1 = (q + 1) * CH < t->T ? (q + 1) * CH : t->T;
+        for (int v = v0; v < v1; v++) stab_node(t, v, cType, &part[q]);
+    }
+    int64_t total = out->n;
+    for (int q = 0; q < nch; q++) total += part[q].n;
+    if (total > out->cap) {
+        out->cap = total > 0 ? total : 1;
+        out->b = realloc(out->b, sizeof(int) * (size_t)out->cap);
+        out->c = realloc(out->c, sizeof(int) * (size_t)out->cap);
+        out->ty = realloc(out->ty, sizeof(int) * (size_t)out->cap);
+        if (!out->b || !out->c || !out->ty) abort();
+    }
+    for (int q = 0; q < nch; q++) {
+        if (part[q].n) {
+            memcpy(out->b + out->n, part[q].b, sizeof(int) * (size_t)part[q].n);
+            memcpy(out->c + out->n, part[q].c, sizeof(int) * (size_t)part[q].n);
+            memcpy(out->ty + out->n, part[q].ty, sizeof(int) * (size_t)part[q].n);
+            out->n += part[q].n;
+        }
+        free(part[q].b);
+        free(part[q].c);
+        free(part[q].ty);
+    }
+    free(part);
 }
 
 /* ------------------------------------------------------------------ */
@@ -776,10 +866,19 @@ static void exclusive_scan_int(int *a, int n) {
     }
 }
 
-/* alphaToInt, oacc:7247-7252 */
-static int alpha_to_int(double a) {
+/* alphaToInt, oacc:7247-7252.  The reference converts 1e9*a to int without
+ * a range check; beyond INT_MAX that conversion is undefined (x86 gives
+ * INT_MIN, and the 9-digit radix sort then indexes its counts with a negative
+ * digit).  Defined here as saturation to INT_MAX, the value the GPU's
+ * conversion gives, and reported through *oor (status bit 16). */
+static int alpha_to_int(double a, int *oor) {
     if (a < SO_EPS) return 0;
-    return (int)((double)1000000000 * a);
+    const double q = (double)1000000000 * a;
+    if (!(q < 2147483647.0)) {
+        *oor = 1;
+        return 2147483647;
+    }
+    return (int)q;
 }
 
 /* radixsort(LONG*,...) over [s,e) with 9 passes, oacc:616-689 as called by
@@ -819,6 +918,7 @@ static void sort_at_edges(const int *qa, const int *sidx, int cStart, const int 
         out[i] = i;
         cp[i] = qa[sidx[i]];
     }
+#pragma omp parallel for schedule(dynamic, 64)
     for (int u = 0; u < nu; u++) {
         int eid = uids[u] / 2 - cStart;
         int s = all[eid], e = all[eid + 1];
@@ -923,8 +1023,10 @@ so_result *so_clip(so_tree *t, int k, int canonical) {
     int nbAug = bSize + bNd[bSize], ncAug = cSize + cNd[cSize];
 
     /* A12: saveIntersectionsParallel, oacc:7257-7341 */
+    int oor = 0; /* a quantised alpha / beta beyond the int range */
     double *inter = so_xcalloc(2 * (size_t)(I > 0 ? I : 1), sizeof(double));
     int *qaB = so_xcalloc((size_t)(I > 0 ? I : 1), sizeof(int)), *qaC = so_xcalloc((size_t)(I > 0 ? I : 1), sizeof(int));
+#pragma omp parallel for schedule(static) reduction(| : oor)
     for (int id = 0; id < I; id++) {
         int bId = bIds[id], cId = cIds[id];
         double alpha, beta;
@@ -949,9 +1051,10 @@ so_result *so_clip(so_tree *t, int k, int canonical) {
             break;
         }
         types[id] = ty;
-        qaB[id] = alpha_to_int(alpha);
-        qaC[id] = alpha_to_int(beta);
+        qaB[id] = alpha_to_int(alpha, &oor);
+        qaC[id] = alpha_to_int(beta, &oor);
     }
+    if (oor) r->status |= 16;
     r->inter = inter;
     r->qa_b = qaB;
     r->qa_c = qaC;
@@ -1044,6 +1147,8 @@ so_result *so_clip(so_tree *t, int k, int canonical) {
 
     /* A15: calculateInitLabelParallel, oacc:7925-7988 */
     int *labB = so_xcalloc((size_t)nbAug, sizeof(int)), *labC = so_xcalloc((size_t)ncAug, sizeof(int));
+    int lab_undef = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : lab_undef)
     for (int bid = 0; bid < bSize; ++bid) {
         int start = bid + bNd[bid], end = bid + bNd[bid + 1] + 1;
         for (int i = start; i < end; i++) {
@@ -1058,7 +1163,7 @@ so_result *so_clip(so_tree *t, int k, int canonical) {
             int pPN = avB[tp] != -100 ? nbB[tp] - 1 : -100;
             int nId = nbB[i] - 1;
             if (nId < 0) {
-                r->status |= 1;
+                lab_undef |= 1;
                 continue;
             }
             int qm = (nId - 1 == -1) ? ncAug - 1 : (nId - 1 == ncAug ? 0 : nId - 1);
@@ -1094,6 +1199,7 @@ so_result *so_clip(so_tree *t, int k, int canonical) {
             labC[nId] = lab;
         }
     }
+    r->status |= lab_undef;
     r->nbaug = nbAug;
     r->ncaug = ncAug;
     r->aug_b = augB;

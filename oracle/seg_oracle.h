@@ -19,6 +19,10 @@
 extern "C" {
 #endif
 
+/* OpenMP threads of the oracle's parallel loops (results do not depend on it) */
+void so_set_threads(int n);
+int so_get_threads(void);
+
 /* one edge record, p_seg_tree/segment_tree/edge.h:141-202 */
 typedef struct {
     int id;       /* 2 * vertex offset of the edge's first vertex in its poly array */
@@ -73,7 +77,8 @@ void so_result_free(so_result *r);
  *   1 an intersection was never assigned a slot (copyIntersectionsParallel),
  *   2 ring/array coordinate walk diverged (copyToLinkedList),
  *   4 a traversal did not close (createResult2),
- *   8 id radix sort truncated (ids wider than the digit count used) */
+ *   8 id radix sort truncated (ids wider than the digit count used),
+ *  16 a quantised alpha or beta beyond the int range (alphaToInt) */
 int so_res_status(const so_result *r);
 int64_t so_res_num_candidates(const so_result *r);  /* K (with duplicates) */
 int so_res_num_pairs(const so_result *r);            /* I */

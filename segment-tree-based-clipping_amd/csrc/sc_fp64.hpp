@@ -218,9 +218,16 @@ __device__ inline YRangeF yrange_empty() {
     return r;
 }
 
+// alphaToInt (oacc:7247-7252).  The reference converts 1e9*a to int with no
+// range check; beyond INT_MAX that is undefined behaviour (on x86 INT_MIN,
+// which its 9-digit radix sort then uses as a negative digit index).  Here it
+// saturates to INT_MAX and alpha_in_range() lets the caller report the job
+// as undefined (status bit 16).
+__host__ __device__ inline bool alpha_in_range(double a) { return a < EPS || 1000000000.0 * a < 2147483647.0; }
 __host__ __device__ inline int alpha_to_int(double a) {
     if (a < EPS) return 0;
-    return int(1000000000.0 * a);
+    const double q = 1000000000.0 * a;
+    return q < 2147483647.0 ? int(q) : 2147483647;
 }
 
 __host__ __device__ inline bool nondeg_b(int t) { return t == 1 || t == 3 || t == 5 || t == 7; }

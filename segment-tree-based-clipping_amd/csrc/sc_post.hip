@@ -38,7 +38,7 @@ inline unsigned grid_for(int64_t n, int per = BLK) {
     return unsigned(g);
 }
 
-enum : int { UB_UNASSIGNED = 1 };
+enum : int { UB_UNASSIGNED = 1, UB_ALPHA_RANGE = 16 };
 
 struct JobArgs {
     int J;
@@ -90,6 +90,7 @@ __global__ void k_pair_decode(JobArgs ja, const uint64_t *__restrict__ pairs, in
             py[p] = y;
             qa[p] = alpha_to_int(al);
             qb[p] = alpha_to_int(be);
+            if (!alpha_in_range(al) || !alpha_in_range(be)) atomicOr(&flags[j], UB_ALPHA_RANGE);
             arr_c[p] = atomicAdd(&all_c[ja.JC[j] + cl], 1); // arrival inside the overlay edge's group
             if (nondeg_c(t)) atomicAdd(&nd_c[ja.JC[j] + cl], 1);
             gb = ja.JB[j] + b;

@@ -78,3 +78,23 @@ def mixed_overlays(seed=40):
                                                (8, 0.3, 0.3, 0.005, 0.01), (5000, 0.1, -0.1, 0.3, 1.2)]):
         out.append(S.star(n, seed + i, cx, cy, r0, r1))
     return out
+
+
+def rings_digest(rings):
+    """64-bit digest of a job's output rings: the ring lengths (int32) then the
+    coordinates (float64 bits), in ring order (blake2b, 8 bytes)."""
+    import hashlib
+    h = hashlib.blake2b(digest_size=8)
+    h.update(np.array([len(r) for r in rings], np.int32).tobytes())
+    for r in rings:
+        h.update(np.ascontiguousarray(r, np.float64).tobytes())
+    return int.from_bytes(h.digest(), "little")
+
+
+def rings_area(rings):
+    """Unsigned area of a job's output: sum of the rings' |shoelace| / 2."""
+    a = 0.0
+    for r in rings:
+        x, y = r[:, 0], r[:, 1]
+        a += abs(float(np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y)))
+    return 0.5 * a

@@ -64,8 +64,19 @@ def lib():
         L.so_classify.argtypes = [dp, dp, dp]
         L.so_rings_area.restype = C.c_double
         L.so_rings_area.argtypes = [ip, dp, C.c_int]
+        L.so_set_threads.argtypes = [C.c_int]
+        L.so_get_threads.restype = C.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (results do not depend on it)."""
+    lib().so_set_threads(int(n))
+
+
+def get_threads():
+    return lib().so_get_threads()
 
 
 EDGE_DTYPE = np.dtype([("id", np.int32), ("type", np.int32), ("start", np.float64), ("end", np.float64),
