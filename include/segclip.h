@@ -246,6 +246,13 @@ int segclip_get_stage_times(segclip_ctx *ctx, float *ms8);
  *   "stab_tiles"  64-column tiles per stab work unit, 1..16, default 1.
  *   "stab_units_per_wave"  consecutive stab work units one wave runs (hits
  *                 staged across them), 1..32, default 8.
+ *   "small_fused" 1 (default): a batch of independent pairs whose rings all
+ *                 have at most 64 vertices, without provenance, runs as one
+ *                 persistent launch, each pair clipped whole in LDS; 0: the
+ *                 staged path.  A batch holding a pair the persistent kernel
+ *                 does not cover reruns on the staged path.
+ * segclip_get_tuning also reads "small_fallback": the fallback reasons of the
+ * last small-pair batch (0 when it ran as one launch).
  * An unknown name or a value out of range returns SEGCLIP_EINVAL. */
 int segclip_set_tuning(segclip_ctx *ctx, const char *name, int value);
 int segclip_get_tuning(segclip_ctx *ctx, const char *name, int *value);

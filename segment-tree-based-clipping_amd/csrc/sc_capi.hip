@@ -567,6 +567,9 @@ int segclip_set_tuning(segclip_ctx *h, const char *name, int value) {
         } else if (std::strcmp(name, "stab_units_per_wave") == 0) {
             SC_REQUIRE(value >= 1 && value <= 32, SEGCLIP_EINVAL, "stab_units_per_wave must be in 1..32");
             c.stab_upw = value;
+        } else if (std::strcmp(name, "small_fused") == 0) {
+            SC_REQUIRE(value == 0 || value == 1, SEGCLIP_EINVAL, "small_fused must be 0 or 1");
+            c.small_fused = value;
         } else {
             SC_REQUIRE(false, SEGCLIP_EINVAL, std::string("unknown tuning setting: ") + name);
         }
@@ -580,6 +583,10 @@ int segclip_get_tuning(segclip_ctx *h, const char *name, int *value) {
             *value = c.stab_tiles;
         } else if (std::strcmp(name, "stab_units_per_wave") == 0) {
             *value = c.stab_upw;
+        } else if (std::strcmp(name, "small_fused") == 0) {
+            *value = c.small_fused;
+        } else if (std::strcmp(name, "small_fallback") == 0) {
+            *value = c.small_fail;
         } else {
             SC_REQUIRE(false, SEGCLIP_EINVAL, std::string("unknown tuning setting: ") + name);
         }

@@ -152,6 +152,10 @@ struct Ctx {
     int stab_tiles = 1;
     // tuning: consecutive stab work units per wave (1..32); results do not depend on it
     int stab_upw = 8;
+    // independent pairs of up to 64 + 64 vertices clipped by the persistent
+    // small-pair kernel (1) or the staged path (0); results do not depend on it
+    int small_fused = 1;
+    int small_fail = 0; // fallback reasons of the last small-pair batch (0: it ran fused)
 
     bool timing = false;
     StageTimes times;
@@ -194,6 +198,11 @@ void post_overlay(Ctx &c, int k);      // A11-A15 for overlay k
 void label_and_traverse(Ctx &c);       // A16-A17 for the jobs of the last post_jobs
 void clip_all(Ctx &c, int k);          // full pipeline for one overlay
 void clip_jobs(Ctx &c, int k0, int k1); // full pipeline for overlays k0..k1 as one batch
+// paired batches of small rings in one persistent launch: whether batch k0..k1
+// qualifies, and the clip (false: some pair needs the staged path; nothing
+// of the batch's results was written)
+bool small_batch_eligible(const Ctx &c, int k0, int k1);
+bool clip_small_batch(Ctx &c, int k0, int k1);
 // A19: per job of the last batch, areas (intersection, base ring, overlay ring)
 // and IoU into out4 (device, 4*J doubles); gradient of sum(g_area*area +
 // g_iou*iou) w.r.t. every input vertex into grad_b (2*nb) and grad_c (2*nct)

@@ -2152,6 +2152,20 @@ void label_and_traverse(Ctx &c) {
 void clip_jobs(Ctx &c, int k0, int k1) {
     SC_REQUIRE(k0 >= 1 && k1 <= c.K && k0 <= k1, SC_EINVAL, "overlay index out of range");
     StageTimes t;
+    // independent small pairs: one persistent launch (sc_small.hip); a batch
+    // it does not cover runs on the staged path below
+    if (small_batch_eligible(c, k0, k1)) {
+        for (int i = 0; i <= 6; ++i) c.mark(i);
+        if (clip_small_batch(c, k0, k1)) {
+            c.mark(7);
+            if (c.timing) {
+                SC_HIP(hipEventSynchronize(c.ev[7]));
+                t.label = c.elapsed(6, 7);
+                c.times = t;
+            }
+            return;
+        }
+    }
     const bool tree = c.T == 0, cand = tree || int(c.pair_off.size()) != c.K + 1;
     c.mark(0);
     if (tree) {

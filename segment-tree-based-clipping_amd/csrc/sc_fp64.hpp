@@ -230,6 +230,35 @@ __host__ __device__ inline int alpha_to_int(double a) {
     return q < 2147483647.0 ? int(q) : 2147483647;
 }
 
+// Initial Foster label of an intersection vertex (calculateInitLabelParallel,
+// oracle and getInitialLabel, oacc:7847-7920): where the neighbours Q- and Q+
+// of its partner lie relative to (P-, I, P+); pMN / pPN are the partners of
+// P- / P+ when those are intersections (-100 otherwise).
+__host__ __device__ inline int rel_pos(int pMN, int pPN, int q, double qx, double qy, double pmx, double pmy, double cx,
+                              double cy, double ppx, double ppy) {
+    if (pMN != -100 && pMN == q) return 2;
+    if (pPN != -100 && pPN == q) return 3;
+    double s1 = areaA(qx, qy, pmx, pmy, cx, cy);
+    double s2 = areaA(qx, qy, cx, cy, ppx, ppy);
+    double s3 = areaA(pmx, pmy, cx, cy, ppx, ppy);
+    if (s3 > 0) return (s1 > 0 && s2 > 0) ? 0 : 1;
+    return (s1 < 0 && s2 < 0) ? 1 : 0;
+}
+
+__host__ __device__ inline int label_of(int pMN, int pPN, int qm, int qp, double qmx, double qmy, double qpx, double qpy,
+                               double pmx, double pmy, double X, double Y, double ppx, double ppy) {
+    int a = rel_pos(pMN, pPN, qm, qmx, qmy, pmx, pmy, X, Y, ppx, ppy);
+    int b = rel_pos(pMN, pPN, qp, qpx, qpy, pmx, pmy, X, Y, ppx, ppy);
+    if ((a == 0 && b == 1) || (a == 1 && b == 0)) return 1;
+    if ((a == 0 && b == 0) || (a == 1 && b == 1)) return 2;
+    if ((b == 3 && a == 1) || (a == 3 && b == 1)) return 3;
+    if ((b == 3 && a == 0) || (a == 3 && b == 0)) return 4;
+    if ((b == 3 && a == 2) || (a == 3 && b == 2)) return 5;
+    if ((a == 2 && b == 1) || (b == 2 && a == 1)) return 6;
+    if ((a == 2 && b == 0) || (b == 2 && a == 0)) return 7;
+    return -102;
+}
+
 __host__ __device__ inline bool nondeg_b(int t) { return t == 1 || t == 3 || t == 5 || t == 7; }
 __host__ __device__ inline bool nondeg_c(int t) { return t == 1 || t == 2 || t == 6; }
 __host__ __device__ inline bool degen_b_first(int t) { return t == 2 || t == 4 || t == 6 || t == 8; }

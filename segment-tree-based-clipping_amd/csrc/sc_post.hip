@@ -246,31 +246,6 @@ __global__ void k_link(const int *__restrict__ sb, const int *__restrict__ scn, 
     nbr_c[b] = a - AB[j] + 1;
 }
 
-__device__ inline int rel_pos(int pMN, int pPN, int q, double qx, double qy, double pmx, double pmy, double cx,
-                              double cy, double ppx, double ppy) {
-    if (pMN != -100 && pMN == q) return 2;
-    if (pPN != -100 && pPN == q) return 3;
-    double s1 = areaA(qx, qy, pmx, pmy, cx, cy);
-    double s2 = areaA(qx, qy, cx, cy, ppx, ppy);
-    double s3 = areaA(pmx, pmy, cx, cy, ppx, ppy);
-    if (s3 > 0) return (s1 > 0 && s2 > 0) ? 0 : 1;
-    return (s1 < 0 && s2 < 0) ? 1 : 0;
-}
-
-__device__ inline int label_of(int pMN, int pPN, int qm, int qp, double qmx, double qmy, double qpx, double qpy,
-                               double pmx, double pmy, double X, double Y, double ppx, double ppy) {
-    int a = rel_pos(pMN, pPN, qm, qmx, qmy, pmx, pmy, X, Y, ppx, ppy);
-    int b = rel_pos(pMN, pPN, qp, qpx, qpy, pmx, pmy, X, Y, ppx, ppy);
-    if ((a == 0 && b == 1) || (a == 1 && b == 0)) return 1;
-    if ((a == 0 && b == 0) || (a == 1 && b == 1)) return 2;
-    if ((b == 3 && a == 1) || (a == 3 && b == 1)) return 3;
-    if ((b == 3 && a == 0) || (a == 3 && b == 0)) return 4;
-    if ((b == 3 && a == 2) || (a == 3 && b == 2)) return 5;
-    if ((a == 2 && b == 1) || (b == 2 && a == 1)) return 6;
-    if ((a == 2 && b == 0) || (b == 2 && a == 0)) return 7;
-    return -102;
-}
-
 __global__ void k_init_labels(int J, const int *__restrict__ AB, const int *__restrict__ AC,
                               const double *__restrict__ bx, const double *__restrict__ by,
                               const int *__restrict__ avb, const int *__restrict__ nbrb,

@@ -11,8 +11,9 @@
   sequential route or are flagged undefined against the live oracle.
 * 16,384 small pairs and 256 large pairs: the whole batch against the per-pair
   summaries; the 8 smallest large pairs bit-exact against the live oracle;
-  recompute-in-backward equal to the stored path bit for bit; a directional
-  central finite-difference check of the gradient (h = 1e-7, rel 1e-6).
+  recompute-in-backward equal to the stored path (areas bit for bit,
+  gradients to summation order); a directional central finite-difference
+  check of the gradient (h = 1e-7, rel 1e-6).
 
 Status bits where the reference is undefined: bit 1 (an intersection never
 gets a slot), 4 (a traversal does not close), 16 (a quantised alpha beyond the
@@ -208,7 +209,6 @@ def test_large_pairs_recompute_and_fd(gpu, large_pairs):
     GPU's own areas on two pairs away from degeneracies."""
     import torch
     import segclip
-    import area_ref as R
     from segclip.autograd import clip_area_iou
     bases, ovs = _smallest(large_pairs, 8)
     bsz, csz = [len(b) for b in bases], [len(o) for o in ovs]
@@ -222,27 +222,38 @@ def test_large_pairs_recompute_and_fd(gpu, large_pairs):
         (a * torch.tensor(w, dtype=torch.float64, device="cuda") + iou).sum().backward()
         out.append((a.detach().cpu().numpy(), iou.detach().cpu().numpy(), bx.grad.cpu().numpy(),
                     ox.grad.cpu().numpy()))
-    for x, y in zip(out[0], out[1]):
+    # areas and IoU bit for bit; gradients to FP64 summation order (contributions
+    # meet at input vertices through atomic adds, whose order varies)
+    for x, y in zip(out[0][:2], out[1][:2]):
         assert _bits_equal(x.reshape(-1), y.reshape(-1))
-    # directional central differences on two pairs clear of the clean-up tolerance
+    for x, y in zip(out[0][2:], out[1][2:]):
+        assert np.max(np.abs(x - y)) <= 1e-12 * max(1.0, np.max(np.abs(x)))
+    # directional central differences on two pairs where the area is smooth
+    # along the direction: rings of 10k-200k vertices have consecutive triples
+    # near the clean-up tolerance (|A| < 1e-9 removes a vertex), and a step
+    # that flips such a decision is a jump, not a slope.  A pair qualifies when
+    # the differences at h and h/2 agree (a jump within h makes them differ).
     rng = np.random.default_rng(5)
     gb = np.split(out[0][2], np.cumsum(bsz)[:-1])
     gc = np.split(out[0][3], np.cumsum(csz)[:-1])
     checked = 0
     for j in range(len(bases)):
-        if R.min_consecutive_area(bases[j]) <= 1e-12 or R.min_consecutive_area(ovs[j]) <= 1e-12:
-            continue
         db, dc = rng.standard_normal(bases[j].shape), rng.standard_normal(ovs[j].shape)
-        h = 1e-7
 
-        def loss(sign):
-            c2 = segclip.Clipper().set_pairs([bases[j] + sign * h * db], [ovs[j] + sign * h * dc]).clip_batch()
+        def loss(t):
+            c2 = segclip.Clipper().set_pairs([bases[j] + t * db], [ovs[j] + t * dc]).clip_batch()
             x = c2.batch_area()
             return w[j] * x[0, 0] + x[3, 0]
 
-        fd = (loss(1.0) - loss(-1.0)) / (2 * h)
+        h = 1e-7
+        fd = (loss(h) - loss(-h)) / (2 * h)
+        fd2 = (loss(0.5 * h) - loss(-0.5 * h)) / h
         an = float(np.sum(gb[j] * db) + np.sum(gc[j] * dc))
-        print("[gpu-full] pair %d directional derivative %.12g fd %.12g" % (j, an, fd), flush=True)
+        smooth = abs(fd - fd2) <= 1e-7 * max(1.0, abs(fd))
+        print("[gpu-full] pair %d directional derivative %.12g fd %.12g (h/2: %.12g) smooth %s"
+              % (j, an, fd, fd2, smooth), flush=True)
+        if not smooth:
+            continue
         assert abs(an - fd) <= 1e-6 * max(1.0, abs(fd)), (j, an, fd)
         checked += 1
         if checked == 2:
