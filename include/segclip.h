@@ -223,6 +223,17 @@ int segclip_stable_sort_by_key_i64(int64_t *keys, int32_t *vals, int64_t n, void
 int segclip_exclusive_scan_i32(int32_t *data, int64_t n, void *stream);
 int segclip_exclusive_scan_i64(int64_t *data, int64_t n, void *stream);
 
+/* segclip_stable_sort_by_key_f64 without any device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same result as segclip_stable_sort_by_key_f64 (stable,
+ * in place, -0.0 == 0.0), for n <= 1,048,576 keys; a larger n returns
+ * SEGCLIP_ERANGE.  Its scratch is one fixed device array of about 38 MB per
+ * GPU, held until the process ends and shared by every call on that GPU:
+ * calls that overlap on one GPU race.  Asynchronous on 'stream' (no host
+ * wait); clears its scratch with kernels, not memset nodes. */
+int segclip_stable_sort_by_key_f64_noalloc(double *keys, int32_t *vals, int64_t n, void *stream);
+
 /* Seeded synthetic rings (host code; see the workload definitions) */
 int segclip_synth_star(double *xy, int n, uint64_t seed, double cx, double cy, double rmin, double rmax,
                        double rot);

@@ -207,6 +207,14 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int b
 void radix_sort_u64_buf(DBuf<uint64_t> &keys, DBuf<uint64_t> &keys_alt, DBuf<uint32_t> *vals,
                         DBuf<uint32_t> *vals_alt, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs, hipStream_t s);
 
+// The same sort on caller-provided memory only (no allocation): work holds
+// radix_fixed_words(n) 32-bit words, cleared here by fill kernels; kalt/valt
+// are scratch of n entries.  Returns true when the sorted data ends in
+// kalt/valt.  No host wait.
+size_t radix_fixed_words(int64_t n);
+bool radix_sort_u64_fixed(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, int64_t n, int bit_lo,
+                          int bit_hi, uint32_t *work, hipStream_t s);
+
 // Order-preserving map of an IEEE double to uint64 (-0.0 and +0.0 map to the
 // same key, so a stable sort on it orders ties by original position as a
 // comparison sort with operator< does).

@@ -593,6 +593,60 @@ bool radix_passes(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt
 }
 } // namespace
 
+size_t radix_fixed_words(int64_t n) {
+    return size_t(RS_MAXPASS) * 256 + RS_MAXPASS + 2 * size_t(ceil_div(n > 0 ? n : 1, RS_TILE)) * 256;
+}
+
+bool radix_sort_u64_fixed(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, int64_t n, int bit_lo,
+                          int bit_hi, uint32_t *work, hipStream_t s) {
+    if (n <= 1 || bit_hi <= bit_lo) return false;
+    SC_REQUIRE(n < (int64_t(1) << 32), SC_ERANGE, "radix sort longer than 2^32 keys");
+    SC_REQUIRE(bit_lo >= 0 && bit_hi <= 64, SC_EINVAL, "radix bit range");
+    const int npass = int(ceil_div(bit_hi - bit_lo, 8));
+    SC_REQUIRE(npass <= RS_MAXPASS, SC_EINVAL, "radix passes");
+    const size_t GH = size_t(RS_MAXPASS) * 256;
+    const int64_t ntiles = ceil_div(n, RS_TILE);
+    uint32_t *gh = work;
+    // 8-byte aligned: GH + RS_MAXPASS words is even
+    unsigned long long *st = reinterpret_cast<unsigned long long *>(work + GH + RS_MAXPASS);
+    {
+        // histograms, tile counters and every status word start at zero;
+        // cleared by a kernel (no memset node when a caller captures the stream)
+        FillBatch fb(s);
+        fb.add(gh, int64_t(GH + RS_MAXPASS), 0u);
+        fb.add(st, 2 * ntiles * 256, 0u);
+        fb.flush();
+    }
+    int ghb = int(ceil_div(n, RS_BLOCK * 16));
+    if (ghb > 1024) ghb = 1024;
+    k_global_hist<<<ghb, RS_BLOCK, 0, s>>>(keys, n, bit_lo, bit_hi, npass, gh);
+    SC_HIP(hipGetLastError());
+    uint64_t *kc = keys, *ko = kalt;
+    uint32_t *vc = vals, *vo = valt;
+    for (int p = 0; p < npass; ++p) {
+        // epochs 1..npass: the words were zeroed above, so none is stale
+        const int sh = bit_lo + 8 * p;
+        const int bits = bit_hi - sh < 8 ? bit_hi - sh : 8;
+        SweepArgs a;
+        a.kin = kc;
+        a.vin = vc;
+        a.kout = ko;
+        a.vout = vo;
+        a.n = n;
+        a.shift = sh;
+        a.mask = (1u << bits) - 1u;
+        a.ghist = gh + size_t(p) * 256;
+        a.status = st;
+        a.epoch = (unsigned long long)(p + 1) << ST_EPOCH_SHIFT;
+        a.tile_counter = gh + GH + p;
+        k_onesweep<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
+        SC_HIP(hipGetLastError());
+        std::swap(kc, ko);
+        std::swap(vc, vo);
+    }
+    return kc != keys;
+}
+
 void radix_sort_u64(uint64_t *keys, uint32_t *vals, int64_t n, int bit_lo, int bit_hi, RadixScratch &rs,
                     hipStream_t s) {
     if (n <= 1 || bit_hi <= bit_lo) return;

@@ -10,6 +10,7 @@
 #include "sc_common.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -25,7 +26,7 @@ struct Scratch {
     sc::DBuf<uint64_t> keys, tmp;
     sc::DBuf<uint32_t> idx;
 };
-Scratch &scratch() {
+Scratch &thread_scratch() {
     thread_local Scratch *p = new Scratch();
     return *p;
 }
@@ -67,7 +68,7 @@ int sort_by_key(K *keys, int32_t *vals, int64_t n, void *stream, KeyK keyk, int 
     if (n <= 1) return SEGCLIP_OK;
     try {
         hipStream_t s = static_cast<hipStream_t>(stream);
-        Scratch &sc_ = scratch();
+        Scratch &sc_ = thread_scratch();
         uint64_t *kk = sc_.keys.ensure(size_t(n));
         uint32_t *ix = sc_.idx.ensure(size_t(n));
         keyk<<<grid_for(n), BLK, 0, s>>>(keys, n, kk, ix);
@@ -119,6 +120,93 @@ int segclip_exclusive_scan_i64(int64_t *data, int64_t n, void *stream) {
         hipStream_t s = static_cast<hipStream_t>(stream);
         sc::scan_exclusive_u64(reinterpret_cast<uint64_t *>(data), reinterpret_cast<uint64_t *>(data), n, nullptr, s);
         SC_HIP(hipStreamSynchronize(s));
+        return SEGCLIP_OK;
+    } catch (const sc::Error &e) {
+        return e.code;
+    }
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of the f64 stable sort (no counterpart in the
+// reference): every buffer beyond the caller's keys and values lives in one
+// file-scope device array, `scratch`, of a fixed size for NOALLOC_MAX keys.
+// Each GPU holds its own copy of the array until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in,
+// not a device address): the device address is looked up once per device
+// with hipGetSymbolAddress, with that device current, and kept per device.
+// Every call on a GPU shares that GPU's array and leaves its contents
+// behind; what has to start at zero (histograms, tile counters, look-back
+// status words) is cleared by fill kernels at the start of each call.  Calls
+// that overlap on one GPU race on the array.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t NOALLOC_MAX = int64_t(1) << 20;
+// layout, bytes: keys and alternates (8+8), values and alternates (4+4),
+// gathered keys and values (8+4), per key; then the radix work words
+constexpr size_t NA_KEYS = 0;
+constexpr size_t NA_KALT = NA_KEYS + 8 * size_t(NOALLOC_MAX);
+constexpr size_t NA_IDX = NA_KALT + 8 * size_t(NOALLOC_MAX);
+constexpr size_t NA_VALT = NA_IDX + 4 * size_t(NOALLOC_MAX);
+constexpr size_t NA_TK = NA_VALT + 4 * size_t(NOALLOC_MAX);
+constexpr size_t NA_TV = NA_TK + 8 * size_t(NOALLOC_MAX);
+constexpr size_t NA_WORK = NA_TV + 4 * size_t(NOALLOC_MAX);
+// radix work words for NOALLOC_MAX keys: 8 x 256 histograms + 8 tile
+// counters, then 256 two-word status words per 4096-key tile
+constexpr size_t NA_WORK_WORDS = 8 * 256 + 8 + 2 * size_t(NOALLOC_MAX / 4096) * 256;
+constexpr size_t NA_BYTES = NA_WORK + 4 * NA_WORK_WORDS;
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mu;
+unsigned char *na_ptr[NA_MAX_DEVICES] = {};
+
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+// device address of this device's copy of `scratch` (looked up on the first call per device)
+unsigned char *noalloc_scratch(int dev) {
+    SC_REQUIRE(dev >= 0 && dev < NA_MAX_DEVICES, sc::SC_EINVAL, "device index");
+    std::lock_guard<std::mutex> g(na_mu);
+    if (!na_ptr[dev]) {
+        void *ptr = nullptr;
+        SC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(::scratch)));
+        SC_REQUIRE(ptr != nullptr, sc::SC_EHIP, "hipGetSymbolAddress gave no address for the scratch array");
+        na_ptr[dev] = static_cast<unsigned char *>(ptr);
+    }
+    return na_ptr[dev];
+}
+} // namespace
+
+extern "C" {
+
+int segclip_stable_sort_by_key_f64_noalloc(double *keys, int32_t *vals, int64_t n, void *stream) {
+    if (n < 0 || (n > 0 && (!keys || !vals))) return SEGCLIP_EINVAL;
+    if (n > NOALLOC_MAX) return SEGCLIP_ERANGE;
+    if (n <= 1) return SEGCLIP_OK;
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        int dev = 0;
+        SC_HIP(hipGetDevice(&dev));
+        unsigned char *base = noalloc_scratch(dev);
+        SC_REQUIRE(sc::radix_fixed_words(n) <= NA_WORK_WORDS, sc::SC_ERANGE, "radix work beyond the scratch array");
+        uint64_t *kk = reinterpret_cast<uint64_t *>(base + NA_KEYS);
+        uint64_t *kalt = reinterpret_cast<uint64_t *>(base + NA_KALT);
+        uint32_t *ix = reinterpret_cast<uint32_t *>(base + NA_IDX);
+        uint32_t *valt = reinterpret_cast<uint32_t *>(base + NA_VALT);
+        double *tk = reinterpret_cast<double *>(base + NA_TK);
+        int32_t *tv = reinterpret_cast<int32_t *>(base + NA_TV);
+        uint32_t *work = reinterpret_cast<uint32_t *>(base + NA_WORK);
+        k_keys_f64<<<grid_for(n), BLK, 0, s>>>(keys, n, kk, ix);
+        SC_HIP(hipGetLastError());
+        if (sc::radix_sort_u64_fixed(kk, ix, kalt, valt, n, 0, 64, work, s)) ix = valt;
+        k_gather<double><<<grid_for(n), BLK, 0, s>>>(keys, vals, ix, n, tk, tv);
+        SC_HIP(hipGetLastError());
+        SC_HIP(hipMemcpyAsync(keys, tk, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, s));
+        SC_HIP(hipMemcpyAsync(vals, tv, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToDevice, s));
         return SEGCLIP_OK;
     } catch (const sc::Error &e) {
         return e.code;
