@@ -15,7 +15,9 @@ GPU loop on a sample of the same pairs):
   one_to_many   base 100,000-vertex harmonic star against 4,096 star overlays
                 (log-uniform 8..20,000 vertices): one tree, every overlay
                 clipped in one batch
-  small_pairs   16,384 pairs of 4..64-vertex rings: clip + area/IoU forward,
+  small_pairs   16,384 pairs of 4..64-vertex rings: clip + area/IoU forward
+                (the persistent small-pair kernel; also replayed as a
+                captured hipGraph, and on the staged path for comparison),
                 and forward + backward of the area / IoU
   large_pairs   256 pairs of 10k..200k-vertex harmonic stars: forward +
                 backward with the clip recomputed in backward, peak HBM
@@ -190,10 +192,25 @@ def bench_small_pairs(steps):
         cl.batch_area_device(out.data_ptr())
 
     dt = timed(fwd, steps, 1)
+    fused = cl.get_tuning("small_fallback") == 0
     import numpy as np
     bx = torch.tensor(np.concatenate(bases), dtype=torch.float64, device="cuda", requires_grad=True)
     ox = torch.tensor(np.concatenate(ovs), dtype=torch.float64, device="cuda", requires_grad=True)
     bsz, csz = [len(b) for b in bases], [len(o) for o in ovs]
+    # the same forward replayed as one captured hipGraph on device-resident rings
+    gcl = segclip.Clipper().set_pairs_device(bx.data_ptr(), bsz, ox.data_ptr(), csz)
+    gout = torch.empty((4, J), dtype=torch.float64, device="cuda")
+    dtg = timed(lambda: gcl.pairs_area_graph(bx.data_ptr(), ox.data_ptr(), gout.data_ptr()), steps, 1)
+    # the staged path (one launch per stage for all pairs) on the same batch
+    scl = segclip.Clipper().set_pairs(bases, ovs).set_tuning("small_fused", 0)
+    scl.apply_tuning(tuning or {})
+
+    def fwd_staged():
+        scl.reset()
+        scl.clip_batch()
+        scl.batch_area_device(out.data_ptr())
+
+    dts = timed(fwd_staged, steps, 1)
 
     def fwd_bwd():
         bx.grad = None
@@ -204,7 +221,10 @@ def bench_small_pairs(steps):
     dtb = timed(fwd_bwd, steps, 1)
     plain, n = plain_loop_rate(bases, ovs, 256)
     return {"workload": "16,384 pairs, 4..64 vertices (convex / star)", "pairs": J,
-            "ms_per_batch": 1e3 * dt, "pairs_per_s": J / dt, "fwd_bwd_ms": 1e3 * dtb, "fwd_bwd_pairs_per_s": J / dtb,
+            "ms_per_batch": 1e3 * dt, "pairs_per_s": J / dt, "persistent_kernel": fused,
+            "ms_per_batch_graph": 1e3 * dtg, "pairs_per_s_graph": J / dtg,
+            "ms_per_batch_staged": 1e3 * dts, "pairs_per_s_staged": J / dts,
+            "fwd_bwd_ms": 1e3 * dtb, "fwd_bwd_pairs_per_s": J / dtb,
             "plain_pairs_per_s": plain, "plain_sample": n, "speedup_vs_plain": (J / dt) / plain, "tuning": tuning}
 
 

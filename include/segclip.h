@@ -141,6 +141,20 @@ int segclip_batch_area(segclip_ctx *ctx, double *out4, int64_t cap, int64_t *nee
 int segclip_batch_area_device(segclip_ctx *ctx, double *out4);
 int segclip_batch_area_backward_device(segclip_ctx *ctx, const double *g_area, const double *g_iou, double *grad_b,
                                        double *grad_c);
+/* Forward areas / IoU (the segclip_batch_area layout, 4*P doubles in out4) of
+ * every pair of a paired context whose rings all have at most 64 vertices,
+ * replayed as one hipGraph on the context's stream: re-layout of the caller's
+ * rings, queue reset and the persistent small-pair kernel with the areas
+ * computed in place.  base_xy / ov_xy are device pointers with the layout and
+ * sizes of the last segclip_set_pairs* call (rings back to back, no closing
+ * vertex); they are read at every replay, so the caller may update them in
+ * place between calls.  The graph is captured on the first call, again when
+ * a pointer changes, and dropped by segclip_set_pairs* / set_polygons*.
+ * After the replay one word is read back: when some pair needs the staged
+ * path, the batch is clipped on it (areas identical either way) and the next
+ * call captures again.  Returns when out4 is written.  No counterpart in the
+ * reference (its per-pair loop launches every stage, clip:1026-1297). */
+int segclip_pairs_area_graph(segclip_ctx *ctx, const double *base_xy, const double *ov_xy, double *out4);
 /* Free every device buffer but the resident input rings (the next call
  * rebuilds what it needs).  segclip_memory reports the device bytes held by
  * the library (all contexts) and their peak since the last reset. */

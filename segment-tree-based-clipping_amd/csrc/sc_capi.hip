@@ -11,6 +11,7 @@ struct segclip_ctx {
 
 namespace sc {
 Ctx::~Ctx() {
+    pairs_graph_reset(*this);
     if (ev_ready)
         for (auto &e : ev) (void)hipEventDestroy(e);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -326,6 +327,9 @@ int segclip_init_labels_arrays(segclip_ctx *h, int nb_aug, int nc_aug, const dou
 }
 int segclip_clip_batch(segclip_ctx *h, int k0, int k1) {
     return guarded(h, [&](sc::Ctx &x) { sc::clip_jobs(x, k0, k1); });
+}
+int segclip_pairs_area_graph(segclip_ctx *h, const double *base_xy, const double *ov_xy, double *out4) {
+    return guarded(h, [&](sc::Ctx &x) { sc::pairs_area_graph(x, base_xy, ov_xy, out4); });
 }
 int segclip_reset(segclip_ctx *h) {
     return guarded(h, [&](sc::Ctx &x) {

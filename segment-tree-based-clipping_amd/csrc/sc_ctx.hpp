@@ -156,6 +156,19 @@ struct Ctx {
     // small-pair kernel (1) or the staged path (0); results do not depend on it
     int small_fused = 1;
     int small_fail = 0; // fallback reasons of the last small-pair batch (0: it ran fused)
+    // the small-pair forward captured as a hipGraph (segclip_pairs_area_graph):
+    // the buffers it reads and writes belong to it, so a replay never sees a
+    // reallocation; destroyed when the polygons change
+    struct PairsGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        const double *bxy = nullptr, *cxy = nullptr;
+        double *out4 = nullptr;
+        DBuf<int> tables;      // job table and the rings' input / output offsets
+        DBuf<uint32_t> words;  // queue counter, fallback word
+        DBuf<int> fail, route, nv, nr, olen;
+        DBuf<double> ox, oy;
+    } pg;
 
     bool timing = false;
     StageTimes times;
@@ -203,6 +216,11 @@ void clip_jobs(Ctx &c, int k0, int k1); // full pipeline for overlays k0..k1 as 
 // of the batch's results was written)
 bool small_batch_eligible(const Ctx &c, int k0, int k1);
 bool clip_small_batch(Ctx &c, int k0, int k1);
+// forward areas of every pair through the captured graph (captured on first
+// use and when the pointers change); a pair the kernel does not cover makes the
+// call rerun the batch on the staged path
+void pairs_area_graph(Ctx &c, const double *bxy, const double *cxy, double *out4);
+void pairs_graph_reset(Ctx &c);
 // A19: per job of the last batch, areas (intersection, base ring, overlay ring)
 // and IoU into out4 (device, 4*J doubles); gradient of sum(g_area*area +
 // g_iou*iou) w.r.t. every input vertex into grad_b (2*nb) and grad_c (2*nct)

@@ -67,6 +67,8 @@ struct SmallLDS {
     int8_t vlab[SP_POOL];
     uint8_t vfl[SP_POOL];                  // bit 0: intersection; bits 1-2: entry/exit status
     int16_t roots[SP_OUTR];
+    int16_t ord[SP_OUTV];                  // output vertices in emission order (record indices)
+    int16_t rlen[SP_OUTR];                 // output ring lengths
 };
 
 struct SmallArgs {
@@ -79,6 +81,8 @@ struct SmallArgs {
     int *job_nv, *job_nr;               // per job: output vertices and rings
     double *out_x, *out_y;              // per job slot of SP_OUTV vertices
     int *out_len;                       // per job slot of SP_OUTR ring lengths
+    double *out4;                       // optional: areas and IoU, 4 x J (the batch-area layout)
+    uint32_t *anyfail;                  // optional: set when some job falls back
 };
 
 __device__ inline bool is_inter(const SmallLDS &L, int v) { return L.vfl[v] & 1; }
@@ -143,6 +147,46 @@ __device__ void ring_remove(SmallLDS &L, int &root, int V) {
 
 __device__ inline double A3(const SmallLDS &L, int a, int b, int c) {
     return areaA(L.vx[a], L.vy[a], L.vx[b], L.vy[b], L.vx[c], L.vy[c]);
+}
+
+// Shoelace sums in the order of the batch-area kernels, so both give the same
+// bits: a ring's terms x_i y_(i+1) - x_(i+1) y_i added per lane over
+// i = lane, lane + 64, ... and then by a shuffle tree (32, 16, ..., 1); an input
+// ring (at most 64 vertices) as that kernel's 256-thread block sum, whose two
+// upper tree steps add zeros.
+__device__ inline double wave_tree(double s) {
+    for (int d = WAVE / 2; d > 0; d >>= 1) s += __shfl_down(s, d, WAVE);
+    return __shfl(s, 0, WAVE);
+}
+__device__ double input_ring_sum2(const double *x, const double *y, int n, int lane) {
+    double s = 0.0;
+    if (lane < n) s += x[lane] * y[lane + 1] - x[lane + 1] * y[lane];
+    s = (s + 0.0) + 0.0;
+    return wave_tree(s);
+}
+__device__ void pair_areas(const SmallArgs &a, const SmallLDS &L, int j, int nb, int nc, int nr, int lane) {
+    double out2 = 0.0;
+    int at = 0;
+    for (int r = 0; r < nr; ++r) {
+        const int len = L.rlen[r];
+        double s = 0.0;
+        for (int i = lane; i < len; i += WAVE) {
+            const int v = L.ord[at + i], w = L.ord[at + (i + 1 == len ? 0 : i + 1)];
+            s += L.vx[v] * L.vy[w] - L.vx[w] * L.vy[v];
+        }
+        out2 += fabs(wave_tree(s));
+        at += len;
+    }
+    const double P2 = input_ring_sum2(L.bx, L.by, nb, lane), Q2 = input_ring_sum2(L.cx, L.cy, nc, lane);
+    if (lane == 0) {
+        const int J = a.J;
+        const double I = 0.5 * out2, P = 0.5 * fabs(P2), Q = 0.5 * fabs(Q2);
+        const double U = P + Q - I;
+        a.out4[j] = I;
+        a.out4[J + j] = P;
+        a.out4[2 * J + j] = Q;
+        a.out4[3 * J + j] = U != 0.0 ? I / U : 0.0;
+    }
 }
 
 // one pair, the whole wave; returns the fallback reasons (0: clipped)
@@ -539,10 +583,12 @@ __device__ int clip_pair(const SmallArgs &a, SmallLDS &L, int j, int lane, int &
                     }
                     ox[nv] = L.vx[V];
                     oy[nv] = L.vy[V];
+                    L.ord[nv] = int16_t(V);
                     ++nv;
                     ++len;
                     V = L.vnext[V];
                 } while (V != R);
+                L.rlen[nr] = int16_t(len);
                 olen[nr++] = len;
             }
         }
@@ -550,6 +596,8 @@ __device__ int clip_pair(const SmallArgs &a, SmallLDS &L, int j, int lane, int &
     fail = __shfl(fail, 0, WAVE);
     nv_out = __shfl(nv, 0, WAVE);
     nr_out = __shfl(nr, 0, WAVE);
+    __builtin_amdgcn_wave_barrier();
+    if (!fail && a.out4) pair_areas(a, L, j, nb, nc, nr_out, lane);
     return fail;
 }
 
@@ -570,6 +618,7 @@ __global__ __launch_bounds__(WAVE) void k_small_pairs(SmallArgs a) {
             a.route[j] = route;
             a.job_nv[j] = f ? 0 : nv;
             a.job_nr[j] = f ? 0 : nr;
+            if (f && a.anyfail) atomicOr(a.anyfail, 1u);
         }
         __syncthreads(); // LDS of this pair read before the next pair's writes
     }
@@ -598,6 +647,32 @@ __global__ void k_small_emit(int J, const int *__restrict__ voff, const int *__r
 // the totals after the exclusive scans of the per-job counts: ring_off[NR] = NV
 __global__ void k_small_tail(int J, const int *__restrict__ voff, const int *__restrict__ roff, int *ring_off) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ring_off[roff[J]] = voff[J];
+}
+
+// rings given back to back without closing vertices (the set_pairs layout)
+// into the resident layout with the closing vertex; one thread per vertex
+__global__ void k_pairs_layout(const double *__restrict__ xy, int64_t n, const int *__restrict__ in_off,
+                               const int *__restrict__ out_off, int R, double *out) {
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int r = seg_of(in_off, R, g);
+    const int local = int(g - in_off[r]);
+    const double x = xy[2 * g], y = xy[2 * g + 1];
+    const int64_t d = int64_t(out_off[r]) + local;
+    out[2 * d] = x;
+    out[2 * d + 1] = y;
+    if (local == 0) {
+        const int64_t cl = int64_t(out_off[r]) + (in_off[r + 1] - in_off[r]);
+        out[2 * cl] = x;
+        out[2 * cl + 1] = y;
+    }
+}
+
+int small_grid(const Ctx &c, int J) {
+    int cus = 256, v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && v > 0) cus = v;
+    // LDS bounds the waves per CU; a few more blocks than fit keep the CUs fed
+    return std::min(J, cus * 8);
 }
 
 struct SmallScratch {
@@ -657,15 +732,9 @@ bool clip_small_batch(Ctx &c, int k0, int k1) {
         fb.add(a.job_nr + J, 1, 0u);
         fb.flush();
     }
-    int dev_cus = 256;
-    {
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && v > 0)
-            dev_cus = v;
-    }
-    // LDS bounds the waves per CU; a few more blocks than fit keep the CUs fed
-    const int nblk = std::min(J, dev_cus * 8);
-    k_small_pairs<<<nblk, WAVE, 0, s>>>(a);
+    a.out4 = nullptr;
+    a.anyfail = nullptr;
+    k_small_pairs<<<small_grid(c, J), WAVE, 0, s>>>(a);
     SC_HIP(hipGetLastError());
     int *voff = S.voff.ensure(size_t(J) + 1), *roff = S.roff.ensure(size_t(J) + 1);
     scan_exclusive_u32(reinterpret_cast<const uint32_t *>(a.job_nv), reinterpret_cast<uint32_t *>(voff), J + 1,
@@ -708,6 +777,120 @@ bool clip_small_batch(Ctx &c, int k0, int k1) {
     js.route = routes;
     js.reason.assign(size_t(J), 0);
     return true;
+}
+
+void pairs_graph_reset(Ctx &c) {
+    Ctx::PairsGraph &g = c.pg;
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+    g.bxy = g.cxy = nullptr;
+    g.out4 = nullptr;
+}
+
+void pairs_area_graph(Ctx &c, const double *bxy, const double *cxy, double *out4) {
+    SC_REQUIRE(c.paired && c.K >= 1, SC_ESTATE, "pairs not set (segclip_set_pairs)");
+    SC_REQUIRE(bxy && cxy && out4, SC_EINVAL, "null pointer");
+    for (int k = 0; k < c.K; ++k)
+        SC_REQUIRE(c.bsz[k] <= SP_NV && c.csz[k] <= SP_NV, SC_EINVAL,
+                   "the graph forward takes rings of at most 64 vertices");
+    hipStream_t s = c.stream;
+    const int J = c.K;
+    Ctx::PairsGraph &g = c.pg;
+    if (!g.exec || g.bxy != bxy || g.cxy != cxy || g.out4 != out4) {
+        pairs_graph_reset(c);
+        // tables: job (bvoff, nb, cvoff, nc), base input / output offsets (J+1, J), overlay ones (J+1, J)
+        std::vector<int> h(4 * size_t(J) + 2 * (2 * size_t(J) + 1));
+        int *t = h.data();
+        for (int j = 0; j < J; ++j) {
+            t[j] = c.bvoff[j];
+            t[J + j] = c.bsz[j];
+            t[2 * J + j] = c.coff[j];
+            t[3 * J + j] = c.csz[j];
+        }
+        int *bin = t + 4 * J, *bout = bin + J + 1, *cin = bout + J, *cout = cin + J + 1;
+        for (int j = 0; j <= J; ++j) {
+            bin[j] = c.beoff[j];
+            cin[j] = c.eoff[j] - c.nb;
+            if (j < J) {
+                bout[j] = c.bvoff[j];
+                cout[j] = c.coff[j];
+            }
+        }
+        int *dt = g.tables.ensure(h.size());
+        SC_HIP(hipMemcpyAsync(dt, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s));
+        uint32_t *words = g.words.ensure(2);
+        SmallArgs a;
+        a.J = J;
+        a.bvoff = dt;
+        a.nb = dt + J;
+        a.cvoff = dt + 2 * J;
+        a.nc = dt + 3 * J;
+        a.bpoly = c.bpoly;
+        a.cpoly = c.cpoly;
+        a.queue = words;
+        a.anyfail = words + 1;
+        a.fail = g.fail.ensure(J);
+        a.route = g.route.ensure(J);
+        a.job_nv = g.nv.ensure(size_t(J) + 1);
+        a.job_nr = g.nr.ensure(size_t(J) + 1);
+        a.out_x = g.ox.ensure(size_t(J) * SP_OUTV);
+        a.out_y = g.oy.ensure(size_t(J) * SP_OUTV);
+        a.out_len = g.olen.ensure(size_t(J) * SP_OUTR);
+        a.out4 = out4;
+        SC_HIP(hipStreamSynchronize(s)); // the tables are on the device before the capture
+        const int64_t nbt = c.nb, nct = c.nct;
+        const int dbin = int(bin - t), dbout = int(bout - t), dcin = int(cin - t), dcout = int(cout - t);
+        SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        hipGraph_t graph = nullptr;
+        try {
+            // fill kernels, no memset nodes, inside the graph
+            FillBatch fb(s);
+            fb.add(words, 2, 0u);
+            fb.flush();
+            k_pairs_layout<<<unsigned(ceil_div(nbt, 256)), 256, 0, s>>>(bxy, nbt, dt + dbin, dt + dbout, J,
+                                                                        c.bpoly);
+            k_pairs_layout<<<unsigned(ceil_div(nct, 256)), 256, 0, s>>>(cxy, nct, dt + dcin, dt + dcout, J,
+                                                                        c.cpoly);
+            k_small_pairs<<<small_grid(c, J), WAVE, 0, s>>>(a);
+            SC_HIP(hipGetLastError());
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &graph); // the stream leaves capture mode either way
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        SC_HIP(hipStreamEndCapture(s, &graph));
+        g.graph = graph;
+        SC_HIP(hipGraphInstantiateWithFlags(&g.exec, g.graph, 0));
+        g.bxy = bxy;
+        g.cxy = cxy;
+        g.out4 = out4;
+    }
+    SC_HIP(hipGraphLaunch(g.exec, s));
+    uint32_t anyfail = 0;
+    {
+        ReadBack rb(c);
+        rb.add(&anyfail, static_cast<const uint32_t *>(g.words.p + 1), 1);
+        rb.run();
+    }
+    if (!anyfail) return;
+    // some pair is not covered: the staged path on the same rings (the
+    // context's rings now hold the caller's coordinates; the set-up is redone
+    // for them, which drops the graph)
+    std::vector<int> bsz = c.bsz, csz = c.csz;
+    ctx_set_pairs(c, bxy, bsz.data(), cxy, csz.data(), J, true);
+    const int keep = c.small_fused;
+    c.small_fused = 0;
+    try {
+        clip_jobs(c, 1, J);
+    } catch (...) {
+        c.small_fused = keep;
+        throw;
+    }
+    c.small_fused = keep;
+    batch_area(c, out4);
+    c.small_fail = 1;
 }
 
 } // namespace sc

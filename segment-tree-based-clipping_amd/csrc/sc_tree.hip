@@ -423,6 +423,7 @@ void reset_derived(Ctx &c) {
 void load(Ctx &c, const double *bxy, const int *bsz, int nbr, const double *cxy, const int *csz, int K,
           bool paired, bool device_ptrs) {
     SC_REQUIRE(bxy && cxy && csz && bsz, SC_EINVAL, "null polygon pointer");
+    pairs_graph_reset(c); // a captured forward refers to the old rings' sizes
     SC_REQUIRE(K >= 1 && nbr >= 1, SC_EINVAL, "at least one base and one overlay polygon are required");
     int64_t nb = 0, nct = 0;
     for (int i = 0; i < nbr; ++i) {

@@ -165,6 +165,13 @@ class Clipper:
     def batch_area_device(self, out_ptr):
         self._check(lib().segclip_batch_area_device(self._h, C.c_void_p(out_ptr)))
 
+    def pairs_area_graph(self, base_ptr, ov_ptr, out_ptr):
+        """Areas / IoU of every pair into out (device, 4 x P doubles) through the
+        captured graph; base_ptr / ov_ptr: device rings laid out as for set_pairs."""
+        self._check(lib().segclip_pairs_area_graph(self._h, C.c_void_p(base_ptr), C.c_void_p(ov_ptr),
+                                                   C.c_void_p(out_ptr)))
+        return self
+
     def batch_area_backward_device(self, g_area_ptr, g_iou_ptr, grad_b_ptr, grad_c_ptr):
         self._check(lib().segclip_batch_area_backward_device(
             self._h, C.c_void_p(g_area_ptr) if g_area_ptr else None, C.c_void_p(g_iou_ptr) if g_iou_ptr else None,

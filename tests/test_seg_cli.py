@@ -17,6 +17,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
+import segclip.io as IO
 import segclip.synth as S
 from fixtures import degenerate_cases
 
@@ -31,29 +32,8 @@ def _seg():
     return SEG
 
 
-def write_count_first(path, ring):
-    with open(path, "w") as f:
-        f.write("%d\n" % len(ring))
-        for x, y in ring:
-            f.write("%.17g,%.17g\n" % (x, y))
-
-
-def read_results(path):
-    """Rings of a result file: 'x y,' lines, ';' closes a component."""
-    rings, cur = [], []
-    with open(path) as f:
-        for line in f:
-            line = line.strip()
-            if not line:
-                continue
-            end = line.endswith(";")
-            x, y = line.rstrip(",;").split()
-            cur.append((float(x), float(y)))
-            if end:
-                rings.append(np.array(cur, np.float64))
-                cur = []
-    assert not cur, "component without ';'"
-    return rings
+write_count_first = IO.write_count_first
+read_results = IO.read_results
 
 
 def test_too_few_arguments(tmp_path):
@@ -119,3 +99,29 @@ def test_clip_files_equal_oracle(gpu, tmp_path, prec):
                 else:
                     w6 = np.array([[float("%.6g" % x), float("%.6g" % y)] for x, y in w])
                     assert np.array_equal(g, w6), (name, k)
+
+
+@pytest.mark.gpu
+def test_wkt_layer_through_seg(gpu, tmp_path):
+    """A shapefile-derived WKT layer (segclip.io.wkt2seg) run through bin/seg as
+    the run scripts do: base = the layer's first polygon, overlays = the rest;
+    results at 17 digits equal the oracle's rings."""
+    polys = [S.harmonic(3000, 11), S.star(300, 12, 0.3, 0.1, 0.3, 0.9), S.star(200, 13, -0.5, -0.2, 0.2, 0.7)]
+    lines = []
+    for i, r in enumerate(polys):
+        pts = ", ".join("%.17g %.17g" % (x, y) for x, y in np.vstack([r, r[:1]]))
+        lines.append("%d,POLYGON ((%s))" % (i, pts))
+    src = tmp_path / "layer.wkt"
+    src.write_text("\n".join(lines) + "\n")
+    run_dir = tmp_path / "run"
+    (tmp_path / "results").mkdir()
+    files = IO.wkt2seg(str(src), str(run_dir))
+    env = dict(os.environ, SEGCLIP_SAVE="1", SEGCLIP_PRECISION="17")
+    r = subprocess.run([_seg()] + files, capture_output=True, text=True, timeout=120, cwd=run_dir, env=env)
+    assert r.returncode == 0, r.stderr
+    tr = O.Tree(polys[0], polys[1:])
+    for k in (1, 2):
+        want = tr.clip(k, canonical=True).rings
+        got = IO.read_results(str(tmp_path / "results" / ("Fig-def-R%d.txt" % k)))
+        assert len(got) == len(want) and all(np.array_equal(g.view(np.uint64), w.view(np.uint64))
+                                             for g, w in zip(got, want))
