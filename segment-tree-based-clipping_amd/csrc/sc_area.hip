@@ -84,14 +84,17 @@ __global__ __launch_bounds__(BLK) void k_out_ring_area2(int nrings, const int *_
     if (lane == 0) out[r] = s;
 }
 
-// twice the unsigned area of each job's output: the rings' |S| in ring order
-__global__ void k_job_out_area2(int J, const int *__restrict__ jring, const double *__restrict__ ring2,
-                                double *out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= J) return;
+// twice the unsigned area of each job's output: one block per job, thread t
+// adds |S| of rings t, t + 256, ... of the job, then a fixed tree
+__global__ __launch_bounds__(BLK) void k_job_out_area2(const int *__restrict__ jring,
+                                                       const double *__restrict__ ring2, double *out) {
+    __shared__ double sh[BLK];
+    const int j = blockIdx.x;
+    const int r0 = jring[j], r1 = jring[j + 1];
     double s = 0.0;
-    for (int r = jring[j]; r < jring[j + 1]; ++r) s += fabs(ring2[r]);
-    out[j] = s;
+    for (int r = r0 + int(threadIdx.x); r < r1; r += BLK) s += fabs(ring2[r]);
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) out[j] = t;
 }
 
 __global__ void k_job_final(int J, int paired, const double *out2, const double *barea2, const double *carea2,
@@ -267,7 +270,7 @@ void batch_area(Ctx &c, double *out4) {
     if (c.nrings > 0)
         k_out_ring_area2<<<grid_for(c.nrings, BLK / WAVE), BLK, 0, s>>>(c.nrings, c.d_ring_off, c.d_ring_x,
                                                                        c.d_ring_y, ring2);
-    k_job_out_area2<<<grid_for(J), BLK, 0, s>>>(J, js.d_ring, ring2, out2);
+    k_job_out_area2<<<J, BLK, 0, s>>>(js.d_ring, ring2, out2);
     if (c.paired) k_ring_area2<<<J, BLK, 0, s>>>(c.bpoly, c.d_bvoff, js.k0 - 1, barea2);
     else k_ring_area2<<<1, BLK, 0, s>>>(c.bpoly, c.d_bvoff, 0, barea2);
     k_ring_area2<<<J, BLK, 0, s>>>(c.cpoly, c.d_coff, js.k0 - 1, carea2);
@@ -333,8 +336,11 @@ void batch_area_backward(Ctx &c, const double *g_area, const double *g_iou, doub
         k_output_grad<<<grid_for(c.n_out), BLK, 0, s>>>(p, 0, c.n_out);
     SC_HIP(hipGetLastError());
     int bad = 0;
-    SC_HIP(hipMemcpyAsync(&bad, p.bad, sizeof(int), hipMemcpyDeviceToHost, s));
-    SC_HIP(hipStreamSynchronize(s));
+    {
+        ReadBack rb(c);
+        rb.add(&bad, static_cast<const int *>(p.bad), 1);
+        rb.run();
+    }
     SC_REQUIRE(!bad, SC_EUNDEF, "an output vertex has no provenance (sequential route midpoint vertex)");
 }
 

@@ -152,8 +152,9 @@ __device__ inline double A3(const SmallLDS &L, int a, int b, int c) {
 // Shoelace sums in the order of the batch-area kernels, so both give the same
 // bits: a ring's terms x_i y_(i+1) - x_(i+1) y_i added per lane over
 // i = lane, lane + 64, ... and then by a shuffle tree (32, 16, ..., 1); an input
-// ring (at most 64 vertices) as that kernel's 256-thread block sum, whose two
-// upper tree steps add zeros.
+// ring (at most 64 vertices) and the sum of a job's |S| over its rings (at most
+// 64) as those kernels' 256-thread block sums, whose two upper tree steps add
+// zeros.
 __device__ inline double wave_tree(double s) {
     for (int d = WAVE / 2; d > 0; d >>= 1) s += __shfl_down(s, d, WAVE);
     return __shfl(s, 0, WAVE);
@@ -165,7 +166,7 @@ __device__ double input_ring_sum2(const double *x, const double *y, int n, int l
     return wave_tree(s);
 }
 __device__ void pair_areas(const SmallArgs &a, const SmallLDS &L, int j, int nb, int nc, int nr, int lane) {
-    double out2 = 0.0;
+    double mine = 0.0; // lane r: |S| of output ring r
     int at = 0;
     for (int r = 0; r < nr; ++r) {
         const int len = L.rlen[r];
@@ -174,9 +175,11 @@ __device__ void pair_areas(const SmallArgs &a, const SmallLDS &L, int j, int nb,
             const int v = L.ord[at + i], w = L.ord[at + (i + 1 == len ? 0 : i + 1)];
             s += L.vx[v] * L.vy[w] - L.vx[w] * L.vy[v];
         }
-        out2 += fabs(wave_tree(s));
+        const double rs = wave_tree(s);
+        if (lane == r) mine += fabs(rs);
         at += len;
     }
+    const double out2 = wave_tree((mine + 0.0) + 0.0);
     const double P2 = input_ring_sum2(L.bx, L.by, nb, lane), Q2 = input_ring_sum2(L.cx, L.cy, nc, lane);
     if (lane == 0) {
         const int J = a.J;

@@ -62,6 +62,43 @@ __global__ void k_layout_overlays(const double *__restrict__ cxy, int nct, const
 }
 
 
+// smallest and largest x of a laid-out ring array (nv vertices), per block;
+// min / max are exact, so the result does not depend on the reduction order
+constexpr int XR_BLOCKS = 256;
+__global__ void k_xrange_blocks(const double *__restrict__ poly, int64_t nv, double *bmin, double *bmax) {
+    __shared__ double smin[BLK], smax[BLK];
+    double lo = INFINITY, hi = -INFINITY;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nv; i += int64_t(gridDim.x) * blockDim.x) {
+        const double x = poly[2 * i];
+        lo = fmin(lo, x);
+        hi = fmax(hi, x);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int d = BLK / 2; d > 0; d >>= 1) {
+        if (int(threadIdx.x) < d) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + d]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + d]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bmin[blockIdx.x] = smin[0];
+        bmax[blockIdx.x] = smax[0];
+    }
+}
+__global__ void k_xrange_final(const double *__restrict__ bmin, const double *__restrict__ bmax, int n, double *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double lo = bmin[0], hi = bmax[0];
+    for (int i = 1; i < n; ++i) {
+        lo = fmin(lo, bmin[i]);
+        hi = fmax(hi, bmax[i]);
+    }
+    out[0] = lo;
+    out[1] = hi;
+}
+
 // Edge records.  In paired mode the x-extents used as tree keys are moved by a
 // per-pair shift so that different pairs occupy disjoint key ranges (no tree
 // node is then covered by edges of two pairs); the endpoints used by every
@@ -465,19 +502,22 @@ void load(Ctx &c, const double *bxy, const int *bsz, int nbr, const double *cxy,
     if (paired) {
         // per-pair key shift: pairs are laid side by side along x, each in a
         // window wider than the span of all rings
+        // x-range of all rings, reduced on the device from the laid-out rings
+        // (their closing vertices repeat first vertices: the same range)
         double lo = 0, hi = 0;
         {
-            std::vector<double> h(2 * size_t(nb + nct));
-            SC_HIP(hipMemcpyAsync(h.data(), bxy, sizeof(double) * 2 * size_t(nb),
-                                  device_ptrs ? hipMemcpyDeviceToHost : hipMemcpyHostToHost, c.stream));
-            SC_HIP(hipMemcpyAsync(h.data() + 2 * nb, cxy, sizeof(double) * 2 * size_t(nct),
-                                  device_ptrs ? hipMemcpyDeviceToHost : hipMemcpyHostToHost, c.stream));
-            SC_HIP(hipStreamSynchronize(c.stream));
-            lo = hi = h[0];
-            for (size_t i = 0; i < h.size(); i += 2) {
-                lo = std::min(lo, h[i]);
-                hi = std::max(hi, h[i]);
-            }
+            double *part = reinterpret_cast<double *>(c.u64b.ensure(4 * XR_BLOCKS + 2));
+            k_xrange_blocks<<<XR_BLOCKS, BLK, 0, c.stream>>>(c.bpoly, nb + nbr, part, part + 2 * XR_BLOCKS);
+            k_xrange_blocks<<<XR_BLOCKS, BLK, 0, c.stream>>>(c.cpoly, nct + K, part + XR_BLOCKS,
+                                                             part + 3 * XR_BLOCKS);
+            k_xrange_final<<<1, 1, 0, c.stream>>>(part, part + 2 * XR_BLOCKS, 2 * XR_BLOCKS, part + 4 * XR_BLOCKS);
+            SC_HIP(hipGetLastError());
+            double h[2];
+            ReadBack rb(c);
+            rb.add(h, static_cast<const double *>(part + 4 * XR_BLOCKS), 2);
+            rb.run();
+            lo = h[0];
+            hi = h[1];
         }
         double span = (hi - lo) + 1.0;
         std::vector<double> sh(static_cast<size_t>(K));

@@ -23,6 +23,7 @@ choose.
 
 import json
 import math
+import os
 import time
 
 # candidate values per setting, and the library's defaults
@@ -103,6 +104,26 @@ def load(path, name):
     settings = {k: int(v) for k, v in (e or {}).get("settings", {}).items()}
     check(settings)
     return settings
+
+
+TUNING_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                           "tuning.json")
+
+
+def workload_for(paired, max_ring):
+    """The tuned workload whose shape a batch has: many small pairs, many large
+    pairs, or one base ring against many overlays."""
+    if not paired:
+        return "one_to_many"
+    return "small_pairs" if max_ring <= 64 else "large_pairs"
+
+
+def settings_for(paired, max_ring, path=None):
+    """Settings persisted for the workload of this shape ({} when none)."""
+    path = path or TUNING_FILE
+    if not os.path.exists(path):
+        return {}
+    return load(path, workload_for(paired, max_ring))
 
 
 def check(settings):
