@@ -20,7 +20,8 @@ GPU loop on a sample of the same pairs):
                 captured hipGraph, and on the staged path for comparison),
                 and forward + backward of the area / IoU
   large_pairs   256 pairs of 10k..200k-vertex harmonic stars: forward +
-                backward with the clip recomputed in backward, peak HBM
+                backward with the clip recomputed in backward and with the
+                forward's state kept, peak HBM of each, the stab's HBM rate
 
 roofline: the stab kernel (the dominant kernel), HBM bound.  Algorithmic
 bytes per launch = 36 (L + E) + 12 K: every cover entry (L) and leaf end entry
@@ -251,6 +252,17 @@ def bench_large_pairs(steps):
     segclip.memory(reset_peak=True)
     dt = timed(fwd_bwd, steps, 0)
     cur, peak = segclip.memory()
+
+    def fwd_bwd_stored():
+        bx.grad = None
+        ox.grad = None
+        a, iou = clip_area_iou(bx, bsz, ox, csz, recompute=False)
+        (a.sum() + iou.sum()).backward()
+
+    fwd_bwd_stored()
+    segclip.memory(reset_peak=True)
+    dts = timed(fwd_bwd_stored, steps, 0)
+    _, peak_stored = segclip.memory()
     # forward only, batched, for the same pairs
     cl = segclip.Clipper().set_pairs(bases, ovs)
     tuning = tuning_for("large_pairs")  # the forward batch (the autograd path keeps the defaults)
@@ -261,12 +273,23 @@ def bench_large_pairs(steps):
         cl.clip_batch()
 
     dtf = timed(fwd, steps, 1)
+    cl.set_timing(True)
+    fwd()
+    st = cl.stage_times()
+    cnt = cl.counts()
+    sb = stab_bytes(cnt["cover_entries"], cnt["leaf_end_entries"], cnt["candidates"])
     cl.release()
     plain, n = plain_loop_rate(bases, ovs, 8)
     edges = sum(bsz) + sum(csz)
     return {"workload": "256 pairs of harmonic stars, 10k..200k vertices", "pairs": J,
             "fwd_bwd_ms": 1e3 * dt, "fwd_bwd_pairs_per_s": J / dt, "fwd_bwd_edges_per_s": edges / dt,
-            "peak_hbm_bytes_fwd_bwd": peak, "fwd_ms": 1e3 * dtf, "fwd_pairs_per_s": J / dtf,
+            "peak_hbm_bytes_fwd_bwd": peak, "fwd_bwd_stored_ms": 1e3 * dts,
+            "peak_hbm_bytes_fwd_bwd_stored": peak_stored,
+            "stab_GBps": round(sb / (st["stab_kernel"] * 1e-3) / 1e9, 1) if st["stab_kernel"] > 0 else None,
+            "stab_roofline_frac": round(sb / (st["stab_kernel"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+            if st["stab_kernel"] > 0 else None,
+            "stage_ms": {k: round(v, 3) for k, v in st.items()},
+            "fwd_ms": 1e3 * dtf, "fwd_pairs_per_s": J / dtf,
             "plain_fwd_pairs_per_s": plain, "plain_sample": n, "speedup_fwd_vs_plain": (J / dtf) / plain,
             "tuning": tuning}
 

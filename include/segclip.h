@@ -237,6 +237,18 @@ int segclip_stable_sort_by_key_i64(int64_t *keys, int32_t *vals, int64_t n, void
 int segclip_exclusive_scan_i32(int32_t *data, int64_t n, void *stream);
 int segclip_exclusive_scan_i64(int64_t *data, int64_t n, void *stream);
 
+/* Candidate pairs of two polygon layers by bounding-box overlap, the filter in
+ * front of many-to-many clipping (the reference joins layers through an R-tree
+ * of polygon MBRs: lib/optimizedFostersAlgorithm/readShapefile.cpp:326-355,
+ * foster/Rtree-Topdown-version; getCMBR's closed test, polyclip_time.h:95-124).
+ * a_xy / b_xy: device pointers to rings back to back (x,y doubles, no closing
+ * vertex), a_sizes / b_sizes: host vertex counts.  pairs (host, 2*cap ints)
+ * receives every (a, b) whose MBRs overlap (closed intervals), sorted by
+ * (a, b); *need = their number (SEGCLIP_ESPACE when cap is smaller).  Returns
+ * after the work on 'stream' is done. */
+int segclip_mbr_join(const double *a_xy, const int *a_sizes, int na, const double *b_xy, const int *b_sizes,
+                     int nb, int32_t *pairs, int64_t cap, int64_t *need, void *stream);
+
 /* segclip_stable_sort_by_key_f64 without any device allocation.  No
  * counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

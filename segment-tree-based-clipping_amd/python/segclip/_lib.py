@@ -29,7 +29,7 @@ SYMBOLS = [
     "segclip_batch_area_backward_device", "segclip_release", "segclip_memory", "segclip_set_allocator", "segclip_save_intersections_ids",
     "segclip_sort_intersections_at_edges", "segclip_copy_intersections_arrays", "segclip_init_labels_arrays",
     "segclip_copy_batch_routes", "segclip_set_tuning", "segclip_get_tuning",
-    "segclip_stable_sort_by_key_f64_noalloc", "segclip_pairs_area_graph",
+    "segclip_stable_sort_by_key_f64_noalloc", "segclip_pairs_area_graph", "segclip_mbr_join",
 ]
 
 
@@ -116,6 +116,7 @@ def lib():
         "segclip_stable_sort_by_key_f64": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
         "segclip_stable_sort_by_key_f64_noalloc": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
         "segclip_pairs_area_graph": (C.c_int, [_vp, _vp, _vp, _vp]),
+        "segclip_mbr_join": (C.c_int, [_vp, _ip, C.c_int, _vp, _ip, C.c_int, _ip, C.c_int64, _i64p, _vp]),
         "segclip_stable_sort_by_key_i32": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
         "segclip_stable_sort_by_key_i64": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
         "segclip_exclusive_scan_i32": (C.c_int, [_vp, C.c_int64, _vp]),
