@@ -77,6 +77,17 @@ int segclip_set_pairs(segclip_ctx *ctx, const double *base_xy, const int *base_s
 int segclip_set_pairs_device(segclip_ctx *ctx, const double *base_xy, const int *base_sizes,
                              const double *ov_xy, const int *ov_sizes, int P);
 
+/* The operation of the following clips: SEGCLIP_OP_INTERSECTION (default, the
+ * reference's bin/seg) or SEGCLIP_OP_UNION (the UNION switch of
+ * foster/polyclip.h:39 with its XORs in labelIntersections2 / createResult2,
+ * polyclip.h:451-470,571-618,669-681,740; the clipper's -union,
+ * lib/optimizedFostersAlgorithm/polyclip.cpp).  Union jobs take the
+ * sequential labelling route; a union output may hold holes, so the
+ * area / IoU entry points refuse it (SEGCLIP_ESTATE). */
+#define SEGCLIP_OP_INTERSECTION 0
+#define SEGCLIP_OP_UNION 1
+int segclip_set_operation(segclip_ctx *ctx, int op);
+
 /* Tree stages.  segclip_build_tree runs elementary, node intervals, end lists
  * and cover lists in that order. */
 int segclip_build_elementary(segclip_ctx *ctx);
@@ -179,7 +190,7 @@ int segclip_copy_batch_status(segclip_ctx *ctx, int32_t *status, int64_t cap, in
  * 1 no crossing vertex, 2 the reference's sequential algorithm) and, for
  * route 2, why (bits: 1 a label other than CROSSING/BOUNCING, 2 a ring without
  * a plain vertex, 4 crossings not alternating, 8 a bouncing pair to split,
- * 16 a neighbour link that is not a crossing) */
+ * 16 a neighbour link that is not a crossing, 32 the union) */
 int segclip_copy_batch_routes(segclip_ctx *ctx, int32_t *route, int32_t *reason, int64_t cap, int64_t *need);
 /* Drop everything derived from the polygons (tree, candidates, results) so
  * that the next segclip_clip recomputes it; the resident polygons stay. */

@@ -4,8 +4,9 @@
  * TEST INFRASTRUCTURE ONLY (see seg_oracle.h).  Restates, on index-linked
  * vertex records, the reference's copyToLinkedList (src/clip.cpp:606-695) and
  * labelIntersections2 / createResult2 / cleanUpResult2
- * (foster/polyclip.h:337-709, 717-759, 767-787) for the intersection case
- * (UNION = false, polyclip.h:39), one base copy per overlay.
+ * (foster/polyclip.h:337-709, 717-759, 767-787), for the intersection
+ * (UNION = false, polyclip.h:39) and, with uni set, the union, one base copy
+ * per overlay.
  *
  * Intended semantics used where the reference is undefined:
  *  - cleanUpResult2 deletes the vertex its iterator stands on and then reads
@@ -125,10 +126,14 @@ static int point_in_poly(const pool *p, const poly *P, double rx, double ry) {
     return (w % 2) != 0;
 }
 
-static int no_crossing_vertex(const pool *p, const poly *P) { /* polygon.h:474-484 */
+/* polygon::noCrossingVertex(union_case), polygon.h:465-475 */
+static int no_crossing_vertex(const pool *p, const poly *P, int uni) {
     int V = P->root;
     do {
-        if (p->v[V].inter && (p->v[V].label == CROSSING || p->v[V].label == DELAYED_CROSSING)) return 0;
+        if (p->v[V].inter) {
+            if (p->v[V].label == CROSSING || p->v[V].label == DELAYED_CROSSING) return 0;
+            if (uni && (p->v[V].label == BOUNCING || p->v[V].label == DELAYED_BOUNCING)) return 0;
+        }
         V = p->v[V].next;
     } while (V != P->root);
     return 1;
@@ -192,8 +197,9 @@ static int toggle(int s) { return s == ENTRY ? EXIT : (s == EXIT ? ENTRY : s); }
 
 int so_foster(const double *augB, int nbAug, const int *avB, const int *labB, const double *augC,
               int ncAug, const int *nbC, const int *labC, const double *bring, int nb,
-              const double *cring, int nc, int *status, int *nrings, int **ring_off,
+              const double *cring, int nc, int uni, int *status, int *nrings, int **ring_off,
               double **ring_xy) {
+    uni = uni ? 1 : 0; /* UNION (foster/polyclip.h:39): the operation's XOR switch */
     pool p = {0};
     poly P = {-1, 0}, Q = {-1, 0};
     for (int i = 0; i < nb; i++) poly_append(&p, &P, bring[2 * i], bring[2 * i + 1], 1);
@@ -305,13 +311,13 @@ int so_foster(const double *augB, int nbAug, const int *avB, const int *labB, co
         /* 3.5) special cases */
         for (int s = 0; s < 2; s++) {
             poly *A = s == 0 ? &P : &Q, *B = s == 0 ? &Q : &P;
-            if (no_crossing_vertex(&p, A)) {
+            if (no_crossing_vertex(&p, A, uni)) {
                 noInter[s] = 1;
                 if (all_on_on(&p, A)) ident[s] = 1;
                 else {
                     double px = 0, py = 0;
                     if (!non_intersection_point(&p, A, &px, &py)) { *status |= 4; goto bail2; }
-                    if (point_in_poly(&p, B, px, py)) PUSH_RES(A->root);
+                    if (point_in_poly(&p, B, px, py) ^ uni) PUSH_RES(A->root); /* polyclip.h:470 */
                 }
             }
         }
@@ -353,14 +359,14 @@ int so_foster(const double *augB, int nbAug, const int *avB, const int *labB, co
                     p.v[I].enex = st;
                     st = toggle(st);
                 }
-                if (lab == BOUNCING && st == EXIT) split[s][nsplit[s]++] = I;
+                if (lab == BOUNCING && ((st == EXIT) ^ uni)) split[s][nsplit[s]++] = I;
                 if (lab == DELAYED_CROSSING) {
                     p.v[I].enex = st;
                     if (first_chain) {
-                        if (st == EXIT) p.v[I].label = CROSSING;
+                        if ((st == EXIT) ^ uni) p.v[I].label = CROSSING;
                         first_chain = 0;
                     } else {
-                        if (st == ENTRY) p.v[I].label = CROSSING;
+                        if ((st == ENTRY) ^ uni) p.v[I].label = CROSSING;
                         first_chain = 1;
                         st = toggle(st);
                     }
@@ -368,10 +374,10 @@ int so_foster(const double *augB, int nbAug, const int *avB, const int *labB, co
                 if (lab == DELAYED_BOUNCING) {
                     p.v[I].enex = st;
                     if (first_chain) {
-                        if (st == EXIT) cross[s][ncross[s]++] = I;
+                        if ((st == EXIT) ^ uni) cross[s][ncross[s]++] = I;
                         first_chain = 0;
                     } else {
-                        if (st == ENTRY) cross[s][ncross[s]++] = I;
+                        if ((st == ENTRY) ^ uni) cross[s][ncross[s]++] = I;
                         first_chain = 1;
                     }
                     st = toggle(st);
@@ -400,7 +406,8 @@ int so_foster(const double *augB, int nbAug, const int *avB, const int *labB, co
         }
         insert_after(&p, VP, IP);
         insert_after(&p, VQ, IQ);
-        p.v[IP].enex = EXIT; p.v[VP].enex = ENTRY; p.v[IQ].enex = EXIT; p.v[VQ].enex = ENTRY;
+        if (!uni) { p.v[IP].enex = EXIT; p.v[VP].enex = ENTRY; p.v[IQ].enex = EXIT; p.v[VQ].enex = ENTRY; }
+        else { p.v[IP].enex = ENTRY; p.v[VP].enex = EXIT; p.v[IQ].enex = ENTRY; p.v[VQ].enex = EXIT; }
         p.v[IP].label = p.v[VP].label = p.v[IQ].label = p.v[VQ].label = CROSSING;
     }
     /* 6) CROSSING candidates */
@@ -431,7 +438,7 @@ int so_foster(const double *augB, int nbAug, const int *avB, const int *labB, co
             do {
                 int st = toggle(p.v[V].enex);
                 do {
-                    V = st == EXIT ? p.v[V].next : p.v[V].prev;
+                    V = ((st == EXIT) ^ uni) ? p.v[V].next : p.v[V].prev;
                     p.v[V].inter = 0;
                     poly_append(&p, &R, p.v[V].x, p.v[V].y, 0);
                     if (++steps > bound) { *status |= 4; goto bail3; }

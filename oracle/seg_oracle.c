@@ -928,7 +928,9 @@ static void sort_at_edges(const int *qa, const int *sidx, int cStart, const int 
     free(cp);
 }
 
-so_result *so_clip(so_tree *t, int k, int canonical) {
+so_result *so_clip(so_tree *t, int k, int canonical) { return so_clip_op(t, k, canonical, 0); }
+
+so_result *so_clip_op(so_tree *t, int k, int canonical, int uni) {
     if (!t || k < 1 || k > t->K) return NULL;
     so_result *r = so_xcalloc(1, sizeof(so_result));
     int bSize = t->nb, cSize = t->csz[k - 1], cStart = t->coff[k - 1];
@@ -1214,7 +1216,7 @@ so_result *so_clip(so_tree *t, int k, int canonical) {
     /* A16/A17 */
     int st = 0;
     so_foster(augB, nbAug, avB, labB, augC, ncAug, nbC, labC, bPoly, bSize, cPoly + 2 * (size_t)cStart,
-              cSize, &st, &r->nrings, &r->ring_off, &r->ring_xy);
+              cSize, uni, &st, &r->nrings, &r->ring_off, &r->ring_xy);
     r->status |= st;
 
     free(bSorted); free(cSorted); free(bAll); free(bNd); free(cAll); free(cNd); free(bU); free(cU);

@@ -71,6 +71,9 @@ const int *so_before_leaf(const so_tree *t);
  *               every edge's intersections sorted by quantised alpha).
  * Returns NULL on failure. */
 so_result *so_clip(so_tree *t, int k, int canonical);
+/* so_clip for an operation: uni = 0 intersection, 1 union (the UNION switch of
+ * foster/polyclip.h:39 and its XORs in labelIntersections2 / createResult2) */
+so_result *so_clip_op(so_tree *t, int k, int canonical, int uni);
 void so_result_free(so_result *r);
 
 /* status: 0 ok; >0 = reference behaviour undefined on this input (bit set):

@@ -54,7 +54,7 @@ int so_lsi(double b1x, double b1y, double b2x, double b2y, double c1x, double c1
 /* Foster labelling / traversal / clean-up on the augmented arrays (foster_oracle.c) */
 int so_foster(const double *aug_b, int nbaug, const int *av_b, const int *lab_b, const double *aug_c,
               int ncaug, const int *nbr_c, const int *lab_c, const double *bring, int nb,
-              const double *cring, int nc, int *status, int *nrings, int **ring_off,
+              const double *cring, int nc, int uni, int *status, int *nrings, int **ring_off,
               double **ring_xy);
 
 void *so_xmalloc(size_t n);

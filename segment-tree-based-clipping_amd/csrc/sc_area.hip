@@ -260,6 +260,7 @@ void job_rings_to_device(Ctx &c) {
 } // namespace
 
 void batch_area(Ctx &c, double *out4) {
+    SC_REQUIRE(!c.uni, SC_ESTATE, "area / IoU are defined for the intersection (a union output may hold holes)");
     JobSet &js = c.jobs;
     job_rings_to_device(c);
     hipStream_t s = c.stream;

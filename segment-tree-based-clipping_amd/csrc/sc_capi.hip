@@ -328,6 +328,16 @@ int segclip_init_labels_arrays(segclip_ctx *h, int nb_aug, int nc_aug, const dou
 int segclip_clip_batch(segclip_ctx *h, int k0, int k1) {
     return guarded(h, [&](sc::Ctx &x) { sc::clip_jobs(x, k0, k1); });
 }
+int segclip_set_operation(segclip_ctx *h, int op) {
+    return guarded(h, [&](sc::Ctx &x) {
+        SC_REQUIRE(op == SEGCLIP_OP_INTERSECTION || op == SEGCLIP_OP_UNION, SEGCLIP_EINVAL, "unknown operation");
+        if (x.uni != (op == SEGCLIP_OP_UNION ? 1 : 0)) {
+            x.uni = op == SEGCLIP_OP_UNION ? 1 : 0;
+            x.jobs.J = 0; // results of the other operation are gone
+            x.jobs.ring_off.clear();
+        }
+    });
+}
 int segclip_pairs_area_graph(segclip_ctx *h, const double *base_xy, const double *ov_xy, double *out4) {
     return guarded(h, [&](sc::Ctx &x) { sc::pairs_area_graph(x, base_xy, ov_xy, out4); });
 }

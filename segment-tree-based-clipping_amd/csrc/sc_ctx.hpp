@@ -154,6 +154,7 @@ struct Ctx {
     int stab_upw = 8;
     // independent pairs of up to 64 + 64 vertices clipped by the persistent
     // small-pair kernel (1) or the staged path (0); results do not depend on it
+    int uni = 0; // operation: 0 intersection, 1 union (UNION of foster/polyclip.h:39)
     int small_fused = 1;
     int small_fail = 0; // fallback reasons of the last small-pair batch (0: it ran fused)
     // the small-pair forward captured as a hipGraph (segclip_pairs_area_graph):

@@ -2,8 +2,8 @@
 // (rows A16-A17) on the GPU, for a batch of jobs in the same launches.
 //
 // Reference: copyToLinkedList (src/clip.cpp:606-695), labelIntersections2,
-// createResult2, cleanUpResult2 (foster/polyclip.h:337-787), intersection
-// case (UNION = false).  The augmented rings built by post_jobs are the linked
+// createResult2, cleanUpResult2 (foster/polyclip.h:337-787), for the
+// intersection (UNION = false) and, on the sequential route, the union.  The augmented rings built by post_jobs are the linked
 // lists of copyToLinkedList laid out in ring order: slot i of a job's base
 // ring P is followed by slot i+1 (mod its length), likewise for the overlay
 // ring Q; a slot is an intersection vertex when its alpha (P) or neighbour
@@ -64,7 +64,7 @@ enum : int { UB_TRAVERSAL = 4 };
 constexpr int SRC_NONE = INT_MIN;
 
 // reasons for taking the sequential route
-enum : int { SLOW_LABEL = 1, SLOW_NOPLAIN = 2, SLOW_ODD = 4, SLOW_SPLIT = 8, SLOW_LINK = 16 };
+enum : int { SLOW_LABEL = 1, SLOW_NOPLAIN = 2, SLOW_ODD = 4, SLOW_SPLIT = 8, SLOW_LINK = 16, SLOW_UNION = 32 };
 // routes
 enum : int { K_CROSS = 0, K_NOCROSS = 1, K_SEQ = 2 };
 
@@ -273,6 +273,7 @@ __global__ void k_sp_enex(int I, int J, SparseP sp, const int *lab, const uint32
 
 struct JobState {
     int J;
+    int uni; // the union: every job takes the sequential route
     const int *AB, *AC;
     int *slow, *firstP, *firstQ, *windP, *windQ, *kind, *mP, *mQ, *cP0, *cQ0;
 };
@@ -290,6 +291,7 @@ __global__ void k_jdecide(JobState s, const uint32_t *ccP, const uint32_t *ccQ) 
     int sl = s.slow[j];
     if (s.firstP[j] < 0 || s.firstQ[j] < 0) sl |= SLOW_NOPLAIN;
     if (mp != mq || (mp & 1)) sl |= SLOW_ODD;
+    if (s.uni) sl |= SLOW_UNION;
     s.slow[j] = sl;
     s.kind[j] = sl ? K_SEQ : (mp == 0 ? K_NOCROSS : K_CROSS);
 }
@@ -957,6 +959,7 @@ struct SeqArgs {
     double *ox, *oy;
     int *vsrc, *osrc; // provenance of inserted vertices / output vertices (slot reference)
     int P0, Q0;       // first slot of the job's rings in the batch arrays
+    int uni;          // 1: the union (UNION of foster/polyclip.h:39 and its XORs)
     int out_cap;
     int *ring_off;
     int ring_cap;
@@ -1183,6 +1186,8 @@ __device__ void seq_run(SeqArgs &a) {
             int V = A;
             do {
                 if (INTER_(V) && (LAB_(V) == L_CROSSING || LAB_(V) == L_DCROSS)) nocross = false;
+                // polygon::noCrossingVertex(union_case), polygon.h:465-475
+                if (a.uni && INTER_(V) && (LAB_(V) == L_BOUNCING || LAB_(V) == L_DBOUNCE)) nocross = false;
                 if (LAB_(V) != L_ON_ON) allon = false;
                 V = NX_(V);
             } while (V != A);
@@ -1220,7 +1225,7 @@ __device__ void seq_run(SeqArgs &a) {
             if (!found) { status |= UB_TRAVERSAL; goto done; }
             // the ring's first plain vertex: its winding against the other ring is known
             const bool in = plain ? ((s == 0 ? a.windP : a.windQ) % 2) != 0 : seq_point_in(a, B, qx, qy);
-            if (in) {
+            if (in != (a.uni != 0)) { // isInside ^ UNION, polyclip.h:470
                 if (rings >= a.ring_cap) { status |= UB_TRAVERSAL; goto done; }
                 a.ring_off[rings++] = outn;
                 int W = A;
@@ -1292,14 +1297,15 @@ __device__ void seq_run(SeqArgs &a) {
                         set_enex(I, st);
                         st = seq_toggle(st);
                     }
-                    if (lab == L_BOUNCING && st == EXIT && nsplit[s] < half / 2) splitL[s][nsplit[s]++] = I;
+                    if (lab == L_BOUNCING && ((st == EXIT) != (a.uni != 0)) && nsplit[s] < half / 2)
+                        splitL[s][nsplit[s]++] = I;
                     if (lab == L_DCROSS) {
                         set_enex(I, st);
                         if (first) {
-                            if (st == EXIT) set_lab(I, L_CROSSING);
+                            if ((st == EXIT) != (a.uni != 0)) set_lab(I, L_CROSSING);
                             first = false;
                         } else {
-                            if (st == ENTRY) set_lab(I, L_CROSSING);
+                            if ((st == ENTRY) != (a.uni != 0)) set_lab(I, L_CROSSING);
                             first = true;
                             st = seq_toggle(st);
                         }
@@ -1307,10 +1313,10 @@ __device__ void seq_run(SeqArgs &a) {
                     if (lab == L_DBOUNCE) {
                         set_enex(I, st);
                         if (first) {
-                            if (st == EXIT && ncross[s] < half / 2) crossL[s][ncross[s]++] = I;
+                            if (((st == EXIT) != (a.uni != 0)) && ncross[s] < half / 2) crossL[s][ncross[s]++] = I;
                             first = false;
                         } else {
-                            if (st == ENTRY && ncross[s] < half / 2) crossL[s][ncross[s]++] = I;
+                            if (((st == ENTRY) != (a.uni != 0)) && ncross[s] < half / 2) crossL[s][ncross[s]++] = I;
                             first = true;
                         }
                         st = seq_toggle(st);
@@ -1342,7 +1348,11 @@ __device__ void seq_run(SeqArgs &a) {
             }
             sinsert(VP, IP);
             sinsert(VQ, IQ);
-            set_enex(IP, EXIT); set_enex(VP, ENTRY); set_enex(IQ, EXIT); set_enex(VQ, ENTRY);
+            if (!a.uni) {
+                set_enex(IP, EXIT); set_enex(VP, ENTRY); set_enex(IQ, EXIT); set_enex(VQ, ENTRY);
+            } else {
+                set_enex(IP, ENTRY); set_enex(VP, EXIT); set_enex(IQ, ENTRY); set_enex(VQ, EXIT);
+            }
             set_lab(IP, L_CROSSING); set_lab(VP, L_CROSSING); set_lab(IQ, L_CROSSING); set_lab(VQ, L_CROSSING);
         }
         // 6) crossing candidates
@@ -1375,14 +1385,15 @@ __device__ void seq_run(SeqArgs &a) {
                 do {
                     int st = seq_toggle(ENEX_(V));
                     do {
-                        V = st == EXIT ? NX_(V) : PV_(V);
+                        const bool fwd = (st == EXIT) != (a.uni != 0); // (status == EXIT) ^ UNION, polyclip.h:740
+                        V = fwd ? NX_(V) : PV_(V);
                         set_inter(V, 0);
                         // within one component the walk is determined by (vertex, direction):
                         // a repeated state can never reach I again (the reference loops forever)
-                        const int dir = st == EXIT ? 1 : 0;
+                        const int dir = fwd ? 1 : 0;
                         if (SEEN_(V, dir) == comp) { status |= UB_TRAVERSAL; goto done; }
                         set_seen(V, dir, comp);
-                        if (++steps > bound || !semit(outn, V, st == EXIT)) { status |= UB_TRAVERSAL; goto done; }
+                        if (++steps > bound || !semit(outn, V, fwd)) { status |= UB_TRAVERSAL; goto done; }
                     } while (!(ENEX_(V) == st) && V != I);
                     if (V != I) {
                         V = NB_(V);
@@ -1428,6 +1439,7 @@ struct SeqBatch {
     uint8_t *inter;
     double *ox, *oy;
     int *osrc, *ring_off, *result;
+    int uni;
 };
 
 // one block per sequential job: its threads lay out the rings, its first wave
@@ -1440,6 +1452,7 @@ __global__ __launch_bounds__(BLK) void k_sequential(SeqBatch b) {
     a.cap = jb.cap;
     a.P0 = jb.P0;
     a.Q0 = jb.Q0;
+    a.uni = b.uni;
     a.nx0 = b.nx0 + jb.node0; a.ny0 = b.ny0 + jb.node0; a.nx1 = b.nx1 + jb.node0; a.ny1 = b.ny1 + jb.node0;
     a.nlen = b.nlen + jb.node0; a.nslot = b.nslot + jb.node0; a.ninter = b.ninter + jb.node0;
     a.nlab = b.nlab + jb.node0; a.nnb = b.nnb + jb.node0;
@@ -1771,6 +1784,7 @@ void label_and_traverse(Ctx &c) {
     SC_HIP(hipMemsetAsync(jw, 0, sizeof(int) * (11 * size_t(J) + 4), s));
     JobState st;
     st.J = J;
+    st.uni = c.uni;
     st.AB = PO;
     st.AC = AC;
     st.slow = jw;
@@ -1926,6 +1940,7 @@ void label_and_traverse(Ctx &c) {
         SC_HIP(hipMemcpyAsync(sidx, hsidx.data(), sizeof(int) * J, hipMemcpyHostToDevice, s));
         SeqBatch b;
         b.jobs = djobs;
+        b.uni = c.uni;
         b.px = c.aug_bx; b.py = c.aug_by; b.qx = c.aug_cx; b.qy = c.aug_cy;
         const int *avP = c.av_b, *labP = c.lab_b, *nbrP = c.nbr_b;
         if (sparse) {

@@ -692,7 +692,7 @@ SmallScratch &sscratch() {
 } // namespace
 
 bool small_batch_eligible(const Ctx &c, int k0, int k1) {
-    if (!c.paired || c.prov || !c.small_fused) return false;
+    if (!c.paired || c.prov || !c.small_fused || c.uni) return false;
     for (int k = k0; k <= k1; ++k)
         if (c.bsz[k - 1] > SP_NV || c.csz[k - 1] > SP_NV) return false;
     return true;
@@ -794,6 +794,7 @@ void pairs_graph_reset(Ctx &c) {
 
 void pairs_area_graph(Ctx &c, const double *bxy, const double *cxy, double *out4) {
     SC_REQUIRE(c.paired && c.K >= 1, SC_ESTATE, "pairs not set (segclip_set_pairs)");
+    SC_REQUIRE(!c.uni, SC_ESTATE, "the area / IoU graph is defined for the intersection");
     SC_REQUIRE(bxy && cxy && out4, SC_EINVAL, "null pointer");
     for (int k = 0; k < c.K; ++k)
         SC_REQUIRE(c.bsz[k] <= SP_NV && c.csz[k] <= SP_NV, SC_EINVAL,

@@ -1,7 +1,10 @@
 // seg_cli.cpp -- bin/seg, the reference's command-line contract on libsegclip.
 //
-//   seg <base polygon> <overlay polygon> [<overlay polygon> ...]
+//   seg [-union] <base polygon> <overlay polygon> [<overlay polygon> ...]
 //
+// -union computes the union instead of the intersection (the UNION switch of
+// foster/polyclip.h:39; the reference's sequential clipper takes the same flag,
+// lib/optimizedFostersAlgorithm/polyclip.cpp).
 // Input files: first line = vertex count n, then n lines "x,y" without the
 // repeated closing vertex (src/clip.cpp:172-311).  Every overlay is clipped
 // against the base (one tree over all rings, as the reference's one-to-many
@@ -44,6 +47,12 @@ static int fail(segclip_ctx *c, const char *what) {
 
 int main(int argc, char **argv) {
     std::printf("MI355X algorithm\n");
+    bool uni = false;
+    if (argc > 1 && std::string(argv[1]) == "-union") {
+        uni = true;
+        --argc;
+        ++argv;
+    }
     if (argc < 3) {
         std::printf("3 parameters required\n./seg <base polygon path> <subject polygon path>\n");
         return 0;
@@ -67,6 +76,7 @@ int main(int argc, char **argv) {
     segclip_ctx *c = nullptr;
     if (segclip_create(&c, 0, nullptr) != SEGCLIP_OK) return fail(nullptr, "segclip_create");
     segclip_set_timing(c, 1);
+    if (uni && segclip_set_operation(c, SEGCLIP_OP_UNION) != SEGCLIP_OK) return fail(c, "segclip_set_operation");
     if (segclip_set_polygons(c, base.data(), int(base.size() / 2), ov.data(), sizes.data(), K) != SEGCLIP_OK)
         return fail(c, "segclip_set_polygons");
     const bool save = std::getenv("SEGCLIP_SAVE") && std::string(std::getenv("SEGCLIP_SAVE")) != "0";

@@ -146,6 +146,11 @@ class Clipper:
                                                    C.c_void_p(ov_ptr), as_ptr(csz, C.c_int32), len(csz)))
         return self
 
+    def set_operation(self, union=False):
+        """Intersection (default) or union of the following clips."""
+        self._check(lib().segclip_set_operation(self._h, 1 if union else 0))
+        return self
+
     def set_provenance(self, on=True):
         self._check(lib().segclip_set_provenance(self._h, 1 if on else 0))
         return self

@@ -30,6 +30,7 @@ SYMBOLS = [
     "segclip_sort_intersections_at_edges", "segclip_copy_intersections_arrays", "segclip_init_labels_arrays",
     "segclip_copy_batch_routes", "segclip_set_tuning", "segclip_get_tuning",
     "segclip_stable_sort_by_key_f64_noalloc", "segclip_pairs_area_graph", "segclip_mbr_join",
+    "segclip_set_operation",
 ]
 
 
@@ -116,6 +117,7 @@ def lib():
         "segclip_stable_sort_by_key_f64": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
         "segclip_stable_sort_by_key_f64_noalloc": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
         "segclip_pairs_area_graph": (C.c_int, [_vp, _vp, _vp, _vp]),
+        "segclip_set_operation": (C.c_int, [_vp, C.c_int]),
         "segclip_mbr_join": (C.c_int, [_vp, _ip, C.c_int, _vp, _ip, C.c_int, _ip, C.c_int64, _i64p, _vp]),
         "segclip_stable_sort_by_key_i32": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
         "segclip_stable_sort_by_key_i64": (C.c_int, [_vp, _vp, C.c_int64, _vp]),

@@ -42,6 +42,8 @@ def lib():
         L.so_edges.argtypes = [vp]
         L.so_clip.restype = vp
         L.so_clip.argtypes = [vp, C.c_int, C.c_int]
+        L.so_clip_op.restype = vp
+        L.so_clip_op.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.so_result_free.argtypes = [vp]
         for n in ["so_res_status", "so_res_num_pairs", "so_res_nb_aug", "so_res_nc_aug", "so_res_num_rings"]:
             getattr(L, n).restype = C.c_int
@@ -145,8 +147,8 @@ class Tree:
         n = self.num_edges
         return (_arr(lib().so_after_leaf(self._h), n, np.int32), _arr(lib().so_before_leaf(self._h), n, np.int32))
 
-    def clip(self, k=1, canonical=True):
-        return Result(self, k, canonical)
+    def clip(self, k=1, canonical=True, union=False):
+        return Result(self, k, canonical, union)
 
     def brute_pairs(self, k=1):
         L = lib()
@@ -162,9 +164,9 @@ class Tree:
 class Result:
     """Oracle clip of the base against overlay k (A8-A17)."""
 
-    def __init__(self, tree, k, canonical):
+    def __init__(self, tree, k, canonical, union=False):
         L = lib()
-        self._h = L.so_clip(tree._h, k, 1 if canonical else 0)
+        self._h = L.so_clip_op(tree._h, k, 1 if canonical else 0, 1 if union else 0)
         if not self._h:
             raise ValueError("oracle clip failed")
         h = self._h
@@ -241,7 +243,7 @@ def read_poly_file(path):
     return rings
 
 
-def run_reference_foster(base, overlay, workdir):
+def run_reference_foster(base, overlay, workdir, union=False):
     """Run the reference's sequential Foster clipper (oracle/_ref/polyclip) and
     return its output rings (6 significant digits, as it prints them)."""
     if not os.path.exists(REF_POLYCLIP):
@@ -251,6 +253,6 @@ def run_reference_foster(base, overlay, workdir):
     r = os.path.join(workdir, "R.poly")
     write_poly_file(p, base)
     write_poly_file(q, overlay)
-    subprocess.run([REF_POLYCLIP, p, q, r], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
-                   cwd=workdir)
+    subprocess.run([REF_POLYCLIP] + (["-union"] if union else []) + [p, q, r], check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=workdir)
     return read_poly_file(r)

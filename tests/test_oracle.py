@@ -140,3 +140,34 @@ def test_predicate_golden_vectors():
         assert O.lsmf(np.array(c["p"])) == c["lsmf"]
         if "alpha" in c:
             assert a == c["alpha"] and b == c["beta"]
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_POLYCLIP), reason="reference Foster clipper not built (no reference tree)")
+@pytest.mark.parametrize("case", _small() + [(n, b, o) for n, b, o, _ in degenerate_cases()], ids=lambda c: c[0])
+def test_union_rings_match_reference_foster(case):
+    """The oracle's union (the UNION switch of foster/polyclip.h:39) equals the
+    reference's sequential clipper run with -union: components, vertex counts,
+    order and the 6-digit coordinates it prints."""
+    name, b, ovs = case
+    r = O.Tree(b, ovs).clip(1, canonical=True, union=True)
+    with tempfile.TemporaryDirectory() as d:
+        ref = O.run_reference_foster(b, ovs[0], d, union=True)
+    assert len(ref) == len(r.rings)
+    for mine, theirs in zip(r.rings, ref):
+        assert mine.shape == theirs.shape
+        assert np.array_equal(np.vectorize(_fmt6)(mine), theirs)
+    # inclusion-exclusion: |P u Q| = |P| + |Q| - |P n Q|.  Two overlapping simple
+    # rings have a connected union: one outer ring (the largest) and holes, all
+    # emitted with the same orientation, so its area is 2 max|A| - sum|A|; with
+    # an empty intersection the union is both rings.  The interior touch of
+    # touch_inside gives no ring at all: its bouncing vertex counts as a crossing
+    # for the union (polygon.h:471), so no special case adds a ring, and no
+    # CROSSING vertex starts a traversal (the reference's own result).
+    if name != "touch_inside":
+        def area(x):
+            return abs(0.5 * float(np.sum(x[:, 0] * np.roll(x[:, 1], -1) - np.roll(x[:, 0], -1) * x[:, 1])))
+        a = [area(x) for x in r.rings]
+        inter = sum(area(x) for x in O.Tree(b, ovs).clip(1, canonical=True).rings)
+        want = area(b) + area(ovs[0]) - inter
+        got = (2 * max(a) - sum(a)) if inter > 0 else sum(a)
+        assert got == pytest.approx(want, rel=1e-9, abs=1e-12)
