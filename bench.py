@@ -422,15 +422,16 @@ def main():
     stab = sum(stab_ms) / len(stab_ms)
     algo = stab_bytes(cnt["cover_entries"], cnt["leaf_end_entries"], cnt["candidates"])
     achieved = algo / (stab * 1e-3) / 1e9 if stab > 0 else 0.0
-    traffic = measured_traffic("k_stab") if (nb, nc) == (200000, 1000000) else None
+    traffic = measured_traffic("stab_stage") if (nb, nc) == (200000, 1000000) else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-            "kernel": "k_stab", "kernel_ms": round(stab, 4), "algorithmic_bytes": algo,
+            "kernel": "stab stage: k_stab + k_stab_rows + k_near_classify", "kernel_ms": round(stab, 4),
+            "algorithmic_bytes": algo, "near_pairs": cnt.get("near_pairs"),
             "work": "36 (L + E) + 12 K bytes with L=%d cover entries, E=%d leaf end entries, K=%d candidates; "
                     "W=%d pair tests" % (cnt["cover_entries"], cnt["leaf_end_entries"], cnt["candidates"],
                                           cnt["tests"]),
-            "traffic_note": "FETCH_SIZE + WRITE_SIZE per launch, scaled by a 1 GiB copy: L2-to-fabric bytes, "
-                            "Infinity Cache hits included"}
+            "traffic_note": "FETCH_SIZE + WRITE_SIZE of the three kernels per stage, scaled by a 1 GiB copy: "
+                            "L2-to-fabric bytes, Infinity Cache hits included"}
     if materialized_end is not None:
         roof["materialized_end_entries"] = materialized_end
 

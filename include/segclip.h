@@ -214,6 +214,7 @@ typedef struct {
                                  16: a quantised alpha or beta beyond the int range */
     int64_t out_vertices;
     int64_t stab_units;       /* work units (waves) of the stab */
+    int64_t near_pairs;       /* stab pairs whose single-precision y-extents meet (classified in FP64) */
 } segclip_counts;
 int segclip_get_counts(segclip_ctx *ctx, segclip_counts *out);
 

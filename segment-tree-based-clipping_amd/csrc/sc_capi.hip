@@ -89,6 +89,11 @@ void release_derived(Ctx &c) {
         b->release();
     for (DBuf<YRangeF> *b : {&c.eyr, &c.lend_yr, &c.cov_yr}) b->release();
     c.lend_bl.release();
+    c.exy.release();
+    c.near.release();
+    c.near_cnt.release();
+    c.srow_off.release();
+    c.srow_node.release();
     c.sp_x.release();
     c.sp_y.release();
     for (DBuf<int> *b : {&c.etype, &c.eid, &c.after_leaf, &c.before_leaf, &c.p_j, &c.p_b, &c.p_c, &c.p_type,
@@ -344,7 +349,7 @@ int segclip_pairs_area_graph(segclip_ctx *h, const double *base_xy, const double
 int segclip_reset(segclip_ctx *h) {
     return guarded(h, [&](sc::Ctx &x) {
         x.epc = x.nleaves = x.T = 0;
-        x.n_lend = x.n_cov = x.n_cand = x.n_pairs = x.n_mend = 0;
+        x.n_lend = x.n_cov = x.n_cand = x.n_pairs = x.n_mend = x.n_near = 0;
         x.cur_k = 0;
         x.pair_off.clear();
         x.jobs.J = 0;
@@ -407,6 +412,7 @@ int segclip_get_counts(segclip_ctx *h, segclip_counts *o) {
     o->undefined = c.undefined;
     o->out_vertices = c.n_out;
     o->stab_units = c.n_units;
+    o->near_pairs = c.n_near;
     return SEGCLIP_OK;
 }
 

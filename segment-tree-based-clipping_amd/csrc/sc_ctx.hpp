@@ -55,6 +55,7 @@ struct Ctx {
     DBuf<double> eymin, eymax;       // y-extent
     DBuf<int> etype, eid;
     DBuf<YRangeF> eyr;               // per edge: y-extent of LSMF's box, rounded outward to single precision
+    DBuf<double4> exy;               // per edge: (x0, y0, x1, y1), one 32-byte record for gathers
 
     // ---- A3 elementary intervals ----
     DBuf<double> ea;
@@ -88,6 +89,11 @@ struct Ctx {
     DBuf<uint64_t> cand;                 // packed (k, b, c) keys
     DBuf<uint64_t> cand_alt;             // its sort partner, same capacity
     int64_t n_cand = 0, cand_cap = 0;
+    DBuf<uint2> near;                    // stab: pairs whose boxes may meet, (row edge | swap bit, column edge),
+                                         // in NEAR_SEGS segments of near_seg_cap entries
+    DBuf<uint64_t> near_cnt;             // per segment: its counter (one per 128-byte line), then 3 totals
+    DBuf<uint32_t> srow_off, srow_node;  // stab rows of small rectangles: per-node offsets, row -> node
+    int64_t n_near = 0, near_seg_cap = 0;
     int64_t n_tests = 0;                 // W
     int64_t n_units = 0;                 // stab work units (waves)
     DBuf<uint64_t> pairs;                // unique packed keys, sorted

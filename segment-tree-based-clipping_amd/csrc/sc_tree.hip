@@ -108,7 +108,7 @@ __global__ void k_edges(const double *__restrict__ bpoly, const double *__restri
                         const int *__restrict__ beoff, const int *__restrict__ bvoff, int nbr,
                         const double *__restrict__ shift, double *x0, double *y0, double *x1, double *y1,
                         double *es, double *ee, double *eyl, double *eyh, int *ety, int *eid, double *uxs,
-                        double *uxe, YRangeF *eyr) {
+                        double *uxe, YRangeF *eyr, double4 *exy) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     const double *P;
@@ -149,6 +149,7 @@ __global__ void k_edges(const double *__restrict__ bpoly, const double *__restri
     ety[e] = ty;
     eid[e] = id;
     eyr[e] = yrange_of(b, d);
+    exy[e] = make_double4(a, b, c, d);
 }
 
 // ---- A3 ----
@@ -452,7 +453,7 @@ void layout_rings(Ctx &c, const double *src, int64_t nverts, const std::vector<i
 
 void reset_derived(Ctx &c) {
     c.epc = c.nleaves = c.T = 0;
-    c.n_lend = c.n_cov = c.n_cand = c.n_pairs = c.n_mend = 0;
+    c.n_lend = c.n_cov = c.n_cand = c.n_pairs = c.n_mend = c.n_near = 0;
     c.cur_k = 0;
     c.pair_off.clear();
 }
@@ -546,7 +547,7 @@ void build_edges(Ctx &c) {
     int ne = c.ne;
     c.ex0.ensure(ne); c.ey0.ensure(ne); c.ex1.ensure(ne); c.ey1.ensure(ne);
     c.estart.ensure(ne); c.eend.ensure(ne); c.etype.ensure(ne); c.eid.ensure(ne);
-    c.eymin.ensure(ne); c.eymax.ensure(ne); c.eyr.ensure(ne);
+    c.eymin.ensure(ne); c.eymax.ensure(ne); c.eyr.ensure(ne); c.exy.ensure(ne);
     if (c.paired) {
         c.uxs.ensure(ne);
         c.uxe.ensure(ne);
@@ -555,7 +556,7 @@ void build_edges(Ctx &c) {
                                                 c.d_bvoff, c.nbr, c.paired ? c.d_shift.p : nullptr, c.ex0, c.ey0,
                                                 c.ex1, c.ey1, c.estart, c.eend, c.eymin, c.eymax, c.etype, c.eid,
                                                 c.paired ? c.uxs.p : nullptr, c.paired ? c.uxe.p : nullptr,
-                                                c.eyr);
+                                                c.eyr, c.exy);
     SC_HIP(hipGetLastError());
 }
 

@@ -45,7 +45,8 @@ class Counts(C.Structure):
                 ("cover_entries", C.c_int64), ("leaf_end_entries", C.c_int64), ("end_entries", C.c_int64),
                 ("tests", C.c_int64), ("candidates", C.c_int64), ("pairs", C.c_int64), ("overlay", C.c_int32),
                 ("overlay_pairs", C.c_int32), ("nb_aug", C.c_int32), ("nc_aug", C.c_int32), ("rings", C.c_int32),
-                ("undefined", C.c_int32), ("out_vertices", C.c_int64), ("stab_units", C.c_int64)]
+                ("undefined", C.c_int32), ("out_vertices", C.c_int64), ("stab_units", C.c_int64),
+                ("near_pairs", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
