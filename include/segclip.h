@@ -292,9 +292,9 @@ int segclip_get_stage_times(segclip_ctx *ctx, float *ms8);
  * oacc_segment_tree.cpp:6250-6526); they serve
  * the autotuner, which times candidate values on a workload and persists the
  * fastest as JSON (segclip.tune).  Results never depend on a setting.
- *   "stab_tiles"  64-column tiles per stab work unit, 1..16, default 1.
- *   "stab_units_per_wave"  consecutive stab work units one wave runs (hits
- *                 staged across them), 1..32, default 8.
+ *   "stab_tiles"  64-column tiles per stab work unit, 1..16, default 2.
+ *   "stab_units_per_wave"  consecutive stab work units one wave runs (near
+ *                 pairs staged across them), 1..32, default 2.
  *   "small_fused" 1 (default): a batch of independent pairs whose rings all
  *                 have at most 64 vertices, without provenance, runs as one
  *                 persistent launch, each pair clipped whole in LDS; 0: the

@@ -211,6 +211,8 @@ void radix_sort_u64_buf(DBuf<uint64_t> &keys, DBuf<uint64_t> &keys_alt, DBuf<uin
 // radix_fixed_words(n) 32-bit words, cleared here by fill kernels; kalt/valt
 // are scratch of n entries.  Returns true when the sorted data ends in
 // kalt/valt.  No host wait.
+// keys per tile of one radix pass (a block of 256 threads, 16 keys each)
+constexpr int RADIX_TILE_KEYS = 4096;
 size_t radix_fixed_words(int64_t n);
 bool radix_sort_u64_fixed(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, int64_t n, int bit_lo,
                           int bit_hi, uint32_t *work, hipStream_t s);

@@ -155,9 +155,9 @@ struct Ctx {
     void ensure_side_stream();
 
     // tuning: 64-column tiles per stab work unit (1..16); results do not depend on it
-    int stab_tiles = 1;
+    int stab_tiles = 2;
     // tuning: consecutive stab work units per wave (1..32); results do not depend on it
-    int stab_upw = 8;
+    int stab_upw = 2;
     // independent pairs of up to 64 + 64 vertices clipped by the persistent
     // small-pair kernel (1) or the staged path (0); results do not depend on it
     int uni = 0; // operation: 0 intersection, 1 union (UNION of foster/polyclip.h:39)

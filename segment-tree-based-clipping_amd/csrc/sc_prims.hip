@@ -148,7 +148,7 @@ template <class T> ScanScratch<T> &scan_scratch() {
 // radix sort
 // ---------------------------------------------------------------------------
 constexpr int RS_BLOCK = 256;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = RADIX_TILE_KEYS / RS_BLOCK;
 constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
 constexpr int RS_WAVES = RS_BLOCK / WAVE;
 constexpr int RS_MAXPASS = 8;

@@ -28,7 +28,7 @@ import time
 
 # candidate values per setting, and the library's defaults
 SETTINGS = {"stab_tiles": (1, 2, 4, 8), "stab_units_per_wave": (1, 2, 4, 8, 16)}
-DEFAULTS = {"stab_tiles": 1, "stab_units_per_wave": 8}
+DEFAULTS = {"stab_tiles": 2, "stab_units_per_wave": 2}
 LIMITS = {"stab_tiles": (1, 16), "stab_units_per_wave": (1, 32)}
 
 
@@ -67,12 +67,32 @@ def autotune(clipper, step, name="stab_tiles", candidates=None, warmup=1, reps=3
 
 
 def autotune_all(clipper, step, warmup=1, reps=3):
-    """Tune every setting in turn (each with the earlier ones fixed at their
-    chosen values).  Returns ({setting: value}, {setting: {value: ms}})."""
-    chosen, timings = {}, {}
-    for name in SETTINGS:
-        chosen[name], timings[name] = autotune(clipper, step, name, warmup=warmup, reps=reps)
-    return chosen, timings
+    """Time every combination of the settings' candidate values (a grid: the
+    settings interact) and keep the fastest.  Returns ({setting: value},
+    {"grid": {"v1,v2,...": ms}}), values in SETTINGS order."""
+    import itertools
+    import torch
+
+    names = list(SETTINGS)
+    grid = {}
+    for combo in itertools.product(*(SETTINGS[n] for n in names)):
+        for n, v in zip(names, combo):
+            clipper.set_tuning(n, v)
+        for _ in range(warmup):
+            step(clipper)
+        torch.cuda.synchronize()
+        best = math.inf
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            step(clipper)
+            torch.cuda.synchronize()
+            best = min(best, 1000.0 * (time.perf_counter() - t0))
+        grid[",".join(str(v) for v in combo)] = best
+    key = min(grid, key=grid.get)
+    chosen = {n: int(v) for n, v in zip(names, key.split(","))}
+    for n, v in chosen.items():
+        clipper.set_tuning(n, v)
+    return chosen, {"grid": grid}
 
 
 def save(path, entries, device=None):

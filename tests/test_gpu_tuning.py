@@ -15,7 +15,7 @@ from fixtures import rings_equal
 pytestmark = pytest.mark.gpu
 
 TILES = (1, 2, 3, 4, 8, 16)
-# (stab_tiles, stab_units_per_wave) combinations beside the defaults (1, 8)
+# (stab_tiles, stab_units_per_wave) combinations beside the defaults (2, 2)
 COMBOS = ((2, 1), (2, 2), (2, 3), (2, 4), (2, 16), (2, 32), (4, 4), (1, 2))
 
 
@@ -33,7 +33,7 @@ def test_stab_tiles_do_not_change_results(gpu):
     o = S.harmonic(60000, 2002, 0.15, -0.1, 1.0, 17.0 * math.pi / 180.0)
     print("[gpu-tuning] harmonic 20k x 60k, stab_tiles", TILES, flush=True)
     cl = segclip.Clipper().set_polygons(b, [o])
-    assert cl.get_tuning("stab_tiles") == 1
+    assert cl.get_tuning("stab_tiles") == 2
     snaps = {}
     for t in TILES:
         print("[gpu-tuning] stab_tiles=%d" % t, flush=True)
@@ -109,10 +109,10 @@ def test_tuning_refuses_bad_settings(gpu):
     for v in (0, 33):
         with pytest.raises(SegclipError):
             cl.set_tuning("stab_units_per_wave", v)
-    assert cl.get_tuning("stab_units_per_wave") == 8
+    assert cl.get_tuning("stab_units_per_wave") == 2
     with pytest.raises(SegclipError):
         cl.set_tuning("no_such_setting", 2)
     with pytest.raises(SegclipError):
         cl.get_tuning("no_such_setting")
-    assert cl.get_tuning("stab_tiles") == 1
+    assert cl.get_tuning("stab_tiles") == 2
     cl.close()
