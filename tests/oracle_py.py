@@ -12,6 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "libsegoracle.so")
 REF_POLYCLIP = os.path.join(ORACLE_DIR, "_ref", "polyclip")
+# the same program with its result rings written at 17 significant digits (oracle/polyclip_full.cpp)
+REF_POLYCLIP17 = os.path.join(ORACLE_DIR, "_ref", "polyclip17")
 
 _lib = None
 
@@ -243,16 +245,18 @@ def read_poly_file(path):
     return rings
 
 
-def run_reference_foster(base, overlay, workdir, union=False):
+def run_reference_foster(base, overlay, workdir, union=False, full_precision=False):
     """Run the reference's sequential Foster clipper (oracle/_ref/polyclip) and
-    return its output rings (6 significant digits, as it prints them)."""
-    if not os.path.exists(REF_POLYCLIP):
+    return its output rings (6 significant digits, as it prints them; 17 with
+    full_precision, oracle/_ref/polyclip17)."""
+    exe = REF_POLYCLIP17 if full_precision else REF_POLYCLIP
+    if not os.path.exists(exe):
         return None
     p = os.path.join(workdir, "P.poly")
     q = os.path.join(workdir, "Q.poly")
     r = os.path.join(workdir, "R.poly")
     write_poly_file(p, base)
     write_poly_file(q, overlay)
-    subprocess.run([REF_POLYCLIP] + (["-union"] if union else []) + [p, q, r], check=True,
+    subprocess.run([exe] + (["-union"] if union else []) + [p, q, r], check=True,
                    stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=workdir)
     return read_poly_file(r)

@@ -67,6 +67,53 @@ def test_rings_match_reference_foster(case):
         assert np.array_equal(printed, theirs)
 
 
+@pytest.mark.skipif(not os.path.exists(O.REF_POLYCLIP17), reason="reference Foster clipper not built (no reference tree)")
+@pytest.mark.parametrize("case", _small() + [(n, b, o) for n, b, o, _ in degenerate_cases()], ids=lambda c: c[0])
+def test_rings_match_reference_foster_full_precision(case):
+    """Rings against the reference's own clipper printed at 17 digits: same
+    components and vertices in the same cyclic order, coordinates within 1e-12
+    relative (the two programs compute intersection points by the same
+    formula but not necessarily with the same rounding)."""
+    name, b, ovs = case
+    r = O.Tree(b, ovs).clip(1, canonical=True)
+    with tempfile.TemporaryDirectory() as d:
+        ref = O.run_reference_foster(b, ovs[0], d, full_precision=True)
+    assert len(ref) == len(r.rings)
+    for mine, theirs in zip(r.rings, ref):
+        assert mine.shape == theirs.shape
+        tol = 1e-12 * np.maximum(1.0, np.abs(theirs))
+        assert np.all(np.abs(mine - theirs) <= tol), np.max(np.abs(mine - theirs))
+
+
+@pytest.mark.parametrize("case", _small() + [(n, b, o) for n, b, o, _ in degenerate_cases()], ids=lambda c: c[0])
+def test_intersection_points_closed_form(case):
+    """A12's points against an independent solution: an X crossing (type 1)
+    is the solution of the 2x2 system b1 + a (b2 - b1) = c1 + t (c2 - c1)
+    (numpy.linalg.solve, within 1e-12 relative); every other type lies on an
+    input vertex exactly: c1 for types 3, 4, 5, 7, 8 and b1 for 2 and 6
+    (saveIntersectionsParallel, oacc_segment_tree.cpp:7257-7341)."""
+    name, b, ovs = case
+    b = np.asarray(b, dtype=np.float64).reshape(-1, 2)
+    c = np.asarray(ovs[0], dtype=np.float64).reshape(-1, 2)
+    r = O.Tree(b, ovs).clip(1, canonical=True)
+    n, m = len(b), len(c)
+    for eid, fid, t, p in zip(r.pair_b, r.pair_c, r.pair_type, r.intersections):
+        e, f = eid // 2, fid // 2  # edge ids are 2 x the first vertex's offset (Edge::id)
+        b1, b2 = b[e], b[(e + 1) % n]
+        c1, c2 = c[f], c[(f + 1) % m]
+        if t == 1:
+            A = np.array([[b2[0] - b1[0], -(c2[0] - c1[0])], [b2[1] - b1[1], -(c2[1] - c1[1])]])
+            a, _ = np.linalg.solve(A, c1 - b1)
+            x = b1 + a * (b2 - b1)
+            scale = max(1.0, float(np.max(np.abs(np.concatenate([b1, b2, c1, c2])))))
+            assert np.all(np.abs(p - x) <= 1e-12 * scale), (name, e, f, p, x)
+        elif t in (3, 4, 5, 7, 8):
+            assert p[0] == c1[0] and p[1] == c1[1], (name, e, f, t)
+        else:
+            assert t in (2, 6), t
+            assert p[0] == b1[0] and p[1] == b1[1], (name, e, f, t)
+
+
 def test_elementary_closed_form():
     b, ovs = workloads_small()[0][1:]
     tr = O.Tree(b, ovs)
