@@ -1152,10 +1152,24 @@ __device__ void seq_run(SeqArgs &a) {
         return true;
     };
     const int rootP = 0, rootQ = a.nP;
-    // 2) intersection chains
-    {
-        int I = rootP;
-        do {
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+    // 2) intersection chains.  Before any insertion ring P is nodes [0, nP) in
+    // ring order from its root, so the chain starts (intersections labelled
+    // LEFT_ON or RIGHT_ON) are found 64 nodes per ballot, in that order, and
+    // walked one after another.  A start that an earlier chain relabelled is
+    // checked again and skipped, and a later block reads the labels the chains
+    // before it wrote: the reference's single walk, step for step.
+    for (int w = 0; w < a.nP; w += WAVE) {
+        const int v = w + lane;
+        bool start = false;
+        if (v < a.nP) {
+            const int lb = a.lab[v];
+            start = a.inter[v] != 0 && (lb == L_LEFT_ON || lb == L_RIGHT_ON);
+        }
+        uint64_t starts = __ballot(start);
+        while (starts) {
+            const int I = w + __builtin_ctzll(starts);
+            starts &= starts - 1ull;
             if (INTER_(I) && (LAB_(I) == L_LEFT_ON || LAB_(I) == L_RIGHT_ON)) {
                 int x = LAB_(I) == L_LEFT_ON ? 0 : 1;
                 int X = I, J = I, guard = 0;
@@ -1172,25 +1186,42 @@ __device__ void seq_run(SeqArgs &a) {
                 if (NB_(X) >= 0) set_lab(NB_(X), ct);
                 if (NB_(J) >= 0) set_lab(NB_(J), ct);
             }
-            I = NX_(I);
-        } while (I != rootP);
+        }
     }
     {
         int rings = 0, outn = 0;
         int roots[2] = {rootP, rootQ};
         bool noInter[2] = {false, false}, ident[2] = {false, false};
-        // 3.5) rings without crossing vertices
+        // rings in index order (no insertion yet): ring s is nodes [ring0[s], ring0[s] + ringn[s])
+        const int ring0[2] = {rootP, rootQ}, ringn[2] = {a.nP, a.nQ};
+        // the ring's first node in ring order that is not an intersection, or -1
+        auto first_plain = [&](int s) {
+            for (int w = 0; w < ringn[s]; w += WAVE) {
+                const int v = ring0[s] + w + lane;
+                const uint64_t m = __ballot(w + lane < ringn[s] && a.inter[v] == 0);
+                if (m) return ring0[s] + w + int(__builtin_ctzll(m));
+            }
+            return -1;
+        };
+        // 3.5) rings without crossing vertices (64 nodes per ballot)
         for (int s = 0; s < 2; ++s) {
             int A = roots[s], B = roots[1 - s];
             bool nocross = true, allon = true;
             int V = A;
-            do {
-                if (INTER_(V) && (LAB_(V) == L_CROSSING || LAB_(V) == L_DCROSS)) nocross = false;
-                // polygon::noCrossingVertex(union_case), polygon.h:465-475
-                if (a.uni && INTER_(V) && (LAB_(V) == L_BOUNCING || LAB_(V) == L_DBOUNCE)) nocross = false;
-                if (LAB_(V) != L_ON_ON) allon = false;
-                V = NX_(V);
-            } while (V != A);
+            for (int w = 0; w < ringn[s] && nocross; w += WAVE) {
+                const int v = ring0[s] + w + lane;
+                bool cr = false, notOn = false;
+                if (w + lane < ringn[s]) {
+                    const int lb = a.lab[v];
+                    const bool it = a.inter[v] != 0;
+                    cr = it && (lb == L_CROSSING || lb == L_DCROSS);
+                    // polygon::noCrossingVertex(union_case), polygon.h:465-475
+                    if (a.uni && it && (lb == L_BOUNCING || lb == L_DBOUNCE)) cr = true;
+                    notOn = lb != L_ON_ON;
+                }
+                if (__ballot(cr)) nocross = false;
+                if (__ballot(notOn)) allon = false;
+            }
             if (!nocross) continue;
             noInter[s] = true;
             if (allon) {
@@ -1199,16 +1230,12 @@ __device__ void seq_run(SeqArgs &a) {
             }
             double qx = 0, qy = 0;
             bool found = false, plain = false;
-            V = A;
-            do {
-                if (!INTER_(V)) {
-                    qx = a.vx[V];
-                    qy = a.vy[V];
-                    found = plain = true;
-                    break;
-                }
-                V = NX_(V);
-            } while (V != A);
+            V = first_plain(s);
+            if (V >= 0) {
+                qx = a.vx[V];
+                qy = a.vy[V];
+                found = plain = true;
+            }
             if (!found) {
                 V = A;
                 do {
@@ -1258,14 +1285,19 @@ __device__ void seq_run(SeqArgs &a) {
         int half = a.cap / 2; // cand[0..half) split lists, [half..cap) crossing lists
         int *splitL[2] = {a.cand, a.cand + half / 2};
         int *crossL[2] = {a.cand + half, a.cand + half + half / 2};
+        // the entry/exit walk's state (status, first of a delayed pair) as 2 bits
+        // x = st | first << 1 (EXIT = 0, ENTRY = 1); a node's effect on it is a
+        // 4-entry table of 2-bit states, and tables compose along the ring
+        auto tapply = [](uint32_t t, int x) { return int((t >> (2 * x)) & 3u); };
+        auto tcompose = [](uint32_t f, uint32_t g) { // f first, then g
+            uint32_t h = 0;
+            for (int x = 0; x < 4; ++x) h |= ((g >> (2 * ((f >> (2 * x)) & 3u))) & 3u) << (2 * x);
+            return h;
+        };
         for (int s = 0; s < 2; ++s) {
             if (noInter[s]) continue;
             int A = roots[s], B = roots[1 - s];
-            int V = -1, W = A;
-            do {
-                if (!INTER_(W)) { V = W; break; }
-                W = NX_(W);
-            } while (W != A);
+            int V = first_plain(s), W = A; // ring s has no inserted node yet
             if (V < 0) {
                 W = A;
                 do {
@@ -1289,6 +1321,72 @@ __device__ void seq_run(SeqArgs &a) {
             const bool in = known ? ((s == 0 ? a.windP : a.windQ) % 2) != 0 : seq_point_in(a, B, a.vx[V], a.vy[V]);
             if (in) st = seq_toggle(st);
             bool first = true;
+            if (!inserted[s]) {
+                // The walk from the node after V around to V, 64 nodes per step: an
+                // inclusive scan of the nodes' tables gives every node the state the
+                // walk reaches it in, then each lane applies its node's effect; list
+                // appends keep ring order (ballot ranks) and the lists' bounds.
+                const int r0 = ring0[s], m = ringn[s], v0 = V - r0, lim = half / 2;
+                const bool uni = a.uni != 0;
+                int state = st | 2; // first = true
+                for (int p0 = 1; p0 < m; p0 += WAVE) {
+                    const int p = p0 + lane;
+                    const bool on = p < m;
+                    int I = -1, lab = L_NONE;
+                    bool it = false;
+                    if (on) {
+                        I = r0 + (v0 + p >= m ? v0 + p - m : v0 + p);
+                        lab = a.lab[I];
+                        it = a.inter[I] != 0;
+                    }
+                    uint32_t tab = 0xE4u; // identity
+                    if (it) {
+                        if (lab == L_CROSSING) tab = 0xB1u;     // status toggles
+                        else if (lab == L_DCROSS) tab = 0x4Bu;  // first: clear it; else set it, toggle
+                        else if (lab == L_DBOUNCE) tab = 0x1Bu; // first flips, status toggles
+                    }
+                    uint32_t P = tab;
+                    for (int d = 1; d < WAVE; d <<= 1) {
+                        const uint32_t o = uint32_t(__shfl_up(int(P), d, WAVE));
+                        if (lane >= d) P = tcompose(o, P);
+                    }
+                    const uint32_t Pex = uint32_t(__shfl_up(int(P), 1, WAVE));
+                    const int sin = lane == 0 ? state : tapply(Pex, state);
+                    const int sst = sin & 1;
+                    const bool sfirst = (sin >> 1) != 0;
+                    bool addS = false, addC = false;
+                    if (it) {
+                        if (lab == L_CROSSING) {
+                            a.enex[I] = sst;
+                        } else if (lab == L_BOUNCING) {
+                            addS = (sst == EXIT) != uni;
+                        } else if (lab == L_DCROSS) {
+                            a.enex[I] = sst;
+                            if (sfirst ? ((sst == EXIT) != uni) : ((sst == ENTRY) != uni)) a.lab[I] = L_CROSSING;
+                        } else if (lab == L_DBOUNCE) {
+                            a.enex[I] = sst;
+                            addC = sfirst ? ((sst == EXIT) != uni) : ((sst == ENTRY) != uni);
+                        }
+                    }
+                    const uint64_t ms = __ballot(addS), mc = __ballot(addC);
+                    if (addS) {
+                        const int q = nsplit[s] + int(__popcll(ms & lt));
+                        if (q < lim) splitL[s][q] = I;
+                    }
+                    if (addC) {
+                        const int q = ncross[s] + int(__popcll(mc & lt));
+                        if (q < lim) crossL[s][q] = I;
+                    }
+                    nsplit[s] = min(nsplit[s] + int(__popcll(ms)), lim);
+                    ncross[s] = min(ncross[s] + int(__popcll(mc)), lim);
+                    const int last = min(WAVE, m - p0) - 1;
+                    state = tapply(uint32_t(__builtin_amdgcn_readlane(int(P), last)), state);
+                }
+                // the lanes' stores, visible to every lane before the window reloads
+                __threadfence_block();
+                w0 = -(1 << 30);
+                continue;
+            }
             int I = NX_(V);
             while (I != V) {
                 if (INTER_(I)) {
