@@ -1035,6 +1035,267 @@ __device__ void seq_init(SeqArgs &a) {
     }
 }
 
+// createResult2's walks, one per lane (see seq_run).  Returns PT_DONE with
+// rings/outn advanced and the rings written, PT_FAIL when the single walk
+// would fail (a ring that revisits a (node, direction) state, a jump to no
+// neighbour, or the output capacities), or PT_FALLBACK after clearing what it
+// wrote.  Candidate ranks live in seen[2v] (rank + 1) and the first ring that
+// visits a candidate in seen[2v + 1] (as SEEN_FIRST - ring, kept by atomicMax),
+// both zero before and after.
+enum : int { PT_DONE = 0, PT_FAIL = 1, PT_FALLBACK = 2 };
+constexpr int PT_MAX_STEPS = 1 << 15; // a longer ring walks in the single walk
+constexpr int SEEN_FIRST = 1 << 30;
+constexpr int PT_CHUNK = 2048; // candidates per round of walks
+
+// one ring's walk from node I by one lane, as createResult2 walks it
+// (foster/polyclip.h:717-759): returns 0 when it closes, 1 when it fails, 2
+// when it is longer than PT_MAX_STEPS, 3 when 'visit' asks it to stop.
+// 'visit' sees every node the walk marks (the start, every node stepped to,
+// every jump target); 'step' sees every node stepped to with its direction
+// (the emitted ones).
+template <class Visit, class Step>
+__device__ inline int pt_walk(const SeqArgs &a, int I, Visit visit, Step step) {
+    const bool uni = a.uni != 0;
+    int V = I;
+    if (visit(V)) return 3;
+    int steps = 0, lam = 0, power = 1, sv = -1, sd = -1; // Brent's cycle test on (node, direction)
+    do {
+        const int e = a.enex[V];
+        const int st = e == ENTRY ? EXIT : (e == EXIT ? ENTRY : e);
+        do {
+            const bool fwd = (st == EXIT) != uni;
+            V = fwd ? a.nx[V] : a.pv[V];
+            const int dir = fwd ? 1 : 0;
+            if (V == sv && dir == sd) return 1; // a state seen again: the walk never closes
+            if (++lam == power) {
+                sv = V;
+                sd = dir;
+                power <<= 1;
+                lam = 0;
+            }
+            if (++steps > PT_MAX_STEPS) return 2;
+            if (visit(V)) return 3;
+            step(V, fwd);
+        } while (!(a.enex[V] == st) && V != I);
+        if (V != I) {
+            V = a.nb[V];
+            if (V < 0) return 1;
+            if (visit(V)) return 3;
+        }
+    } while (V != I);
+    return 0;
+}
+
+__device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int lane, uint64_t lt) {
+    const int n0 = a.nP + a.nQ; // nodes before any insertion; inserted nodes follow
+    if (n - n0 > 4096) return PT_FALLBACK;
+    const int half = a.cap / 2;
+    int *cand = a.cand, *info = a.cand + half;
+    // 1) the candidates in ring order from the root: original P nodes in index
+    //    order, each followed by the nodes inserted after it
+    auto is_cand = [&](int v) { return a.inter[v] != 0 && a.lab[v] == L_CROSSING; };
+    int ncand = 0;
+    // back to all-zero state words (listed candidates only carry marks)
+    auto clear = [&]() {
+        __threadfence_block();
+        for (int q = lane; q < ncand; q += WAVE) {
+            const int v = cand[q];
+            a.seen[2 * v] = 0;
+            a.seen[2 * v + 1] = 0;
+        }
+        __threadfence_block();
+    };
+    for (int w = 0; w < a.nP; w += WAVE) {
+        const int v = w + lane;
+        int c = 0;
+        if (v < a.nP) {
+            c = is_cand(v) ? 1 : 0;
+            for (int x = a.nx[v], g = 0; x >= n0 && g <= n - n0; x = a.nx[x], ++g) c += is_cand(x) ? 1 : 0;
+        }
+        int incl = c;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int o = __shfl_up(incl, d, WAVE);
+            if (lane >= d) incl += o;
+        }
+        const int tot = __shfl(incl, WAVE - 1, WAVE);
+        if (ncand + tot > half) {
+            clear();
+            return PT_FALLBACK;
+        }
+        int q = ncand + incl - c;
+        if (v < a.nP) {
+            if (is_cand(v)) {
+                cand[q] = v;
+                a.seen[2 * v] = ++q;
+            }
+            for (int x = a.nx[v], g = 0; x >= n0 && g <= n - n0; x = a.nx[x], ++g)
+                if (is_cand(x)) {
+                    cand[q] = x;
+                    a.seen[2 * x] = ++q;
+                }
+        }
+        ncand += tot;
+    }
+    __threadfence_block();
+    // 2)-4) in chunks of PT_CHUNK candidates in rank order, so that the
+    // single walk's first failing ring is found without walking every
+    // candidate after it.
+    __shared__ int s_next, s_qend, s_long;
+    int nring = 0;    // rings of the chunks before
+    int64_t nout = 0; // their vertices
+    for (int c0 = 0; c0 < ncand; c0 += PT_CHUNK) {
+        const int c1 = min(ncand, c0 + PT_CHUNK);
+        // 2) every candidate walks without writing until it marks a candidate
+        //    of smaller rank (then it does not own its ring); an owner's walk
+        //    gives its ring's vertex count or its failure.  The single walk
+        //    ends at the first failing ring: nothing after that owner's rank
+        //    matters.  Lanes take candidates from a counter in LDS as their
+        //    walks end, so a long walk holds one lane, not 63 others.
+        if (lane == 0) {
+            s_next = c0 + WAVE;
+            s_qend = c1;
+            s_long = 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int q = c0 + lane; q < c1;) {
+            if (q >= __hip_atomic_load(&s_qend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+            int cnt = 0;
+            const int res = pt_walk(
+                a, cand[q],
+                [&](int v) {
+                    const int r = a.seen[2 * v];
+                    return r > 0 && r < q + 1;
+                },
+                [&](int v, bool) { cnt += a.vlen[v]; });
+            if (res == 2) atomicOr(&s_long, 1);
+            if (res == 1) atomicMin(&s_qend, q + 1);
+            info[q] = res == 3 ? -1 : (res == 1 ? -2 : cnt);
+            q = atomicAdd(&s_next, 1);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __threadfence_block();
+        if (__hip_atomic_load(&s_long, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            clear();
+            return PT_FALLBACK;
+        }
+        const int qend = __hip_atomic_load(&s_qend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const bool fail = qend < c1 || info[c1 - 1] == -2;
+        // 3) ring order: owner r (in the order of the starts) gets ring index
+        //    r (kept in its seen[2v], still positive) and its output place
+        //    from a running sum (kept in info); then the owners walk again in
+        //    any order, and every candidate a walk marks keeps the first ring
+        //    that marked it
+        const int nring0 = nring;
+        for (int q0 = c0; q0 < qend; q0 += WAVE) {
+            const int q = q0 + lane;
+            const int inf = q < qend ? info[q] : -1;
+            const bool owner = inf != -1;
+            const uint64_t om = __ballot(owner);
+            const int r = nring + int(__popcll(om & lt));
+            const int c = owner && inf > 0 ? inf : 0;
+            int incl = c;
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const int o = __shfl_up(incl, d, WAVE);
+                if (lane >= d) incl += o;
+            }
+            const int at = outn + int(nout) + incl - c;
+            if (owner) {
+                const bool write = !fail && rings + r < a.ring_cap && int64_t(outn) + nout + incl <= a.out_cap;
+                if (write) a.ring_off[rings + r] = at;
+                a.seen[2 * cand[q]] = r + 1;
+                info[q] = write ? at : -3; // -3: an owner that writes nothing
+            }
+            nring += int(__popcll(om));
+            nout += __shfl(incl, WAVE - 1, WAVE);
+        }
+        __threadfence_block();
+        if (lane == 0) s_next = c0 + WAVE;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int q = c0 + lane; q < qend; q = atomicAdd(&s_next, 1)) {
+            const int inf = info[q];
+            if (inf == -1) continue;
+            const int X = cand[q];
+            const int tag = SEEN_FIRST - (a.seen[2 * X] - 1);
+            const bool write = inf >= 0;
+            int at = write ? inf : 0;
+            (void)pt_walk(
+                a, X,
+                [&](int v) {
+                    if (a.seen[2 * v] > 0) atomicMax(&a.seen[2 * v + 1], tag);
+                    return false;
+                },
+                [&](int v, bool fwd) {
+                    const int len = a.vlen[v];
+                    if (write) {
+                        const int vs = a.vslot[v];
+                        const bool onP = v < a.nP;
+                        for (int t = 0; t < len; ++t) {
+                            const int u = fwd ? t : len - 1 - t;
+                            double x, y;
+                            if (vs >= 0 && len > 1) {
+                                x = onP ? a.px[vs + u] : a.qx[vs + u];
+                                y = onP ? a.py[vs + u] : a.qy[vs + u];
+                            } else {
+                                x = a.vx[v];
+                                y = a.vy[v];
+                            }
+                            a.ox[at + t] = x;
+                            a.oy[at + t] = y;
+                            a.osrc[at + t] = vs >= 0 ? (onP ? a.P0 + vs + u : ~(a.Q0 + vs + u)) : a.vsrc[v];
+                        }
+                    }
+                    at += len;
+                });
+        }
+        __builtin_amdgcn_wave_barrier();
+        __threadfence_block();
+        // 4) the check: an owner is first marked by its own ring, any other
+        //    candidate by a ring whose start comes before it.  At the first
+        //    candidate where that does not hold, a candidate no earlier ring
+        //    marked is where the single walk starts its next ring: if a full
+        //    walk from it fails, the single walk fails there too.
+        int before = nring0;
+        for (int q0 = c0; q0 < qend; q0 += WAVE) {
+            const int q = q0 + lane;
+            const int inf = q < qend ? info[q] : -1;
+            const bool owner = inf != -1;
+            const uint64_t om = __ballot(owner);
+            const int r = before + int(__popcll(om & lt));
+            bool bad = false, unmarked = false;
+            if (q < qend) {
+                const int fv = a.seen[2 * cand[q] + 1];
+                const int first = fv > 0 ? SEEN_FIRST - fv : INT_MAX;
+                bad = owner ? first != r : first >= r;
+                unmarked = !owner && first >= r;
+            }
+            const uint64_t bm = __ballot(bad);
+            if (bm) {
+                const int l = int(__builtin_ctzll(bm));
+                const bool um = __shfl(int(unmarked), l, WAVE) != 0;
+                int res = 0;
+                if (um && lane == 0)
+                    res = pt_walk(
+                        a, cand[q0 + l], [&](int) { return false; }, [&](int, bool) {});
+                res = __shfl(res, 0, WAVE);
+                clear();
+                return um && res == 1 ? PT_FAIL : PT_FALLBACK;
+            }
+            before += int(__popcll(om));
+        }
+        if (fail) {
+            clear();
+            return PT_FAIL;
+        }
+    }
+    clear();
+    if (rings + nring > a.ring_cap || int64_t(outn) + nout > a.out_cap) return PT_FAIL;
+    rings += nring;
+    outn += int(nout);
+    return PT_DONE;
+}
+
 // The reference's sequential algorithm on the job's rings, run by one wave
 // with uniform control flow: every lane holds the same walk state.  Lane i
 // caches the hot fields of node w0 + i (a window of 64 consecutive nodes,
@@ -1463,8 +1724,20 @@ __device__ void seq_run(SeqArgs &a) {
         }
         for (int j = 0; j < nsplit[1]; ++j) set_seen(splitL[1][j], 0, 0);
         for (int j = 0; j < ncross[1]; ++j) set_seen(crossL[1][j], 1, 0);
-        // createResult2
-        {
+        // createResult2.  First the walks run one per lane (seq_par_traverse):
+        // every candidate start (a P crossing, in ring order from the root)
+        // walks its ring without writing; the candidate with the smallest
+        // rank on a ring owns it; the owners walk again and write the rings.
+        // The reference's single walk starts a ring at the first candidate
+        // that no earlier ring visited, so the result is the same when every
+        // owner is first visited by its own ring and every other candidate by
+        // a ring that starts before it; that is checked, and otherwise (or
+        // when a walk is too long or too many nodes were inserted) the single
+        // walk below runs.
+        const int ptr = seq_par_traverse(a, n, rings, outn, lane, lt);
+        if (ptr == PT_FAIL) { status |= UB_TRAVERSAL; goto done; }
+        if (ptr == PT_FALLBACK) w0 = -(1 << 30); // the window reloads what the attempt wrote
+        if (ptr == PT_FALLBACK) {
             int root = rootP, I = root;
             bool have = INTER_(I) && LAB_(I) == L_CROSSING;
             if (!have) {
