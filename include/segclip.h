@@ -300,6 +300,11 @@ int segclip_get_stage_times(segclip_ctx *ctx, float *ms8);
  *                 persistent launch, each pair clipped whole in LDS; 0: the
  *                 staged path.  A batch holding a pair the persistent kernel
  *                 does not cover reruns on the staged path.
+ *   "seq_parallel_walks"  1 (default): the traversal of a job on the
+ *                 sequential route walks its rings one per lane and checks
+ *                 that they are the ones the reference's single walk finds
+ *                 (taking the single walk when they may not be); 0: the
+ *                 single walk always.
  * segclip_get_tuning also reads "small_fallback": the fallback reasons of the
  * last small-pair batch (0 when it ran as one launch).
  * An unknown name or a value out of range returns SEGCLIP_EINVAL. */

@@ -590,6 +590,9 @@ int segclip_set_tuning(segclip_ctx *h, const char *name, int value) {
         } else if (std::strcmp(name, "small_fused") == 0) {
             SC_REQUIRE(value == 0 || value == 1, SEGCLIP_EINVAL, "small_fused must be 0 or 1");
             c.small_fused = value;
+        } else if (std::strcmp(name, "seq_parallel_walks") == 0) {
+            SC_REQUIRE(value == 0 || value == 1, SEGCLIP_EINVAL, "seq_parallel_walks must be 0 or 1");
+            c.seq_par = value;
         } else {
             SC_REQUIRE(false, SEGCLIP_EINVAL, std::string("unknown tuning setting: ") + name);
         }
@@ -605,6 +608,8 @@ int segclip_get_tuning(segclip_ctx *h, const char *name, int *value) {
             *value = c.stab_upw;
         } else if (std::strcmp(name, "small_fused") == 0) {
             *value = c.small_fused;
+        } else if (std::strcmp(name, "seq_parallel_walks") == 0) {
+            *value = c.seq_par;
         } else if (std::strcmp(name, "small_fallback") == 0) {
             *value = c.small_fail;
         } else {

@@ -162,6 +162,9 @@ struct Ctx {
     // small-pair kernel (1) or the staged path (0); results do not depend on it
     int uni = 0; // operation: 0 intersection, 1 union (UNION of foster/polyclip.h:39)
     int small_fused = 1;
+    // sequential-route rings walked one per lane (1) or by the single walk
+    // (0); results do not depend on it
+    int seq_par = 1;
     int small_fail = 0; // fallback reasons of the last small-pair batch (0: it ran fused)
     // the small-pair forward captured as a hipGraph (segclip_pairs_area_graph):
     // the buffers it reads and writes belong to it, so a replay never sees a

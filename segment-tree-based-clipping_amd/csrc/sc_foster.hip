@@ -960,6 +960,7 @@ struct SeqArgs {
     int *vsrc, *osrc; // provenance of inserted vertices / output vertices (slot reference)
     int P0, Q0;       // first slot of the job's rings in the batch arrays
     int uni;          // 1: the union (UNION of foster/polyclip.h:39 and its XORs)
+    int par;          // 1: createResult2's rings walked one per lane first
     int out_cap;
     int *ring_off;
     int ring_cap;
@@ -1734,7 +1735,7 @@ __device__ void seq_run(SeqArgs &a) {
         // a ring that starts before it; that is checked, and otherwise (or
         // when a walk is too long or too many nodes were inserted) the single
         // walk below runs.
-        const int ptr = seq_par_traverse(a, n, rings, outn, lane, lt);
+        const int ptr = a.par ? seq_par_traverse(a, n, rings, outn, lane, lt) : PT_FALLBACK;
         if (ptr == PT_FAIL) { status |= UB_TRAVERSAL; goto done; }
         if (ptr == PT_FALLBACK) w0 = -(1 << 30); // the window reloads what the attempt wrote
         if (ptr == PT_FALLBACK) {
@@ -1811,6 +1812,7 @@ struct SeqBatch {
     double *ox, *oy;
     int *osrc, *ring_off, *result;
     int uni;
+    int par;
 };
 
 // one block per sequential job: its threads lay out the rings, its first wave
@@ -1824,6 +1826,7 @@ __global__ __launch_bounds__(BLK) void k_sequential(SeqBatch b) {
     a.P0 = jb.P0;
     a.Q0 = jb.Q0;
     a.uni = b.uni;
+    a.par = b.par;
     a.nx0 = b.nx0 + jb.node0; a.ny0 = b.ny0 + jb.node0; a.nx1 = b.nx1 + jb.node0; a.ny1 = b.ny1 + jb.node0;
     a.nlen = b.nlen + jb.node0; a.nslot = b.nslot + jb.node0; a.ninter = b.ninter + jb.node0;
     a.nlab = b.nlab + jb.node0; a.nnb = b.nnb + jb.node0;
@@ -2312,6 +2315,7 @@ void label_and_traverse(Ctx &c) {
         SeqBatch b;
         b.jobs = djobs;
         b.uni = c.uni;
+        b.par = c.seq_par;
         b.px = c.aug_bx; b.py = c.aug_by; b.qx = c.aug_cx; b.qy = c.aug_cy;
         const int *avP = c.av_b, *labP = c.lab_b, *nbrP = c.nbr_b;
         if (sparse) {

@@ -157,6 +157,35 @@ def test_one_to_many_flagged_jobs_vs_oracle(gpu):
         assert err is None, "overlay %d: %s" % (i, err)
 
 
+def test_one_to_many_sequential_walks_agree(gpu):
+    """The sequential route's rings walked one per lane (the default) equal
+    the reference's single walk (seq_parallel_walks = 0) on every job of the
+    one-to-many batch that takes that route, failures included."""
+    import segclip
+    base, ovs = S.one_to_many(100000, 4096, 3000)
+    cl = segclip.Clipper().set_polygons(base, ovs)
+    cl.clip_batch(1, len(ovs))
+    route, _ = cl.batch_routes()
+    pick = np.nonzero(route == 2)[0].tolist()
+    assert len(pick) >= 8
+    sub = [ovs[i] for i in pick]
+    got = {}
+    for par in (1, 0):
+        print("[gpu-full] sequential route, seq_parallel_walks=%d, %d jobs" % (par, len(sub)), flush=True)
+        c = segclip.Clipper().set_polygons(base, sub)
+        c.set_tuning("seq_parallel_walks", par)
+        assert c.get_tuning("seq_parallel_walks") == par
+        c.clip_batch(1, len(sub))
+        got[par] = (c.batch_rings(), c.batch_status())
+    assert np.array_equal(got[0][1], got[1][1])
+    assert np.count_nonzero(got[0][1] & 4) >= 1  # some jobs end in a failed walk
+    for q in range(len(sub)):
+        err = rings_equal(got[1][0][q], got[0][0][q])
+        assert err is None, "job %d: %s" % (pick[q], err)
+        for a_, b_ in zip(got[1][0][q], got[0][0][q]):
+            assert np.array_equal(a_.view(np.uint64), b_.view(np.uint64)), pick[q]
+
+
 def test_small_pairs_full_batch(gpu):
     import segclip
     bases, ovs = S.small_pairs(16384, 4000)
