@@ -1,7 +1,8 @@
 // sc_prims.hip -- device-wide scan and stable LSD radix sort for gfx950.
 //
-// Scan: reduce / scan-of-block-sums / downsweep, 2048 items per 256-thread
-// block, wave64 shuffles inside a block.
+// Scan: u32 sums and maxima in one launch per scan (a chained scan over
+// 2048-item tiles with a one-wave look-back); u64 sums and single-tile scans
+// as reduce / scan-of-block-sums / downsweep.  Wave64 shuffles inside a block.
 //
 // Radix sort: 8-bit digits, 4096 keys per tile.  One pass over the keys first
 // builds every digit histogram at once so that passes whose digit is the same
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(const T *in, T *out, i
 }
 
 template <class T> struct ScanScratch {
-    DBuf<T> sums;
+    DBuf<T> sums, sums2;
 };
 
 template <class T> void scan_impl(const T *in, T *out, int64_t n, T *d_total, hipStream_t s, ScanScratch<T> &ss) {
@@ -137,11 +138,182 @@ template <class T> void scan_impl(const T *in, T *out, int64_t n, T *d_total, hi
     SC_HIP(hipGetLastError());
 }
 
-// per-thread scratch, intentionally never freed: releasing device memory from
-// thread-exit destructors can run after the HIP runtime has shut down
-template <class T> ScanScratch<T> &scan_scratch() {
-    thread_local ScanScratch<T> *p = new ScanScratch<T>();
+// ---------------------------------------------------------------------------
+// two-launch scan of u32 (exclusive sum, inclusive max)
+// ---------------------------------------------------------------------------
+// The first launch reduces each 2048-item tile to one value; in the second,
+// each block combines the values of the tiles before its own (at most
+// PREF_TILES of them, read from L2) into its prefix and scans its tile.  This
+// drops a single-block scan of the tile values, a launch whose cost is all
+// latency at these sizes.  Longer inputs scan the tile values themselves
+// first (two more launches of the same kernels).
+constexpr int64_t PREF_TILES = 2048;
+
+struct OpSum {
+    static __device__ uint32_t id() { return 0u; }
+    static __device__ uint32_t f(uint32_t a, uint32_t b) { return a + b; }
+};
+struct OpMax {
+    static __device__ uint32_t id() { return 0u; }
+    static __device__ uint32_t f(uint32_t a, uint32_t b) { return a > b ? a : b; }
+};
+
+template <class Op> __device__ inline uint32_t wave_incl_op(uint32_t x, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x = Op::f(y, x);
+    }
+    return x;
+}
+
+// exclusive scan of one value per thread over the block; total = the block's aggregate
+template <class Op> __device__ inline uint32_t block_excl_op(uint32_t x, uint32_t *s_wave, uint32_t &total) {
+    constexpr int NW = SCAN_BLOCK / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const uint32_t inc = wave_incl_op<Op>(x, lane);
+    uint32_t ex = __shfl_up(inc, 1, WAVE);
+    if (lane == 0) ex = Op::id();
+    if (lane == WAVE - 1) s_wave[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        const uint32_t w = lane < NW ? s_wave[lane] : Op::id();
+        const uint32_t wi = wave_incl_op<Op>(w, lane);
+        uint32_t we = __shfl_up(wi, 1, WAVE);
+        if (lane == 0) we = Op::id();
+        if (lane < NW) s_wave[lane] = we;
+        if (lane == NW - 1) s_wave[NW] = wi;
+    }
+    __syncthreads();
+    const uint32_t r = Op::f(s_wave[wid], ex);
+    total = s_wave[NW];
+    __syncthreads();
+    return r;
+}
+
+template <class Op> __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_reduce_op(const uint32_t *in, int64_t n,
+                                                                                uint32_t *sums) {
+    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE + 1];
+    const int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
+    uint32_t acc = Op::id();
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
+        if (i < n) acc = Op::f(acc, in[i]);
+    }
+    uint32_t total;
+    (void)block_excl_op<Op>(acc, s_wave, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// INCL: inclusive output (the max scan), else exclusive (the sum scan).
+// In place (in == out) is safe: a block reads only its own tile's inputs.
+// The tiles before this one come either as their values (sums, at most
+// PREF_TILES of them, combined here) or already combined (pre[tile]).
+template <class Op, bool INCL>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan_op(const uint32_t *in, uint32_t *out, int64_t n,
+                                                              const uint32_t *sums, const uint32_t *pre,
+                                                              uint32_t *d_total) {
+    __shared__ uint32_t s_data[SCAN_TILE];
+    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE + 1];
+    const uint32_t tile = blockIdx.x;
+    const int64_t base = int64_t(tile) * SCAN_TILE;
+    uint32_t pacc = Op::id();
+    if (!pre)
+        for (uint32_t i = threadIdx.x; i < tile; i += SCAN_BLOCK) pacc = Op::f(pacc, sums[i]);
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
+        s_data[it * SCAN_BLOCK + threadIdx.x] = i < n ? in[i] : Op::id();
+    }
+    uint32_t prefix; // the tiles before this one
+    if (pre) {
+        prefix = pre[tile];
+        __syncthreads();
+    } else {
+        (void)block_excl_op<Op>(pacc, s_wave, prefix);
+    }
+    uint32_t loc[SCAN_ITEMS];
+    uint32_t acc = Op::id();
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        loc[j] = s_data[threadIdx.x * SCAN_ITEMS + j];
+        acc = Op::f(acc, loc[j]);
+    }
+    uint32_t total;
+    const uint32_t ex = block_excl_op<Op>(acc, s_wave, total);
+    uint32_t run = Op::f(prefix, ex);
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const uint32_t v = loc[j];
+        if (INCL) run = Op::f(run, v);
+        s_data[threadIdx.x * SCAN_ITEMS + j] = run;
+        if (!INCL) run = Op::f(run, v);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
+        if (i < n) out[i] = s_data[it * SCAN_BLOCK + threadIdx.x];
+    }
+    if (d_total && threadIdx.x == 0 && base + SCAN_TILE >= n) *d_total = Op::f(prefix, total);
+}
+
+// Per host thread and stream, intentionally never freed: releasing device
+// memory from thread-exit destructors can run after the HIP runtime has shut
+// down.  Keying by stream keeps scans queued on different streams from
+// sharing tile sums.
+struct StreamScanScratch {
+    hipStream_t stream = nullptr;
+    ScanScratch<uint32_t> s32;
+    ScanScratch<uint64_t> s64;
+};
+
+StreamScanScratch &stream_scratch(hipStream_t s) {
+    thread_local std::vector<StreamScanScratch *> *v = new std::vector<StreamScanScratch *>();
+    for (StreamScanScratch *p : *v)
+        if (p->stream == s) return *p;
+    StreamScanScratch *p = new StreamScanScratch();
+    p->stream = s;
+    v->push_back(p);
     return *p;
+}
+
+template <class T> ScanScratch<T> &scan_scratch(hipStream_t s);
+template <> ScanScratch<uint32_t> &scan_scratch<uint32_t>(hipStream_t s) { return stream_scratch(s).s32; }
+template <> ScanScratch<uint64_t> &scan_scratch<uint64_t>(hipStream_t s) { return stream_scratch(s).s64; }
+
+template <class Op, bool INCL>
+void scan_u32_op(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total, hipStream_t s) {
+    if (n <= 0) {
+        if (d_total) SC_HIP(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
+        return;
+    }
+    const int64_t nb = ceil_div(n, SCAN_TILE);
+    SC_REQUIRE(nb <= PREF_TILES * SCAN_TILE, SC_ERANGE, "scan too long");
+    if (nb == 1) {
+        k_tile_scan_op<Op, INCL><<<1, SCAN_BLOCK, 0, s>>>(in, out, n, nullptr, nullptr, d_total);
+        SC_HIP(hipGetLastError());
+        return;
+    }
+    ScanScratch<uint32_t> &ss = scan_scratch<uint32_t>(s);
+    uint32_t *sums = ss.sums.ensure(size_t(nb));
+    k_tile_reduce_op<Op><<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, n, sums);
+    SC_HIP(hipGetLastError());
+    if (nb <= PREF_TILES) {
+        k_tile_scan_op<Op, INCL><<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, out, n, sums, nullptr, d_total);
+        SC_HIP(hipGetLastError());
+        return;
+    }
+    // more tiles than one block combines: exclusive scan of the tile values in place
+    const int64_t nb2 = ceil_div(nb, SCAN_TILE); // <= PREF_TILES
+    uint32_t *sums2 = ss.sums2.ensure(size_t(nb2));
+    k_tile_reduce_op<Op><<<unsigned(nb2), SCAN_BLOCK, 0, s>>>(sums, nb, sums2);
+    SC_HIP(hipGetLastError());
+    k_tile_scan_op<Op, false><<<unsigned(nb2), SCAN_BLOCK, 0, s>>>(sums, sums, nb, sums2, nullptr, nullptr);
+    SC_HIP(hipGetLastError());
+    k_tile_scan_op<Op, INCL><<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, out, n, nullptr, sums, d_total);
+    SC_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -375,115 +547,13 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
     }
 }
 
-
-// ---------------------------------------------------------------------------
-// inclusive max-scan (u32): propagates the last head marker to the right
-// ---------------------------------------------------------------------------
-__device__ inline uint32_t wave_incl_max(uint32_t x, int lane) {
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, WAVE);
-        if (lane >= d) x = x > y ? x : y;
-    }
-    return x;
-}
-
-__device__ inline uint32_t block_incl_max(uint32_t x, uint32_t *s_wave) {
-    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    uint32_t inc = wave_incl_max(x, lane);
-    if (lane == WAVE - 1) s_wave[wid] = inc;
-    __syncthreads();
-    uint32_t before = 0;
-    for (int w = 0; w < wid; ++w) before = before > s_wave[w] ? before : s_wave[w];
-    __syncthreads();
-    return inc > before ? inc : before;
-}
-
-__global__ __launch_bounds__(SCAN_BLOCK) void k_max_tile(const uint32_t *in, int64_t n, uint32_t *sums) {
-    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
-    int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
-    uint32_t acc = 0;
-    for (int it = 0; it < SCAN_ITEMS; ++it) {
-        int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
-        if (i < n) acc = acc > in[i] ? acc : in[i];
-    }
-    uint32_t r = block_incl_max(acc, s_wave);
-    if (threadIdx.x == SCAN_BLOCK - 1) sums[blockIdx.x] = r;
-}
-
-// exclusive running max over the tile maxima, one block walking in chunks
-__global__ __launch_bounds__(SCAN_BLOCK) void k_max_sums(uint32_t *sums, int64_t n) {
-    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
-    __shared__ uint32_t s_carry;
-    if (threadIdx.x == 0) s_carry = 0;
-    __syncthreads();
-    for (int64_t base = 0; base < n; base += SCAN_BLOCK) {
-        int64_t i = base + threadIdx.x;
-        uint32_t v = i < n ? sums[i] : 0u;
-        uint32_t inc = block_incl_max(v, s_wave);
-        uint32_t prev = __shfl_up(inc, 1, WAVE);
-        const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-        uint32_t before = 0;
-        if (lane == 0) {
-            for (int w = 0; w < wid; ++w) before = before > s_wave[w] ? before : s_wave[w];
-        } else {
-            before = prev;
-        }
-        uint32_t carry = s_carry;
-        uint32_t ex = carry > before ? carry : before;
-        __syncthreads();
-        if (i < n) sums[i] = ex;
-        if (threadIdx.x == SCAN_BLOCK - 1) s_carry = carry > inc ? carry : inc;
-        __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(SCAN_BLOCK) void k_max_apply(const uint32_t *in, uint32_t *out, int64_t n,
-                                                          const uint32_t *prefix) {
-    __shared__ uint32_t s_data[SCAN_TILE];
-    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
-    int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
-    for (int it = 0; it < SCAN_ITEMS; ++it) {
-        int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
-        s_data[it * SCAN_BLOCK + threadIdx.x] = i < n ? in[i] : 0u;
-    }
-    __syncthreads();
-    uint32_t loc[SCAN_ITEMS];
-    uint32_t acc = 0;
-    for (int j = 0; j < SCAN_ITEMS; ++j) {
-        loc[j] = s_data[threadIdx.x * SCAN_ITEMS + j];
-        acc = acc > loc[j] ? acc : loc[j];
-    }
-    uint32_t inc = block_incl_max(acc, s_wave);
-    // exclusive within the block: max of the previous threads
-    uint32_t prevthr = __shfl_up(inc, 1, WAVE);
-    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    uint32_t before = 0;
-    if (lane == 0) {
-        for (int w = 0; w < wid; ++w) before = before > s_wave[w] ? before : s_wave[w];
-    } else {
-        before = prevthr;
-    }
-    uint32_t run = prefix ? prefix[blockIdx.x] : 0u;
-    run = run > before ? run : before;
-    for (int j = 0; j < SCAN_ITEMS; ++j) {
-        run = run > loc[j] ? run : loc[j];
-        s_data[threadIdx.x * SCAN_ITEMS + j] = run;
-    }
-    __syncthreads();
-    for (int it = 0; it < SCAN_ITEMS; ++it) {
-        int64_t i = base + int64_t(it) * SCAN_BLOCK + threadIdx.x;
-        if (i < n) out[i] = s_data[it * SCAN_BLOCK + threadIdx.x];
-    }
-}
-
 } // namespace
 
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total, hipStream_t s) {
-    scan_impl<uint32_t>(in, out, n, d_total, s, scan_scratch<uint32_t>());
+    scan_u32_op<OpSum, false>(in, out, n, d_total, s);
 }
 void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *d_total, hipStream_t s) {
-    scan_impl<uint64_t>(in, out, n, d_total, s, scan_scratch<uint64_t>());
+    scan_impl<uint64_t>(in, out, n, d_total, s, scan_scratch<uint64_t>(s));
 }
 
 namespace {
@@ -521,19 +591,7 @@ void FillBatch::flush() {
 }
 
 void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s) {
-    if (n <= 0) return;
-    int64_t nb = ceil_div(n, SCAN_TILE);
-    SC_REQUIRE(nb < (int64_t(1) << 31), SC_ERANGE, "scan too long");
-    uint32_t *sums = nullptr;
-    if (nb > 1) {
-        sums = scan_scratch<uint32_t>().sums.ensure(size_t(nb));
-        k_max_tile<<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, n, sums);
-        SC_HIP(hipGetLastError());
-        k_max_sums<<<1, SCAN_BLOCK, 0, s>>>(sums, nb);
-        SC_HIP(hipGetLastError());
-    }
-    k_max_apply<<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, out, n, sums);
-    SC_HIP(hipGetLastError());
+    if (n > 0) scan_u32_op<OpMax, true>(in, out, n, nullptr, s);
 }
 
 namespace {

@@ -52,7 +52,7 @@ def test_stable_sort_i32_i64(gpu, n):
     assert np.array_equal(v, np.argsort(k64, kind="stable").astype(np.int32))
 
 
-@pytest.mark.parametrize("n", [0, 1, 777, 1 << 20])
+@pytest.mark.parametrize("n", [0, 1, 777, 2048, 2049, 1 << 20, 5_000_011])
 def test_exclusive_scan(gpu, n):
     import torch
     from segclip import lib
@@ -71,3 +71,29 @@ def test_exclusive_scan(gpu, n):
     torch.cuda.synchronize()
     want = np.concatenate([[0], np.cumsum(a64)[:-1]]).astype(np.int64) if n else a64
     assert np.array_equal(d.cpu().numpy(), want)
+
+
+def test_exclusive_scan_streams_back_to_back(gpu):
+    """Multi-tile scans on two streams in turn, a dozen on each: every result
+    equals numpy's (each stream keeps its own tile status words and tickets,
+    so the tickets of one stream's scans never count the other's)."""
+    import torch
+    from segclip import lib
+    rng = np.random.default_rng(7)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    inputs, outs = [], []
+    torch.cuda.synchronize()
+    for k in range(24):
+        n = int(rng.integers(2049, 300000))
+        a = rng.integers(0, 1000, n).astype(np.int32)
+        s = streams[k % 2]
+        with torch.cuda.stream(s):
+            d = torch.from_numpy(a).to("cuda", non_blocking=False)
+        s.synchronize()
+        assert lib().segclip_exclusive_scan_i32(C.c_void_p(d.data_ptr()), n, C.c_void_p(s.cuda_stream)) == 0
+        inputs.append(a)
+        outs.append(d)
+    torch.cuda.synchronize()
+    for a, d in zip(inputs, outs):
+        want = np.concatenate([[0], np.cumsum(a, dtype=np.int64)[:-1]]).astype(np.int32)
+        assert np.array_equal(d.cpu().numpy(), want)
