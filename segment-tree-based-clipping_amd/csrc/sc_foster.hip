@@ -981,7 +981,7 @@ __device__ inline void seq_vertex(const SeqArgs &a, int V, int t, double &x, dou
 }
 
 // point-in-polygon over the ring from root, every vertex of every node
-__device__ bool seq_point_in(SeqArgs &a, int root, double rx, double ry) {
+__device__ __attribute__((always_inline)) bool seq_point_in(SeqArgs &a, int root, double rx, double ry) {
     int w = 0, one = root;
     bool have = false;
     double fx = 0, fy = 0, px = 0, py = 0;
@@ -1014,28 +1014,6 @@ __device__ inline int seq_src(const SeqArgs &a, int V) {
 __device__ int seq_toggle(int s) { return s == ENTRY ? EXIT : (s == EXIT ? ENTRY : s); }
 
 // node records -> the job's linked rings (all threads of the block)
-__device__ void seq_init(SeqArgs &a) {
-    const int nn = a.nP + a.nQ;
-    for (int v = threadIdx.x; v < 2 * a.cap; v += blockDim.x) a.seen[v] = 0;
-    for (int v = threadIdx.x; v < nn; v += blockDim.x) {
-        a.vx[v] = a.nx0[v];
-        a.vy[v] = a.ny0[v];
-        a.vx1[v] = a.nx1[v];
-        a.vy1[v] = a.ny1[v];
-        a.vlen[v] = a.nlen[v];
-        a.vslot[v] = a.nslot[v];
-        a.vsrc[v] = SRC_NONE;
-        a.lab[v] = a.nlab[v];
-        a.inter[v] = a.ninter[v] ? 1 : 0;
-        a.enex[v] = NEITHER;
-        a.nb[v] = a.ninter[v] ? a.nnb[v] : -1;
-        const bool onP = v < a.nP;
-        const int r0 = onP ? 0 : a.nP, m = onP ? a.nP : a.nQ, i = v - r0;
-        a.nx[v] = r0 + (i + 1 == m ? 0 : i + 1);
-        a.pv[v] = r0 + (i == 0 ? m - 1 : i - 1);
-    }
-}
-
 // createResult2's walks, one per lane (see seq_run).  Returns PT_DONE with
 // rings/outn advanced and the rings written, PT_FAIL when the single walk
 // would fail (a ring that revisits a (node, direction) state, a jump to no
@@ -1055,7 +1033,7 @@ constexpr int PT_CHUNK = 2048; // candidates per round of walks
 // every jump target); 'step' sees every node stepped to with its direction
 // (the emitted ones).
 template <class Visit, class Step>
-__device__ inline int pt_walk(const SeqArgs &a, int I, Visit visit, Step step) {
+__device__ __attribute__((always_inline)) int pt_walk(const SeqArgs &a, int I, Visit visit, Step step) {
     const bool uni = a.uni != 0;
     int V = I;
     if (visit(V)) return 3;
@@ -1087,9 +1065,62 @@ __device__ inline int pt_walk(const SeqArgs &a, int I, Visit visit, Step step) {
     return 0;
 }
 
-__device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int lane, uint64_t lt) {
+// what the walks need from the wave that ran the steps before them, and
+// what they give back (in LDS, one per block)
+struct SeqShared {
+    int go;            // 1: the walks run (else the job's result is written)
+    int n, status;     // nodes (inserted ones included), status bits so far
+    int rings, outn;   // rings and output vertices written so far
+    int ptr;           // the walks' outcome (PT_*)
+};
+
+constexpr int SEQ_BLK = 1024; // threads of a sequential job's block
+
+// exclusive scan of one value per thread over the block, in thread order;
+// tot = the block's sum.  Every thread of the block calls it.
+template <class T> __device__ inline T seq_block_scan(T c, T &tot, T *s_w) {
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    T incl = c;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const T o = __shfl_up(incl, d, WAVE);
+        if (lane >= d) incl += o;
+    }
+    if (lane == WAVE - 1) s_w[wid] = incl;
+    __syncthreads();
+    T before = 0, t = 0;
+    for (int w = 0; w < nw; ++w) {
+        const T v = s_w[w];
+        if (w < wid) before += v;
+        t += v;
+    }
+    __syncthreads(); // s_w is free for the next call
+    tot = t;
+    return before + incl - c;
+}
+
+// minimum of one value per thread over the block.  Every thread calls it.
+__device__ inline int seq_block_min(int x, int *s_w) {
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    for (int d = 1; d < WAVE; d <<= 1) x = min(x, __shfl_xor(x, d, WAVE));
+    if (lane == 0) s_w[wid] = x;
+    __syncthreads();
+    int m = INT_MAX;
+    for (int w = 0; w < nw; ++w) m = min(m, s_w[w]);
+    __syncthreads();
+    return m;
+}
+
+// Run by every thread of the block, with the same control flow in every
+// wave (all loop bounds and branches that reach a barrier come from LDS or
+// from values every thread computes alike).
+__device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqShared &sh) {
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int n = sh.n, rings = sh.rings, outn = sh.outn;
     const int n0 = a.nP + a.nQ; // nodes before any insertion; inserted nodes follow
     if (n - n0 > 4096) return PT_FALLBACK;
+    __shared__ int s_w[SEQ_BLK / WAVE];
+    __shared__ int64_t s_w64[SEQ_BLK / WAVE];
+    __shared__ int s_next, s_qend, s_long, s_res;
     const int half = a.cap / 2;
     int *cand = a.cand, *info = a.cand + half;
     // 1) the candidates in ring order from the root: original P nodes in index
@@ -1098,32 +1129,28 @@ __device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int la
     int ncand = 0;
     // back to all-zero state words (listed candidates only carry marks)
     auto clear = [&]() {
-        __threadfence_block();
-        for (int q = lane; q < ncand; q += WAVE) {
+        __syncthreads();
+        for (int q = tid; q < ncand; q += nthr) {
             const int v = cand[q];
             a.seen[2 * v] = 0;
             a.seen[2 * v + 1] = 0;
         }
-        __threadfence_block();
+        __syncthreads();
     };
-    for (int w = 0; w < a.nP; w += WAVE) {
-        const int v = w + lane;
+    for (int w = 0; w < a.nP; w += nthr) {
+        const int v = w + tid;
         int c = 0;
         if (v < a.nP) {
             c = is_cand(v) ? 1 : 0;
             for (int x = a.nx[v], g = 0; x >= n0 && g <= n - n0; x = a.nx[x], ++g) c += is_cand(x) ? 1 : 0;
         }
-        int incl = c;
-        for (int d = 1; d < WAVE; d <<= 1) {
-            const int o = __shfl_up(incl, d, WAVE);
-            if (lane >= d) incl += o;
-        }
-        const int tot = __shfl(incl, WAVE - 1, WAVE);
+        int tot;
+        const int ex = seq_block_scan(c, tot, s_w);
         if (ncand + tot > half) {
             clear();
             return PT_FALLBACK;
         }
-        int q = ncand + incl - c;
+        int q = ncand + ex;
         if (v < a.nP) {
             if (is_cand(v)) {
                 cand[q] = v;
@@ -1137,11 +1164,10 @@ __device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int la
         }
         ncand += tot;
     }
-    __threadfence_block();
+    __syncthreads();
     // 2)-4) in chunks of PT_CHUNK candidates in rank order, so that the
     // single walk's first failing ring is found without walking every
     // candidate after it.
-    __shared__ int s_next, s_qend, s_long;
     int nring = 0;    // rings of the chunks before
     int64_t nout = 0; // their vertices
     for (int c0 = 0; c0 < ncand; c0 += PT_CHUNK) {
@@ -1150,16 +1176,15 @@ __device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int la
         //    of smaller rank (then it does not own its ring); an owner's walk
         //    gives its ring's vertex count or its failure.  The single walk
         //    ends at the first failing ring: nothing after that owner's rank
-        //    matters.  Lanes take candidates from a counter in LDS as their
-        //    walks end, so a long walk holds one lane, not 63 others.
-        if (lane == 0) {
-            s_next = c0 + WAVE;
+        //    matters.  Threads take candidates from a counter in LDS as their
+        //    walks end, so a long walk holds one lane, not the others.
+        if (tid == 0) {
+            s_next = c0 + nthr;
             s_qend = c1;
             s_long = 0;
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        for (int q = c0 + lane; q < c1;) {
+        __syncthreads();
+        for (int q = c0 + tid; q < c1;) {
             if (q >= __hip_atomic_load(&s_qend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
             int cnt = 0;
             const int res = pt_walk(
@@ -1174,13 +1199,13 @@ __device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int la
             info[q] = res == 3 ? -1 : (res == 1 ? -2 : cnt);
             q = atomicAdd(&s_next, 1);
         }
-        __builtin_amdgcn_wave_barrier();
-        __threadfence_block();
-        if (__hip_atomic_load(&s_long, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+        __syncthreads();
+        const int qend = s_qend;
+        const bool lng = s_long != 0;
+        if (lng) {
             clear();
             return PT_FALLBACK;
         }
-        const int qend = __hip_atomic_load(&s_qend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const bool fail = qend < c1 || info[c1 - 1] == -2;
         // 3) ring order: owner r (in the order of the starts) gets ring index
         //    r (kept in its seen[2v], still positive) and its output place
@@ -1188,33 +1213,29 @@ __device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int la
         //    any order, and every candidate a walk marks keeps the first ring
         //    that marked it
         const int nring0 = nring;
-        for (int q0 = c0; q0 < qend; q0 += WAVE) {
-            const int q = q0 + lane;
+        for (int q0 = c0; q0 < qend; q0 += nthr) {
+            const int q = q0 + tid;
             const int inf = q < qend ? info[q] : -1;
             const bool owner = inf != -1;
-            const uint64_t om = __ballot(owner);
-            const int r = nring + int(__popcll(om & lt));
-            const int c = owner && inf > 0 ? inf : 0;
-            int incl = c;
-            for (int d = 1; d < WAVE; d <<= 1) {
-                const int o = __shfl_up(incl, d, WAVE);
-                if (lane >= d) incl += o;
-            }
-            const int at = outn + int(nout) + incl - c;
+            int otot;
+            const int oex = seq_block_scan(owner ? 1 : 0, otot, s_w);
+            const int64_t c = owner && inf > 0 ? inf : 0;
+            int64_t ctot;
+            const int64_t cex = seq_block_scan(c, ctot, s_w64);
+            const int r = nring + oex;
+            const int64_t at = int64_t(outn) + nout + cex;
             if (owner) {
-                const bool write = !fail && rings + r < a.ring_cap && int64_t(outn) + nout + incl <= a.out_cap;
-                if (write) a.ring_off[rings + r] = at;
+                const bool write = !fail && rings + r < a.ring_cap && at + c <= a.out_cap;
+                if (write) a.ring_off[rings + r] = int(at);
                 a.seen[2 * cand[q]] = r + 1;
-                info[q] = write ? at : -3; // -3: an owner that writes nothing
+                info[q] = write ? int(at) : -3; // -3: an owner that writes nothing
             }
-            nring += int(__popcll(om));
-            nout += __shfl(incl, WAVE - 1, WAVE);
+            nring += otot;
+            nout += ctot;
         }
-        __threadfence_block();
-        if (lane == 0) s_next = c0 + WAVE;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        for (int q = c0 + lane; q < qend; q = atomicAdd(&s_next, 1)) {
+        if (tid == 0) s_next = c0 + nthr;
+        __syncthreads();
+        for (int q = c0 + tid; q < qend; q = atomicAdd(&s_next, 1)) {
             const int inf = info[q];
             if (inf == -1) continue;
             const int X = cand[q];
@@ -1250,20 +1271,19 @@ __device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int la
                     at += len;
                 });
         }
-        __builtin_amdgcn_wave_barrier();
-        __threadfence_block();
+        __syncthreads();
         // 4) the check: an owner is first marked by its own ring, any other
         //    candidate by a ring whose start comes before it.  At the first
         //    candidate where that does not hold, a candidate no earlier ring
         //    marked is where the single walk starts its next ring: if a full
         //    walk from it fails, the single walk fails there too.
         int before = nring0;
-        for (int q0 = c0; q0 < qend; q0 += WAVE) {
-            const int q = q0 + lane;
+        for (int q0 = c0; q0 < qend; q0 += nthr) {
+            const int q = q0 + tid;
             const int inf = q < qend ? info[q] : -1;
             const bool owner = inf != -1;
-            const uint64_t om = __ballot(owner);
-            const int r = before + int(__popcll(om & lt));
+            int otot;
+            const int r = before + seq_block_scan(owner ? 1 : 0, otot, s_w);
             bool bad = false, unmarked = false;
             if (q < qend) {
                 const int fv = a.seen[2 * cand[q] + 1];
@@ -1271,19 +1291,21 @@ __device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int la
                 bad = owner ? first != r : first >= r;
                 unmarked = !owner && first >= r;
             }
-            const uint64_t bm = __ballot(bad);
-            if (bm) {
-                const int l = int(__builtin_ctzll(bm));
-                const bool um = __shfl(int(unmarked), l, WAVE) != 0;
-                int res = 0;
-                if (um && lane == 0)
-                    res = pt_walk(
-                        a, cand[q0 + l], [&](int) { return false; }, [&](int, bool) {});
-                res = __shfl(res, 0, WAVE);
+            const int fb = seq_block_min(bad ? tid : INT_MAX, s_w);
+            if (fb != INT_MAX) {
+                if (tid == fb) {
+                    int res = 0;
+                    if (unmarked)
+                        res = pt_walk(
+                            a, cand[q], [&](int) { return false; }, [&](int, bool) {});
+                    s_res = unmarked && res == 1 ? PT_FAIL : PT_FALLBACK;
+                }
+                __syncthreads();
+                const int out = s_res;
                 clear();
-                return um && res == 1 ? PT_FAIL : PT_FALLBACK;
+                return out;
             }
-            before += int(__popcll(om));
+            before += otot;
         }
         if (fail) {
             clear();
@@ -1292,19 +1314,24 @@ __device__ int seq_par_traverse(SeqArgs &a, int n, int &rings, int &outn, int la
     }
     clear();
     if (rings + nring > a.ring_cap || int64_t(outn) + nout > a.out_cap) return PT_FAIL;
-    rings += nring;
-    outn += int(nout);
+    if (tid == 0) {
+        sh.rings = rings + nring;
+        sh.outn = outn + int(nout);
+    }
     return PT_DONE;
 }
 
 // The reference's sequential algorithm on the job's rings, run by one wave
-// with uniform control flow: every lane holds the same walk state.  Lane i
+// with uniform control flow: every lane holds the same walk state.  Part 0
+// runs everything before createResult2's walks and leaves its state in sh
+// (sh.go = 1), or writes the job's result; part 1 takes the walks' outcome
+// (sh.ptr), runs the single walk when they fell back, and writes the result.  Lane i
 // caches the hot fields of node w0 + i (a window of 64 consecutive nodes,
 // w0 a multiple of 64), so a walk in ring order reads registers and reloads
 // one coalesced window every 64 nodes.  Every write goes to memory (from all
 // lanes, the same value) and to the window.  Output runs are written one
 // vertex per lane.
-__device__ void seq_run(SeqArgs &a) {
+__device__ __attribute__((always_inline)) void seq_run(SeqArgs &a, SeqShared &sh, int part) {
     const int lane = threadIdx.x & (WAVE - 1);
     int status = 0;
     // ring nodes already laid out by seq_init: P = [0,nP), Q = [nP,nP+nQ)
@@ -1415,6 +1442,56 @@ __device__ void seq_run(SeqArgs &a) {
     };
     const int rootP = 0, rootQ = a.nP;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+    if (part == 1) {
+        n = sh.n;
+        status = sh.status;
+        int rings = sh.rings, outn = sh.outn;
+        if (sh.ptr == PT_FAIL) goto done;
+        if (sh.ptr == PT_FALLBACK) {
+            int root = rootP, I = root;
+            bool have = INTER_(I) && LAB_(I) == L_CROSSING;
+            if (!have) {
+                do { I = NX_(I); } while (!(INTER_(I) && LAB_(I) == L_CROSSING) && I != root);
+                have = I != root;
+            }
+            long bound = 4L * n + 16;
+            int comp = 0;
+            while (have) {
+                if (rings >= a.ring_cap) goto done;
+                a.ring_off[rings++] = outn;
+                int V = I;
+                set_inter(V, 0);
+                long steps = 0;
+                ++comp;
+                do {
+                    int st = seq_toggle(ENEX_(V));
+                    do {
+                        const bool fwd = (st == EXIT) != (a.uni != 0); // (status == EXIT) ^ UNION, polyclip.h:740
+                        V = fwd ? NX_(V) : PV_(V);
+                        set_inter(V, 0);
+                        // within one component the walk is determined by (vertex, direction):
+                        // a repeated state can never reach I again (the reference loops forever)
+                        const int dir = fwd ? 1 : 0;
+                        if (SEEN_(V, dir) == comp) goto done;
+                        set_seen(V, dir, comp);
+                        if (++steps > bound || !semit(outn, V, fwd)) goto done;
+                    } while (!(ENEX_(V) == st) && V != I);
+                    if (V != I) {
+                        V = NB_(V);
+                        if (V < 0) goto done;
+                        set_inter(V, 0);
+                    }
+                } while (V != I);
+                do { I = NX_(I); } while (!(INTER_(I) && LAB_(I) == L_CROSSING) && I != root);
+                have = I != root;
+            }
+        }
+        a.ring_off[rings] = outn;
+        a.result[0] = rings;
+        a.result[1] = outn;
+        a.result[2] = status;
+        return;
+    }
     // 2) intersection chains.  Before any insertion ring P is nodes [0, nP) in
     // ring order from its root, so the chain starts (intersections labelled
     // LEFT_ON or RIGHT_ON) are found 64 nodes per ballot, in that order, and
@@ -1725,62 +1802,24 @@ __device__ void seq_run(SeqArgs &a) {
         }
         for (int j = 0; j < nsplit[1]; ++j) set_seen(splitL[1][j], 0, 0);
         for (int j = 0; j < ncross[1]; ++j) set_seen(crossL[1][j], 1, 0);
-        // createResult2.  First the walks run one per lane (seq_par_traverse):
-        // every candidate start (a P crossing, in ring order from the root)
-        // walks its ring without writing; the candidate with the smallest
-        // rank on a ring owns it; the owners walk again and write the rings.
-        // The reference's single walk starts a ring at the first candidate
-        // that no earlier ring visited, so the result is the same when every
-        // owner is first visited by its own ring and every other candidate by
-        // a ring that starts before it; that is checked, and otherwise (or
-        // when a walk is too long or too many nodes were inserted) the single
-        // walk below runs.
-        const int ptr = a.par ? seq_par_traverse(a, n, rings, outn, lane, lt) : PT_FALLBACK;
-        if (ptr == PT_FAIL) { status |= UB_TRAVERSAL; goto done; }
-        if (ptr == PT_FALLBACK) w0 = -(1 << 30); // the window reloads what the attempt wrote
-        if (ptr == PT_FALLBACK) {
-            int root = rootP, I = root;
-            bool have = INTER_(I) && LAB_(I) == L_CROSSING;
-            if (!have) {
-                do { I = NX_(I); } while (!(INTER_(I) && LAB_(I) == L_CROSSING) && I != root);
-                have = I != root;
-            }
-            long bound = 4L * n + 16;
-            int comp = 0;
-            while (have) {
-                if (rings >= a.ring_cap) { status |= UB_TRAVERSAL; goto done; }
-                a.ring_off[rings++] = outn;
-                int V = I;
-                set_inter(V, 0);
-                long steps = 0;
-                ++comp;
-                do {
-                    int st = seq_toggle(ENEX_(V));
-                    do {
-                        const bool fwd = (st == EXIT) != (a.uni != 0); // (status == EXIT) ^ UNION, polyclip.h:740
-                        V = fwd ? NX_(V) : PV_(V);
-                        set_inter(V, 0);
-                        // within one component the walk is determined by (vertex, direction):
-                        // a repeated state can never reach I again (the reference loops forever)
-                        const int dir = fwd ? 1 : 0;
-                        if (SEEN_(V, dir) == comp) { status |= UB_TRAVERSAL; goto done; }
-                        set_seen(V, dir, comp);
-                        if (++steps > bound || !semit(outn, V, fwd)) { status |= UB_TRAVERSAL; goto done; }
-                    } while (!(ENEX_(V) == st) && V != I);
-                    if (V != I) {
-                        V = NB_(V);
-                        if (V < 0) { status |= UB_TRAVERSAL; goto done; }
-                        set_inter(V, 0);
-                    }
-                } while (V != I);
-                do { I = NX_(I); } while (!(INTER_(I) && LAB_(I) == L_CROSSING) && I != root);
-                have = I != root;
-            }
+        // createResult2.  First the walks run one per lane (seq_par_traverse,
+        // every wave of the block): every candidate start (a P crossing, in
+        // ring order from the root) walks its ring without writing; the
+        // candidate with the smallest rank on a ring owns it; the owners walk
+        // again and write the rings.  The reference's single walk starts a
+        // ring at the first candidate that no earlier ring visited, so the
+        // result is the same when every owner is first visited by its own
+        // ring and every other candidate by a ring that starts before it;
+        // that is checked, and otherwise (or when a walk is too long or too
+        // many nodes were inserted) the single walk of part 1 runs.
+        __threadfence_block(); // this wave's stores, before the other waves read them
+        if (lane == 0) {
+            sh.n = n;
+            sh.status = status;
+            sh.rings = rings;
+            sh.outn = outn;
+            sh.go = 1;
         }
-        a.ring_off[rings] = outn;
-        a.result[0] = rings;
-        a.result[1] = outn;
-        a.result[2] = status;
         return;
     }
 done:
@@ -1817,7 +1856,7 @@ struct SeqBatch {
 
 // one block per sequential job: its threads lay out the rings, its first wave
 // runs the algorithm (no barrier after that point)
-__global__ __launch_bounds__(BLK) void k_sequential(SeqBatch b) {
+__global__ __launch_bounds__(SEQ_BLK) void k_sequential(SeqBatch b) {
     const SeqJob jb = b.jobs[blockIdx.x];
     SeqArgs a;
     a.nP = jb.nPn;
@@ -1851,10 +1890,50 @@ __global__ __launch_bounds__(BLK) void k_sequential(SeqBatch b) {
         }
         return;
     }
-    seq_init(a);
+    __shared__ SeqShared sh;
+    if (threadIdx.x == 0) sh.go = 0;
     __syncthreads();
-    if (threadIdx.x >= WAVE) return; // wave 0 runs the walk
-    seq_run(a);
+    if (threadIdx.x < WAVE) seq_run(a, sh, 0); // wave 0: the steps before the walks
+    __syncthreads();
+    if (!sh.go) return; // the job's result is written
+    const int ptr = a.par ? seq_par_traverse(a, sh) : PT_FALLBACK; // every wave
+    __syncthreads();
+    if (threadIdx.x == 0) sh.ptr = ptr;
+    __syncthreads();
+    if (threadIdx.x < WAVE) seq_run(a, sh, 1); // wave 0: the single walk if needed, the result
+}
+
+// Every sequential job's working rings before any insertion: its P and Q
+// rings in node order (node v of the job at slot vbase + v), nothing
+// inserted, no walk state.  Job q owns nodes [nodeoff[2q], nodeoff[2q+2]).
+__global__ void k_seq_init(SeqBatch b, const int64_t *__restrict__ nodeoff, int nseq, int64_t N) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    int lo = 0, hi = nseq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (nodeoff[2 * mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const SeqJob &jb = b.jobs[lo];
+    const int v = int(t - jb.node0);
+    const int64_t w = jb.vbase + v;
+    b.vx[w] = b.nx0[t];
+    b.vy[w] = b.ny0[t];
+    b.vx1[w] = b.nx1[t];
+    b.vy1[w] = b.ny1[t];
+    b.vlen[w] = b.nlen[t];
+    b.vslot[w] = b.nslot[t];
+    b.vsrc[w] = SRC_NONE;
+    b.lab[w] = b.nlab[t];
+    const bool it = b.ninter[t] != 0;
+    b.inter[w] = it ? 1 : 0;
+    b.enex[w] = NEITHER;
+    b.nb[w] = it ? b.nnb[t] : -1;
+    const bool onP = v < jb.nPn;
+    const int r0 = onP ? 0 : jb.nPn, m = onP ? jb.nPn : jb.nQn, i = v - r0;
+    b.nx[w] = r0 + (i + 1 == m ? 0 : i + 1);
+    b.pv[w] = r0 + (i == 0 ? m - 1 : i - 1);
 }
 
 // Ring nodes of the sequential jobs: every intersection slot is a node, every
@@ -2399,11 +2478,19 @@ void label_and_traverse(Ctx &c) {
         b.ox = f.sox.ensure(ob); b.oy = f.soy.ensure(ob); b.osrc = f.sosrc.ensure(ob);
         b.ring_off = f.sroff.ensure(rb);
         b.result = f.sres.ensure(3 * int64_t(nseq));
+        // the jobs' working rings and zeroed walk state, spread over the device
+        {
+            FillBatch fb(s);
+            fb.add(b.seen, 2 * vb, 0u);
+            fb.flush();
+        }
+        k_seq_init<<<grid_for(N), BLK, 0, s>>>(b, dnodeoff, nseq, N);
+        SC_HIP(hipGetLastError());
         // the sequential jobs run beside the crossing jobs' labelling
         c.ensure_side_stream();
         SC_HIP(hipEventRecord(c.ev_fork, s));
         SC_HIP(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
-        k_sequential<<<nseq, BLK, 0, c.stream2>>>(b);
+        k_sequential<<<nseq, SEQ_BLK, 0, c.stream2>>>(b);
         SC_HIP(hipGetLastError());
         SC_HIP(hipEventRecord(c.ev_join, c.stream2));
     }
