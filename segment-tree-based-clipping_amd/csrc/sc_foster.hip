@@ -956,6 +956,7 @@ struct SeqArgs {
     uint8_t *inter;
     int *cand; // scratch list, cap entries
     int *seen; // 2 per vertex: the traversal component that last walked it in each direction
+    int4 *rec; // per node for createResult2's walks: nx, pv, nb, enex | vlen << 2
     double *ox, *oy;
     int *vsrc, *osrc; // provenance of inserted vertices / output vertices (slot reference)
     int P0, Q0;       // first slot of the job's rings in the batch arrays
@@ -1031,19 +1032,21 @@ constexpr int PT_CHUNK = 2048; // candidates per round of walks
 // when it is longer than PT_MAX_STEPS, 3 when 'visit' asks it to stop.
 // 'visit' sees every node the walk marks (the start, every node stepped to,
 // every jump target); 'step' sees every node stepped to with its direction
-// (the emitted ones).
+// and vertex count (the emitted ones).  A node's links, label and length come
+// in one 16-byte record, so a step waits for one load.
 template <class Visit, class Step>
 __device__ __attribute__((always_inline)) int pt_walk(const SeqArgs &a, int I, Visit visit, Step step) {
     const bool uni = a.uni != 0;
     int V = I;
     if (visit(V)) return 3;
+    int4 r = a.rec[V];
     int steps = 0, lam = 0, power = 1, sv = -1, sd = -1; // Brent's cycle test on (node, direction)
     do {
-        const int e = a.enex[V];
+        const int e = r.w & 3;
         const int st = e == ENTRY ? EXIT : (e == EXIT ? ENTRY : e);
         do {
             const bool fwd = (st == EXIT) != uni;
-            V = fwd ? a.nx[V] : a.pv[V];
+            V = fwd ? r.x : r.y;
             const int dir = fwd ? 1 : 0;
             if (V == sv && dir == sd) return 1; // a state seen again: the walk never closes
             if (++lam == power) {
@@ -1053,13 +1056,15 @@ __device__ __attribute__((always_inline)) int pt_walk(const SeqArgs &a, int I, V
                 lam = 0;
             }
             if (++steps > PT_MAX_STEPS) return 2;
+            r = a.rec[V];
             if (visit(V)) return 3;
-            step(V, fwd);
-        } while (!(a.enex[V] == st) && V != I);
+            step(V, fwd, r.w >> 2);
+        } while (!((r.w & 3) == st) && V != I);
         if (V != I) {
-            V = a.nb[V];
+            V = r.z;
             if (V < 0) return 1;
             if (visit(V)) return 3;
+            r = a.rec[V];
         }
     } while (V != I);
     return 0;
@@ -1123,6 +1128,8 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
     __shared__ int s_next, s_qend, s_long, s_res;
     const int half = a.cap / 2;
     int *cand = a.cand, *info = a.cand + half;
+    // the walks' node records (the walks change none of these fields)
+    for (int v = tid; v < n; v += nthr) a.rec[v] = make_int4(a.nx[v], a.pv[v], a.nb[v], a.enex[v] | (a.vlen[v] << 2));
     // 1) the candidates in ring order from the root: original P nodes in index
     //    order, each followed by the nodes inserted after it
     auto is_cand = [&](int v) { return a.inter[v] != 0 && a.lab[v] == L_CROSSING; };
@@ -1193,7 +1200,7 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
                     const int r = a.seen[2 * v];
                     return r > 0 && r < q + 1;
                 },
-                [&](int v, bool) { cnt += a.vlen[v]; });
+                [&](int, bool, int len) { cnt += len; });
             if (res == 2) atomicOr(&s_long, 1);
             if (res == 1) atomicMin(&s_qend, q + 1);
             info[q] = res == 3 ? -1 : (res == 1 ? -2 : cnt);
@@ -1248,8 +1255,7 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
                     if (a.seen[2 * v] > 0) atomicMax(&a.seen[2 * v + 1], tag);
                     return false;
                 },
-                [&](int v, bool fwd) {
-                    const int len = a.vlen[v];
+                [&](int v, bool fwd, int len) {
                     if (write) {
                         const int vs = a.vslot[v];
                         const bool onP = v < a.nP;
@@ -1297,7 +1303,7 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
                     int res = 0;
                     if (unmarked)
                         res = pt_walk(
-                            a, cand[q], [&](int) { return false; }, [&](int, bool) {});
+                            a, cand[q], [&](int) { return false; }, [&](int, bool, int) {});
                     s_res = unmarked && res == 1 ? PT_FAIL : PT_FALLBACK;
                 }
                 __syncthreads();
@@ -1848,6 +1854,7 @@ struct SeqBatch {
     double *vx, *vy, *vx1, *vy1;
     int *vlen, *vslot, *nx, *pv, *nb, *lab, *enex, *vsrc, *cand, *seen;
     uint8_t *inter;
+    int4 *rec;
     double *ox, *oy;
     int *osrc, *ring_off, *result;
     int uni;
@@ -1877,6 +1884,7 @@ __global__ __launch_bounds__(SEQ_BLK) void k_sequential(SeqBatch b) {
     a.nx = b.nx + jb.vbase; a.pv = b.pv + jb.vbase; a.nb = b.nb + jb.vbase; a.lab = b.lab + jb.vbase;
     a.enex = b.enex + jb.vbase; a.vsrc = b.vsrc + jb.vbase; a.cand = b.cand + jb.vbase;
     a.seen = b.seen + 2 * jb.vbase;
+    a.rec = b.rec + jb.vbase;
     a.inter = b.inter + jb.vbase;
     a.ox = b.ox + jb.obase; a.oy = b.oy + jb.obase; a.osrc = b.osrc + jb.obase; a.out_cap = jb.out_cap;
     a.ring_off = b.ring_off + jb.rbase; a.ring_cap = jb.ring_cap;
@@ -2064,6 +2072,7 @@ struct FosterScratch {
     DBuf<double> vx, vy, sox, soy;
     DBuf<int> nx, pv, nb, lab, enex, vsrc, cand;
     DBuf<uint8_t> inter;
+    DBuf<int4> srec;
 };
 
 FosterScratch &fscratch() {
@@ -2475,6 +2484,7 @@ void label_and_traverse(Ctx &c) {
         b.enex = f.enex.ensure(vb); b.vsrc = f.vsrc.ensure(vb); b.cand = f.cand.ensure(vb);
         b.seen = f.seen.ensure(2 * vb);
         b.inter = f.inter.ensure(vb);
+        b.rec = f.srec.ensure(vb);
         b.ox = f.sox.ensure(ob); b.oy = f.soy.ensure(ob); b.osrc = f.sosrc.ensure(ob);
         b.ring_off = f.sroff.ensure(rb);
         b.result = f.sres.ensure(3 * int64_t(nseq));
