@@ -957,6 +957,8 @@ struct SeqArgs {
     int *cand; // scratch list, cap entries
     int *seen; // 2 per vertex: the traversal component that last walked it in each direction
     int4 *rec; // per node for createResult2's walks: nx, pv, nb, enex | vlen << 2
+    int *pstep; // per walk candidate: its walk's node count, then its first step record
+    int2 *emit; // 2 * cap step records (node << 1 | forward, output place) of one round of walks
     double *ox, *oy;
     int *vsrc, *osrc; // provenance of inserted vertices / output vertices (slot reference)
     int P0, Q0;       // first slot of the job's rings in the batch arrays
@@ -1193,17 +1195,21 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
         __syncthreads();
         for (int q = c0 + tid; q < c1;) {
             if (q >= __hip_atomic_load(&s_qend, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-            int cnt = 0;
+            int cnt = 0, nst = 0;
             const int res = pt_walk(
                 a, cand[q],
                 [&](int v) {
                     const int r = a.seen[2 * v];
                     return r > 0 && r < q + 1;
                 },
-                [&](int, bool, int len) { cnt += len; });
+                [&](int, bool, int len) {
+                    cnt += len;
+                    ++nst;
+                });
             if (res == 2) atomicOr(&s_long, 1);
             if (res == 1) atomicMin(&s_qend, q + 1);
             info[q] = res == 3 ? -1 : (res == 1 ? -2 : cnt);
+            a.pstep[q] = nst;
             q = atomicAdd(&s_next, 1);
         }
         __syncthreads();
@@ -1220,6 +1226,9 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
         //    any order, and every candidate a walk marks keeps the first ring
         //    that marked it
         const int nring0 = nring;
+        const int64_t out0 = int64_t(outn) + nout; // the chunk's first output vertex
+        int nrec = 0;                              // the chunk's step records
+        bool allw = !fail;                         // every owner of the chunk writes
         for (int q0 = c0; q0 < qend; q0 += nthr) {
             const int q = q0 + tid;
             const int inf = q < qend ? info[q] : -1;
@@ -1229,16 +1238,30 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
             const int64_t c = owner && inf > 0 ? inf : 0;
             int64_t ctot;
             const int64_t cex = seq_block_scan(c, ctot, s_w64);
+            int stot;
+            const int sex = seq_block_scan(owner ? a.pstep[q] : 0, stot, s_w);
             const int r = nring + oex;
             const int64_t at = int64_t(outn) + nout + cex;
+            bool write = true;
             if (owner) {
-                const bool write = !fail && rings + r < a.ring_cap && at + c <= a.out_cap;
+                write = !fail && rings + r < a.ring_cap && at + c <= a.out_cap;
                 if (write) a.ring_off[rings + r] = int(at);
                 a.seen[2 * cand[q]] = r + 1;
                 info[q] = write ? int(at) : -3; // -3: an owner that writes nothing
+                a.pstep[q] = nrec + sex;
             }
+            int wtot;
+            (void)seq_block_scan(write ? 0 : 1, wtot, s_w);
+            if (wtot) allw = false;
             nring += otot;
             nout += ctot;
+            nrec += stot;
+        }
+        // rings that share (node, direction) states can exceed the records'
+        // room (2 per node); the single walk then decides
+        if (allw && nrec > 2 * a.cap) {
+            clear();
+            return PT_FALLBACK;
         }
         if (tid == 0) s_next = c0 + nthr;
         __syncthreads();
@@ -1247,8 +1270,8 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
             if (inf == -1) continue;
             const int X = cand[q];
             const int tag = SEEN_FIRST - (a.seen[2 * X] - 1);
-            const bool write = inf >= 0;
-            int at = write ? inf : 0;
+            const bool write = inf >= 0 && allw;
+            int at = inf, k = a.pstep[q];
             (void)pt_walk(
                 a, X,
                 [&](int v) {
@@ -1256,26 +1279,41 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
                     return false;
                 },
                 [&](int v, bool fwd, int len) {
-                    if (write) {
-                        const int vs = a.vslot[v];
-                        const bool onP = v < a.nP;
-                        for (int t = 0; t < len; ++t) {
-                            const int u = fwd ? t : len - 1 - t;
-                            double x, y;
-                            if (vs >= 0 && len > 1) {
-                                x = onP ? a.px[vs + u] : a.qx[vs + u];
-                                y = onP ? a.py[vs + u] : a.qy[vs + u];
-                            } else {
-                                x = a.vx[v];
-                                y = a.vy[v];
-                            }
-                            a.ox[at + t] = x;
-                            a.oy[at + t] = y;
-                            a.osrc[at + t] = vs >= 0 ? (onP ? a.P0 + vs + u : ~(a.Q0 + vs + u)) : a.vsrc[v];
-                        }
-                    }
+                    if (write) a.emit[k++] = make_int2(v << 1 | (fwd ? 1 : 0), at);
                     at += len;
                 });
+        }
+        __syncthreads();
+        // the rings' vertices, one thread per output vertex: the step record
+        // of vertex o is the last one placed at or before it (records follow
+        // the output order)
+        if (allw && nrec > 0) {
+            const int O = int(int64_t(outn) + nout - out0);
+            for (int o = tid; o < O; o += nthr) {
+                const int oo = int(out0) + o;
+                int lo = 0, hi = nrec - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (a.emit[mid].y <= oo) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int2 e = a.emit[lo];
+                const int v = e.x >> 1, t = oo - e.y, len = a.rec[v].w >> 2;
+                const int u = (e.x & 1) ? t : len - 1 - t;
+                const int vs = a.vslot[v];
+                const bool onP = v < a.nP;
+                double x, y;
+                if (vs >= 0 && len > 1) {
+                    x = onP ? a.px[vs + u] : a.qx[vs + u];
+                    y = onP ? a.py[vs + u] : a.qy[vs + u];
+                } else {
+                    x = a.vx[v];
+                    y = a.vy[v];
+                }
+                a.ox[oo] = x;
+                a.oy[oo] = y;
+                a.osrc[oo] = vs >= 0 ? (onP ? a.P0 + vs + u : ~(a.Q0 + vs + u)) : a.vsrc[v];
+            }
         }
         __syncthreads();
         // 4) the check: an owner is first marked by its own ring, any other
@@ -1855,6 +1893,8 @@ struct SeqBatch {
     int *vlen, *vslot, *nx, *pv, *nb, *lab, *enex, *vsrc, *cand, *seen;
     uint8_t *inter;
     int4 *rec;
+    int *pstep;
+    int2 *emit;
     double *ox, *oy;
     int *osrc, *ring_off, *result;
     int uni;
@@ -1885,6 +1925,8 @@ __global__ __launch_bounds__(SEQ_BLK) void k_sequential(SeqBatch b) {
     a.enex = b.enex + jb.vbase; a.vsrc = b.vsrc + jb.vbase; a.cand = b.cand + jb.vbase;
     a.seen = b.seen + 2 * jb.vbase;
     a.rec = b.rec + jb.vbase;
+    a.pstep = b.pstep + jb.vbase;
+    a.emit = b.emit + 2 * jb.vbase;
     a.inter = b.inter + jb.vbase;
     a.ox = b.ox + jb.obase; a.oy = b.oy + jb.obase; a.osrc = b.osrc + jb.obase; a.out_cap = jb.out_cap;
     a.ring_off = b.ring_off + jb.rbase; a.ring_cap = jb.ring_cap;
@@ -2073,6 +2115,8 @@ struct FosterScratch {
     DBuf<int> nx, pv, nb, lab, enex, vsrc, cand;
     DBuf<uint8_t> inter;
     DBuf<int4> srec;
+    DBuf<int> spstep;
+    DBuf<int2> semit;
 };
 
 FosterScratch &fscratch() {
@@ -2485,6 +2529,8 @@ void label_and_traverse(Ctx &c) {
         b.seen = f.seen.ensure(2 * vb);
         b.inter = f.inter.ensure(vb);
         b.rec = f.srec.ensure(vb);
+        b.pstep = f.spstep.ensure(vb);
+        b.emit = f.semit.ensure(2 * vb);
         b.ox = f.sox.ensure(ob); b.oy = f.soy.ensure(ob); b.osrc = f.sosrc.ensure(ob);
         b.ring_off = f.sroff.ensure(rb);
         b.result = f.sres.ensure(3 * int64_t(nseq));
