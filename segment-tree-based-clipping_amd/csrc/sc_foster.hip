@@ -1075,13 +1075,26 @@ __device__ __attribute__((always_inline)) int pt_walk(const SeqArgs &a, int I, V
 // what the walks need from the wave that ran the steps before them, and
 // what they give back (in LDS, one per block)
 struct SeqShared {
-    int go;            // 1: the walks run (else the job's result is written)
-    int n, status;     // nodes (inserted ones included), status bits so far
-    int rings, outn;   // rings and output vertices written so far
-    int ptr;           // the walks' outcome (PT_*)
+    int go;                // 2: the entry/exit walks run, 1: createResult2's walks run, 0: result written
+    int n, status;         // nodes (inserted ones included), status bits so far
+    int rings, outn;       // rings and output vertices written so far
+    int ptr;               // the walks' outcome (PT_*)
+    int scan[2], sV[2];    // ring s: its entry/exit walk runs on every wave, from node sV[s]
+    int sst[2];            //   with this status at sV[s]
+    int nsplit[2], ncross[2]; // the split / crossing candidate lists' lengths
 };
 
 constexpr int SEQ_BLK = 1024; // threads of a sequential job's block
+constexpr int SEQ_UNROLL = 4;  // 64-node blocks whose labels one wave loads at once
+
+// arr[k] with the array kept in registers (every index a constant)
+template <class T, int N> __device__ inline T seq_pick(const T (&arr)[N], int k) {
+    T r = arr[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i)
+        if (k == i) r = arr[i];
+    return r;
+}
 
 // exclusive scan of one value per thread over the block, in thread order;
 // tot = the block's sum.  Every thread of the block calls it.
@@ -1365,11 +1378,130 @@ __device__ __attribute__((always_inline)) int seq_par_traverse(SeqArgs &a, SeqSh
     return PT_DONE;
 }
 
+// the entry/exit walk's state (status, first of a delayed pair) as 2 bits
+// x = st | first << 1 (EXIT = 0, ENTRY = 1); a node's effect on it is a
+// 4-entry table of 2-bit states, and tables compose along the ring
+__device__ inline int seq_tapply(uint32_t t, int x) { return int((t >> (2 * x)) & 3u); }
+__device__ inline uint32_t seq_tcompose(uint32_t f, uint32_t g) { // f first, then g
+    uint32_t h = 0;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) h |= ((g >> (2 * ((f >> (2 * x)) & 3u))) & 3u) << (2 * x);
+    return h;
+}
+
+// The entry/exit walks of the rings marked in sh.scan, on every thread of the
+// block: ring s from the node after sh.sV[s] around to sh.sV[s], WAVE x
+// waves nodes per round.  Each wave scans its nodes' tables, the waves'
+// composed tables give every wave the state the walk enters its nodes in
+// (LDS), and each lane applies its node's effect; list appends keep ring
+// order (ballot ranks within a wave, counts across waves) and the lists'
+// bounds.  Every branch that reaches a barrier depends on LDS values only.
+__device__ __attribute__((always_inline)) void seq_enex_block(SeqArgs &a, SeqShared &sh) {
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE, nw = blockDim.x / WAVE;
+    const int nthr = blockDim.x;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
+    __shared__ uint32_t s_tab[SEQ_BLK / WAVE];
+    __shared__ int s_cs[SEQ_BLK / WAVE], s_cc[SEQ_BLK / WAVE];
+    const int half = a.cap / 2, lim = half / 2;
+    const bool uni = a.uni != 0;
+    for (int s = 0; s < 2; ++s) {
+        if (!sh.scan[s]) continue;
+        int *splitL = s ? a.cand + half / 2 : a.cand;
+        int *crossL = s ? a.cand + half + half / 2 : a.cand + half;
+        const int r0 = s ? a.nP : 0, m = s ? a.nQ : a.nP, v0 = sh.sV[s] - r0;
+        int state = sh.sst[s] | 2; // first = true
+        int nsplit = sh.nsplit[s], ncross = sh.ncross[s];
+        for (int pb = 1; pb < m; pb += nthr) {
+            const int p = pb + wid * WAVE + lane;
+            const bool on = p < m;
+            int I = -1, lab = L_NONE;
+            bool it = false;
+            if (on) {
+                I = r0 + (v0 + p >= m ? v0 + p - m : v0 + p);
+                lab = a.lab[I];
+                it = a.inter[I] != 0;
+            }
+            uint32_t tab = 0xE4u; // identity
+            if (it) {
+                if (lab == L_CROSSING) tab = 0xB1u;     // status toggles
+                else if (lab == L_DCROSS) tab = 0x4Bu;  // first: clear it; else set it, toggle
+                else if (lab == L_DBOUNCE) tab = 0x1Bu; // first flips, status toggles
+            }
+            uint32_t P = tab;
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const uint32_t o = uint32_t(__shfl_up(int(P), d, WAVE));
+                if (lane >= d) P = seq_tcompose(o, P);
+            }
+            // lanes past the ring's end hold the identity: lane 63 has the wave's table
+            if (lane == WAVE - 1) s_tab[wid] = P;
+            __syncthreads();
+            int sw = state, send = state;
+            for (int w = 0; w < nw; ++w) {
+                const uint32_t t = s_tab[w];
+                if (w < wid) sw = seq_tapply(t, sw);
+                send = seq_tapply(t, send);
+            }
+            const uint32_t Pex = uint32_t(__shfl_up(int(P), 1, WAVE));
+            const int sin = lane == 0 ? sw : seq_tapply(Pex, sw);
+            const int sst = sin & 1;
+            const bool sfirst = (sin >> 1) != 0;
+            bool addS = false, addC = false;
+            if (it) {
+                if (lab == L_CROSSING) {
+                    a.enex[I] = sst;
+                } else if (lab == L_BOUNCING) {
+                    addS = (sst == EXIT) != uni;
+                } else if (lab == L_DCROSS) {
+                    a.enex[I] = sst;
+                    if (sfirst ? ((sst == EXIT) != uni) : ((sst == ENTRY) != uni)) a.lab[I] = L_CROSSING;
+                } else if (lab == L_DBOUNCE) {
+                    a.enex[I] = sst;
+                    addC = sfirst ? ((sst == EXIT) != uni) : ((sst == ENTRY) != uni);
+                }
+            }
+            const uint64_t ms = __ballot(addS), mc = __ballot(addC);
+            if (lane == 0) {
+                s_cs[wid] = int(__popcll(ms));
+                s_cc[wid] = int(__popcll(mc));
+            }
+            __syncthreads();
+            int bs = 0, bc = 0, ts = 0, tc = 0;
+            for (int w = 0; w < nw; ++w) {
+                const int cs = s_cs[w], cc = s_cc[w];
+                if (w < wid) {
+                    bs += cs;
+                    bc += cc;
+                }
+                ts += cs;
+                tc += cc;
+            }
+            if (addS) {
+                const int q = nsplit + bs + int(__popcll(ms & lt));
+                if (q < lim) splitL[q] = I;
+            }
+            if (addC) {
+                const int q = ncross + bc + int(__popcll(mc & lt));
+                if (q < lim) crossL[q] = I;
+            }
+            nsplit = min(nsplit + ts, lim);
+            ncross = min(ncross + tc, lim);
+            state = send;
+            __syncthreads(); // s_tab, s_cs and s_cc are rewritten next round
+        }
+        if (tid == 0) {
+            sh.nsplit[s] = nsplit;
+            sh.ncross[s] = ncross;
+        }
+    }
+}
+
 // The reference's sequential algorithm on the job's rings, run by one wave
 // with uniform control flow: every lane holds the same walk state.  Part 0
-// runs everything before createResult2's walks and leaves its state in sh
-// (sh.go = 1), or writes the job's result; part 1 takes the walks' outcome
-// (sh.ptr), runs the single walk when they fell back, and writes the result.  Lane i
+// runs the steps before the entry/exit walks and leaves its state in sh
+// (sh.go = 2), part 2 the steps after them up to createResult2's walks
+// (sh.go = 1); either writes the job's result instead when it fails (sh.go =
+// 0).  Part 1 takes the walks' outcome (sh.ptr), runs the single walk when
+// they fell back, and writes the result.  Lane i
 // caches the hot fields of node w0 + i (a window of 64 consecutive nodes,
 // w0 a multiple of 64), so a walk in ring order reads registers and reloads
 // one coalesced window every 64 nodes.  Every write goes to memory (from all
@@ -1536,20 +1668,97 @@ __device__ __attribute__((always_inline)) void seq_run(SeqArgs &a, SeqShared &sh
         a.result[2] = status;
         return;
     }
+    if (part == 2) {
+        n = sh.n;
+        status = sh.status;
+        const int rings = sh.rings, outn = sh.outn;
+        const int half = a.cap / 2;
+        const int nsplit[2] = {sh.nsplit[0], sh.nsplit[1]}, ncross[2] = {sh.ncross[0], sh.ncross[1]};
+        int *splitL[2] = {a.cand, a.cand + half / 2};
+        int *crossL[2] = {a.cand + half, a.cand + half + half / 2};
+        // membership of the Q lists, marked in the traversal's (still zero)
+        // state words: -1 in slot 2V for the split list, in 2V+1 for the crossing list
+        const int nmark = n;
+        for (int j = 0; j < nsplit[1]; ++j) set_seen(splitL[1][j], 0, -1);
+        for (int j = 0; j < ncross[1]; ++j) set_seen(crossL[1][j], 1, -1);
+        // 5) split interior bouncing pairs
+        for (int i = 0; i < nsplit[0]; ++i) {
+            int IP = splitL[0][i], IQ = NB_(IP);
+            if (!(IQ >= 0 && IQ < nmark && SEEN_(IQ, 0) == -1)) continue;
+            int VP = snew(a.vx[IP], a.vy[IP], seq_src(a, IP));
+            int VQ = snew(a.vx[IQ], a.vy[IQ], seq_src(a, IQ));
+            if (VP < 0 || VQ < 0) { status |= UB_TRAVERSAL; goto done; }
+            // the vertex before IP is the last vertex of its previous node, the one after the first of the next
+            double sP = areaA(a.vx1[PV_(IP)], a.vy1[PV_(IP)], a.vx[IP], a.vy[IP], a.vx[NX_(IP)], a.vy[NX_(IP)]);
+            double sQ = areaA(a.vx1[PV_(IQ)], a.vy1[PV_(IQ)], a.vx[IQ], a.vy[IQ], a.vx[NX_(IQ)], a.vy[NX_(IQ)]);
+            if (sP * sQ > 0) {
+                set_nb(IP, VQ); set_nb(VQ, IP); set_inter(IP, 1); set_inter(VQ, 1);
+                set_nb(IQ, VP); set_nb(VP, IQ); set_inter(IQ, 1); set_inter(VP, 1);
+            } else {
+                set_nb(VP, VQ); set_nb(VQ, VP); set_inter(VP, 1); set_inter(VQ, 1);
+            }
+            sinsert(VP, IP);
+            sinsert(VQ, IQ);
+            if (!a.uni) {
+                set_enex(IP, EXIT); set_enex(VP, ENTRY); set_enex(IQ, EXIT); set_enex(VQ, ENTRY);
+            } else {
+                set_enex(IP, ENTRY); set_enex(VP, EXIT); set_enex(IQ, ENTRY); set_enex(VQ, EXIT);
+            }
+            set_lab(IP, L_CROSSING); set_lab(VP, L_CROSSING); set_lab(IQ, L_CROSSING); set_lab(VQ, L_CROSSING);
+        }
+        // 6) crossing candidates
+        for (int i = 0; i < ncross[0]; ++i) {
+            int IP = crossL[0][i], IQ = NB_(IP);
+            if (IQ >= 0 && IQ < nmark && SEEN_(IQ, 1) == -1) {
+                set_lab(IP, L_CROSSING);
+                set_lab(IQ, L_CROSSING);
+            }
+        }
+        for (int j = 0; j < nsplit[1]; ++j) set_seen(splitL[1][j], 0, 0);
+        for (int j = 0; j < ncross[1]; ++j) set_seen(crossL[1][j], 1, 0);
+        // createResult2.  First the walks run one per lane (seq_par_traverse,
+        // every wave of the block): every candidate start (a P crossing, in
+        // ring order from the root) walks its ring without writing; the
+        // candidate with the smallest rank on a ring owns it; the owners walk
+        // again and write the rings.  The reference's single walk starts a
+        // ring at the first candidate that no earlier ring visited, so the
+        // result is the same when every owner is first visited by its own
+        // ring and every other candidate by a ring that starts before it;
+        // that is checked, and otherwise (or when a walk is too long or too
+        // many nodes were inserted) the single walk of part 1 runs.
+        __threadfence_block(); // this wave's stores, before the other waves read them
+        if (lane == 0) {
+            sh.n = n;
+            sh.status = status;
+            sh.rings = rings;
+            sh.outn = outn;
+            sh.go = 1;
+        }
+        return;
+    }
     // 2) intersection chains.  Before any insertion ring P is nodes [0, nP) in
     // ring order from its root, so the chain starts (intersections labelled
     // LEFT_ON or RIGHT_ON) are found 64 nodes per ballot, in that order, and
     // walked one after another.  A start that an earlier chain relabelled is
-    // checked again and skipped, and a later block reads the labels the chains
-    // before it wrote: the reference's single walk, step for step.
-    for (int w = 0; w < a.nP; w += WAVE) {
-        const int v = w + lane;
-        bool start = false;
-        if (v < a.nP) {
-            const int lb = a.lab[v];
-            start = a.inter[v] != 0 && (lb == L_LEFT_ON || lb == L_RIGHT_ON);
+    // checked again and skipped: the reference's single walk, step for step.
+    // The labels of SEQ_UNROLL blocks are loaded at once; a chain never
+    // labels a node LEFT_ON or RIGHT_ON, so a start found in the loaded
+    // labels is a start or was relabelled since (and is skipped).
+    for (int w0b = 0; w0b < a.nP; w0b += SEQ_UNROLL * WAVE) {
+        uint64_t sb[SEQ_UNROLL];
+#pragma unroll
+        for (int k = 0; k < SEQ_UNROLL; ++k) {
+            const int v = w0b + k * WAVE + lane;
+            bool start = false;
+            if (v < a.nP) {
+                const int lb = a.lab[v];
+                start = a.inter[v] != 0 && (lb == L_LEFT_ON || lb == L_RIGHT_ON);
+            }
+            sb[k] = __ballot(start);
         }
-        uint64_t starts = __ballot(start);
+        for (int k = 0; k < SEQ_UNROLL; ++k) {
+        const int w = w0b + k * WAVE;
+        uint64_t starts = seq_pick(sb, k);
         while (starts) {
             const int I = w + __builtin_ctzll(starts);
             starts &= starts - 1ull;
@@ -1569,6 +1778,7 @@ __device__ __attribute__((always_inline)) void seq_run(SeqArgs &a, SeqShared &sh
                 if (NB_(X) >= 0) set_lab(NB_(X), ct);
                 if (NB_(J) >= 0) set_lab(NB_(J), ct);
             }
+        }
         }
     }
     {
@@ -1671,12 +1881,7 @@ __device__ __attribute__((always_inline)) void seq_run(SeqArgs &a, SeqShared &sh
         // the entry/exit walk's state (status, first of a delayed pair) as 2 bits
         // x = st | first << 1 (EXIT = 0, ENTRY = 1); a node's effect on it is a
         // 4-entry table of 2-bit states, and tables compose along the ring
-        auto tapply = [](uint32_t t, int x) { return int((t >> (2 * x)) & 3u); };
-        auto tcompose = [](uint32_t f, uint32_t g) { // f first, then g
-            uint32_t h = 0;
-            for (int x = 0; x < 4; ++x) h |= ((g >> (2 * ((f >> (2 * x)) & 3u))) & 3u) << (2 * x);
-            return h;
-        };
+        if (lane == 0) sh.scan[0] = sh.scan[1] = 0;
         for (int s = 0; s < 2; ++s) {
             if (noInter[s]) continue;
             int A = roots[s], B = roots[1 - s];
@@ -1705,69 +1910,14 @@ __device__ __attribute__((always_inline)) void seq_run(SeqArgs &a, SeqShared &sh
             if (in) st = seq_toggle(st);
             bool first = true;
             if (!inserted[s]) {
-                // The walk from the node after V around to V, 64 nodes per step: an
-                // inclusive scan of the nodes' tables gives every node the state the
-                // walk reaches it in, then each lane applies its node's effect; list
-                // appends keep ring order (ballot ranks) and the lists' bounds.
-                const int r0 = ring0[s], m = ringn[s], v0 = V - r0, lim = half / 2;
-                const bool uni = a.uni != 0;
-                int state = st | 2; // first = true
-                for (int p0 = 1; p0 < m; p0 += WAVE) {
-                    const int p = p0 + lane;
-                    const bool on = p < m;
-                    int I = -1, lab = L_NONE;
-                    bool it = false;
-                    if (on) {
-                        I = r0 + (v0 + p >= m ? v0 + p - m : v0 + p);
-                        lab = a.lab[I];
-                        it = a.inter[I] != 0;
-                    }
-                    uint32_t tab = 0xE4u; // identity
-                    if (it) {
-                        if (lab == L_CROSSING) tab = 0xB1u;     // status toggles
-                        else if (lab == L_DCROSS) tab = 0x4Bu;  // first: clear it; else set it, toggle
-                        else if (lab == L_DBOUNCE) tab = 0x1Bu; // first flips, status toggles
-                    }
-                    uint32_t P = tab;
-                    for (int d = 1; d < WAVE; d <<= 1) {
-                        const uint32_t o = uint32_t(__shfl_up(int(P), d, WAVE));
-                        if (lane >= d) P = tcompose(o, P);
-                    }
-                    const uint32_t Pex = uint32_t(__shfl_up(int(P), 1, WAVE));
-                    const int sin = lane == 0 ? state : tapply(Pex, state);
-                    const int sst = sin & 1;
-                    const bool sfirst = (sin >> 1) != 0;
-                    bool addS = false, addC = false;
-                    if (it) {
-                        if (lab == L_CROSSING) {
-                            a.enex[I] = sst;
-                        } else if (lab == L_BOUNCING) {
-                            addS = (sst == EXIT) != uni;
-                        } else if (lab == L_DCROSS) {
-                            a.enex[I] = sst;
-                            if (sfirst ? ((sst == EXIT) != uni) : ((sst == ENTRY) != uni)) a.lab[I] = L_CROSSING;
-                        } else if (lab == L_DBOUNCE) {
-                            a.enex[I] = sst;
-                            addC = sfirst ? ((sst == EXIT) != uni) : ((sst == ENTRY) != uni);
-                        }
-                    }
-                    const uint64_t ms = __ballot(addS), mc = __ballot(addC);
-                    if (addS) {
-                        const int q = nsplit[s] + int(__popcll(ms & lt));
-                        if (q < lim) splitL[s][q] = I;
-                    }
-                    if (addC) {
-                        const int q = ncross[s] + int(__popcll(mc & lt));
-                        if (q < lim) crossL[s][q] = I;
-                    }
-                    nsplit[s] = min(nsplit[s] + int(__popcll(ms)), lim);
-                    ncross[s] = min(ncross[s] + int(__popcll(mc)), lim);
-                    const int last = min(WAVE, m - p0) - 1;
-                    state = tapply(uint32_t(__builtin_amdgcn_readlane(int(P), last)), state);
+                // the walk from the node after V around to V runs on every wave
+                // (seq_enex_block); no other step before it reads this ring's
+                // labels or lists
+                if (lane == 0) {
+                    sh.scan[s] = 1;
+                    sh.sV[s] = V;
+                    sh.sst[s] = st;
                 }
-                // the lanes' stores, visible to every lane before the window reloads
-                __threadfence_block();
-                w0 = -(1 << 30);
                 continue;
             }
             int I = NX_(V);
@@ -1806,67 +1956,22 @@ __device__ __attribute__((always_inline)) void seq_run(SeqArgs &a, SeqShared &sh
                 I = NX_(I);
             }
         }
-        // membership of the Q lists, marked in the traversal's (still zero)
-        // state words: -1 in slot 2V for the split list, in 2V+1 for the crossing list
-        const int nmark = n;
-        for (int j = 0; j < nsplit[1]; ++j) set_seen(splitL[1][j], 0, -1);
-        for (int j = 0; j < ncross[1]; ++j) set_seen(crossL[1][j], 1, -1);
-        // 5) split interior bouncing pairs
-        for (int i = 0; i < nsplit[0]; ++i) {
-            int IP = splitL[0][i], IQ = NB_(IP);
-            if (!(IQ >= 0 && IQ < nmark && SEEN_(IQ, 0) == -1)) continue;
-            int VP = snew(a.vx[IP], a.vy[IP], seq_src(a, IP));
-            int VQ = snew(a.vx[IQ], a.vy[IQ], seq_src(a, IQ));
-            if (VP < 0 || VQ < 0) { status |= UB_TRAVERSAL; goto done; }
-            // the vertex before IP is the last vertex of its previous node, the one after the first of the next
-            double sP = areaA(a.vx1[PV_(IP)], a.vy1[PV_(IP)], a.vx[IP], a.vy[IP], a.vx[NX_(IP)], a.vy[NX_(IP)]);
-            double sQ = areaA(a.vx1[PV_(IQ)], a.vy1[PV_(IQ)], a.vx[IQ], a.vy[IQ], a.vx[NX_(IQ)], a.vy[NX_(IQ)]);
-            if (sP * sQ > 0) {
-                set_nb(IP, VQ); set_nb(VQ, IP); set_inter(IP, 1); set_inter(VQ, 1);
-                set_nb(IQ, VP); set_nb(VP, IQ); set_inter(IQ, 1); set_inter(VP, 1);
-            } else {
-                set_nb(VP, VQ); set_nb(VQ, VP); set_inter(VP, 1); set_inter(VQ, 1);
-            }
-            sinsert(VP, IP);
-            sinsert(VQ, IQ);
-            if (!a.uni) {
-                set_enex(IP, EXIT); set_enex(VP, ENTRY); set_enex(IQ, EXIT); set_enex(VQ, ENTRY);
-            } else {
-                set_enex(IP, ENTRY); set_enex(VP, EXIT); set_enex(IQ, ENTRY); set_enex(VQ, EXIT);
-            }
-            set_lab(IP, L_CROSSING); set_lab(VP, L_CROSSING); set_lab(IQ, L_CROSSING); set_lab(VQ, L_CROSSING);
-        }
-        // 6) crossing candidates
-        for (int i = 0; i < ncross[0]; ++i) {
-            int IP = crossL[0][i], IQ = NB_(IP);
-            if (IQ >= 0 && IQ < nmark && SEEN_(IQ, 1) == -1) {
-                set_lab(IP, L_CROSSING);
-                set_lab(IQ, L_CROSSING);
-            }
-        }
-        for (int j = 0; j < nsplit[1]; ++j) set_seen(splitL[1][j], 0, 0);
-        for (int j = 0; j < ncross[1]; ++j) set_seen(crossL[1][j], 1, 0);
-        // createResult2.  First the walks run one per lane (seq_par_traverse,
-        // every wave of the block): every candidate start (a P crossing, in
-        // ring order from the root) walks its ring without writing; the
-        // candidate with the smallest rank on a ring owns it; the owners walk
-        // again and write the rings.  The reference's single walk starts a
-        // ring at the first candidate that no earlier ring visited, so the
-        // result is the same when every owner is first visited by its own
-        // ring and every other candidate by a ring that starts before it;
-        // that is checked, and otherwise (or when a walk is too long or too
-        // many nodes were inserted) the single walk of part 1 runs.
         __threadfence_block(); // this wave's stores, before the other waves read them
         if (lane == 0) {
             sh.n = n;
             sh.status = status;
             sh.rings = rings;
             sh.outn = outn;
-            sh.go = 1;
+            sh.nsplit[0] = nsplit[0];
+            sh.nsplit[1] = nsplit[1];
+            sh.ncross[0] = ncross[0];
+            sh.ncross[1] = ncross[1];
+            sh.go = 2;
         }
         return;
     }
 done:
+    if (lane == 0) sh.go = 0;
     a.result[0] = 0;
     a.result[1] = 0;
     a.result[2] = status | UB_TRAVERSAL;
@@ -1943,9 +2048,14 @@ __global__ __launch_bounds__(SEQ_BLK) void k_sequential(SeqBatch b) {
     __shared__ SeqShared sh;
     if (threadIdx.x == 0) sh.go = 0;
     __syncthreads();
-    if (threadIdx.x < WAVE) seq_run(a, sh, 0); // wave 0: the steps before the walks
+    if (threadIdx.x < WAVE) seq_run(a, sh, 0); // wave 0: the steps before the entry/exit walks
     __syncthreads();
     if (!sh.go) return; // the job's result is written
+    seq_enex_block(a, sh); // every wave
+    __syncthreads();
+    if (threadIdx.x < WAVE) seq_run(a, sh, 2); // wave 0: split and crossing candidates
+    __syncthreads();
+    if (!sh.go) return;
     const int ptr = a.par ? seq_par_traverse(a, sh) : PT_FALLBACK; // every wave
     __syncthreads();
     if (threadIdx.x == 0) sh.ptr = ptr;
