@@ -1085,16 +1085,6 @@ struct SeqShared {
 };
 
 constexpr int SEQ_BLK = 1024; // threads of a sequential job's block
-constexpr int SEQ_UNROLL = 4;  // 64-node blocks whose labels one wave loads at once
-
-// arr[k] with the array kept in registers (every index a constant)
-template <class T, int N> __device__ inline T seq_pick(const T (&arr)[N], int k) {
-    T r = arr[0];
-#pragma unroll
-    for (int i = 1; i < N; ++i)
-        if (k == i) r = arr[i];
-    return r;
-}
 
 // exclusive scan of one value per thread over the block, in thread order;
 // tot = the block's sum.  Every thread of the block calls it.
@@ -1740,25 +1730,16 @@ __device__ __attribute__((always_inline)) void seq_run(SeqArgs &a, SeqShared &sh
     // ring order from its root, so the chain starts (intersections labelled
     // LEFT_ON or RIGHT_ON) are found 64 nodes per ballot, in that order, and
     // walked one after another.  A start that an earlier chain relabelled is
-    // checked again and skipped: the reference's single walk, step for step.
-    // The labels of SEQ_UNROLL blocks are loaded at once; a chain never
-    // labels a node LEFT_ON or RIGHT_ON, so a start found in the loaded
-    // labels is a start or was relabelled since (and is skipped).
-    for (int w0b = 0; w0b < a.nP; w0b += SEQ_UNROLL * WAVE) {
-        uint64_t sb[SEQ_UNROLL];
-#pragma unroll
-        for (int k = 0; k < SEQ_UNROLL; ++k) {
-            const int v = w0b + k * WAVE + lane;
-            bool start = false;
-            if (v < a.nP) {
-                const int lb = a.lab[v];
-                start = a.inter[v] != 0 && (lb == L_LEFT_ON || lb == L_RIGHT_ON);
-            }
-            sb[k] = __ballot(start);
+    // checked again and skipped, and a later block reads the labels the chains
+    // before it wrote: the reference's single walk, step for step.
+    for (int w = 0; w < a.nP; w += WAVE) {
+        const int v = w + lane;
+        bool start = false;
+        if (v < a.nP) {
+            const int lb = a.lab[v];
+            start = a.inter[v] != 0 && (lb == L_LEFT_ON || lb == L_RIGHT_ON);
         }
-        for (int k = 0; k < SEQ_UNROLL; ++k) {
-        const int w = w0b + k * WAVE;
-        uint64_t starts = seq_pick(sb, k);
+        uint64_t starts = __ballot(start);
         while (starts) {
             const int I = w + __builtin_ctzll(starts);
             starts &= starts - 1ull;
@@ -1778,7 +1759,6 @@ __device__ __attribute__((always_inline)) void seq_run(SeqArgs &a, SeqShared &sh
                 if (NB_(X) >= 0) set_lab(NB_(X), ct);
                 if (NB_(J) >= 0) set_lab(NB_(J), ct);
             }
-        }
         }
     }
     {
