@@ -231,22 +231,20 @@ __global__ void k_sp_wind_fix(int I, int J, SparseP sp, const double *__restrict
     const int e = sp.edge[i];
     auto is_nd = [&](int k) { return sp.slot[k] >= 0 && sp.slot[k] == sp.edge[k] + 1 + sp.ndx[k]; };
     if (!is_nd(i)) return;
-    if (i > sp.P0[j] && sp.edge[i - 1] == e && is_nd(i - 1)) return; // not the first piece point
+    // Base edge e = (a, b) with the run of points p1..pm inserted on it counts
+    // -w(a, b) + w(a, p1) + w(p1, p2) + ... + w(pm, b) instead of w(a, b):
+    // each point adds the term of the piece that ends at it, the last point
+    // also the closing piece, the first the removal of the whole edge.
     const double rx = qq[2 * j], ry = qq[2 * j + 1];
-    const int nb = sp.nbv[j];
     const double *V = sp.bpoly + 2 * int64_t(sp.bvoff[j]);
     const double ax = V[2 * e], ay = V[2 * e + 1], bx = V[2 * e + 2], by = V[2 * e + 3]; // closing vertex repeated
-    (void)nb;
-    int w = -wind_edge(ax, ay, bx, by, rx, ry);
-    double px = ax, py = ay;
-    int k = i;
-    while (k < sp.P0[j + 1] && sp.edge[k] == e && is_nd(k)) {
-        w += wind_edge(px, py, sp.x[k], sp.y[k], rx, ry);
-        px = sp.x[k];
-        py = sp.y[k];
-        ++k;
-    }
-    w += wind_edge(px, py, bx, by, rx, ry);
+    const bool first = !(i > sp.P0[j] && sp.edge[i - 1] == e && is_nd(i - 1));
+    const bool last = !(i + 1 < sp.P0[j + 1] && sp.edge[i + 1] == e && is_nd(i + 1));
+    const double px = first ? ax : sp.x[i - 1], py = first ? ay : sp.y[i - 1];
+    const double x = sp.x[i], y = sp.y[i];
+    int w = wind_edge(px, py, x, y, rx, ry);
+    if (first) w -= wind_edge(ax, ay, bx, by, rx, ry);
+    if (last) w += wind_edge(x, y, bx, by, rx, ry);
     if (w) atomicAdd(&wind[j], w);
 }
 
