@@ -715,8 +715,11 @@ __global__ void k_keep0(int64_t n, const int *ringid, const int *off, const int 
 // unless it is collinear with the last vertex kept before it (under the
 // previous round's flags) and its successor (the root, for the ring's last
 // vertex); the rounds repeat until no flag changes.
-// Tiles of KEEP_TILE vertices are taken in launch order from a counter (every
-// earlier tile belongs to a block already running).  A tile publishes the
+// Tile t is block t.  A block waits only for tiles of lower index, and every
+// XCD dispatches its blocks in index order, so the lowest block not yet
+// dispatched waits for nothing that does not finish: every wait ends (and a
+// wait is also bounded, reported as an error).  A ticket counter instead of
+// the block index costs one contended atomic per tile.  A tile publishes the
 // largest mark (kept index + 1) among its vertices under the previous round's
 // flags; wave 0 then walks back over earlier tiles' words, 64 per round trip,
 // until one carries an inclusive prefix.  Each vertex so learns the last
@@ -737,7 +740,6 @@ struct KeepRound {
     uint32_t *keep;
     unsigned long long *status; // per tile; no word carries this round's epoch before the launch
     unsigned long long epoch;   // 1..KEEP_ROUNDS, shifted into place
-    uint32_t *tile_counter;     // zero before the launch
     uint32_t *changed;          // set when a flag changes
     uint32_t *err;              // set when a wait gives up (the result is then invalid)
     uint32_t *tstate;           // per tile: flags changed in its last round, the prefix that round saw
@@ -745,12 +747,10 @@ struct KeepRound {
 };
 
 __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
-    __shared__ uint32_t s_tile, s_prefix, s_skip, s_ch;
+    __shared__ uint32_t s_prefix, s_skip, s_ch;
     __shared__ uint32_t s_wmax[BLK / WAVE];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
-    if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = blockIdx.x;
     const int64_t i0 = int64_t(tile) * KEEP_TILE + int64_t(tid) * KEEP_ITEMS;
     uint32_t kp[KEEP_ITEMS];
     uint32_t mine = 0; // marks grow with the index: the thread's largest is its last kept vertex
@@ -2245,11 +2245,11 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
     // clean-up rounds, KEEP_BATCH per look at the per-round change flags; a
     // round after one without change changes nothing either
     const int64_t ntiles = (n + KEEP_TILE - 1) / KEEP_TILE;
-    uint32_t *ctl = f.kctl.ensure(2 * KEEP_ROUNDS + 1); // changed flags, error word, tile counters
+    uint32_t *ctl = f.kctl.ensure(KEEP_ROUNDS + 1); // changed flags, error word
     unsigned long long *kst = f.kstatus.ensure(size_t(ntiles));
     {
         FillBatch fb(s);
-        fb.add(ctl, 2 * KEEP_ROUNDS + 1, 0u);
+        fb.add(ctl, KEEP_ROUNDS + 1, 0u);
         fb.add(kst, 2 * ntiles, 0u);
         fb.flush();
     }
@@ -2269,7 +2269,6 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
         const int end = std::min(it + KEEP_BATCH, KEEP_ROUNDS);
         for (; it < end; ++it) {
             kr.epoch = (unsigned long long)(it + 1) << KS_EPOCH_SHIFT;
-            kr.tile_counter = ctl + KEEP_ROUNDS + 1 + it;
             kr.changed = ctl + it;
             kr.first = it == 0 ? 1 : 0;
             k_keep_round<<<unsigned(ntiles), BLK, 0, s>>>(kr);

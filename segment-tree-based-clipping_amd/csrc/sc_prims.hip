@@ -347,8 +347,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_global_hist(const uint64_t *keys, 
 }
 
 // One pass of the LSD radix sort in one launch (a chained scan over tiles).
-// A block takes the next tile number from a counter, so every tile before
-// it belongs to a block that is already running.  Wave w of the block owns
+// Tile t is block t: a tile waits only for tiles of lower index, and every
+// XCD dispatches its blocks in index order, so the lowest block not yet
+// dispatched waits for nothing that does not finish (a ticket counter
+// instead costs one contended atomic per tile; a wait is also bounded, see
+// the look-back below).  Wave w of the block owns
 // the RS_TILE/RS_WAVES consecutive keys starting at w*RS_TILE/RS_WAVES, 64 at
 // a time; keys are ranked within their wave and digit (peers found with
 // eight ballots, a per-wave digit counter in LDS) and a block scan over
@@ -380,7 +383,6 @@ struct SweepArgs {
     const uint32_t *ghist;       // this pass's 256 digit counts over all keys
     unsigned long long *status;  // ntiles * 256 words, none of them tagged with this epoch
     unsigned long long epoch;    // 1..255, shifted into place
-    uint32_t *tile_counter;      // zero before the launch
 };
 
 __device__ inline uint32_t block_excl_sum256(uint32_t x, uint32_t *s_wsum, uint32_t &total) {
@@ -411,14 +413,12 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
     __shared__ uint16_t s_cnt[256 * RS_WAVES];
     __shared__ uint32_t s_gbase[256];          // output offset of the digit minus its tile-local start
     __shared__ uint32_t s_wsum[RS_WAVES];
-    __shared__ uint32_t s_tile;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
-    if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
     for (int i = tid; i < 256 * RS_WAVES; i += RS_BLOCK) s_cnt[i] = 0;
     // a pass in which every key has the same digit keeps the order: copy the tile
     const bool trivial = __syncthreads_or(uint64_t(a.ghist[tid]) == uint64_t(a.n));
-    const uint32_t tile = s_tile;
+    const uint32_t tile = blockIdx.x;
     const int64_t base = int64_t(tile) * RS_TILE;
     if (trivial) {
         for (int64_t i = base + tid; i < a.n && i < base + RS_TILE; i += RS_BLOCK) {
@@ -602,10 +602,10 @@ bool radix_passes(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt
     SC_REQUIRE(bit_lo >= 0 && bit_hi <= 64, SC_EINVAL, "radix bit range");
     int npass = int(ceil_div(bit_hi - bit_lo, 8));
     SC_REQUIRE(npass <= RS_MAXPASS, SC_EINVAL, "radix passes");
-    // [0, GH): per pass 256 digit counts; [GH, GH+RS_MAXPASS): per pass tile counter
+    // per pass 256 digit counts
     const size_t GH = size_t(RS_MAXPASS) * 256;
-    uint32_t *gh = rs.global_hist.ensure(GH + RS_MAXPASS);
-    SC_HIP(hipMemsetAsync(gh, 0, sizeof(uint32_t) * (GH + RS_MAXPASS), s));
+    uint32_t *gh = rs.global_hist.ensure(GH);
+    SC_HIP(hipMemsetAsync(gh, 0, sizeof(uint32_t) * GH, s));
     int ghb = int(ceil_div(n, RS_BLOCK * 16));
     if (ghb > 1024) ghb = 1024;
     k_global_hist<<<ghb, RS_BLOCK, 0, s>>>(keys, n, bit_lo, bit_hi, npass, gh);
@@ -641,7 +641,6 @@ bool radix_passes(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt
         a.ghist = gh + size_t(p) * 256;
         a.status = st;
         a.epoch = (unsigned long long)rs.epoch << ST_EPOCH_SHIFT;
-        a.tile_counter = gh + GH + p;
         k_onesweep<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
         SC_HIP(hipGetLastError());
         std::swap(kc, ko);
@@ -652,7 +651,7 @@ bool radix_passes(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt
 } // namespace
 
 size_t radix_fixed_words(int64_t n) {
-    return size_t(RS_MAXPASS) * 256 + RS_MAXPASS + 2 * size_t(ceil_div(n > 0 ? n : 1, RS_TILE)) * 256;
+    return size_t(RS_MAXPASS) * 256 + 2 * size_t(ceil_div(n > 0 ? n : 1, RS_TILE)) * 256;
 }
 
 bool radix_sort_u64_fixed(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt, int64_t n, int bit_lo,
@@ -665,13 +664,13 @@ bool radix_sort_u64_fixed(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32
     const size_t GH = size_t(RS_MAXPASS) * 256;
     const int64_t ntiles = ceil_div(n, RS_TILE);
     uint32_t *gh = work;
-    // 8-byte aligned: GH + RS_MAXPASS words is even
-    unsigned long long *st = reinterpret_cast<unsigned long long *>(work + GH + RS_MAXPASS);
+    // 8-byte aligned: GH words is even
+    unsigned long long *st = reinterpret_cast<unsigned long long *>(work + GH);
     {
-        // histograms, tile counters and every status word start at zero;
-        // cleared by a kernel (no memset node when a caller captures the stream)
+        // histograms and every status word start at zero; cleared by a
+        // kernel (no memset node when a caller captures the stream)
         FillBatch fb(s);
-        fb.add(gh, int64_t(GH + RS_MAXPASS), 0u);
+        fb.add(gh, int64_t(GH), 0u);
         fb.add(st, 2 * ntiles * 256, 0u);
         fb.flush();
     }
@@ -696,7 +695,6 @@ bool radix_sort_u64_fixed(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32
         a.ghist = gh + size_t(p) * 256;
         a.status = st;
         a.epoch = (unsigned long long)(p + 1) << ST_EPOCH_SHIFT;
-        a.tile_counter = gh + GH + p;
         k_onesweep<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
         SC_HIP(hipGetLastError());
         std::swap(kc, ko);

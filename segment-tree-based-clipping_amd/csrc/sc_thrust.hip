@@ -153,9 +153,9 @@ constexpr size_t NA_VALT = NA_IDX + 4 * size_t(NOALLOC_MAX);
 constexpr size_t NA_TK = NA_VALT + 4 * size_t(NOALLOC_MAX);
 constexpr size_t NA_TV = NA_TK + 8 * size_t(NOALLOC_MAX);
 constexpr size_t NA_WORK = NA_TV + 4 * size_t(NOALLOC_MAX);
-// radix work words for NOALLOC_MAX keys: 8 x 256 histograms + 8 tile
-// counters, then 256 two-word status words per radix tile
-constexpr size_t NA_WORK_WORDS = 8 * 256 + 8 + 2 * size_t(NOALLOC_MAX / sc::RADIX_TILE_KEYS) * 256;
+// radix work words for NOALLOC_MAX keys: 8 x 256 histograms, then 256
+// two-word status words per radix tile
+constexpr size_t NA_WORK_WORDS = 8 * 256 + 2 * size_t(NOALLOC_MAX / sc::RADIX_TILE_KEYS) * 256;
 constexpr size_t NA_BYTES = NA_WORK + 4 * NA_WORK_WORDS;
 
 constexpr int NA_MAX_DEVICES = 64;
