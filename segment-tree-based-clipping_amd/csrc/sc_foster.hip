@@ -449,7 +449,14 @@ __global__ void k_jump_init(int mP, int M, const uint32_t *val, const uint32_t *
     rec[x] = make_uint4(val[x], 0u, w, n2);
 }
 
-__global__ void k_jump_rank(int mP, const uint4 *__restrict__ rec, uint4 *__restrict__ rec2) {
+// Round r doubles every window (2^r crossings to 2^(r+1)) and sets moved[r]
+// when some window's minimum changed.  Values on a cycle are distinct, so a
+// round that moves no minimum finds every cycle no longer than the windows
+// it read: minima and distances are final in both buffers (the round copied
+// them unchanged), and every later round returns at once.
+__global__ void k_jump_rank(int mP, const uint4 *__restrict__ rec, uint4 *__restrict__ rec2, uint32_t *moved,
+                            int r) {
+    if (r > 0 && moved[r - 1] == 0u) return;
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= mP) return;
     const uint4 a = rec[x];
@@ -460,6 +467,7 @@ __global__ void k_jump_rank(int mP, const uint4 *__restrict__ rec, uint4 *__rest
     o.z = a.z + b.z;
     o.w = b.w;
     rec2[x] = o;
+    if (o.x != a.x) moved[r] = 1u;
 }
 
 // Per crossing (P and Q): its cycle's minimum, the list start of a traversed
@@ -2203,6 +2211,7 @@ struct FosterScratch {
     DBuf<int> nx, pv, nb, lab, enex, vsrc, cand;
     DBuf<uint8_t> inter;
     DBuf<int4> srec;
+    DBuf<uint32_t> jmoved; // per doubling round: some cycle minimum moved
     DBuf<int> spstep;
     DBuf<int2> semit;
 };
@@ -2646,8 +2655,14 @@ void label_and_traverse(Ctx &c) {
             k_jump_init<<<grid_for(mP), BLK, 0, s>>>(mP, M, pa.val, pa.succ, pa.plen, rec);
             // a cycle of a traversed job holds at most maxmP P crossings
             const int rounds = ilog2_exact(maxmP);
+            uint32_t *moved = f.jmoved.ensure(size_t(rounds > 0 ? rounds : 1));
+            {
+                FillBatch fb(s);
+                fb.add(moved, rounds, 0u);
+                fb.flush();
+            }
             for (int r = 0; r < rounds; ++r) {
-                k_jump_rank<<<grid_for(mP), BLK, 0, s>>>(mP, rec, rec2);
+                k_jump_rank<<<grid_for(mP), BLK, 0, s>>>(mP, rec, rec2, moved, r);
                 std::swap(rec, rec2);
             }
         }
