@@ -163,6 +163,11 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *d_total, hipStream_t s);
 // Inclusive running maximum of n values (in == out allowed).
 void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s);
+// two independent arrays in the launches of one scan (same results as two calls)
+void scan_exclusive_u32_2(const uint32_t *in0, uint32_t *out0, int64_t n0, uint32_t *t0, const uint32_t *in1,
+                          uint32_t *out1, int64_t n1, uint32_t *t1, hipStream_t s);
+void scan_inclusive_max_u32_2(const uint32_t *in0, uint32_t *out0, int64_t n0, const uint32_t *in1,
+                              uint32_t *out1, int64_t n1, hipStream_t s);
 
 // Several 32-bit spans set to a value each, in one launch: add() queues a
 // span and launches when FILL_MAX are queued, flush() launches the rest (the

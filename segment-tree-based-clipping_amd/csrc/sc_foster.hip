@@ -2415,8 +2415,7 @@ void label_and_traverse(Ctx &c) {
     }
     uint32_t *ccP = f.ccP.ensure(nP + 1), *ccQ = f.ccQ.ensure(nQ + 1);
     uint32_t *tot = c.d_u32_total.ensure(2);
-    scan_exclusive_u32(crossP, ccP, nP + 1, tot, s);
-    scan_exclusive_u32(crossQ, ccQ, nQ + 1, tot + 1, s);
+    scan_exclusive_u32_2(crossP, ccP, nP + 1, tot, crossQ, ccQ, nQ + 1, tot + 1, s);
     // point-in-polygon of each ring's first plain vertex w.r.t. the other ring
     double *qpts = f.qpts.ensure(4 * size_t(J));
     int *qoks = f.qoks.ensure(2 * size_t(J));
@@ -2682,8 +2681,7 @@ void label_and_traverse(Ctx &c) {
             fb.flush();
         }
         SC_HIP(hipGetLastError());
-        scan_exclusive_u32(base, base, int64_t(mP) + 1, nullptr, s);
-        scan_exclusive_u32(isstart, ridx, int64_t(mP) + 1, nullptr, s);
+        scan_exclusive_u32_2(base, base, int64_t(mP) + 1, nullptr, isstart, ridx, int64_t(mP) + 1, nullptr, s);
     }
     // per-job output sizes before clean-up, and their offsets
     uint32_t *jv = f.jv.ensure(J + 1), *jr = f.jr.ensure(J + 1);
@@ -2710,8 +2708,7 @@ void label_and_traverse(Ctx &c) {
         fb.add(jr + J, 1, 0u);
         fb.flush();
     }
-    scan_exclusive_u32(jv, jv, int64_t(J) + 1, tot, s);
-    scan_exclusive_u32(jr, jr, int64_t(J) + 1, tot + 1, s);
+    scan_exclusive_u32_2(jv, jv, int64_t(J) + 1, tot, jr, jr, int64_t(J) + 1, tot + 1, s);
     d2h_sync(c, htot, tot, 2);
     const int64_t NV = htot[0];
     const int NR = int(htot[1]);

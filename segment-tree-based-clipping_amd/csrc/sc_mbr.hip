@@ -170,8 +170,7 @@ int64_t mbr_join(const double *axy, const int *asz, int na, const double *bxy, c
                                                nullptr);
     SC_HIP(hipGetLastError());
     uint32_t *oa = m.oa.ensure(size_t(na) + 1), *ob = m.ob.ensure(size_t(nb) + 1), *tot = m.tot.ensure(2);
-    scan_exclusive_u32(ca, oa, int64_t(na) + 1, tot, s);
-    scan_exclusive_u32(cb, ob, int64_t(nb) + 1, tot + 1, s);
+    scan_exclusive_u32_2(ca, oa, int64_t(na) + 1, tot, cb, ob, int64_t(nb) + 1, tot + 1, s);
     uint32_t h[2] = {0, 0};
     {
         // the totals: one pageable copy and a stream wait (a stateless call has no context staging)

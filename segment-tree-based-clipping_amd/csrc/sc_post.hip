@@ -550,8 +550,8 @@ void post_jobs(Ctx &c, int k0, int k1) {
         fb.flush();
     }
     uint32_t *tot = c.d_u32_total.ensure(2);
-    scan_exclusive_u32(reinterpret_cast<uint32_t *>(nba), reinterpret_cast<uint32_t *>(nba), J + 1, tot, s);
-    scan_exclusive_u32(reinterpret_cast<uint32_t *>(nca), reinterpret_cast<uint32_t *>(nca), J + 1, tot + 1, s);
+    scan_exclusive_u32_2(reinterpret_cast<uint32_t *>(nba), reinterpret_cast<uint32_t *>(nba), J + 1, tot,
+                         reinterpret_cast<uint32_t *>(nca), reinterpret_cast<uint32_t *>(nca), J + 1, tot + 1, s);
     js.AB.resize(J + 1);
     js.AC.resize(J + 1);
     uint32_t hseg[2 * SEG_CTL] = {};

@@ -191,10 +191,10 @@ template <class Op> __device__ inline uint32_t block_excl_op(uint32_t x, uint32_
     return r;
 }
 
-template <class Op> __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_reduce_op(const uint32_t *in, int64_t n,
-                                                                                uint32_t *sums) {
-    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE + 1];
-    const int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
+template <class Op>
+__device__ inline void tile_reduce_body(const uint32_t *in, int64_t n, uint32_t *sums, uint32_t tile,
+                                        uint32_t *s_wave) {
+    const int64_t base = int64_t(tile) * SCAN_TILE;
     uint32_t acc = Op::id();
 #pragma unroll
     for (int it = 0; it < SCAN_ITEMS; ++it) {
@@ -203,7 +203,7 @@ template <class Op> __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_reduce_
     }
     uint32_t total;
     (void)block_excl_op<Op>(acc, s_wave, total);
-    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) sums[tile] = total;
 }
 
 // INCL: inclusive output (the max scan), else exclusive (the sum scan).
@@ -211,12 +211,9 @@ template <class Op> __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_reduce_
 // The tiles before this one come either as their values (sums, at most
 // PREF_TILES of them, combined here) or already combined (pre[tile]).
 template <class Op, bool INCL>
-__global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan_op(const uint32_t *in, uint32_t *out, int64_t n,
-                                                              const uint32_t *sums, const uint32_t *pre,
-                                                              uint32_t *d_total) {
-    __shared__ uint32_t s_data[SCAN_TILE];
-    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE + 1];
-    const uint32_t tile = blockIdx.x;
+__device__ inline void tile_scan_body(const uint32_t *in, uint32_t *out, int64_t n, const uint32_t *sums,
+                                      const uint32_t *pre, uint32_t *d_total, uint32_t tile, uint32_t *s_data,
+                                      uint32_t *s_wave) {
     const int64_t base = int64_t(tile) * SCAN_TILE;
     uint32_t pacc = Op::id();
     if (!pre)
@@ -257,6 +254,50 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan_op(const uint32_t *in,
         if (i < n) out[i] = s_data[it * SCAN_BLOCK + threadIdx.x];
     }
     if (d_total && threadIdx.x == 0 && base + SCAN_TILE >= n) *d_total = Op::f(prefix, total);
+}
+
+template <class Op>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_tile_reduce_op(const uint32_t *in, int64_t n, uint32_t *sums) {
+    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE + 1];
+    tile_reduce_body<Op>(in, n, sums, blockIdx.x, s_wave);
+}
+
+template <class Op, bool INCL>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan_op(const uint32_t *in, uint32_t *out, int64_t n,
+                                                              const uint32_t *sums, const uint32_t *pre,
+                                                              uint32_t *d_total) {
+    __shared__ uint32_t s_data[SCAN_TILE];
+    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE + 1];
+    tile_scan_body<Op, INCL>(in, out, n, sums, pre, d_total, blockIdx.x, s_data, s_wave);
+}
+
+// Several independent arrays scanned by the same two launches: blockIdx.y
+// picks the array, blocks past its last tile return at once.
+constexpr int SCAN_SET = 4;
+struct ScanItem {
+    const uint32_t *in;
+    uint32_t *out;
+    int64_t n;
+    uint32_t *d_total; // or null
+    uint32_t *sums;    // its tiles' values
+};
+struct ScanSet {
+    ScanItem it[SCAN_SET];
+};
+
+template <class Op> __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_reduce_set(ScanSet S) {
+    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE + 1];
+    const ScanItem q = S.it[blockIdx.y];
+    if (int64_t(blockIdx.x) * SCAN_TILE >= q.n) return;
+    tile_reduce_body<Op>(q.in, q.n, q.sums, blockIdx.x, s_wave);
+}
+
+template <class Op, bool INCL> __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan_set(ScanSet S) {
+    __shared__ uint32_t s_data[SCAN_TILE];
+    __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE + 1];
+    const ScanItem q = S.it[blockIdx.y];
+    if (int64_t(blockIdx.x) * SCAN_TILE >= q.n) return;
+    tile_scan_body<Op, INCL>(q.in, q.out, q.n, q.sums, nullptr, q.d_total, blockIdx.x, s_data, s_wave);
 }
 
 // Per host thread and stream, intentionally never freed: releasing device
@@ -547,6 +588,43 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
     }
 }
 
+// up to SCAN_SET arrays of at most PREF_TILES tiles each in two launches (one
+// when every array fits one tile); longer arrays take scan_u32_op
+template <class Op, bool INCL>
+void scan_u32_set(int count, const uint32_t *const *in, uint32_t *const *out, const int64_t *n,
+                  uint32_t *const *d_total, hipStream_t s) {
+    SC_REQUIRE(count >= 1 && count <= SCAN_SET, SC_EINVAL, "scan set size");
+    ScanSet S{};
+    int m = 0;
+    int64_t maxnb = 0, nsum = 0;
+    for (int k = 0; k < count; ++k) {
+        const int64_t nb = ceil_div(n[k], SCAN_TILE);
+        if (n[k] <= 0 || nb > PREF_TILES) {
+            scan_u32_op<Op, INCL>(in[k], out[k], n[k], d_total[k], s);
+            continue;
+        }
+        S.it[m].in = in[k];
+        S.it[m].out = out[k];
+        S.it[m].n = n[k];
+        S.it[m].d_total = d_total[k];
+        S.it[m].sums = reinterpret_cast<uint32_t *>(uintptr_t(nsum)); // offset, placed below
+        nsum += nb;
+        maxnb = nb > maxnb ? nb : maxnb;
+        ++m;
+    }
+    if (m == 0) return;
+    if (maxnb > 1) {
+        uint32_t *sums = scan_scratch<uint32_t>(s).sums.ensure(size_t(nsum));
+        for (int k = 0; k < m; ++k) S.it[k].sums = sums + uintptr_t(S.it[k].sums);
+        k_tile_reduce_set<Op><<<dim3(unsigned(maxnb), unsigned(m)), SCAN_BLOCK, 0, s>>>(S);
+        SC_HIP(hipGetLastError());
+    } else {
+        for (int k = 0; k < m; ++k) S.it[k].sums = nullptr; // one tile each: no earlier tiles
+    }
+    k_tile_scan_set<Op, INCL><<<dim3(unsigned(maxnb), unsigned(m)), SCAN_BLOCK, 0, s>>>(S);
+    SC_HIP(hipGetLastError());
+}
+
 } // namespace
 
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total, hipStream_t s) {
@@ -592,6 +670,21 @@ void FillBatch::flush() {
 
 void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s) {
     if (n > 0) scan_u32_op<OpMax, true>(in, out, n, nullptr, s);
+}
+void scan_exclusive_u32_2(const uint32_t *in0, uint32_t *out0, int64_t n0, uint32_t *t0, const uint32_t *in1,
+                          uint32_t *out1, int64_t n1, uint32_t *t1, hipStream_t s) {
+    const uint32_t *in[2] = {in0, in1};
+    uint32_t *out[2] = {out0, out1}, *tot[2] = {t0, t1};
+    const int64_t n[2] = {n0, n1};
+    scan_u32_set<OpSum, false>(2, in, out, n, tot, s);
+}
+void scan_inclusive_max_u32_2(const uint32_t *in0, uint32_t *out0, int64_t n0, const uint32_t *in1,
+                              uint32_t *out1, int64_t n1, hipStream_t s) {
+    const uint32_t *in[2] = {in0, in1};
+    uint32_t *out[2] = {out0, out1}, *tot[2] = {nullptr, nullptr};
+    const int64_t n[2] = {n0, n1};
+    if (n0 <= 0 && n1 <= 0) return;
+    scan_u32_set<OpMax, true>(2, in, out, n, tot, s);
 }
 
 namespace {

@@ -740,10 +740,9 @@ bool clip_small_batch(Ctx &c, int k0, int k1) {
     k_small_pairs<<<small_grid(c, J), WAVE, 0, s>>>(a);
     SC_HIP(hipGetLastError());
     int *voff = S.voff.ensure(size_t(J) + 1), *roff = S.roff.ensure(size_t(J) + 1);
-    scan_exclusive_u32(reinterpret_cast<const uint32_t *>(a.job_nv), reinterpret_cast<uint32_t *>(voff), J + 1,
-                       nullptr, s);
-    scan_exclusive_u32(reinterpret_cast<const uint32_t *>(a.job_nr), reinterpret_cast<uint32_t *>(roff), J + 1,
-                       nullptr, s);
+    scan_exclusive_u32_2(reinterpret_cast<const uint32_t *>(a.job_nv), reinterpret_cast<uint32_t *>(voff), J + 1,
+                         nullptr, reinterpret_cast<const uint32_t *>(a.job_nr), reinterpret_cast<uint32_t *>(roff),
+                         J + 1, nullptr, s);
     // one read-back: ring offsets per job, vertex total, fallback reasons, routes
     JobSet &js = c.jobs;
     js.ring_off.assign(size_t(J) + 1, 0);

@@ -758,8 +758,7 @@ void find_candidates(Ctx &c) {
     }
     uint32_t *tot = c.d_u32_total.ensure(2);
     uint64_t *tot64 = c.d_u64_total.ensure(2); // later: the candidate counter
-    scan_exclusive_u32(units, units, int64_t(T) + 1, tot, s);
-    scan_exclusive_u32(srows, srows, int64_t(T) + 1, tot + 1, s);
+    scan_exclusive_u32_2(units, units, int64_t(T) + 1, tot, srows, srows, int64_t(T) + 1, tot + 1, s);
     scan_exclusive_u64(tests, tests, int64_t(T) + 1, tot64, s);
     uint32_t tot2[2] = {0, 0};
     uint64_t ntests = 0;
@@ -773,19 +772,25 @@ void find_candidates(Ctx &c) {
     c.n_tests = int64_t(ntests);
     c.n_units = nunits;
     uint32_t *inode = c.srow_node.ensure(size_t(nitems) + 1);
-    if (nitems > 0) {
-        SC_HIP(hipMemsetAsync(inode, 0, sizeof(uint32_t) * nitems, s));
-        k_unit_heads<<<grid_for(T), BLK, 0, s>>>(T, srows, inode);
-        SC_HIP(hipGetLastError());
-        scan_inclusive_max_u32(inode, inode, nitems, s);
-    }
     uint32_t *unode = c.u32c.ensure(size_t(nunits) + 1);
-    if (nunits > 0) {
-        SC_HIP(hipMemsetAsync(unode, 0, sizeof(uint32_t) * nunits, s));
-        k_unit_heads<<<grid_for(T), BLK, 0, s>>>(T, units, unode);
-        SC_HIP(hipGetLastError());
-        scan_inclusive_max_u32(unode, unode, nunits, s);
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(c.d_u64_total.p);
+    unsigned long long *ncnt =
+        reinterpret_cast<unsigned long long *>(c.near_cnt.ensure(size_t(NEAR_SEGS) * NEAR_PAD + 2));
+    unsigned long long *summary = ncnt + size_t(NEAR_SEGS) * NEAR_PAD;
+    // the node maps and the stab's counters start at zero (one fill launch;
+    // the totals above are already read back)
+    {
+        FillBatch fb(s);
+        fb.add(inode, nitems, 0u);
+        fb.add(unode, nunits, 0u);
+        fb.add(counter, 2, 0u);
+        fb.add(ncnt, 2 * (int64_t(NEAR_SEGS) * NEAR_PAD + 2), 0u);
+        fb.flush();
     }
+    if (nitems > 0) k_unit_heads<<<grid_for(T), BLK, 0, s>>>(T, srows, inode);
+    if (nunits > 0) k_unit_heads<<<grid_for(T), BLK, 0, s>>>(T, units, unode);
+    SC_HIP(hipGetLastError());
+    scan_inclusive_max_u32_2(inode, inode, nitems, unode, unode, nunits, s);
     UnitDesc *desc = reinterpret_cast<UnitDesc *>(c.unit_desc.ensure(size_t(nunits > 0 ? nunits : 1) * 4));
     if (nunits > 0) {
         k_unit_desc<<<grid_for(nunits), BLK, 0, s>>>(nunits, unode, units, c.cov_off, c.cov_split, c.lend_off,
@@ -795,10 +800,6 @@ void find_candidates(Ctx &c) {
 
     if (c.cand_cap == 0) c.cand_cap = std::max<int64_t>(1 << 20, 4 * int64_t(c.ne));
     if (c.near_seg_cap == 0) c.near_seg_cap = std::max<int64_t>(1 << 13, 16 * int64_t(c.ne) / NEAR_SEGS);
-    unsigned long long *counter = reinterpret_cast<unsigned long long *>(c.d_u64_total.p);
-    unsigned long long *ncnt =
-        reinterpret_cast<unsigned long long *>(c.near_cnt.ensure(size_t(NEAR_SEGS) * NEAR_PAD + 2));
-    unsigned long long *summary = ncnt + size_t(NEAR_SEGS) * NEAR_PAD;
     int cus = 256, v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && v > 0) cus = v;
     // a buffer that overflows is counted in full: it is resized to the count
@@ -808,8 +809,12 @@ void find_candidates(Ctx &c) {
         c.near.ensure(size_t(c.near_seg_cap) * NEAR_SEGS);
         c.cand.ensure(size_t(c.cand_cap));
         c.cand_alt.ensure(size_t(c.cand_cap));
-        SC_HIP(hipMemsetAsync(counter, 0, sizeof(uint64_t), s));
-        SC_HIP(hipMemsetAsync(ncnt, 0, sizeof(uint64_t) * (size_t(NEAR_SEGS) * NEAR_PAD + 2), s));
+        if (attempt > 0) { // the counters of the run that overflowed
+            FillBatch fb(s);
+            fb.add(counter, 2, 0u);
+            fb.add(ncnt, 2 * (int64_t(NEAR_SEGS) * NEAR_PAD + 2), 0u);
+            fb.flush();
+        }
         StabArgs a;
         a.desc = desc; a.cov_e = c.cov_e; a.lend_e = c.lend_e;
         a.lend_bl = c.lend_bl;
