@@ -22,9 +22,24 @@ Ctx::~Ctx() {
 namespace {
 struct PinnedStage {
     void *p = nullptr;
+    char *dev = nullptr; // the same memory as seen by the device
     size_t cap = 0;
 };
 thread_local PinnedStage tl_pin; // never freed: a read-back has finished with it when it returns
+
+// A read-back's items copied into the pinned stage by one launch (the
+// device's stores reach host memory before the stream reports completion).
+struct GatherArgs {
+    const uint32_t *src[RB_MAX];
+    uint32_t *dst[RB_MAX];
+    uint32_t words[RB_MAX];
+    int n;
+};
+__global__ void k_readback(GatherArgs a) {
+    for (int q = 0; q < a.n; ++q)
+        for (uint32_t i = threadIdx.x; i < a.words[q]; i += blockDim.x) a.dst[q][i] = a.src[q][i];
+}
+constexpr size_t RB_GATHER_MAX = 65536; // larger items go through the copy engine path
 } // namespace
 void *Ctx::pinned(size_t bytes) {
     PinnedStage &ps = tl_pin;
@@ -34,6 +49,9 @@ void *Ctx::pinned(size_t bytes) {
         ps.cap = 0;
         const size_t want = bytes < 65536 ? 65536 : bytes + bytes / 4;
         SC_HIP(hipHostMalloc(&ps.p, want, hipHostMallocDefault));
+        void *dp = nullptr;
+        SC_HIP(hipHostGetDevicePointer(&dp, ps.p, 0));
+        ps.dev = static_cast<char *>(dp);
         ps.cap = want;
     }
     return ps.p;
@@ -49,10 +67,24 @@ void ReadBack::run() {
     size_t tot = 0;
     for (int i = 0; i < n; ++i) tot += (bytes[i] + 15) & ~size_t(15);
     char *st = static_cast<char *>(c.pinned(tot));
+    char *sd = tl_pin.dev;
+    GatherArgs g;
+    g.n = 0;
     size_t at = 0;
     for (int i = 0; i < n; ++i) {
-        if (bytes[i]) SC_HIP(hipMemcpyAsync(st + at, src[i], bytes[i], hipMemcpyDeviceToHost, c.stream));
+        if (bytes[i] && bytes[i] % 4 == 0 && bytes[i] <= RB_GATHER_MAX && (uintptr_t(src[i]) & 3) == 0) {
+            g.src[g.n] = static_cast<const uint32_t *>(src[i]);
+            g.dst[g.n] = reinterpret_cast<uint32_t *>(sd + at);
+            g.words[g.n] = uint32_t(bytes[i] / 4);
+            ++g.n;
+        } else if (bytes[i]) {
+            SC_HIP(hipMemcpyAsync(st + at, src[i], bytes[i], hipMemcpyDeviceToHost, c.stream));
+        }
         at += (bytes[i] + 15) & ~size_t(15);
+    }
+    if (g.n) {
+        k_readback<<<1, 256, 0, c.stream>>>(g);
+        SC_HIP(hipGetLastError());
     }
     c.wait();
     at = 0;
