@@ -2245,7 +2245,11 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
     int *rstart = f.rstart.ensure(nr);
     int *ringid = f.ringid.ensure(size_t(n));
     k_root_loop<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, f.ox, f.oy, rstart);
-    SC_HIP(hipMemsetAsync(ringid, 0, sizeof(int) * size_t(n), s));
+    {
+        FillBatch fb(s);
+        fb.add(ringid, n, 0u);
+        fb.flush();
+    }
     k_ring_heads<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, reinterpret_cast<uint32_t *>(ringid));
     scan_inclusive_max_u32(reinterpret_cast<uint32_t *>(ringid), reinterpret_cast<uint32_t *>(ringid), n, s);
     uint32_t *keep = f.keep.ensure(size_t(n) + 1);
@@ -2383,7 +2387,11 @@ void label_and_traverse(Ctx &c) {
     const Side Q{AC, AB, c.aug_cx, c.aug_cy, c.lab_c, c.nbr_c, nullptr};
     // per-job words
     int *jw = f.jst.ensure(11 * int64_t(J) + 4);
-    SC_HIP(hipMemsetAsync(jw, 0, sizeof(int) * (11 * size_t(J) + 4), s));
+    {
+        FillBatch fb(s);
+        fb.add(jw, 11 * int64_t(J) + 4, 0u);
+        fb.flush();
+    }
     JobState st;
     st.J = J;
     st.uni = c.uni;
@@ -2719,7 +2727,11 @@ void label_and_traverse(Ctx &c) {
     if (maxm > 0 && NV > 0) {
         k_jring_offsets<<<grid_for(mP), BLK, 0, s>>>(mP, f.isstart, ridx, base, pa.pj, st.cP0, jv, jr, off);
         uint32_t *head = f.head.ensure(NV + 1), *piece_at = f.piece_at.ensure(NV + 1);
-        SC_HIP(hipMemsetAsync(head, 0, sizeof(uint32_t) * size_t(NV + 1), s));
+        {
+            FillBatch fb(s);
+            fb.add(head, NV + 1, 0u);
+            fb.flush();
+        }
         k_jheads<<<grid_for(M), BLK, 0, s>>>(M, inode, dist, base, pa.pj, st.cP0, jv, head, piece_at);
         SC_HIP(hipGetLastError());
         scan_inclusive_max_u32(head, head, NV, s);

@@ -878,7 +878,11 @@ void find_candidates(Ctx &c) {
     const int64_t n = c.n_cand;
     radix_sort_u64_buf(c.cand, c.cand_alt, nullptr, nullptr, n, 0, kb, c.rs, s);
     uint32_t *flag = c.u32b.ensure(size_t(n) + 1);
-    SC_HIP(hipMemsetAsync(flag, 0, sizeof(uint32_t) * (size_t(n) + 1), s));
+    {
+        FillBatch fb(s);
+        fb.add(flag, n + 1, 0u);
+        fb.flush();
+    }
     if (n > 0) {
         k_diff_flags64<<<grid_for(n), BLK, 0, s>>>(c.cand, n, flag);
         SC_HIP(hipGetLastError());

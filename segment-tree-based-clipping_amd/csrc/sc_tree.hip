@@ -572,7 +572,13 @@ void build_elementary(Ctx &c) {
     radix_sort_u64(keys, sidx, m, 0, 64, c.rs, s);
     // vertical-edge bookkeeping
     uint32_t *vl = c.u32b.ensure(ne + 1);
-    SC_HIP(hipMemsetAsync(vl, 0, sizeof(uint32_t) * (ne + 1), s));
+    uint32_t *flag = c.u32d.ensure(m + 1);
+    {
+        FillBatch fb(s); // one launch for both (a memset of an odd size is two)
+        fb.add(vl, ne + 1, 0u);
+        fb.add(flag, m + 1, 0u);
+        fb.flush();
+    }
     k_vertical_flags<<<grid_for(ne), BLK, 0, s>>>(sidx, c.estart, c.eend, ne, vl);
     SC_HIP(hipGetLastError());
     scan_exclusive_u32(vl, vl, ne + 1, nullptr, s);
@@ -580,9 +586,7 @@ void build_elementary(Ctx &c) {
     k_vertical_loc<<<grid_for(ne), BLK, 0, s>>>(vl, ne, vloc);
     SC_HIP(hipGetLastError());
     // unique flags, compaction
-    uint32_t *flag = c.u32d.ensure(m + 1);
     double *val3 = reinterpret_cast<double *>(c.u64b.ensure(m));
-    SC_HIP(hipMemsetAsync(flag, 0, sizeof(uint32_t) * (m + 1), s));
     k_unique_flags<<<grid_for(m), BLK, 0, s>>>(sidx, c.estart, c.eend, ne, vl, vloc, flag, val3);
     SC_HIP(hipGetLastError());
     uint32_t *tot = c.d_u32_total.ensure(1);
