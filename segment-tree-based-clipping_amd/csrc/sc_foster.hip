@@ -2260,10 +2260,13 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
     const int64_t ntiles = (n + KEEP_TILE - 1) / KEEP_TILE;
     uint32_t *ctl = f.kctl.ensure(KEEP_ROUNDS + 1); // changed flags, error word
     unsigned long long *kst = f.kstatus.ensure(size_t(ntiles));
+    uint32_t *nonempty = f.nonempty.ensure(nr + 1);
     {
         FillBatch fb(s);
         fb.add(ctl, KEEP_ROUNDS + 1, 0u);
         fb.add(kst, 2 * ntiles, 0u);
+        fb.add(keep + n, 1, 0u);         // closing entries of the two scans below
+        fb.add(nonempty + nr, 1, 0u);
         fb.flush();
     }
     KeepRound kr;
@@ -2298,18 +2301,16 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
         SC_HIP(hipGetLastError());
     }
     // compaction
-    SC_HIP(hipMemsetAsync(keep + n, 0, sizeof(uint32_t), s));
     uint32_t *kscan = f.kscan.ensure(size_t(n) + 1);
     uint32_t *tot = c.d_u32_total.ensure(2);
     scan_exclusive_u32(keep, kscan, n + 1, tot, s);
-    uint32_t *rcnt = f.rcnt.ensure(nr + 1), *nonempty = f.nonempty.ensure(nr + 1);
+    uint32_t *rcnt = f.rcnt.ensure(nr + 1);
     k_ring_counts<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, kscan, rcnt);
     SC_HIP(hipGetLastError());
     // surviving rings: exclusive scan of the non-empty flags; both totals and
     // the jobs' first rings come back in one copy
     k_nonempty<<<grid_for(nr), BLK, 0, s>>>(nr, rcnt, nonempty);
     SC_HIP(hipGetLastError());
-    SC_HIP(hipMemsetAsync(nonempty + nr, 0, sizeof(uint32_t), s));
     scan_exclusive_u32(nonempty, nonempty, int64_t(nr) + 1, tot + 1, s);
     uint32_t *jring = f.jring.ensure(size_t(J) + 3);
     k_job_rings<<<grid_for(int64_t(J) + 1), BLK, 0, s>>>(J, jr, nonempty, jring);

@@ -334,6 +334,7 @@ __global__ void k_cover(const double *__restrict__ es, const double *__restrict_
                         const double *__restrict__ ea, int epc, int nleaves, uint32_t *__restrict__ cnt,
                         const uint32_t *__restrict__ off, uint64_t *__restrict__ keys, int2 *__restrict__ lr) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!EMIT && e == 0) cnt[ne] = 0u; // the count scan's closing entry
     if (e >= ne) return;
     int L, R;
     if (!EMIT) {
@@ -691,7 +692,6 @@ void build_cover_lists(Ctx &c) {
     k_cover<false><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, cnt, nullptr,
                                                 nullptr, lr);
     SC_HIP(hipGetLastError());
-    SC_HIP(hipMemsetAsync(cnt + ne, 0, sizeof(uint32_t), s));
     uint32_t *tot = c.d_u32_total.ensure(1);
     scan_exclusive_u32(cnt, cnt, int64_t(ne) + 1, tot, s);
     c.n_cov = read_u32(c, tot);
