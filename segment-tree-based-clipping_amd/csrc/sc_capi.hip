@@ -121,6 +121,7 @@ void release_derived(Ctx &c) {
         b->release();
     for (DBuf<YRangeF> *b : {&c.eyr, &c.lend_yr, &c.cov_yr}) b->release();
     c.lend_bl.release();
+    c.ep_lohi.release();
     c.exy.release();
     c.near.release();
     c.near_cnt.release();

@@ -60,6 +60,8 @@ struct Ctx {
     // ---- A3 elementary intervals ----
     DBuf<double> ea;
     int epc = 0, nleaves = 0, T = 0, depth = 0;
+    int ea_count = 0;                    // elementary points before padding
+    DBuf<uint2> ep_lohi;                 // per endpoint: its elementary index range, or NIL (search)
     // ---- A4 ----
     DBuf<double> niv;
     // ---- A6 leaf end entries, split by class (0 base, 1 overlay) ----

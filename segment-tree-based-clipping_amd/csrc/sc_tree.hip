@@ -227,6 +227,33 @@ __global__ void k_scatter_elementary(const uint32_t *__restrict__ pos, const dou
     if (pos[i] != pos[i + 1]) ea[pos[i]] = val3[i];
 }
 
+// Per endpoint (2e: the start of edge e, 2e + 1: its end) the elementary
+// indices of its value: lo = the first index with ea[i] >= x, hi = the first
+// with ea[i] > x, from its run of equal values among the sorted endpoints and
+// the exclusive scan of the unique flags (pos).  Every run but the last two
+// ends in a flag carrying its own value (the last flag carries the value
+// before it, see k_unique_flags), so ea[lo] == x and nothing after hi equals
+// x.  ER_NIL where the binary search must decide instead: the last two
+// runs, NaN, runs longer than ER_WALK either side.
+constexpr int ER_WALK = 8;
+constexpr uint32_t ER_NIL = 0xffffffffu;
+__global__ void k_endpoint_ranges(int64_t m, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ sidx,
+                                  const uint32_t *__restrict__ pos, uint2 *lohi) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint64_t k = keys[j], klast = keys[m - 1];
+    // order keys of NaN: exponent all ones and a non-zero mantissa, either sign
+    const uint64_t bits = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    bool ok = !((bits & 0x7ff0000000000000ull) == 0x7ff0000000000000ull && (bits & 0x000fffffffffffffull));
+    int64_t j0 = j, j1 = j;
+    for (int t = 0; t < ER_WALK && j0 > 0 && keys[j0 - 1] == k; ++t) --j0;
+    for (int t = 0; t < ER_WALK && j1 + 1 < m && keys[j1 + 1] == k; ++t) ++j1;
+    if (j0 > 0 && keys[j0 - 1] == k) ok = false;
+    if (j1 + 1 < m && keys[j1 + 1] == k) ok = false;
+    if (j1 + 1 >= m || keys[j1 + 1] == klast) ok = false; // the last run or the one before it
+    lohi[sidx[j]] = ok ? make_uint2(pos[j0], pos[j1 + 1]) : make_uint2(ER_NIL, ER_NIL);
+}
+
 __global__ void k_pad_elementary(double *ea, int cnt, int round) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (cnt + i >= round) return;
@@ -287,13 +314,22 @@ __device__ inline void first_point2(const double *ea, int epc, double q1, double
 // its leaf end entries; sentinel key 2*nleaves marks "not inserted"
 __global__ void k_leaf_targets(const double *__restrict__ es, const double *__restrict__ ee,
                                const int *__restrict__ ety, int ne, const double *__restrict__ ea, int epc,
-                               int nleaves, int *after, int *before, uint64_t *keys, uint32_t *vals) {
+                               int nleaves, const uint2 *__restrict__ lohi, int *after, int *before,
+                               uint64_t *keys, uint32_t *vals) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     const int half = nleaves, T = 2 * nleaves;
     int a = -1, b = -1;
     int ie, is;
-    first_point2(ea, epc, ee[e], es[e], ie, is);
+    const uint2 rE = lohi[2 * int64_t(e) + 1], rS = lohi[2 * int64_t(e)];
+    if (rE.x != ER_NIL && rS.x != ER_NIL) {
+        // the search over [1, epc) finds lo when lo >= 1; for the first point
+        // it looks at index 1 (epc >= 2)
+        ie = rE.x >= 1u ? int(rE.x) : (ea[1] == ee[e] ? 1 : -1);
+        is = rS.x >= 1u ? int(rS.x) : (ea[1] == es[e] ? 1 : -1);
+    } else {
+        first_point2(ea, epc, ee[e], es[e], ie, is);
+    }
     if (ie >= 0) {
         int rl = half + ie;
         if (ie != 1 && ie + 1 <= epc - 1 && ea[ie] == ea[ie + 1]) rl = half + ie + 1;
@@ -331,14 +367,23 @@ __global__ void k_key_ends(const uint64_t *__restrict__ keys, int64_t n, uint64_
 // keeps L,R), EMIT = true writes (node, class) keys edge by edge.
 template <bool EMIT>
 __global__ void k_cover(const double *__restrict__ es, const double *__restrict__ ee, int ne, int nb,
-                        const double *__restrict__ ea, int epc, int nleaves, uint32_t *__restrict__ cnt,
-                        const uint32_t *__restrict__ off, uint64_t *__restrict__ keys, int2 *__restrict__ lr) {
+                        const double *__restrict__ ea, int epc, int nleaves, const uint2 *__restrict__ lohi,
+                        int ecount, uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
+                        uint64_t *__restrict__ keys, int2 *__restrict__ lr) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (!EMIT && e == 0) cnt[ne] = 0u; // the count scan's closing entry
     if (e >= ne) return;
     int L, R;
     if (!EMIT) {
-        search2<true>(ea, 0, epc, es[e], ee[e], L, R);
+        const uint2 rS = lohi[2 * int64_t(e)], rE = lohi[2 * int64_t(e) + 1];
+        if (rS.x != ER_NIL && rE.x != ER_NIL) {
+            // first index >= start; first index > end, the padding (copies
+            // of the last point) counted when nothing real is larger
+            L = int(rS.x);
+            R = rE.y == uint32_t(ecount) ? epc : int(rE.y);
+        } else {
+            search2<true>(ea, 0, epc, es[e], ee[e], L, R);
+        }
         R -= 1; // last point <= end
         lr[e] = make_int2(L, R);
     } else {
@@ -453,7 +498,7 @@ void layout_rings(Ctx &c, const double *src, int64_t nverts, const std::vector<i
 }
 
 void reset_derived(Ctx &c) {
-    c.epc = c.nleaves = c.T = 0;
+    c.epc = c.nleaves = c.T = c.ea_count = 0;
     c.n_lend = c.n_cov = c.n_cand = c.n_pairs = c.n_mend = c.n_near = 0;
     c.cur_k = 0;
     c.pair_off.clear();
@@ -592,6 +637,8 @@ void build_elementary(Ctx &c) {
     SC_HIP(hipGetLastError());
     uint32_t *tot = c.d_u32_total.ensure(1);
     scan_exclusive_u32(flag, flag, m + 1, tot, s);
+    k_endpoint_ranges<<<grid_for(m), BLK, 0, s>>>(m, keys, sidx, flag, c.ep_lohi.ensure(size_t(m)));
+    SC_HIP(hipGetLastError());
     int cnt = int(read_u32(c, tot));
     SC_REQUIRE(cnt >= 1, SC_EINVAL, "degenerate input: fewer than two distinct x values");
     int round = next_pow2_int(cnt) + 1;
@@ -603,6 +650,7 @@ void build_elementary(Ctx &c) {
         SC_HIP(hipGetLastError());
     }
     c.epc = round;
+    c.ea_count = cnt;
     c.nleaves = round - 1;
     c.T = 2 * c.nleaves;
     c.depth = ilog2_exact(c.T);
@@ -654,8 +702,8 @@ void build_leaf_end_entries(Ctx &c) {
     c.before_leaf.ensure(ne);
     uint64_t *keys = c.lend_key.ensure(m);
     uint32_t *vals = c.lend_e.ensure(m);
-    k_leaf_targets<<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, c.etype, ne, c.ea, c.epc, nl, c.after_leaf,
-                                                c.before_leaf, keys, vals);
+    k_leaf_targets<<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, c.etype, ne, c.ea, c.epc, nl, c.ep_lohi,
+                                                c.after_leaf, c.before_leaf, keys, vals);
     SC_HIP(hipGetLastError());
     int kb = ilog2_exact(2 * int64_t(nl) + 1);
     radix_sort_u64_buf(c.lend_key, c.lend_key_alt, &c.lend_e, &c.lend_e_alt, m, 0, kb, c.rs, s);
@@ -689,8 +737,8 @@ void build_cover_lists(Ctx &c) {
     // per-edge entry counts -> edge-major offsets
     uint32_t *cnt = c.u32b.ensure(size_t(ne) + 1);
     int2 *lr = reinterpret_cast<int2 *>(c.u64b.ensure(ne));
-    k_cover<false><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, cnt, nullptr,
-                                                nullptr, lr);
+    k_cover<false><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, c.ep_lohi,
+                                                c.ea_count, cnt, nullptr, nullptr, lr);
     SC_HIP(hipGetLastError());
     uint32_t *tot = c.d_u32_total.ensure(1);
     scan_exclusive_u32(cnt, cnt, int64_t(ne) + 1, tot, s);
@@ -702,8 +750,8 @@ void build_cover_lists(Ctx &c) {
     uint64_t *keys = c.cov_key.ensure(size_t(L > 0 ? L : 1));
     c.cov_e.ensure(size_t(L > 0 ? L : 1));
     if (L > 0) {
-        k_cover<true><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, nullptr, cnt,
-                                                   keys, lr);
+        k_cover<true><<<grid_for(ne), BLK, 0, s>>>(c.estart, c.eend, ne, c.nb, c.ea, c.epc, c.nleaves, c.ep_lohi,
+                                                   c.ea_count, nullptr, cnt, keys, lr);
         SC_HIP(hipGetLastError());
         const int kb = ilog2_exact(T) + 1;
         SC_REQUIRE(kb <= 32, SC_ERANGE, "cover key too wide");

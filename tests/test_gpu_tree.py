@@ -21,9 +21,26 @@ def _bits_equal(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
+def _comb(teeth):
+    """A ring whose zigzag puts `teeth` vertices at x = 1 and as many at x = 2
+    (runs of equal endpoint values longer than the library's short walk),
+    crossed by a tilted rectangle."""
+    pts = [(0.0, 0.0), (4.0, 0.0), (4.0, float(teeth) + 1.0)]
+    for i in range(teeth, 0, -1):
+        pts.append((2.0, float(i) + 0.5))
+        pts.append((1.0, float(i)))
+    base = np.array(pts, np.float64)
+    th = 0.3
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    rect = np.array([[-0.5, -0.2], [3.5, -0.2], [3.5, 0.6], [-0.5, 0.6]]) @ R.T + np.array([0.2, 2.1])
+    return base, rect
+
+
 def _cases():
     out = [(n, b, o) for n, b, o in workloads_small()]
     out += [(n, b, o) for n, b, o, _ in degenerate_cases()]
+    b, r = _comb(24)
+    out.append(("comb24", b, [r]))
     return out
 
 
