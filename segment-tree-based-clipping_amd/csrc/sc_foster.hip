@@ -750,42 +750,58 @@ struct KeepRound {
     unsigned long long epoch;   // 1..KEEP_ROUNDS, shifted into place
     uint32_t *changed;          // set when a flag changes
     uint32_t *err;              // set when a wait gives up (the result is then invalid)
-    uint32_t *tstate;           // per tile: flags changed in its last round, the prefix that round saw
+    uint32_t *tstate;           // per tile: flags changed in its last round, the prefix that round saw, its largest mark
     int first;                  // the first round: every tile decides
 };
 
 __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
-    __shared__ uint32_t s_prefix, s_skip, s_ch;
+    __shared__ uint32_t s_prefix, s_skip, s_ch, s_fast, s_agg;
     __shared__ uint32_t s_wmax[BLK / WAVE];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
     const uint32_t tile = blockIdx.x;
     const int64_t i0 = int64_t(tile) * KEEP_TILE + int64_t(tid) * KEEP_ITEMS;
-    uint32_t kp[KEEP_ITEMS];
-    uint32_t mine = 0; // marks grow with the index: the thread's largest is its last kept vertex
-#pragma unroll
-    for (int it = 0; it < KEEP_ITEMS; ++it) {
-        const int64_t i = i0 + it;
-        kp[it] = i < a.n ? a.keep[i] : 0u;
-        if (kp[it]) mine = uint32_t(i) + 1u;
+    // a tile whose flags did not change in its last round still has the
+    // largest mark it computed then: it publishes that without reading its
+    // flags, and reads them only if its incoming prefix changed
+    if (tid == 0) {
+        const bool f = !a.first && a.tstate[3 * tile] == 0u;
+        s_fast = f ? 1u : 0u;
+        s_agg = f ? a.tstate[3 * tile + 2] : 0u;
     }
-    // exclusive max over the threads before this one, and the tile's maximum
-    uint32_t inc = mine;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        const uint32_t y = __shfl_up(inc, d, WAVE);
-        if (lane >= d) inc = inc > y ? inc : y;
-    }
-    uint32_t ex = __shfl_up(inc, 1, WAVE);
-    if (lane == 0) ex = 0;
-    if (lane == WAVE - 1) s_wmax[w] = inc;
     __syncthreads();
-    uint32_t agg = 0;
+    const bool fast = s_fast != 0u;
+    uint32_t kp[KEEP_ITEMS];
+    uint32_t ex = 0, agg = 0;
+    // the tile's flags, the largest mark before each thread's vertices and
+    // the tile's largest (every thread of the block calls it)
+    auto load_scan = [&]() {
+        uint32_t mine = 0; // marks grow with the index: the thread's largest is its last kept vertex
 #pragma unroll
-    for (int q = 0; q < BLK / WAVE; ++q) {
-        const uint32_t m = s_wmax[q];
-        if (q < w) ex = ex > m ? ex : m;
-        agg = agg > m ? agg : m;
-    }
+        for (int it = 0; it < KEEP_ITEMS; ++it) {
+            const int64_t i = i0 + it;
+            kp[it] = i < a.n ? a.keep[i] : 0u;
+            if (kp[it]) mine = uint32_t(i) + 1u;
+        }
+        uint32_t inc = mine;
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const uint32_t y = __shfl_up(inc, d, WAVE);
+            if (lane >= d) inc = inc > y ? inc : y;
+        }
+        ex = __shfl_up(inc, 1, WAVE);
+        if (lane == 0) ex = 0;
+        if (lane == WAVE - 1) s_wmax[w] = inc;
+        __syncthreads();
+        agg = 0;
+#pragma unroll
+        for (int q = 0; q < BLK / WAVE; ++q) {
+            const uint32_t m = s_wmax[q];
+            if (q < w) ex = ex > m ? ex : m;
+            agg = agg > m ? agg : m;
+        }
+    };
+    if (fast) agg = s_agg;
+    else load_scan();
     // marks grow with the index, so a tile holding a kept vertex knows its
     // inclusive prefix already (its own largest mark) and publishes it as one;
     // a look-back then normally stops at the nearest predecessor
@@ -846,15 +862,16 @@ __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
             // a tile whose flags did not change in its last round and whose
             // incoming prefix is the same sees the same last-kept vertices:
             // its decisions, and so its flags, stay as they are
-            s_skip = !a.first && a.tstate[2 * tile] == 0u && a.tstate[2 * tile + 1] == prefix ? 1u : 0u;
+            s_skip = fast && a.tstate[3 * tile + 1] == prefix ? 1u : 0u;
             s_ch = 0u;
         }
     }
     __syncthreads();
     if (s_skip) {
-        if (tid == 0) a.tstate[2 * tile] = 0u;
+        if (tid == 0) a.tstate[3 * tile] = 0u;
         return;
     }
+    if (fast) load_scan(); // its maximum is the one published
     uint32_t cur = s_prefix > ex ? s_prefix : ex; // mark of the last vertex kept before i
     bool ch = false;
 #pragma unroll
@@ -881,8 +898,9 @@ __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
     }
     __syncthreads();
     if (tid == 0) {
-        a.tstate[2 * tile] = s_ch;
-        a.tstate[2 * tile + 1] = s_prefix;
+        a.tstate[3 * tile] = s_ch;
+        a.tstate[3 * tile + 1] = s_prefix;
+        a.tstate[3 * tile + 2] = agg;
     }
 }
 
@@ -2279,7 +2297,7 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
     kr.keep = keep;
     kr.status = kst;
     kr.err = ctl + KEEP_ROUNDS;
-    kr.tstate = f.ktstate.ensure(2 * size_t(ntiles)); // written by every tile in the first round
+    kr.tstate = f.ktstate.ensure(3 * size_t(ntiles)); // written by every tile in the first round
     bool converged = false;
     for (int it = 0; it < KEEP_ROUNDS && !converged;) {
         const int end = std::min(it + KEEP_BATCH, KEEP_ROUNDS);
