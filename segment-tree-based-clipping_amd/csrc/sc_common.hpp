@@ -161,6 +161,8 @@ __device__ inline int seg_of(const int *off, int n, int64_t i) {
 // total through *d_total when non-null (device pointer).
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total, hipStream_t s);
 void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *d_total, hipStream_t s);
+// Sum of n values into *d_total (device pointer).
+void sum_u64(const uint64_t *in, int64_t n, uint64_t *d_total, hipStream_t s);
 // Inclusive running maximum of n values (in == out allowed).
 void scan_inclusive_max_u32(const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s);
 // two independent arrays in the launches of one scan (same results as two calls)

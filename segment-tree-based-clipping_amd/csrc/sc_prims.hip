@@ -633,6 +633,19 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 void scan_exclusive_u64(const uint64_t *in, uint64_t *out, int64_t n, uint64_t *d_total, hipStream_t s) {
     scan_impl<uint64_t>(in, out, n, d_total, s, scan_scratch<uint64_t>(s));
 }
+void sum_u64(const uint64_t *in, int64_t n, uint64_t *d_total, hipStream_t s) {
+    if (n <= 0) {
+        SC_HIP(hipMemsetAsync(d_total, 0, sizeof(uint64_t), s));
+        return;
+    }
+    const int64_t nb = ceil_div(n, SCAN_TILE);
+    SC_REQUIRE(nb < (int64_t(1) << 31), SC_ERANGE, "sum too long");
+    uint64_t *sums = scan_scratch<uint64_t>(s).sums.ensure(size_t(nb));
+    k_tile_reduce<uint64_t><<<unsigned(nb), SCAN_BLOCK, 0, s>>>(in, n, sums);
+    SC_HIP(hipGetLastError());
+    k_scan_sums<uint64_t><<<1, SCAN_BLOCK, 0, s>>>(sums, nb, d_total); // the total; the scanned sums go unused
+    SC_HIP(hipGetLastError());
+}
 
 namespace {
 struct FillArgs {

@@ -759,7 +759,7 @@ void find_candidates(Ctx &c) {
     uint32_t *tot = c.d_u32_total.ensure(2);
     uint64_t *tot64 = c.d_u64_total.ensure(2); // later: the candidate counter
     scan_exclusive_u32_2(units, units, int64_t(T) + 1, tot, srows, srows, int64_t(T) + 1, tot + 1, s);
-    scan_exclusive_u64(tests, tests, int64_t(T) + 1, tot64, s);
+    sum_u64(tests, int64_t(T) + 1, tot64, s); // only the total (pair tests W) is used
     uint32_t tot2[2] = {0, 0};
     uint64_t ntests = 0;
     {
