@@ -445,19 +445,23 @@ __global__ void k_diff_flags(const uint64_t *__restrict__ k, int64_t n, uint32_t
     if (i == n - 1) f[n] = 0u;
 }
 
+// Keys are sorted by node, so the last key of each node's run writes where the
+// node's distinct entries end (pos after it); a max-scan over nodes then gives
+// every node's offset, empty nodes included. (A counter per node took one
+// same-address atomic per entry: the root's entries all serialised.)
 __global__ void k_mend_scatter(const uint64_t *__restrict__ k, const uint32_t *__restrict__ pos, int64_t n,
                                int idb, int tb, const int *__restrict__ eoff, const int *__restrict__ coff,
-                               uint32_t *out_e, uint32_t *cnt) {
+                               uint32_t *out_e, uint32_t *node_end) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (pos[i] == pos[i + 1]) return;
     uint64_t key = k[i];
+    int node = int(key >> (idb + tb));
+    if (i == n - 1 || int(k[i + 1] >> (idb + tb)) != node) node_end[node] = pos[i + 1];
+    if (pos[i] == pos[i + 1]) return;
     int type = int(key & ((1ull << tb) - 1ull));
     int id = int((key >> tb) & ((1ull << idb) - 1ull));
-    int node = int(key >> (idb + tb));
     int e = type == 0 ? id / 2 : eoff[type - 1] + (id / 2 - coff[type - 1]);
     out_e[pos[i]] = uint32_t(e);
-    atomicAdd(&cnt[node], 1u);
 }
 
 } // namespace
@@ -801,13 +805,14 @@ void materialize_end_lists(Ctx &c) {
     scan_exclusive_u32(flag, flag, n + 1, tot, s);
     c.n_mend = read_u32(c, tot);
     c.mend_e.ensure(size_t(c.n_mend));
-    uint32_t *cnt = c.mend_off.ensure(size_t(c.T) + 1);
-    SC_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (size_t(c.T) + 1), s));
+    // off[v + 1] = end of node v's entries; off[v] after the max-scan = entries of nodes < v
+    uint32_t *off = c.mend_off.ensure(size_t(c.T) + 1);
+    SC_HIP(hipMemsetAsync(off, 0, sizeof(uint32_t) * (size_t(c.T) + 1), s));
     if (n > 0) {
-        k_mend_scatter<<<grid_for(n), BLK, 0, s>>>(keys, flag, n, idb, tb, c.d_eoff, c.d_coff, c.mend_e, cnt);
+        k_mend_scatter<<<grid_for(n), BLK, 0, s>>>(keys, flag, n, idb, tb, c.d_eoff, c.d_coff, c.mend_e, off + 1);
         SC_HIP(hipGetLastError());
     }
-    scan_exclusive_u32(cnt, cnt, int64_t(c.T) + 1, nullptr, s);
+    scan_inclusive_max_u32(off + 1, off + 1, c.T, s);
 }
 
 } // namespace sc
