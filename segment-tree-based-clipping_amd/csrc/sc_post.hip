@@ -507,9 +507,9 @@ void post_jobs(Ctx &c, int k0, int k1) {
                              int nseg, DBuf<uint64_t> &K, DBuf<uint32_t> &wl, DBuf<uint32_t> &bl, uint32_t *ctl,
                              int *ord) {
             k_seg_scatter<<<grid_for(I), BLK, 0, s>>>(I, edge_of, c.p_j, JX, scan, arr, q, K.ensure(Ia));
-            // listed groups hold more than SEG_THREAD (wave list) or SEG_WAVE (block list) pairs each
+            // listed groups hold more than SEG_THREAD (wave list) or SEG_WAVE_MAX (block list) pairs each
             const int64_t nw = std::min<int64_t>(nseg, int64_t(I) / (SEG_THREAD + 1)) + 1;
-            const int64_t nb = std::min<int64_t>(nseg, int64_t(I) / (SEG_WAVE + 1)) + 1;
+            const int64_t nb = std::min<int64_t>(nseg, int64_t(I) / (SEG_WAVE_MAX + 1)) + 1;
             seg_sort<false>(nseg, scan, K, ord, wl.ensure(size_t(nw)), bl.ensure(size_t(nb)), ctl, s);
         };
         if (sparse) radix_order(true);

@@ -65,12 +65,14 @@ def _sawtooth(T):
     return np.array(pts, dtype=np.float64)
 
 
-@pytest.mark.parametrize("T", [40, 2500])
+@pytest.mark.parametrize("T", [32, 33, 40, 64, 65, 128, 129, 256, 257, 2048, 2500])
 @pytest.mark.parametrize("swap", [False, True])
 def test_many_intersections_on_one_edge(gpu, T, swap):
-    """2T intersections on each horizontal edge of a rectangle: per-edge groups
-    ranked by one block (80 pairs) and groups beyond the LDS limit (5000
-    pairs), on the base side and (swapped) on the overlay side."""
+    """2T intersections on each horizontal edge of a rectangle, on the base side
+    and (swapped) on the overlay side: per-edge groups sorted by one wave with
+    1, 2, 4 or 8 keys per lane (64 up to 512 pairs, each side of every
+    boundary), by one block (514 and 4096 pairs) and beyond the LDS limit
+    (5000 pairs)."""
     import segclip
     rect = np.array([(-0.5, 0.0), (1.5, 0.0), (1.5, 0.5), (-0.5, 0.5)], dtype=np.float64)
     saw = _sawtooth(T)
