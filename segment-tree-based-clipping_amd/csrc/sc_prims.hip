@@ -1,12 +1,13 @@
 // sc_prims.hip -- device-wide scan and stable LSD radix sort for gfx950.
 //
-// Scan: u32 sums and maxima in one launch per scan (a chained scan over
-// 2048-item tiles with a one-wave look-back); u64 sums and single-tile scans
-// as reduce / scan-of-block-sums / downsweep.  Wave64 shuffles inside a block.
+// Scan: u32 sums and maxima in two launches per scan (tile totals, then
+// each tile combines the totals of the tiles before it itself), up to four
+// arrays per launch pair; u64 scans as reduce / scan-of-block-sums /
+// downsweep.  Wave64 shuffles inside a block.
 //
-// Radix sort: 8-bit digits, 4096 keys per tile.  One pass over the keys first
-// builds every digit histogram at once so that passes whose digit is the same
-// for all keys are skipped.  The scatter ranks keys stably inside a tile with
+// Radix sort: 8-bit digits, 4096 keys per tile, one launch per pass.  One
+// pass over the keys first builds every digit histogram at once, so a pass
+// whose digit is the same for all keys only copies its tiles.  The scatter ranks keys stably inside a tile with
 // a 64-lane match (eight ballots per key) and per-wave digit counts in LDS.
 #include "sc_common.hpp"
 
@@ -397,11 +398,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_global_hist(const uint64_t *keys, 
 // a time; keys are ranked within their wave and digit (peers found with
 // eight ballots, a per-wave digit counter in LDS) and a block scan over
 // (digit, wave) counters gives every key its place in the tile's digit
-// order.  The tile publishes its per-digit counts, then thread d walks back
-// over earlier tiles' published words until one carries an inclusive
-// prefix, and publishes its own.  The tile is sorted in LDS and each digit's
-// run is written out contiguously at global digit base + prefix.  Within a
-// digit keys keep their input order (the sort is stable).
+// order.  The tile publishes its per-digit counts and is sorted into LDS in
+// digit order; then thread d walks back over earlier tiles' published words
+// until one carries an inclusive prefix, and publishes its own, and each
+// digit's run is written out contiguously at global digit base + prefix.
+// Within a digit keys keep their input order (the sort is stable).
 constexpr int RS_PER_WAVE = RS_TILE / RS_WAVES;
 constexpr int RS_IT = RS_PER_WAVE / WAVE; // = RS_ITEMS
 static_assert(RS_BLOCK == 256, "one thread per 8-bit digit");
@@ -446,9 +447,9 @@ __device__ inline uint32_t block_excl_sum256(uint32_t x, uint32_t *s_wsum, uint3
     return ex;
 }
 
-__global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
+template <bool VALS> __device__ __forceinline__ void onesweep_tile(const SweepArgs &a) {
     __shared__ uint64_t s_k[RS_TILE];
-    __shared__ uint32_t s_v[RS_TILE];
+    __shared__ uint32_t s_v[VALS ? RS_TILE : 1];
     // index digit*RS_WAVES + wave; counts and tile offsets stay below RS_TILE, so
     // 16 bits suffice (the smaller footprint lets three blocks share a CU)
     __shared__ uint16_t s_cnt[256 * RS_WAVES];
@@ -464,17 +465,17 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
     if (trivial) {
         for (int64_t i = base + tid; i < a.n && i < base + RS_TILE; i += RS_BLOCK) {
             a.kout[i] = a.kin[i];
-            if (a.vout) a.vout[i] = a.vin[i];
+            if (VALS) a.vout[i] = a.vin[i];
         }
         return;
     }
     uint64_t k[RS_IT];
-    uint32_t v[RS_IT], rk[RS_IT];
+    uint32_t v[VALS ? RS_IT : 1], rk[RS_IT];
 #pragma unroll
     for (int it = 0; it < RS_IT; ++it) {
         const int64_t i = base + w * RS_PER_WAVE + it * WAVE + lane;
         k[it] = i < a.n ? a.kin[i] : 0ull;
-        v[it] = (i < a.n && a.vin) ? a.vin[i] : 0u;
+        if (VALS) v[it] = i < a.n ? a.vin[i] : 0u;
     }
 #pragma unroll
     for (int it = 0; it < RS_IT; ++it) {
@@ -518,6 +519,19 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
     }
     // global base of the digit: exclusive scan of the pass histogram
     const uint32_t gx = block_excl_sum256(a.ghist[tid], s_wsum, tot);
+    // the tile in digit order in LDS (the places inside the tile do not
+    // depend on earlier tiles, and the keys' registers are free during the
+    // look-back)
+#pragma unroll
+    for (int it = 0; it < RS_IT; ++it) {
+        const int64_t i = base + w * RS_PER_WAVE + it * WAVE + lane;
+        if (i < a.n) {
+            const uint32_t d = uint32_t(k[it] >> a.shift) & a.mask;
+            const uint32_t pos = s_cnt[d * RS_WAVES + w] + rk[it];
+            s_k[pos] = k[it];
+            if (VALS) s_v[pos] = v[it];
+        }
+    }
     // look-back over earlier tiles
     unsigned long long prefix = 0;
     if (tile > 0) {
@@ -568,24 +582,21 @@ __global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) {
         __hip_atomic_store(my, a.epoch | ST_PRE | (prefix + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     s_gbase[tid] = gx + uint32_t(prefix) - ex;
-#pragma unroll
-    for (int it = 0; it < RS_IT; ++it) {
-        const int64_t i = base + w * RS_PER_WAVE + it * WAVE + lane;
-        if (i < a.n) {
-            const uint32_t d = uint32_t(k[it] >> a.shift) & a.mask;
-            const uint32_t pos = s_cnt[d * RS_WAVES + w] + rk[it];
-            s_k[pos] = k[it];
-            s_v[pos] = v[it];
-        }
-    }
     __syncthreads();
     const int m = int(a.n - base < RS_TILE ? a.n - base : RS_TILE);
     for (int i = tid; i < m; i += RS_BLOCK) {
         const uint64_t key = s_k[i];
         const uint32_t o = s_gbase[uint32_t(key >> a.shift) & a.mask] + uint32_t(i);
         a.kout[o] = key;
-        if (a.vout) a.vout[o] = s_v[i];
+        if (VALS) a.vout[o] = s_v[i];
     }
+}
+
+// keys with values; keys only without the values' LDS and registers, which
+// lets four blocks share a CU (three with values)
+__global__ __launch_bounds__(RS_BLOCK) void k_onesweep(SweepArgs a) { onesweep_tile<true>(a); }
+__global__ __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_onesweep_keys(SweepArgs a) {
+    onesweep_tile<false>(a);
 }
 
 // up to SCAN_SET arrays of at most PREF_TILES tiles each in two launches (one
@@ -747,7 +758,8 @@ bool radix_passes(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32_t *valt
         a.ghist = gh + size_t(p) * 256;
         a.status = st;
         a.epoch = (unsigned long long)rs.epoch << ST_EPOCH_SHIFT;
-        k_onesweep<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
+        if (vc) k_onesweep<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
+        else k_onesweep_keys<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
         SC_HIP(hipGetLastError());
         std::swap(kc, ko);
         std::swap(vc, vo);
@@ -801,7 +813,8 @@ bool radix_sort_u64_fixed(uint64_t *keys, uint32_t *vals, uint64_t *kalt, uint32
         a.ghist = gh + size_t(p) * 256;
         a.status = st;
         a.epoch = (unsigned long long)(p + 1) << ST_EPOCH_SHIFT;
-        k_onesweep<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
+        if (vc) k_onesweep<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
+        else k_onesweep_keys<<<unsigned(ntiles), RS_BLOCK, 0, s>>>(a);
         SC_HIP(hipGetLastError());
         std::swap(kc, ko);
         std::swap(vc, vo);
