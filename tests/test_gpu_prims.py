@@ -28,7 +28,7 @@ def _sort(name, keys):
     return dk.cpu().numpy(), dv.cpu().numpy()
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 1000, 300001])
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 3072, 3073, 4096, 4097, 300001])
 def test_stable_sort_f64(gpu, n):
     rng = np.random.default_rng(n)
     keys = np.round(rng.standard_normal(n) * 50) / 4.0  # many ties, negatives
