@@ -229,7 +229,9 @@ int segclip_copy_cover_lists(segclip_ctx *ctx, uint32_t *off, uint32_t *edges, i
 int segclip_copy_end_lists(segclip_ctx *ctx, uint32_t *off, uint32_t *edges, int64_t cap, int64_t *need);
 /* per edge "after"/"before" leaf node of its leaf end entries (-1 = none) */
 int segclip_copy_leaf_targets(segclip_ctx *ctx, int32_t *after, int32_t *before, int64_t cap, int64_t *need);
-/* unique pairs of overlay k as reference ids (2*vertex offset) and types */
+/* unique pairs of overlay k as reference ids (2*vertex offset) and types;
+ * the types are those of segclip_clip(k) when k is the overlay clipped last,
+ * else -1 (a batch keeps no per-pair types) */
 int segclip_copy_pairs(segclip_ctx *ctx, int k, int32_t *b_id, int32_t *c_id, int32_t *type, int64_t cap,
                        int64_t *need);
 /* intersections of the current overlay, in pair order */

@@ -56,6 +56,15 @@ def degenerate_cases():
     return cases
 
 
+def status_agrees(gpu_status, oracle_status):
+    """A job's status against the oracle's.  Where an intersection got no slot
+    (bit 1) the reference's later steps are undefined, so only bits 1 and 16
+    must agree; otherwise the statuses are equal."""
+    if oracle_status & 1:
+        return (gpu_status & 17) == (oracle_status & 17)
+    return gpu_status == oracle_status
+
+
 def rings_equal(g, o):
     """None when the two ring lists are bit-identical, else what differs."""
     if len(g) != len(o):

@@ -10,7 +10,7 @@ import pytest
 
 import oracle_py as O
 import segclip.synth as S
-from fixtures import degenerate_cases, mixed_overlays, rings_equal
+from fixtures import degenerate_cases, mixed_overlays, rings_equal, status_agrees
 
 pytestmark = pytest.mark.gpu
 
@@ -82,6 +82,74 @@ def test_pairs_batch_equals_each_pair_alone(gpu):
         r = O.Tree(b, [o]).clip(1, canonical=True)
         err = rings_equal(br[i], r.rings)
         assert err is None, "%s: batch vs oracle: %s" % (name, err)
+
+
+def _rect(x0, y0, w, h, t):
+    c, s = np.cos(t), np.sin(t)
+    pts = np.array([(0.0, 0.0), (w, 0.0), (w, h), (0.0, h)], dtype=np.float64)
+    return pts @ np.array([[c, s], [-s, c]]) + (x0, y0)
+
+
+def _far_pairs(count, seed):
+    """Small rings at longitude/latitude-like places (stars 1e-4 to 1e-2
+    across); every fourth pair is two rectangles turned by about 1e-9 rad,
+    whose long edges span about 1e-12 in x: below the spacing of the x keys
+    once a pair's keys are shifted by up to count * 361."""
+    rng = np.random.default_rng(seed)
+    bases, ovs = [], []
+    for i in range(count):
+        cx, cy = rng.uniform(-180.0, 180.0), rng.uniform(-85.0, 85.0)
+        s = 10.0 ** rng.uniform(-4.0, -2.0)
+        if i % 4 == 0:
+            t = 1e-9 * (1.0 + rng.uniform())
+            bases.append(_rect(cx, cy, 0.5 * s, s, t))
+            ovs.append(_rect(cx + 0.25 * s, cy + 0.3 * s, 0.5 * s, 0.2 * s, t))
+        else:
+            bases.append(S.star(int(rng.integers(8, 40)), 7000 + i, cx, cy, 0.5 * s, s))
+            ovs.append(S.star(int(rng.integers(8, 40)), 9000 + i, cx + 0.3 * s, cy + 0.2 * s, 0.5 * s, s))
+    return bases, ovs
+
+
+def test_pairs_batch_far_apart_keys(gpu):
+    """1,024 pairs spread like longitude/latitude data on the staged path (one
+    tree for all pairs, each pair's x keys shifted into its own window): every
+    pair's pair set, rings and status equal clipping it alone, also for the
+    edges whose shifted x keys coincide although their x coordinates differ;
+    its status agrees with the oracle's, and its rings equal the oracle's
+    wherever the reference's result is defined (every intersection got a
+    slot; 60 of these pairs have one that did not)."""
+    import segclip
+    bases, ovs = _far_pairs(1024, 77)
+    # the shifted keys of some edges coincide: the case this test is about
+    allx = np.concatenate([r[:, 0] for r in bases + ovs])
+    span = (allx.max() - allx.min()) + 1.0
+    merged = 0
+    for k, pair in enumerate(zip(bases, ovs)):
+        sh = k * span - allx.min()
+        for r in pair:
+            x0, x1 = r[:, 0], np.roll(r[:, 0], -1)
+            merged += int(np.sum((x0 != x1) & (x0 + sh == x1 + sh)))
+    assert merged > 100
+    print("[gpu-batch] far-apart pairs", len(bases), "merged keys", merged, flush=True)
+    cl = segclip.Clipper().set_pairs(bases, ovs).set_tuning("small_fused", 0)
+    cl.clip_batch()
+    br = cl.batch_rings()
+    st = cl.batch_status()
+    one = segclip.Clipper()
+    for i, (b, o) in enumerate(zip(bases, ovs)):
+        one.set_polygons(b, [o]).clip(1)
+        err = rings_equal(br[i], one.rings())
+        assert err is None, "pair %d: batch vs alone: %s" % (i, err)
+        assert (st[i] & 1) == (one.counts()["undefined"] & 1), i
+        pb, pc, _ = cl.pairs(i + 1)  # a batch keeps no per-pair types
+        qb, qc, _ = one.pairs(1)
+        assert np.array_equal(pb, qb) and np.array_equal(pc, qc), i
+        r = O.Tree(b, [o]).clip(1, canonical=True)
+        assert status_agrees(int(st[i]), r.status), (i, int(st[i]), r.status)
+        if r.status & 1:
+            continue
+        err = rings_equal(br[i], r.rings)
+        assert err is None, "pair %d: batch vs oracle: %s" % (i, err)
 
 
 def test_pooled_allocator_release_and_rebuild(gpu):

@@ -27,7 +27,7 @@ import pytest
 
 import oracle_py as O
 import segclip.synth as S
-from fixtures import rings_area, rings_digest, rings_equal
+from fixtures import rings_area, rings_digest, rings_equal, status_agrees
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
@@ -45,12 +45,6 @@ def _threads():
         return max(1, min(16, len(os.sched_getaffinity(0))))
     except AttributeError:
         return 8
-
-
-def _status_agrees(gpu_status, oracle_status):
-    if oracle_status & 1:
-        return (gpu_status & 17) == (oracle_status & 17)
-    return gpu_status == oracle_status
 
 
 def test_harmonic_200k_1m_bit_exact(gpu):
@@ -108,7 +102,7 @@ def _check_batch(cl, g, J, areas=None):
     assert len(rings) == len(st) == J
     bad = []
     for j in range(J):
-        ok = _status_agrees(int(st[j]), int(g["status"][j]))
+        ok = status_agrees(int(st[j]), int(g["status"][j]))
         if ok and not (g["status"][j] & 1):
             ok = (len(rings[j]) == g["rings"][j] and sum(len(x) for x in rings[j]) == g["verts"][j]
                   and rings_digest(rings[j]) == int(g["digest"][j]))
@@ -150,7 +144,7 @@ def test_one_to_many_flagged_jobs_vs_oracle(gpu):
     O.set_threads(_threads())
     for q, i in enumerate(pick):
         r = O.Tree(base, [ovs[i]]).clip(1, canonical=True)
-        assert _status_agrees(int(st[q]), r.status), (i, int(st[q]), r.status)
+        assert status_agrees(int(st[q]), r.status), (i, int(st[q]), r.status)
         if r.status & 1:
             continue
         err = rings_equal(rings[q], r.rings)
