@@ -703,8 +703,10 @@ __global__ void k_root_loop(int nr, const int *off, const double *x, const doubl
     rstart[r] = root;
 }
 
+// keep0 keeps the decisions (bytes) for the clean-up rounds: a vertex whose
+// last kept predecessor is the vertex before it is decided exactly so again
 __global__ void k_keep0(int64_t n, const int *ringid, const int *off, const int *rstart, const double *x,
-                        const double *y, uint32_t *keep) {
+                        const double *y, uint32_t *keep, uint8_t *keep0) {
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int r = ringid[i];
@@ -717,6 +719,7 @@ __global__ void k_keep0(int64_t n, const int *ringid, const int *off, const int 
         k = collinear(x, y, int(i) - 1, int(i), nx) ? 0u : 1u;
     }
     keep[i] = k;
+    keep0[i] = uint8_t(k);
 }
 
 // One clean-up round in one launch.  A vertex after its ring's root stays
@@ -746,6 +749,7 @@ struct KeepRound {
     const int *ringid, *off, *rstart;
     const double *x, *y;
     uint32_t *keep;
+    const uint8_t *keep0;       // k_keep0's decisions (predecessor = the vertex before)
     unsigned long long *status; // per tile; no word carries this round's epoch before the launch
     unsigned long long epoch;   // 1..KEEP_ROUNDS, shifted into place
     uint32_t *changed;          // set when a flag changes
@@ -873,21 +877,37 @@ __global__ __launch_bounds__(BLK) void k_keep_round(KeepRound a) {
     }
     if (fast) load_scan(); // its maximum is the one published
     uint32_t cur = s_prefix > ex ? s_prefix : ex; // mark of the last vertex kept before i
+    // the thread's initial decisions, one byte per vertex (i0 is a multiple of 8)
+    uint64_t k0w = 0;
+    if (i0 + KEEP_ITEMS <= a.n) {
+        k0w = *reinterpret_cast<const uint64_t *>(a.keep0 + i0);
+    } else {
+        for (int it = 0; it < KEEP_ITEMS; ++it)
+            if (i0 + it < a.n) k0w |= uint64_t(a.keep0[i0 + it]) << (8 * it);
+    }
     bool ch = false;
 #pragma unroll
     for (int it = 0; it < KEEP_ITEMS; ++it) {
         const int64_t i = i0 + it;
         if (i < a.n) {
-            const int r = a.ringid[i];
-            const int rs = a.rstart[r], last = a.off[r + 1] - 1;
-            if (i > rs) {
-                const int pv = int(cur) - 1; // >= rs: the ring's root is always kept
-                const int nx = i == last ? rs : int(i) + 1;
-                const uint32_t k = collinear(a.x, a.y, pv, int(i), nx) ? 0u : 1u;
-                if (k != kp[it]) {
-                    a.keep[i] = k;
-                    ch = true;
+            uint32_t k = kp[it];
+            if (cur == uint32_t(i)) {
+                // the last vertex kept before i is i - 1: the initial decision's
+                // inputs (for a vertex up to its ring's root that decision is
+                // its flag, which no round changes)
+                k = uint32_t(k0w >> (8 * it)) & 0xffu;
+            } else {
+                const int r = a.ringid[i];
+                const int rs = a.rstart[r], last = a.off[r + 1] - 1;
+                if (i > rs) {
+                    const int pv = int(cur) - 1; // >= rs: the ring's root is always kept
+                    const int nx = i == last ? rs : int(i) + 1;
+                    k = collinear(a.x, a.y, pv, int(i), nx) ? 0u : 1u;
                 }
+            }
+            if (k != kp[it]) {
+                a.keep[i] = k;
+                ch = true;
             }
         }
         if (kp[it]) cur = uint32_t(i) + 1u;
@@ -2221,6 +2241,7 @@ struct FosterScratch {
     DBuf<uint32_t> nflag, nidx;
     DBuf<int> off, rstart, ringid;
     DBuf<uint32_t> keep, kscan, rcnt, nonempty, kctl, ktstate;
+    DBuf<uint8_t> keep0;
     DBuf<unsigned long long> kstatus; // clean-up rounds: per tile status words
     // sequential route
     DBuf<SeqJob> sjobs;
@@ -2271,7 +2292,8 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
     k_ring_heads<<<grid_for(nr), BLK, 0, s>>>(nr, f.off, reinterpret_cast<uint32_t *>(ringid));
     scan_inclusive_max_u32(reinterpret_cast<uint32_t *>(ringid), reinterpret_cast<uint32_t *>(ringid), n, s);
     uint32_t *keep = f.keep.ensure(size_t(n) + 1);
-    k_keep0<<<grid_for(n), BLK, 0, s>>>(n, ringid, f.off, rstart, f.ox, f.oy, keep);
+    uint8_t *keep0 = f.keep0.ensure(size_t(n) + KEEP_ITEMS);
+    k_keep0<<<grid_for(n), BLK, 0, s>>>(n, ringid, f.off, rstart, f.ox, f.oy, keep, keep0);
     SC_HIP(hipGetLastError());
     // clean-up rounds, KEEP_BATCH per look at the per-round change flags; a
     // round after one without change changes nothing either
@@ -2295,6 +2317,7 @@ void cleanup_rings(Ctx &c, FosterScratch &f, int nr, int64_t n, int J, const uin
     kr.x = f.ox;
     kr.y = f.oy;
     kr.keep = keep;
+    kr.keep0 = keep0;
     kr.status = kst;
     kr.err = ctl + KEEP_ROUNDS;
     kr.tstate = f.ktstate.ensure(3 * size_t(ntiles)); // written by every tile in the first round
