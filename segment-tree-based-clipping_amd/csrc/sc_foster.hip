@@ -2136,6 +2136,12 @@ __device__ inline bool node_inter(const NodeArgs &a, int g, int s) {
     return (g & 1) ? a.nbrQ[jb.Q0 + s] != -100 : a.avP[jb.P0 + s] != -100;
 }
 
+// node offset of every segment: the node-start scan at the segment's first slot
+__global__ void k_seg_node_offsets(int nseg, const int64_t *segoff, const uint32_t *nidx, int64_t *nodeoff) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g <= nseg) nodeoff[g] = int64_t(nidx[segoff[g]]);
+}
+
 __global__ void k_node_flags(NodeArgs a, int64_t S, uint32_t *flag) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= S) return;
@@ -2627,18 +2633,17 @@ void label_and_traverse(Ctx &c) {
         SC_HIP(hipGetLastError());
         SC_HIP(hipMemsetAsync(flag + S, 0, sizeof(uint32_t), s));
         scan_exclusive_u32(flag, nidx, S + 1, nullptr, s);
-        // node offsets of the segments
+        // node offsets of the segments, gathered on the device and read back at once
         std::vector<int64_t> nodeoff(size_t(nseg) + 1);
+        int64_t *dnodeoff = f.nodeoff.ensure(nseg + 1);
+        k_seg_node_offsets<<<grid_for(int64_t(nseg) + 1), BLK, 0, s>>>(nseg, dseg, nidx, dnodeoff);
+        SC_HIP(hipGetLastError());
         {
-            std::vector<uint32_t> h(size_t(nseg) + 1);
-            for (int g = 0; g <= nseg; ++g)
-                SC_HIP(hipMemcpyAsync(&h[g], nidx + segoff[g], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-            SC_HIP(hipStreamSynchronize(s));
-            for (int g = 0; g <= nseg; ++g) nodeoff[g] = h[g];
+            ReadBack rb(c);
+            rb.add(nodeoff.data(), static_cast<const int64_t *>(dnodeoff), size_t(nseg) + 1);
+            rb.run();
         }
         const int64_t N = nodeoff[nseg];
-        int64_t *dnodeoff = f.nodeoff.ensure(nseg + 1);
-        SC_HIP(hipMemcpyAsync(dnodeoff, nodeoff.data(), sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice, s));
         int *nslot = f.nslot.ensure(N), *ninter = f.ninter.ensure(N), *nlab = f.nlab.ensure(N);
         int *nnb = f.nnb.ensure(N), *nlen = f.nlen.ensure(N);
         double *nx0 = f.nx0.ensure(N), *ny0 = f.ny0.ensure(N), *nx1 = f.nx1.ensure(N), *ny1 = f.ny1.ensure(N);
