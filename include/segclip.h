@@ -166,9 +166,11 @@ int segclip_batch_area_backward_device(segclip_ctx *ctx, const double *g_area, c
  * call captures again.  Returns when out4 is written.  No counterpart in the
  * reference (its per-pair loop launches every stage, clip:1026-1297). */
 int segclip_pairs_area_graph(segclip_ctx *ctx, const double *base_xy, const double *ov_xy, double *out4);
-/* Free every device buffer but the resident input rings (the next call
- * rebuilds what it needs).  segclip_memory reports the device bytes held by
- * the library (all contexts) and their peak since the last reset. */
+/* Free every device buffer of the context but the resident input rings, the
+ * last results (output rings and status words, so they stay readable) and a
+ * captured pairs graph's own buffers; the next call rebuilds what it needs.
+ * segclip_memory reports the device bytes held by the library (all contexts)
+ * and their peak since the last reset. */
 int segclip_release(segclip_ctx *ctx);
 int segclip_memory(int64_t *current, int64_t *peak, int reset_peak);
 /* Device allocator for the library's buffers (process-wide), for programs

@@ -131,9 +131,6 @@ struct Ctx {
     DBuf<double> sp_x, sp_y;
     DBuf<int> sv_b, sp_b, sv_c, sp_c;    // per slot: source vertex (local) or pair index, else -1
     DBuf<int> flags;                     // device status words
-    // labelling / traversal
-    DBuf<int> nx_b, pv_b, nx_c, pv_c;    // ring links over augmented slots (+ appended vertices)
-    DBuf<int> enex_b, enex_c, inter_b, inter_c;
     int nrings = 0;
     std::vector<int> ring_off;           // host copy of the result
     std::vector<double> ring_xy;

@@ -2391,8 +2391,13 @@ void release_foster_scratch() {
     FosterScratch &f = fscratch();
     for (DBuf<uint32_t> *b : {&f.jring, &f.crossP, &f.crossQ, &f.ccP, &f.ccQ, &f.succ, &f.val, &f.val2,
                               &f.inode, &f.dist, &f.tot, &f.isstart, &f.ringidx, &f.head, &f.piece_at,
-                              &f.jv, &f.jr, &f.keep, &f.kscan, &f.rcnt, &f.nonempty, &f.kctl, &f.ktstate})
+                              &f.jv, &f.jr, &f.keep, &f.kscan, &f.rcnt, &f.nonempty, &f.kctl, &f.ktstate,
+                              &f.jmoved})
         b->release();
+    f.keep0.release();
+    f.srec.release();
+    f.spstep.release();
+    f.semit.release();
     f.jrec.release();
     f.kstatus.release();
     f.jrec2.release();
