@@ -66,22 +66,61 @@ __global__ __launch_bounds__(BLK) void k_ring_area2(const double *__restrict__ p
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
-// twice the signed area of every output ring; one wave per ring, lanes
-// stride the ring and a fixed shuffle tree adds their sums
-__global__ __launch_bounds__(BLK) void k_out_ring_area2(int nrings, const int *__restrict__ roff,
-                                                        const double *__restrict__ x, const double *__restrict__ y,
-                                                        double *out) {
-    const int r = int(blockIdx.x) * (BLK / WAVE) + int(threadIdx.x) / WAVE;
+// Twice the signed area of every output ring, in chunks of AREA_CHUNK
+// vertices: one wave per chunk, lanes stride the chunk and a fixed shuffle
+// tree adds their sums; a ring's chunk sums are then added in chunk order.
+// A ring of at most AREA_CHUNK vertices is one chunk, summed in the order of
+// the small-pair kernel (one wave over the ring); a longer one no longer
+// waits on one wave.
+constexpr int AREA_CHUNK = 64 * WAVE;
+
+// chunk count per ring (cnt[nrings] = 0 closes the scan)
+__global__ void k_ring_chunks(int nrings, const int *__restrict__ roff, uint32_t *cnt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nrings) return;
+    if (r == nrings) {
+        cnt[r] = 0u;
+        return;
+    }
+    const int64_t len = roff[r + 1] - roff[r];
+    cnt[r] = len <= AREA_CHUNK ? 1u : uint32_t((len + AREA_CHUNK - 1) / AREA_CHUNK);
+}
+
+// chunk q of the *total (choff: exclusive scan of the chunk counts)
+__global__ __launch_bounds__(BLK) void k_out_chunk_area2(int nrings, const int *__restrict__ roff,
+                                                         const uint32_t *__restrict__ choff,
+                                                         const uint32_t *__restrict__ total, int64_t cap,
+                                                         const double *__restrict__ x, const double *__restrict__ y,
+                                                         double *part) {
+    const int64_t q = int64_t(blockIdx.x) * (BLK / WAVE) + int64_t(threadIdx.x) / WAVE;
     const int lane = int(threadIdx.x) & (WAVE - 1);
-    if (r >= nrings) return; // whole wave
-    const int64_t a = roff[r], b = roff[r + 1];
+    if (q >= int64_t(*total) || q >= cap) return; // whole wave
+    int lo = 0, hi = nrings - 1; // the ring: the last r with choff[r] <= q (counts are >= 1)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (int64_t(choff[mid]) <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t a = roff[lo], b = roff[lo + 1];
+    const int64_t c0 = a + (q - int64_t(choff[lo])) * AREA_CHUNK;
+    const int64_t c1 = c0 + AREA_CHUNK < b ? c0 + AREA_CHUNK : b;
     double s = 0.0;
-    for (int64_t i = a + lane; i < b; i += WAVE) {
+    for (int64_t i = c0 + lane; i < c1; i += WAVE) {
         const int64_t nx = i + 1 == b ? a : i + 1;
         s += x[i] * y[nx] - x[nx] * y[i];
     }
     for (int d = WAVE / 2; d > 0; d >>= 1) s += __shfl_down(s, d, WAVE);
-    if (lane == 0) out[r] = s;
+    if (lane == 0) part[q] = s;
+}
+
+__global__ void k_ring_chunk_sum(int nrings, const uint32_t *__restrict__ choff, int64_t cap,
+                                 const double *__restrict__ part, double *out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrings) return;
+    const int64_t k0 = choff[r], k1 = int64_t(choff[r + 1]) < cap ? int64_t(choff[r + 1]) : cap;
+    double s = k0 < k1 ? part[k0] : 0.0;
+    for (int64_t k = k0 + 1; k < k1; ++k) s += part[k];
+    out[r] = s;
 }
 
 // twice the unsigned area of each job's output: one block per job, thread t
@@ -240,8 +279,9 @@ __global__ void k_output_grad(ProvArgs a, int64_t v0, int64_t v1) {
 }
 
 struct AreaScratch {
-    DBuf<double> out2, ring2, barea2, carea2, out4, cI, cQ, cPr;
+    DBuf<double> out2, ring2, barea2, carea2, out4, cI, cQ, cPr, part;
     DBuf<int> bad, bin_off;
+    DBuf<uint32_t> chunks, choff, tot;
 };
 
 AreaScratch &ascratch(Ctx &) {
@@ -268,9 +308,19 @@ void batch_area(Ctx &c, double *out4) {
     const int J = js.J;
     double *out2 = a.out2.ensure(J), *barea2 = a.barea2.ensure(c.nbr), *carea2 = a.carea2.ensure(J);
     double *ring2 = a.ring2.ensure(size_t(c.nrings > 0 ? c.nrings : 1));
-    if (c.nrings > 0)
-        k_out_ring_area2<<<grid_for(c.nrings, BLK / WAVE), BLK, 0, s>>>(c.nrings, c.d_ring_off, c.d_ring_x,
-                                                                       c.d_ring_y, ring2);
+    if (c.nrings > 0) {
+        const int nr = c.nrings;
+        uint32_t *cnt = a.chunks.ensure(size_t(nr) + 1), *choff = a.choff.ensure(size_t(nr) + 1);
+        uint32_t *tot = a.tot.ensure(1);
+        // at most one chunk per ring plus one per AREA_CHUNK output vertices
+        const int64_t cap = int64_t(nr) + c.n_out / AREA_CHUNK + 1;
+        double *part = a.part.ensure(size_t(cap));
+        k_ring_chunks<<<grid_for(int64_t(nr) + 1), BLK, 0, s>>>(nr, c.d_ring_off, cnt);
+        scan_exclusive_u32(cnt, choff, int64_t(nr) + 1, tot, s);
+        k_out_chunk_area2<<<grid_for(cap, BLK / WAVE), BLK, 0, s>>>(nr, c.d_ring_off, choff, tot, cap, c.d_ring_x,
+                                                                    c.d_ring_y, part);
+        k_ring_chunk_sum<<<grid_for(nr), BLK, 0, s>>>(nr, choff, cap, part, ring2);
+    }
     k_job_out_area2<<<J, BLK, 0, s>>>(js.d_ring, ring2, out2);
     if (c.paired) k_ring_area2<<<J, BLK, 0, s>>>(c.bpoly, c.d_bvoff, js.k0 - 1, barea2);
     else k_ring_area2<<<1, BLK, 0, s>>>(c.bpoly, c.d_bvoff, 0, barea2);
