@@ -57,8 +57,20 @@ __global__ __launch_bounds__(BLK) void k_ring_area2(const double *__restrict__ p
     __shared__ double sh[BLK];
     const int r = r0 + blockIdx.x;
     const int64_t a = voff[r], n = voff[r + 1] - a - 1;
+    // thread t adds the terms of vertices t, t + BLK, ... in that order; four
+    // terms are loaded before they are added (the same order and bits)
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += BLK) {
+    int64_t i = threadIdx.x;
+    for (; i + 3 * BLK < n; i += 4 * BLK) {
+        const double *p0 = poly + 2 * (a + i), *p1 = p0 + 2 * BLK, *p2 = p1 + 2 * BLK, *p3 = p2 + 2 * BLK;
+        const double t0 = p0[0] * p0[3] - p0[2] * p0[1], t1 = p1[0] * p1[3] - p1[2] * p1[1];
+        const double t2 = p2[0] * p2[3] - p2[2] * p2[1], t3 = p3[0] * p3[3] - p3[2] * p3[1];
+        s += t0;
+        s += t1;
+        s += t2;
+        s += t3;
+    }
+    for (; i < n; i += BLK) {
         const double *p = poly + 2 * (a + i);
         s += p[0] * p[3] - p[2] * p[1];
     }
@@ -66,15 +78,42 @@ __global__ __launch_bounds__(BLK) void k_ring_area2(const double *__restrict__ p
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
-// Twice the signed area of every output ring, in chunks of AREA_CHUNK
-// vertices: one wave per chunk, lanes stride the chunk and a fixed shuffle
-// tree adds their sums; a ring's chunk sums are then added in chunk order.
-// A ring of at most AREA_CHUNK vertices is one chunk, summed in the order of
-// the small-pair kernel (one wave over the ring); a longer one no longer
-// waits on one wave.
+// Twice the signed area of every output ring.  A ring of at most AREA_CHUNK
+// vertices is summed by one wave: lanes stride the ring and a fixed shuffle
+// tree adds their sums (the order of the small-pair kernel); the waves of a
+// fixed grid take rings in turn, so a batch of many small rings costs no
+// launch of a wave per ring.  A longer ring is summed in chunks of
+// AREA_CHUNK vertices the same way, one wave per chunk, and its chunk sums
+// are added in chunk order.
 constexpr int AREA_CHUNK = 64 * WAVE;
+constexpr int AREA_WAVES = 8192; // waves of the short-ring grid (32 per CU)
 
-// chunk count per ring (cnt[nrings] = 0 closes the scan)
+__device__ inline double wave_ring_sum(const double *__restrict__ x, const double *__restrict__ y, int64_t a,
+                                       int64_t b, int64_t c0, int64_t c1, int lane) {
+    double s = 0.0;
+    for (int64_t i = c0 + lane; i < c1; i += WAVE) {
+        const int64_t nx = i + 1 == b ? a : i + 1;
+        s += x[i] * y[nx] - x[nx] * y[i];
+    }
+    for (int d = WAVE / 2; d > 0; d >>= 1) s += __shfl_down(s, d, WAVE);
+    return s;
+}
+
+__global__ __launch_bounds__(BLK) void k_out_ring_area2(int nrings, const int *__restrict__ roff,
+                                                        const double *__restrict__ x, const double *__restrict__ y,
+                                                        double *out) {
+    const int lane = int(threadIdx.x) & (WAVE - 1);
+    const int stride = int(gridDim.x) * (BLK / WAVE);
+    for (int r = int(blockIdx.x) * (BLK / WAVE) + int(threadIdx.x) / WAVE; r < nrings; r += stride) {
+        const int64_t a = roff[r], b = roff[r + 1];
+        if (b - a > AREA_CHUNK) continue; // chunked below
+        const double s = wave_ring_sum(x, y, a, b, a, b, lane);
+        if (lane == 0) out[r] = s;
+    }
+}
+
+// chunks of each long ring (0 for a ring of at most AREA_CHUNK vertices;
+// cnt[nrings] = 0 closes the scan)
 __global__ void k_ring_chunks(int nrings, const int *__restrict__ roff, uint32_t *cnt) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r > nrings) return;
@@ -83,7 +122,7 @@ __global__ void k_ring_chunks(int nrings, const int *__restrict__ roff, uint32_t
         return;
     }
     const int64_t len = roff[r + 1] - roff[r];
-    cnt[r] = len <= AREA_CHUNK ? 1u : uint32_t((len + AREA_CHUNK - 1) / AREA_CHUNK);
+    cnt[r] = len <= AREA_CHUNK ? 0u : uint32_t((len + AREA_CHUNK - 1) / AREA_CHUNK);
 }
 
 // chunk q of the *total (choff: exclusive scan of the chunk counts)
@@ -95,21 +134,16 @@ __global__ __launch_bounds__(BLK) void k_out_chunk_area2(int nrings, const int *
     const int64_t q = int64_t(blockIdx.x) * (BLK / WAVE) + int64_t(threadIdx.x) / WAVE;
     const int lane = int(threadIdx.x) & (WAVE - 1);
     if (q >= int64_t(*total) || q >= cap) return; // whole wave
-    int lo = 0, hi = nrings - 1; // the ring: the last r with choff[r] <= q (counts are >= 1)
+    int lo = 0, hi = nrings - 1; // the ring: the first r with choff[r + 1] > q
     while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (int64_t(choff[mid]) <= q) lo = mid;
-        else hi = mid - 1;
+        const int mid = (lo + hi) >> 1;
+        if (int64_t(choff[mid + 1]) > q) hi = mid;
+        else lo = mid + 1;
     }
     const int64_t a = roff[lo], b = roff[lo + 1];
     const int64_t c0 = a + (q - int64_t(choff[lo])) * AREA_CHUNK;
     const int64_t c1 = c0 + AREA_CHUNK < b ? c0 + AREA_CHUNK : b;
-    double s = 0.0;
-    for (int64_t i = c0 + lane; i < c1; i += WAVE) {
-        const int64_t nx = i + 1 == b ? a : i + 1;
-        s += x[i] * y[nx] - x[nx] * y[i];
-    }
-    for (int d = WAVE / 2; d > 0; d >>= 1) s += __shfl_down(s, d, WAVE);
+    const double s = wave_ring_sum(x, y, a, b, c0, c1, lane);
     if (lane == 0) part[q] = s;
 }
 
@@ -118,7 +152,8 @@ __global__ void k_ring_chunk_sum(int nrings, const uint32_t *__restrict__ choff,
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrings) return;
     const int64_t k0 = choff[r], k1 = int64_t(choff[r + 1]) < cap ? int64_t(choff[r + 1]) : cap;
-    double s = k0 < k1 ? part[k0] : 0.0;
+    if (k0 >= k1) return; // a short ring
+    double s = part[k0];
     for (int64_t k = k0 + 1; k < k1; ++k) s += part[k];
     out[r] = s;
 }
@@ -310,10 +345,13 @@ void batch_area(Ctx &c, double *out4) {
     double *ring2 = a.ring2.ensure(size_t(c.nrings > 0 ? c.nrings : 1));
     if (c.nrings > 0) {
         const int nr = c.nrings;
+        const int64_t wgrid = std::min<int64_t>(ceil_div(int64_t(nr), BLK / WAVE), AREA_WAVES / (BLK / WAVE));
+        k_out_ring_area2<<<unsigned(wgrid), BLK, 0, s>>>(nr, c.d_ring_off, c.d_ring_x, c.d_ring_y, ring2);
         uint32_t *cnt = a.chunks.ensure(size_t(nr) + 1), *choff = a.choff.ensure(size_t(nr) + 1);
         uint32_t *tot = a.tot.ensure(1);
-        // at most one chunk per ring plus one per AREA_CHUNK output vertices
-        const int64_t cap = int64_t(nr) + c.n_out / AREA_CHUNK + 1;
+        // a long ring has at most one chunk more than its AREA_CHUNK-vertex
+        // multiples, and there are at most n_out / AREA_CHUNK long rings
+        const int64_t cap = 2 * (c.n_out / AREA_CHUNK) + 1;
         double *part = a.part.ensure(size_t(cap));
         k_ring_chunks<<<grid_for(int64_t(nr) + 1), BLK, 0, s>>>(nr, c.d_ring_off, cnt);
         scan_exclusive_u32(cnt, choff, int64_t(nr) + 1, tot, s);
