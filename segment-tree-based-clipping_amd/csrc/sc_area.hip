@@ -225,6 +225,7 @@ __global__ void k_input_ring_grad(int64_t n, const int *__restrict__ off, int R,
 struct ProvArgs {
     int J, nrings;
     const int *jring, *roff, *src;
+    const uint32_t *rid; // output ring of every output vertex
     const double *ox, *oy;
     const double *cI;
     const double *ring2; // signed 2A of every output ring
@@ -285,10 +286,17 @@ __device__ void pair_grad(const ProvArgs &a, int j, int p, bool onP, double gx, 
     add_c(a, j, c + 1 == nc ? 0 : c + 1, gc2x, gc2y);
 }
 
+// ring heads for the output ring index (rid zero-filled, then max-scanned)
+__global__ void k_out_ring_heads(int nrings, const int *__restrict__ roff, uint32_t *rid) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrings) return;
+    if (roff[r + 1] > roff[r]) rid[roff[r]] = uint32_t(r);
+}
+
 __global__ void k_output_grad(ProvArgs a, int64_t v0, int64_t v1) {
     int64_t i = v0 + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= v1) return;
-    const int r = seg_of(a.roff, a.nrings, i);
+    const int r = int(a.rid[i]);
     const int j = seg_of(a.jring, a.J, r);
     const int64_t rb = a.roff[r], re = a.roff[r + 1];
     const int64_t nx = i + 1 == re ? rb : i + 1, pv = i == rb ? re - 1 : i - 1;
@@ -316,7 +324,7 @@ __global__ void k_output_grad(ProvArgs a, int64_t v0, int64_t v1) {
 struct AreaScratch {
     DBuf<double> out2, ring2, barea2, carea2, out4, cI, cQ, cPr, part;
     DBuf<int> bad, bin_off;
-    DBuf<uint32_t> chunks, choff, tot;
+    DBuf<uint32_t> chunks, choff, tot, rid;
 };
 
 AreaScratch &ascratch(Ctx &) {
@@ -421,8 +429,19 @@ void batch_area_backward(Ctx &c, const double *g_area, const double *g_iou, doub
     p.grad_c = grad_c;
     p.bad = a.bad.ensure(1);
     SC_HIP(hipMemsetAsync(p.bad, 0, sizeof(int), s));
-    if (c.n_out > 0 && c.nrings > 0)
+    if (c.n_out > 0 && c.nrings > 0) {
+        // the ring of every output vertex: ring heads, then a max-scan
+        uint32_t *rid = a.rid.ensure(size_t(c.n_out));
+        {
+            FillBatch fb(s);
+            fb.add(rid, c.n_out, 0u);
+            fb.flush();
+        }
+        k_out_ring_heads<<<grid_for(c.nrings), BLK, 0, s>>>(c.nrings, c.d_ring_off, rid);
+        scan_inclusive_max_u32(rid, rid, c.n_out, s);
+        p.rid = rid;
         k_output_grad<<<grid_for(c.n_out), BLK, 0, s>>>(p, 0, c.n_out);
+    }
     SC_HIP(hipGetLastError());
     int bad = 0;
     {
